@@ -1,0 +1,816 @@
+// C ABI of libborn: plan management, argument validation and dispatch.
+//
+// Replaces the MEX gateway combined_quick_gpu (combined_quick_gpu.cu:66-1094):
+// the validation and error identifiers follow cu:201-441 and h:31-44, the
+// observation ordering cu:773/:978, the forward occupancy rule cu:454-499.
+// Tables are converted and uploaded once per plan instead of once per call
+// (cu:33 TODO, cu:548-593, :817-847).
+#include "born.h"
+#include "born_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace born {
+
+static thread_local std::string g_last_error;
+
+struct Error : std::runtime_error {
+  int status;
+  Error(int s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+#define HIP_CHECK(expr)                                                                          \
+  do {                                                                                           \
+    hipError_t e_ = (expr);                                                                      \
+    if (e_ != hipSuccess)                                                                        \
+      throw Error(BORN_ERR_DEVICE, std::string("HIP error '") + hipGetErrorString(e_) + "' in " + \
+                                       #expr + " at " + __FILE__ + ":" + std::to_string(__LINE__)); \
+  } while (0)
+
+template <typename T>
+static void dfree(T*& p) {
+  if (p) hipFree((void*)p);
+  p = nullptr;
+}
+
+template <typename T>
+static T* dalloc(size_t n) {
+  if (n == 0) n = 1;
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, n * sizeof(T));
+  if (e != hipSuccess) {
+    hipGetLastError();
+    throw Error(BORN_ERR_OUT_OF_MEMORY, "device allocation of " + std::to_string(n * sizeof(T)) + " bytes failed");
+  }
+  return (T*)p;
+}
+
+// Small index tables: synchronous copies, so the host vectors may die on return.
+template <typename T>
+static T* dupload(const std::vector<T>& v, hipStream_t) {
+  T* p = dalloc<T>(v.size());
+  if (!v.empty()) HIP_CHECK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return p;
+}
+
+static void free_workspace(Plan& P) {
+  dfree(P.d_chunks);
+  dfree(P.d_col_pair);
+  dfree(P.d_pr_chunk);
+  dfree(P.d_pr_pos);
+  dfree(P.d_vpart);
+  dfree(P.d_V);
+  dfree(P.d_W);
+  dfree(P.d_gm);
+  P.ws_rhs = -1;
+  P.chunks.clear();
+  P.col_pair.clear();
+}
+
+Plan::~Plan() {
+  hipSetDevice(device);
+  if (stream) hipStreamSynchronize(stream);
+  free_workspace(*this);
+  dfree(d_h);
+  dfree(d_p);
+  dfree(d_shift);
+  dfree(d_ext);
+  dfree(d_pair_fg);
+  dfree(d_pair_meas);
+  dfree(d_obs_mk);
+  dfree(d_obs_f);
+  dfree(d_mk_meta);
+  dfree(d_mk_off);
+  dfree(d_fmeas);
+  dfree(d_act);
+  dfree(d_pref);
+  dfree(d_meas_pair_base);
+  dfree(d_pair_list_off);
+  dfree(d_pair_list);
+  dfree(d_slot);
+  if (d_io_in) hipFree(d_io_in);
+  if (d_io_out) hipFree(d_io_out);
+  if (stream) hipStreamDestroy(stream);
+}
+
+// ------------------------------------------------------------ validation
+static int idx1(double v, int n, const char* what) {
+  // 1-based double index as MATLAB holds it (cu:317, :329-333)
+  if (!(v >= 1.0) || !(v <= (double)n) || v != std::floor(v))
+    throw Error(BORN_ERR_INVALID_ARGUMENT, std::string(what) + " index out of range or not an integer");
+  return (int)v - 1;
+}
+
+// Measurement/mix index structures shared by both creation paths.
+static void build_index_structures(Plan& P, int N_meas, const born_measurement_desc* meas) {
+  if (N_meas <= 0 || !meas) throw Error(BORN_ERR_NO_OPERATOR_BORN, "no selected measurements");
+  P.meas_pair_base.assign(N_meas, 0);
+  P.meas_Nfm.assign(N_meas, 0);
+  P.pair_fg.clear();
+  P.pair_meas.clear();
+  for (int m = 0; m < N_meas; ++m) {
+    const born_measurement_desc& M = meas[m];
+    if (M.N_f_unique <= 0 || !M.indices_f_to_unique)
+      throw Error(BORN_ERR_INVALID_ARGUMENT, "measurement without unique frequencies");
+    P.meas_pair_base[m] = (int)P.pair_fg.size();
+    P.meas_Nfm[m] = M.N_f_unique;
+    for (int fm = 0; fm < M.N_f_unique; ++fm) {
+      P.pair_fg.push_back(idx1(M.indices_f_to_unique[fm], P.N_fu, "indices_f_to_unique{m}"));
+      P.pair_meas.push_back(m);
+    }
+  }
+  P.N_pairs = (int)P.pair_fg.size();
+  // pairs grouped by global frequency
+  P.fg_pair_off.assign(P.N_fu + 1, 0);
+  for (int fg : P.pair_fg) P.fg_pair_off[fg + 1]++;
+  for (int f = 0; f < P.N_fu; ++f) P.fg_pair_off[f + 1] += P.fg_pair_off[f];
+  P.fg_pairs.assign(P.N_pairs, 0);
+  {
+    std::vector<int> fill(P.fg_pair_off.begin(), P.fg_pair_off.end() - 1);
+    for (int pr = 0; pr < P.N_pairs; ++pr) P.fg_pairs[fill[P.pair_fg[pr]]++] = pr;
+  }
+  // mixes
+  P.mk_meas.clear(); P.mk_Nf.clear(); P.mk_Nact.clear();
+  P.mk_obs_off.clear(); P.mk_f_off.clear(); P.mk_a_off.clear(); P.mk_pref_off.clear();
+  P.fmeas.clear(); P.act.clear(); P.pref.clear();
+  int64_t obs = 0;
+  for (int m = 0; m < N_meas; ++m) {
+    const born_measurement_desc& M = meas[m];
+    if (M.N_mix <= 0 || !M.mixes) throw Error(BORN_ERR_INVALID_ARGUMENT, "measurement without mixes");
+    for (int k = 0; k < M.N_mix; ++k) {
+      const born_mix_desc& X = M.mixes[k];
+      if (X.N_f <= 0 || X.N_active <= 0 || !X.indices_f_to_unique || !X.indices_active || !X.prefactors)
+        throw Error(BORN_ERR_INVALID_ARGUMENT, "incomplete mix description");
+      P.mk_meas.push_back(m);
+      P.mk_Nf.push_back(X.N_f);
+      P.mk_Nact.push_back(X.N_active);
+      P.mk_obs_off.push_back(obs);
+      P.mk_f_off.push_back((int64_t)P.fmeas.size());
+      P.mk_a_off.push_back((int64_t)P.act.size());
+      P.mk_pref_off.push_back((int64_t)(P.pref.size() / 2));
+      for (int f = 0; f < X.N_f; ++f) P.fmeas.push_back(idx1(X.indices_f_to_unique[f], M.N_f_unique, "indices_f_to_unique{k}"));
+      std::vector<char> seen(P.N_e, 0);
+      for (int a = 0; a < X.N_active; ++a) {
+        const int e = idx1(X.indices_active[a], P.N_e, "indices_active");
+        if (seen[e]) throw Error(BORN_ERR_UNSUPPORTED, "an element appears twice in one mix");
+        seen[e] = 1;
+        P.act.push_back(e);
+      }
+      P.pref.insert(P.pref.end(), X.prefactors, X.prefactors + 2 * (size_t)X.N_f * X.N_active);
+      obs += X.N_f;
+    }
+  }
+  P.N_mk = (int)P.mk_meas.size();
+  P.N_obs = obs;
+}
+
+static void upload_index_structures(Plan& P, hipStream_t s) {
+  P.d_pair_fg = dupload(P.pair_fg, s);
+  P.d_pair_meas = dupload(P.pair_meas, s);
+  P.d_meas_pair_base = dupload(P.meas_pair_base, s);
+  std::vector<int> obs_mk(P.N_obs), obs_f(P.N_obs);
+  std::vector<int> meta(4 * (size_t)P.N_mk);
+  std::vector<int64_t> off(4 * (size_t)P.N_mk);
+  for (int mk = 0; mk < P.N_mk; ++mk) {
+    meta[4 * mk + 0] = P.mk_meas[mk];
+    meta[4 * mk + 1] = P.mk_Nf[mk];
+    meta[4 * mk + 2] = P.mk_Nact[mk];
+    meta[4 * mk + 3] = 0;
+    off[4 * mk + 0] = P.mk_obs_off[mk];
+    off[4 * mk + 1] = P.mk_f_off[mk];
+    off[4 * mk + 2] = P.mk_a_off[mk];
+    off[4 * mk + 3] = P.mk_pref_off[mk];
+    for (int f = 0; f < P.mk_Nf[mk]; ++f) {
+      obs_mk[P.mk_obs_off[mk] + f] = mk;
+      obs_f[P.mk_obs_off[mk] + f] = f;
+    }
+  }
+  P.d_obs_mk = dupload(obs_mk, s);
+  P.d_obs_f = dupload(obs_f, s);
+  P.d_mk_meta = dupload(meta, s);
+  P.d_mk_off = dupload(off, s);
+  P.d_fmeas = dupload(P.fmeas, s);
+  P.d_act = dupload(P.act, s);
+  std::vector<double2> pref(P.pref.size() / 2);
+  for (size_t i = 0; i < pref.size(); ++i) pref[i] = make_double2(P.pref[2 * i], P.pref[2 * i + 1]);
+  P.d_pref = dupload(pref, s);
+  // un-mixing gather lists
+  std::vector<int> cnt(P.N_pairs + 1, 0);
+  for (int mk = 0; mk < P.N_mk; ++mk)
+    for (int f = 0; f < P.mk_Nf[mk]; ++f) cnt[P.meas_pair_base[P.mk_meas[mk]] + P.fmeas[P.mk_f_off[mk] + f] + 1]++;
+  for (int i = 0; i < P.N_pairs; ++i) cnt[i + 1] += cnt[i];
+  std::vector<int2> lst(cnt[P.N_pairs]);
+  {
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for (int mk = 0; mk < P.N_mk; ++mk)
+      for (int f = 0; f < P.mk_Nf[mk]; ++f) {
+        const int pr = P.meas_pair_base[P.mk_meas[mk]] + P.fmeas[P.mk_f_off[mk] + f];
+        lst[fill[pr]++] = make_int2(mk, f);
+      }
+  }
+  P.d_pair_list_off = dupload(cnt, s);
+  P.d_pair_list = dupload(lst, s);
+  std::vector<int> slot((size_t)P.N_mk * P.N_e, -1);
+  for (int mk = 0; mk < P.N_mk; ++mk)
+    for (int a = 0; a < P.mk_Nact[mk]; ++a) slot[(size_t)mk * P.N_e + P.act[P.mk_a_off[mk] + a]] = a;
+  P.d_slot = dupload(slot, s);
+}
+
+// The lateral Toeplitz structure of setup_grid_symmetric.shift_lateral
+// (setup_grid_symmetric.m:95-241): in 2-D, element e sees point (i, z) at
+// h_ref point (ext[i - e*Np + (N_e-1)*Np], z).  Derived from the map itself
+// and verified entry by entry; any mismatch selects the general path.
+static void detect_structure(Plan& P) {
+  P.structured = false;
+  if (P.nd != 2 || P.Nex != P.N_e || P.N_e > 128) return;
+  const int Nx = P.Nx, Nz = P.Nz;
+  const int* sh = P.h_shift.data();
+  int Np = 1;
+  if (P.N_e >= 2) {
+    const int p = P.N_pts - 1;
+    Np = p - sh[(size_t)1 * P.N_pts + p];
+    if (Np < 1 || Np > Nx - 1) return;
+  }
+  const int N_ext = Nx + (P.N_e - 1) * Np;
+  std::vector<int> ext(N_ext, -1);
+  for (int e = 0; e < P.N_e; ++e)
+    for (int i = 0; i < Nx; ++i) {
+      const int t = i - e * Np + (P.N_e - 1) * Np;
+      const int v = sh[(size_t)e * P.N_pts + i];
+      if (v < 0 || v >= Nx) return;
+      if (ext[t] < 0) ext[t] = v;
+      else if (ext[t] != v) return;
+    }
+  for (int t = 0; t < N_ext; ++t)
+    if (ext[t] < 0) return;
+  for (int e = 0; e < P.N_e; ++e)
+    for (int z = 0; z < Nz; ++z)
+      for (int i = 0; i < Nx; ++i) {
+        const int t = i - e * Np + (P.N_e - 1) * Np;
+        if (sh[(size_t)e * P.N_pts + (size_t)z * Nx + i] != z * Nx + ext[t]) return;
+      }
+  P.Np = Np;
+  P.N_ext = N_ext;
+  P.Ne_pad = ((P.N_e + 15) / 16) * 16;
+  // forward LDS: polyphase row + column tile (and the cross-wave reduction)
+  const int S = (Nx + 15) / 16, XP = 16 * S;
+  const int QL = (XP - 1) / Np + P.Ne_pad;
+  const size_t body = ((size_t)Np * QL + (size_t)XP * (NC + 1)) * sizeof(float2);
+  const size_t red = (size_t)4 * (P.Ne_pad / 16) * 8 * 64 * sizeof(float);
+  if (std::max(body, red) > 160 * 1024) return;
+  P.d_ext = dupload(ext, P.stream);
+  P.structured = true;
+}
+
+static bool use_structured(const Plan& P) { return P.structured && !P.force_general; }
+
+static void ensure_shift_on_device(Plan& P) {
+  if (P.d_shift) return;
+  P.d_shift = dupload(P.h_shift, P.stream);
+}
+
+// Chunks of <= NC (pair, rhs) columns per frequency, rhs-major.
+static void ensure_workspace(Plan& P, int N_rhs) {
+  if (P.ws_rhs == N_rhs) return;
+  free_workspace(P);
+  P.chunks.clear();
+  P.col_pair.clear();
+  std::vector<int> pr_chunk((size_t)P.N_pairs * N_rhs, -1), pr_pos((size_t)P.N_pairs * N_rhs, -1);
+  for (int r = 0; r < N_rhs; ++r)
+    for (int fg = 0; fg < P.N_fu; ++fg) {
+      const int b = P.fg_pair_off[fg], e = P.fg_pair_off[fg + 1];
+      for (int j = b; j < e; j += NC) {
+        Chunk c;
+        c.fg = fg;
+        c.r = r;
+        c.col_start = (int)P.col_pair.size();
+        c.ncols = std::min(NC, e - j);
+        for (int q = 0; q < c.ncols; ++q) {
+          const int pr = P.fg_pairs[j + q];
+          pr_chunk[(size_t)pr * N_rhs + r] = (int)P.chunks.size();
+          pr_pos[(size_t)pr * N_rhs + r] = q;
+          P.col_pair.push_back(pr);
+        }
+        P.chunks.push_back(c);
+      }
+    }
+  const int n_chunks = (int)P.chunks.size();
+  const int target = 4096;
+  int NZB = std::max(1, std::min(P.Nz, target / std::max(1, n_chunks)));
+  P.ZR = (P.Nz + NZB - 1) / NZB;
+  P.NZB = (P.Nz + P.ZR - 1) / P.ZR;
+  P.d_chunks = dupload(P.chunks, P.stream);
+  P.d_col_pair = dupload(P.col_pair, P.stream);
+  P.d_pr_chunk = dupload(pr_chunk, P.stream);
+  P.d_pr_pos = dupload(pr_pos, P.stream);
+  if (P.structured) P.d_vpart = dalloc<float2>((size_t)n_chunks * P.NZB * P.Ne_pad * NC);
+  P.d_V = dalloc<double2>((size_t)P.N_pairs * P.N_e * N_rhs);
+  P.d_W = dalloc<float2>((size_t)P.N_pairs * P.N_e * N_rhs);
+  P.d_gm = dalloc<float2>((size_t)N_rhs * P.N_pts);
+  HIP_CHECK(hipDeviceSynchronize());  // workspace ready before work on any stream
+  P.ws_rhs = N_rhs;
+}
+
+static void ensure_io(Plan& P, size_t in_bytes, size_t out_bytes) {
+  const size_t need = std::max(in_bytes, out_bytes);
+  if (P.io_bytes >= need) return;
+  if (P.d_io_in) hipFree(P.d_io_in);
+  if (P.d_io_out) hipFree(P.d_io_out);
+  P.d_io_in = P.d_io_out = nullptr;
+  P.io_bytes = 0;
+  P.d_io_in = dalloc<char>(need);
+  P.d_io_out = dalloc<char>(need);
+  P.io_bytes = need;
+}
+
+static void common_dims(Plan& P, int nd, const int32_t* Npa, const int32_t* Nea, int N_points, int N_elements) {
+  if (nd != 2 && nd != 3) throw Error(BORN_ERR_NO_SYMMETRIC_GRIDS_SETUP, "grid must have 2 or 3 dimensions");
+  P.nd = nd;
+  P.Nx = Npa[0];
+  if (nd == 2) {
+    P.Ny = 1;
+    P.Nz = Npa[1];
+  } else {
+    P.Ny = Npa[1];
+    P.Nz = Npa[2];
+  }
+  P.Nex = Nea[0];
+  P.Ney = (nd == 3) ? Nea[1] : 1;
+  if (P.Nx <= 0 || P.Ny <= 0 || P.Nz <= 0 || (int64_t)P.Nx * P.Ny * P.Nz != N_points)
+    throw Error(BORN_ERR_INVALID_ARGUMENT, "N_points_axis does not match N_points");
+  if (P.Nex <= 0 || P.Ney <= 0 || P.Nex * P.Ney != N_elements)
+    throw Error(BORN_ERR_INVALID_ARGUMENT, "N_elements_axis does not match N_elements");
+  P.N_pts = N_points;
+  P.N_e = N_elements;
+}
+
+static void finish_plan(Plan& P) {
+  detect_structure(P);
+  if (!P.structured) ensure_shift_on_device(P);
+  set_max_lds(160 * 1024);
+  HIP_CHECK(hipStreamSynchronize(P.stream));
+}
+
+// ------------------------------------------------------------ plan paths
+static Plan* create_tables(const born_tables_desc* d) {
+  if (!d) throw Error(BORN_ERR_NUMBER_ARGUMENTS, "descriptor is NULL");
+  if (!d->h_ref) throw Error(BORN_ERR_NO_COMPLEX_DOUBLES, "operator_born.sequence.h_ref must be complex doubles!");
+  if (!d->indices_grid_FOV_shift)
+    throw Error(BORN_ERR_NO_SYMMETRIC_GRIDS_SETUP, "sequence.setup must be scattering.sequences.setups.setup_grid_symmetric!");
+  if (d->precision != BORN_PRECISION_FP32 && d->precision != BORN_PRECISION_SPLIT_BF16 &&
+      d->precision != BORN_PRECISION_SPLIT_FP16)
+    throw Error(BORN_ERR_INVALID_ARGUMENT, "unknown precision");
+  std::unique_ptr<Plan> P(new Plan());
+  P->device = d->device;
+  int ndev = 0;
+  HIP_CHECK(hipGetDeviceCount(&ndev));
+  if (d->device < 0 || d->device >= ndev) throw Error(BORN_ERR_DEVICE, "invalid device index");
+  HIP_CHECK(hipSetDevice(d->device));
+  HIP_CHECK(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
+  P->precision = d->precision;
+  if (d->N_f_unique <= 0) throw Error(BORN_ERR_INVALID_ARGUMENT, "N_f_unique must be positive");
+  P->N_fu = d->N_f_unique;
+  common_dims(*P, d->N_dimensions, d->N_points_axis, d->N_elements_axis, d->N_points, d->N_elements);
+  build_index_structures(*P, d->N_measurements, d->measurements);
+  for (int m = 0; m < d->N_measurements; ++m)
+    if (!d->measurements[m].p_incident)
+      throw Error(BORN_ERR_NO_COMPLEX_DOUBLES, "operator_born.incident_waves.p_incident must be complex doubles!");
+  // shift map, 1-based double -> 0-based int [e][p]
+  P->h_shift.resize((size_t)P->N_e * P->N_pts);
+  for (int e = 0; e < P->N_e; ++e)
+    for (int p = 0; p < P->N_pts; ++p)
+      P->h_shift[(size_t)e * P->N_pts + p] = idx1(d->indices_grid_FOV_shift[(size_t)e * P->N_pts + p], P->N_pts,
+                                                  "indices_grid_FOV_shift");
+  upload_index_structures(*P, P->stream);
+  // tables: FP64 column-major -> FP32 row-major on the device, through one
+  // staging buffer sized for the largest matrix
+  size_t max_rows = P->N_fu;
+  for (int m = 0; m < d->N_measurements; ++m) max_rows = std::max(max_rows, (size_t)d->measurements[m].N_f_unique);
+  double2* stage = dalloc<double2>(max_rows * P->N_pts);
+  try {
+    P->d_h = dalloc<float2>((size_t)P->N_fu * P->N_pts);
+    HIP_CHECK(hipMemcpyAsync(stage, d->h_ref, sizeof(double2) * (size_t)P->N_fu * P->N_pts, hipMemcpyHostToDevice,
+                             P->stream));
+    launch_convert_transpose(stage, P->N_fu, P->N_pts, P->d_h, P->stream);
+    P->d_p = dalloc<float2>((size_t)P->N_pairs * P->N_pts);
+    for (int m = 0; m < d->N_measurements; ++m) {
+      const int nf = d->measurements[m].N_f_unique;
+      HIP_CHECK(hipMemcpyAsync(stage, d->measurements[m].p_incident, sizeof(double2) * (size_t)nf * P->N_pts,
+                               hipMemcpyHostToDevice, P->stream));
+      launch_convert_transpose(stage, nf, P->N_pts, P->d_p + (size_t)P->meas_pair_base[m] * P->N_pts, P->stream);
+    }
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(P->stream));
+  } catch (...) {
+    hipFree(stage);
+    throw;
+  }
+  hipFree(stage);
+  finish_plan(*P);
+  return P.release();
+}
+
+// shift_lateral restated (setup_grid_symmetric.m:171-233): subscripts shifted
+// by (element subscript - 1) * points-per-pitch, mirrored when <= 0 about
+// index_offset = N_points_axis - (N_elements_axis - 1) * Np + 1.
+static void compute_shift_map(Plan& P, const int* Np_axis) {
+  const int nlat = P.nd - 1;
+  const int Na[3] = {P.Nx, P.Ny, P.Nz};
+  const int Ne[2] = {P.Nex, P.Ney};
+  P.h_shift.resize((size_t)P.N_e * P.N_pts);
+  for (int d = 0; d < nlat; ++d)
+    if (Na[d] - (Ne[d] - 1) * Np_axis[d] < 1)
+      throw Error(BORN_ERR_UNSUPPORTED, "lateral FOV narrower than the aperture: the mirror map is undefined");
+  for (int e = 0; e < P.N_e; ++e) {
+    const int es[2] = {e % P.Nex, e / P.Nex};
+    for (int p = 0; p < P.N_pts; ++p) {
+      int s[3];
+      s[0] = p % P.Nx;
+      const int rest = p / P.Nx;
+      if (P.nd == 3) {
+        s[1] = rest % P.Ny;
+        s[2] = rest / P.Ny;
+      } else {
+        s[1] = rest;
+      }
+      int lin = 0, mul = 1;
+      for (int d = 0; d < P.nd; ++d) {
+        int v = s[d] + 1;  // 1-based subscript
+        if (d < nlat) {
+          v -= es[d] * Np_axis[d];
+          if (v <= 0) v = (Na[d] - (Ne[d] - 1) * Np_axis[d] + 1) - v;
+        }
+        lin += (v - 1) * mul;
+        mul *= (P.nd == 3) ? Na[d] : (d == 0 ? P.Nx : P.Nz);
+      }
+      P.h_shift[(size_t)e * P.N_pts + p] = lin;
+    }
+  }
+}
+
+static Plan* create_geometry(const born_geometry_desc* d) {
+  if (!d) throw Error(BORN_ERR_NUMBER_ARGUMENTS, "descriptor is NULL");
+  std::unique_ptr<Plan> P(new Plan());
+  int ndev = 0;
+  HIP_CHECK(hipGetDeviceCount(&ndev));
+  if (d->device < 0 || d->device >= ndev) throw Error(BORN_ERR_DEVICE, "invalid device index");
+  P->device = d->device;
+  HIP_CHECK(hipSetDevice(d->device));
+  HIP_CHECK(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
+  P->precision = d->precision;
+  if (d->N_f_unique <= 0 || !d->frequencies || !d->wavenumbers)
+    throw Error(BORN_ERR_INVALID_ARGUMENT, "frequencies and wavenumbers are required");
+  P->N_fu = d->N_f_unique;
+  int Np_axis[2] = {1, 1};
+  const int nlat = d->N_dimensions - 1;
+  const int N_elem = d->N_elements_axis[0] * (d->N_dimensions == 3 ? d->N_elements_axis[1] : 1);
+  int N_points = d->N_points_axis[0] * d->N_points_axis[1] * (d->N_dimensions == 3 ? d->N_points_axis[2] : 1);
+  common_dims(*P, d->N_dimensions, d->N_points_axis, d->N_elements_axis, N_points, N_elem);
+  for (int a = 0; a < nlat; ++a) {
+    const double r = d->pitch_axis[a] / d->delta_axis[a];
+    const int n = (int)std::lround(r);
+    if (n < 1 || std::fabs(r - n) > 1e-9 * n)
+      throw Error(BORN_ERR_NO_SYMMETRIC_GRIDS_SETUP, "lateral spacing is not an integer fraction of the pitch");
+    Np_axis[a] = n;
+  }
+  build_index_structures(*P, d->N_measurements, d->measurements);
+  compute_shift_map(*P, Np_axis);
+  upload_index_structures(*P, P->stream);
+  hipStream_t s = P->stream;
+
+  GeomDev g;
+  g.nd = d->N_dimensions;
+  g.Nx = P->Nx; g.Ny = P->Ny; g.Nz = P->Nz;
+  g.dx = d->delta_axis[0];
+  g.dy = (g.nd == 3) ? d->delta_axis[1] : 0.0;
+  g.dz = d->delta_axis[g.nd - 1];
+  g.z0 = d->z0;
+  g.Nex = P->Nex; g.Ney = P->Ney;
+  g.pitch_x = d->pitch_axis[0];
+  g.pitch_y = (g.nd == 3) ? d->pitch_axis[1] : 0.0;
+  g.Nfx = d->N_face_points_axis[0];
+  g.Nfy = (g.nd == 3) ? d->N_face_points_axis[1] : 1;
+  g.dfx = d->face_delta_axis[0];
+  g.dfy = (g.nd == 3) ? d->face_delta_axis[1] : 0.0;
+  if (g.Nfx <= 0 || g.Nfy <= 0) throw Error(BORN_ERR_INVALID_ARGUMENT, "face discretization must be positive");
+
+  std::vector<double2> kv(P->N_fu);
+  for (int f = 0; f < P->N_fu; ++f) kv[f] = make_double2(d->wavenumbers[2 * f], d->wavenumbers[2 * f + 1]);
+  double2* dk = dupload(kv, s);
+  double2* dfw = nullptr;
+  if (d->face_weights) {
+    const size_t nfw = (size_t)g.Nfx * g.Nfy * P->N_fu;
+    std::vector<double2> fw(nfw);
+    for (size_t i = 0; i < nfw; ++i) fw[i] = make_double2(d->face_weights[2 * i], d->face_weights[2 * i + 1]);
+    dfw = dupload(fw, s);
+  }
+  double2* h64 = nullptr;
+  double2* h64tx = nullptr;
+  int* d_meas_Nfm = nullptr;
+  int* d_tx_off = nullptr;
+  int* d_tx_el = nullptr;
+  double2* d_vd = nullptr;
+  int* d_vd_off = nullptr;
+  try {
+    h64 = dalloc<double2>((size_t)P->N_fu * P->N_pts);
+    g.filt_kind = d->filter_rx_kind;
+    g.filt_param = d->filter_rx_param;
+    launch_build_h(g, P->N_fu, dk, dfw, h64, s);
+    P->d_h = dalloc<float2>((size_t)P->N_fu * P->N_pts);
+    launch_f64_to_f32(h64, (size_t)P->N_fu * P->N_pts, P->d_h, s);
+    const double2* htx = h64;
+    if (d->filter_tx_kind != d->filter_rx_kind || d->filter_tx_param != d->filter_rx_param) {
+      h64tx = dalloc<double2>((size_t)P->N_fu * P->N_pts);
+      g.filt_kind = d->filter_tx_kind;
+      g.filt_param = d->filter_tx_param;
+      launch_build_h(g, P->N_fu, dk, dfw, h64tx, s);
+      htx = h64tx;
+    }
+    // transmit lists and normal velocities
+    if (!d->N_tx_active || !d->tx_indices_active || !d->v_d)
+      throw Error(BORN_ERR_INVALID_ARGUMENT, "transmit settings are required");
+    std::vector<int> tx_off(d->N_measurements + 1, 0), tx_el, vd_off(d->N_measurements, 0);
+    std::vector<double2> vd;
+    for (int m = 0; m < d->N_measurements; ++m) {
+      const int n = d->N_tx_active[m];
+      if (n <= 0) throw Error(BORN_ERR_INVALID_ARGUMENT, "no active transmit element");
+      for (int a = 0; a < n; ++a) tx_el.push_back(idx1(d->tx_indices_active[m][a], P->N_e, "tx indices_active"));
+      tx_off[m + 1] = tx_off[m] + n;
+      vd_off[m] = (int)vd.size();
+      const size_t cnt = (size_t)n * P->meas_Nfm[m];
+      for (size_t i = 0; i < cnt; ++i) vd.push_back(make_double2(d->v_d[m][2 * i], d->v_d[m][2 * i + 1]));
+    }
+    d_meas_Nfm = dupload(P->meas_Nfm, s);
+    d_tx_off = dupload(tx_off, s);
+    d_tx_el = dupload(tx_el, s);
+    d_vd = dupload(vd, s);
+    d_vd_off = dupload(vd_off, s);
+    ensure_shift_on_device(*P);
+    P->d_p = dalloc<float2>((size_t)P->N_pairs * P->N_pts);
+    launch_synth_pinc(*P, htx, d_meas_Nfm, d_tx_off, d_tx_el, d_vd, d_vd_off, s);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(s));
+  } catch (...) {
+    hipStreamSynchronize(s);
+    dfree(dk); dfree(dfw); dfree(h64); dfree(h64tx);
+    dfree(d_meas_Nfm); dfree(d_tx_off); dfree(d_tx_el); dfree(d_vd); dfree(d_vd_off);
+    throw;
+  }
+  dfree(dk); dfree(dfw); dfree(h64); dfree(h64tx);
+  dfree(d_meas_Nfm); dfree(d_tx_off); dfree(d_tx_el); dfree(d_vd); dfree(d_vd_off);
+  finish_plan(*P);
+  if (P->structured) dfree(P->d_shift);  // only the general path reads it
+  return P.release();
+}
+
+// ----------------------------------------------------------- apply paths
+static void forward_device(Plan& P, const float2* gm, int N_rhs, void* u, bool out_f64, hipStream_t s) {
+  if (use_structured(P)) {
+    launch_fwd_structured(P, gm, N_rhs, s);
+    launch_fwd_reduce(P, N_rhs, s);
+  } else {
+    ensure_shift_on_device(P);
+    launch_fwd_general(P, gm, N_rhs, s);
+  }
+  launch_fwd_mix(P, N_rhs, u, out_f64, s);
+  HIP_CHECK(hipGetLastError());
+}
+
+static void adjoint_device(Plan& P, const float2* u, int N_rhs, float2* g, hipStream_t s) {
+  launch_adj_unmix(P, u, N_rhs, s);
+  if (use_structured(P)) {
+    launch_adj_structured(P, N_rhs, g, s);
+  } else {
+    ensure_shift_on_device(P);
+    launch_adj_general(P, N_rhs, g, s);
+  }
+  HIP_CHECK(hipGetLastError());
+}
+
+static void check_precision(const Plan& P) {
+  if (P.precision != BORN_PRECISION_FP32)
+    throw Error(BORN_ERR_UNSUPPORTED, "split-precision contraction is not available in this build");
+}
+
+static void host_forward(Plan& P, const double* gamma, int N_rhs, double* u) {
+  if (!gamma || !u) throw Error(BORN_ERR_NO_VALID_DOUBLE_MATRIX, "input must be a double matrix with N_points rows!");
+  if (N_rhs <= 0) throw Error(BORN_ERR_NO_VALID_DOUBLE_MATRIX, "input must be a nonempty double matrix!");
+  check_precision(P);
+  HIP_CHECK(hipSetDevice(P.device));
+  const size_t npr = (size_t)P.N_pts * N_rhs;
+  // occupancy on the doubles, as the MEX does before converting (cu:466-488)
+  std::vector<float2> buf(npr);
+  size_t n_occ = 0;
+  for (int p = 0; p < P.N_pts; ++p) {
+    bool occ = false;
+    for (int r = 0; r < N_rhs && !occ; ++r) {
+      const double* v = gamma + 2 * ((size_t)r * P.N_pts + p);
+      occ = std::fabs(v[0]) > 2.220446049250313080847e-16 || std::fabs(v[1]) > 2.220446049250313080847e-16;
+    }
+    n_occ += occ;
+    for (int r = 0; r < N_rhs; ++r) {
+      const double* v = gamma + 2 * ((size_t)r * P.N_pts + p);
+      buf[(size_t)r * P.N_pts + p] = occ ? make_float2((float)v[0], (float)v[1]) : make_float2(0.f, 0.f);
+    }
+  }
+  const size_t nout = (size_t)P.N_obs * N_rhs;
+  if (n_occ == 0) {  // quick exit (cu:496): zeros
+    std::memset(u, 0, sizeof(double) * 2 * nout);
+    return;
+  }
+  ensure_workspace(P, N_rhs);
+  ensure_io(P, npr * sizeof(float2), nout * sizeof(double2));
+  HIP_CHECK(hipMemcpyAsync(P.d_io_in, buf.data(), npr * sizeof(float2), hipMemcpyHostToDevice, P.stream));
+  forward_device(P, (const float2*)P.d_io_in, N_rhs, P.d_io_out, true, P.stream);
+  HIP_CHECK(hipMemcpyAsync(u, P.d_io_out, nout * sizeof(double2), hipMemcpyDeviceToHost, P.stream));
+  HIP_CHECK(hipStreamSynchronize(P.stream));
+}
+
+static void host_adjoint(Plan& P, const double* uin, int N_rhs, double* g) {
+  if (!uin || !g) throw Error(BORN_ERR_NO_VALID_DOUBLE_MATRIX, "input must be a double matrix with N_observations rows!");
+  if (N_rhs <= 0) throw Error(BORN_ERR_NO_VALID_DOUBLE_MATRIX, "input must be a nonempty double matrix!");
+  check_precision(P);
+  HIP_CHECK(hipSetDevice(P.device));
+  const size_t nin = (size_t)P.N_obs * N_rhs, nout = (size_t)P.N_pts * N_rhs;
+  std::vector<float2> buf(std::max(nin, nout));
+  for (size_t i = 0; i < nin; ++i) buf[i] = make_float2((float)uin[2 * i], (float)uin[2 * i + 1]);
+  ensure_workspace(P, N_rhs);
+  ensure_io(P, nin * sizeof(float2), nout * sizeof(float2));
+  HIP_CHECK(hipMemcpyAsync(P.d_io_in, buf.data(), nin * sizeof(float2), hipMemcpyHostToDevice, P.stream));
+  adjoint_device(P, (const float2*)P.d_io_in, N_rhs, (float2*)P.d_io_out, P.stream);
+  HIP_CHECK(hipMemcpyAsync(buf.data(), P.d_io_out, nout * sizeof(float2), hipMemcpyDeviceToHost, P.stream));
+  HIP_CHECK(hipStreamSynchronize(P.stream));
+  for (size_t i = 0; i < nout; ++i) {
+    g[2 * i] = buf[i].x;
+    g[2 * i + 1] = buf[i].y;
+  }
+}
+
+template <typename F>
+static int guarded(F&& f) {
+  try {
+    f();
+    return BORN_OK;
+  } catch (const Error& e) {
+    g_last_error = e.what();
+    return e.status;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return BORN_ERR_OUT_OF_MEMORY;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return BORN_ERR_DEVICE;
+  }
+}
+
+}  // namespace born
+
+using namespace born;
+
+extern "C" {
+
+int born_plan_create_tables(const born_tables_desc* desc, born_plan_t* plan) {
+  if (!plan) return BORN_ERR_NUMBER_ARGUMENTS;
+  *plan = nullptr;
+  return guarded([&] { *plan = reinterpret_cast<born_plan_t>(create_tables(desc)); });
+}
+
+int born_plan_create_geometry(const born_geometry_desc* desc, born_plan_t* plan) {
+  if (!plan) return BORN_ERR_NUMBER_ARGUMENTS;
+  *plan = nullptr;
+  return guarded([&] { *plan = reinterpret_cast<born_plan_t>(create_geometry(desc)); });
+}
+
+int born_plan_destroy(born_plan_t plan) {
+  return guarded([&] { delete reinterpret_cast<Plan*>(plan); });
+}
+
+int born_plan_size(born_plan_t plan, int64_t* N_observations, int64_t* N_points) {
+  return guarded([&] {
+    if (!plan) throw Error(BORN_ERR_NO_OPERATOR_BORN, "plan is NULL");
+    Plan* P = reinterpret_cast<Plan*>(plan);
+    if (N_observations) *N_observations = P->N_obs;
+    if (N_points) *N_points = P->N_pts;
+  });
+}
+
+int born_plan_set_precision(born_plan_t plan, int32_t precision) {
+  return guarded([&] {
+    if (!plan) throw Error(BORN_ERR_NO_OPERATOR_BORN, "plan is NULL");
+    if (precision < 0 || precision > 2) throw Error(BORN_ERR_INVALID_ARGUMENT, "unknown precision");
+    reinterpret_cast<Plan*>(plan)->precision = precision;
+  });
+}
+
+int born_forward(born_plan_t plan, const double* gamma, int32_t N_rhs, double* u) {
+  return guarded([&] {
+    if (!plan) throw Error(BORN_ERR_NO_OPERATOR_BORN, "operator_born must be a single scattering.operator_born!");
+    host_forward(*reinterpret_cast<Plan*>(plan), gamma, N_rhs, u);
+  });
+}
+
+int born_adjoint(born_plan_t plan, const double* u, int32_t N_rhs, double* gamma_hat) {
+  return guarded([&] {
+    if (!plan) throw Error(BORN_ERR_NO_OPERATOR_BORN, "operator_born must be a single scattering.operator_born!");
+    host_adjoint(*reinterpret_cast<Plan*>(plan), u, N_rhs, gamma_hat);
+  });
+}
+
+int born_forward_dev(born_plan_t plan, const void* gamma_c64, int32_t N_rhs, void* u_c64, void* stream) {
+  return guarded([&] {
+    if (!plan) throw Error(BORN_ERR_NO_OPERATOR_BORN, "plan is NULL");
+    if (!gamma_c64 || !u_c64 || N_rhs <= 0) throw Error(BORN_ERR_NO_VALID_DOUBLE_MATRIX, "invalid device buffers");
+    Plan& P = *reinterpret_cast<Plan*>(plan);
+    check_precision(P);
+    HIP_CHECK(hipSetDevice(P.device));
+    hipStream_t s = (hipStream_t)stream;
+    ensure_workspace(P, N_rhs);
+    launch_mask_gamma((const float2*)gamma_c64, P.N_pts, N_rhs, P.d_gm, s);
+    forward_device(P, P.d_gm, N_rhs, u_c64, false, s);
+  });
+}
+
+int born_adjoint_dev(born_plan_t plan, const void* u_c64, int32_t N_rhs, void* gamma_hat_c64, void* stream) {
+  return guarded([&] {
+    if (!plan) throw Error(BORN_ERR_NO_OPERATOR_BORN, "plan is NULL");
+    if (!u_c64 || !gamma_hat_c64 || N_rhs <= 0) throw Error(BORN_ERR_NO_VALID_DOUBLE_MATRIX, "invalid device buffers");
+    Plan& P = *reinterpret_cast<Plan*>(plan);
+    check_precision(P);
+    HIP_CHECK(hipSetDevice(P.device));
+    ensure_workspace(P, N_rhs);
+    adjoint_device(P, (const float2*)u_c64, N_rhs, (float2*)gamma_hat_c64, (hipStream_t)stream);
+  });
+}
+
+int born_plan_get_table(born_plan_t plan, int32_t which, int32_t index, void* out_c64, int64_t capacity) {
+  return guarded([&] {
+    if (!plan || !out_c64) throw Error(BORN_ERR_INVALID_ARGUMENT, "plan and output are required");
+    Plan& P = *reinterpret_cast<Plan*>(plan);
+    HIP_CHECK(hipSetDevice(P.device));
+    const float2* src = nullptr;
+    int rows = 0;
+    if (which == 0) {
+      src = P.d_h;
+      rows = P.N_fu;
+    } else if (which == 1) {
+      if (index < 0 || index >= (int)P.meas_Nfm.size()) throw Error(BORN_ERR_INVALID_ARGUMENT, "measurement index");
+      src = P.d_p + (size_t)P.meas_pair_base[index] * P.N_pts;
+      rows = P.meas_Nfm[index];
+    } else {
+      throw Error(BORN_ERR_INVALID_ARGUMENT, "unknown table");
+    }
+    const size_t n = (size_t)rows * P.N_pts;
+    if ((int64_t)n > capacity) throw Error(BORN_ERR_INVALID_ARGUMENT, "output capacity too small");
+    std::vector<float2> tmp(n);
+    HIP_CHECK(hipMemcpy(tmp.data(), src, n * sizeof(float2), hipMemcpyDeviceToHost));
+    float2* o = (float2*)out_c64;
+    for (int r = 0; r < rows; ++r)
+      for (int p = 0; p < P.N_pts; ++p) o[(size_t)p * rows + r] = tmp[(size_t)r * P.N_pts + p];
+  });
+}
+
+int born_plan_path(born_plan_t plan) {
+  if (!plan) return -1;
+  return use_structured(*reinterpret_cast<Plan*>(plan)) ? 0 : 1;
+}
+
+int born_plan_force_general(born_plan_t plan, int32_t on) {
+  return guarded([&] {
+    if (!plan) throw Error(BORN_ERR_INVALID_ARGUMENT, "plan is NULL");
+    Plan& P = *reinterpret_cast<Plan*>(plan);
+    HIP_CHECK(hipSetDevice(P.device));
+    P.force_general = on != 0;
+    if (P.force_general) ensure_shift_on_device(P);
+    HIP_CHECK(hipStreamSynchronize(P.stream));
+  });
+}
+
+const char* born_last_error(void) { return g_last_error.c_str(); }
+
+const char* born_status_id(int status) {
+  switch (status) {
+    case BORN_OK: return "";
+    case BORN_ERR_NUMBER_ARGUMENTS: return "combined_quick_gpu:NumberArguments";
+    case BORN_ERR_NO_SYMMETRIC_GRIDS_SETUP: return "combined_quick_gpu:NoSymmetricGridsSetup";
+    case BORN_ERR_NO_GPU_ACTIVE: return "combined_quick_gpu:NoScatteringOptionsGPUActive";
+    case BORN_ERR_NO_COMPLEX_DOUBLES: return "combined_quick_gpu:NoComplexDoubles";
+    case BORN_ERR_NO_OPERATOR_BORN: return "combined_quick_gpu:NoOperatorBorn";
+    case BORN_ERR_NO_NUMERIC_SCALAR_MODE: return "combined_quick_gpu:NoNumericScalarMode";
+    case BORN_ERR_NO_VALID_DOUBLE_MATRIX: return "combined_quick_gpu:NoValidDoubleMatrix";
+    case BORN_ERR_CONVERSION_FAILED: return "combined_quick_gpu:ConversionFailed";
+    case BORN_ERR_DEVICE: return "adjoint_quick_gpu:ErrorCUDA";
+    case BORN_ERR_INVALID_ARGUMENT: return "born:InvalidArgument";
+    case BORN_ERR_UNSUPPORTED: return "born:Unsupported";
+    case BORN_ERR_OUT_OF_MEMORY: return "born:OutOfMemory";
+    default: return "born:Unknown";
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,127 @@
+// Internal plan layout and kernel launchers of libborn (not part of the ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace born {
+
+// Column tiles of the grid contraction: one chunk is (frequency, rhs, <=16
+// measurement/frequency pairs).
+constexpr int NC = 16;
+
+struct Chunk {
+  int fg;         // global unique frequency index
+  int r;          // right-hand side
+  int col_start;  // into col_pair
+  int ncols;      // <= NC
+};
+
+// Device tables and index maps.  All device arrays are owned by the plan.
+struct Plan {
+  int device = 0;
+  int precision = 0;
+  bool force_general = false;
+  bool structured = false;  // 2-D lateral Toeplitz path usable
+
+  int nd = 2;
+  int Nx = 0, Ny = 1, Nz = 0;
+  int N_pts = 0;
+  int N_e = 0, Nex = 0, Ney = 1;
+  int N_fu = 0;
+  int Np = 1;       // grid points per pitch (lateral, 2-D structured path)
+  int N_ext = 0;    // extended lateral length Nx + (Nex-1)*Np
+
+  // (measurement, unique-frequency-of-measurement) pairs: one p_inc row each
+  int N_pairs = 0;
+  std::vector<int> pair_fg, pair_meas;
+  std::vector<int> meas_pair_base;  // first pair of each measurement
+  std::vector<int> meas_Nfm;
+  // pairs grouped by global frequency
+  std::vector<int> fg_pair_off, fg_pairs;
+
+  // mixes, flattened over (measurement, mix)
+  int N_mk = 0;
+  int64_t N_obs = 0;
+  std::vector<int> mk_meas, mk_Nf, mk_Nact;
+  std::vector<int64_t> mk_obs_off, mk_f_off, mk_a_off, mk_pref_off;
+  std::vector<int> fmeas;   // 0-based measurement-unique freq per (mk, f)
+  std::vector<int> act;     // 0-based element per (mk, a)
+  std::vector<double> pref; // complex (re, im) per (mk, f, a), f fastest
+
+  // device copies
+  float2* d_h = nullptr;          // [N_fu][N_pts]
+  float2* d_p = nullptr;          // [N_pairs][N_pts]
+  int* d_shift = nullptr;         // [N_e][N_pts], 0-based (general path)
+  int* d_ext = nullptr;           // [N_ext] lateral map (structured path)
+  int* d_pair_fg = nullptr;
+  int* d_pair_meas = nullptr;
+  int* d_obs_mk = nullptr;        // [N_obs]
+  int* d_obs_f = nullptr;         // [N_obs]
+  int* d_mk_meta = nullptr;       // [N_mk][4]: meas, Nf, Nact, pad
+  int64_t* d_mk_off = nullptr;    // [N_mk][4]: obs_off, f_off, a_off, pref_off
+  int* d_fmeas = nullptr;
+  int* d_act = nullptr;
+  double2* d_pref = nullptr;
+  int* d_meas_pair_base = nullptr;
+  // adjoint un-mixing gather lists: for each pair, the (mk, f) that read it
+  int* d_pair_list_off = nullptr; // [N_pairs+1]
+  int2* d_pair_list = nullptr;    // (mk, f)
+  int* d_slot = nullptr;          // [N_mk][N_e]: active slot of element or -1
+  std::vector<int> h_shift;       // host copy of the 0-based shift map [N_e][N_pts]
+
+  // per-N_rhs workspace
+  int ws_rhs = -1;
+  std::vector<Chunk> chunks;
+  std::vector<int> col_pair;
+  Chunk* d_chunks = nullptr;
+  int* d_col_pair = nullptr;
+  int* d_pr_chunk = nullptr;      // [N_pairs*N_rhs]: chunk of (pair, r)
+  int* d_pr_pos = nullptr;        // [N_pairs*N_rhs]: column position in chunk
+  int NZB = 1, ZR = 1;            // forward z-blocking
+  int Ne_pad = 16;
+  float2* d_vpart = nullptr;      // [n_chunks][NZB][Ne_pad][NC]
+  double2* d_V = nullptr;         // [N_pairs][N_e][N_rhs]
+  float2* d_W = nullptr;          // [N_pairs][N_e][N_rhs]
+  float2* d_gm = nullptr;         // masked gamma [N_rhs][N_pts]
+  size_t io_bytes = 0;
+  void* d_io_in = nullptr;        // host-path staging
+  void* d_io_out = nullptr;
+  hipStream_t stream = nullptr;   // host-path stream
+
+  ~Plan();
+};
+
+// ------------------------------------------------------------------ kernels
+void launch_convert_transpose(const double2* src, int rows, int cols, float2* dst, hipStream_t s);
+void launch_f64_to_f32(const double2* src, size_t n, float2* dst, hipStream_t s);
+void launch_mask_gamma(const float2* g, int N_pts, int N_rhs, float2* gm, hipStream_t s);
+
+void launch_fwd_structured(const Plan& P, const float2* gm, int N_rhs, hipStream_t s);
+void launch_fwd_general(const Plan& P, const float2* gm, int N_rhs, hipStream_t s);
+void launch_fwd_reduce(const Plan& P, int N_rhs, hipStream_t s);
+void launch_fwd_mix(const Plan& P, int N_rhs, void* u, bool out_f64, hipStream_t s);
+
+void launch_adj_unmix(const Plan& P, const float2* u, int N_rhs, hipStream_t s);
+void launch_adj_structured(const Plan& P, int N_rhs, float2* g_out, hipStream_t s);
+void launch_adj_general(const Plan& P, int N_rhs, float2* g_out, hipStream_t s);
+
+struct GeomDev {
+  int nd;
+  int Nx, Ny, Nz;
+  double dx, dy, dz, z0;
+  int Nex, Ney;
+  double pitch_x, pitch_y;
+  int Nfx, Nfy;
+  double dfx, dfy;
+  int filt_kind;
+  double filt_param;
+};
+void launch_build_h(const GeomDev& g, int N_fu, const double2* k, const double2* face_w,
+                    double2* h64, hipStream_t s);
+void launch_synth_pinc(const Plan& P, const double2* h64, const int* d_meas_Nfm, const int* d_tx_off,
+                       const int* d_tx_el, const double2* d_vd, const int* d_vd_off, hipStream_t s);
+void set_max_lds(size_t bytes);
+
+}  // namespace born

@@ -1,0 +1,305 @@
+"""Host-side mirror of scattering.operator_born's operator path.
+
+`OperatorBorn` owns a libborn plan (device-resident tables).  It is built
+either from the tables an operator_born object holds -- exactly what the MEX
+reads (combined_quick_gpu.cu:201-391, :817-823) -- or from a `SequenceSpec`,
+in which case the transfer function and the incident fields are built on the
+device.  `combined_quick_gpu` keeps the MEX's calling convention and error
+identifiers (combined_quick_gpu.m:11-23, cu:201-441); `op_A` is the solver
+contract of operator.combined_scalar (operator.m:1019-1072).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import BornError, check, dptr
+from . import physics
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _c128_colmajor(mat) -> np.ndarray:
+    """Column-major memory of a 2-D complex matrix as an interleaved float64
+    buffer (MATLAB's layout)."""
+    m = np.asarray(mat, dtype=np.complex128)
+    if m.ndim == 1:
+        m = m[:, None]
+    return np.ascontiguousarray(m.T).view(np.float64)
+
+
+class _DescKeeper:
+    """Keeps the numpy buffers a descriptor points at alive for one call."""
+
+    def __init__(self):
+        self.refs = []
+
+    def keep(self, a):
+        self.refs.append(a)
+        return a
+
+
+def _measurement_descs(keeper, meas_list, with_p_inc: bool):
+    """Build born_measurement_desc[] from objects with the attributes of the
+    operator's tables: idx_f_to_global (1-based), p_incident (N_f_meas x
+    N_pts), mixes[k].idx_f_to_meas / idx_active (1-based) / prefactors."""
+    M = (_lib.MeasDesc * len(meas_list))()
+    for m, ms in enumerate(meas_list):
+        mixes = (_lib.MixDesc * len(ms.mixes))()
+        for k, mx in enumerate(ms.mixes):
+            fi = keeper.keep(_f64(mx.idx_f_to_meas))
+            ai = keeper.keep(_f64(mx.idx_active))
+            pf = keeper.keep(_c128_colmajor(mx.prefactors))
+            mixes[k] = _lib.MixDesc(len(fi), dptr(fi), len(ai), dptr(ai), dptr(pf))
+        keeper.keep(mixes)
+        fg = keeper.keep(_f64(ms.idx_f_to_global))
+        p = keeper.keep(_c128_colmajor(ms.p_incident)) if with_p_inc else None
+        M[m] = _lib.MeasDesc(len(fg), dptr(fg), dptr(p), len(ms.mixes), mixes)
+    keeper.keep(M)
+    return M
+
+
+class _Mix:
+    def __init__(self, idx_f_to_meas, idx_active, prefactors):
+        self.idx_f_to_meas = idx_f_to_meas
+        self.idx_active = idx_active
+        self.prefactors = prefactors
+
+
+class _Meas:
+    def __init__(self, idx_f_to_global, mixes):
+        self.idx_f_to_global = idx_f_to_global
+        self.p_incident = None
+        self.mixes = mixes
+
+
+class OperatorBorn:
+    """Born scattering operator A (N_obs x N_pts) on one MI355X."""
+
+    def __init__(self, plan: int, n_obs: int, n_pts: int, device: int, meta: Optional[dict] = None):
+        self._plan = C.c_void_p(plan)
+        self.N_observations = n_obs
+        self.N_points = n_pts
+        self.device = device
+        self.meta = meta or {}
+
+    # ---------------------------------------------------------- creation
+    @classmethod
+    def from_tables(cls, T, device: int = 0, precision: str = "fp32") -> "OperatorBorn":
+        """Plan from an operator's tables: T.h_ref (N_f_unique x N_pts),
+        T.shift (N_pts x N_e, 1-based), T.meas[m] (idx_f_to_global,
+        p_incident, mixes), T.N_points_axis, T.N_elements_axis."""
+        L = _lib.lib()
+        keeper = _DescKeeper()
+        h = keeper.keep(_c128_colmajor(T.h_ref))
+        sh = keeper.keep(np.ascontiguousarray(np.asarray(T.shift, dtype=np.float64).T))
+        M = _measurement_descs(keeper, T.meas, True)
+        npa = list(T.N_points_axis) + [0] * (3 - len(T.N_points_axis))
+        nea = list(T.N_elements_axis) + [1] * (2 - len(T.N_elements_axis))
+        n_pts, n_e = np.asarray(T.shift).shape
+        d = _lib.TablesDesc(len(T.N_points_axis), (C.c_int32 * 3)(*npa), (C.c_int32 * 2)(*nea),
+                            n_pts, n_e, np.asarray(T.h_ref).shape[0], dptr(h), dptr(sh.reshape(-1)),
+                            len(T.meas), M, device, _lib.PRECISIONS[precision])
+        plan = C.c_void_p()
+        check(L.born_plan_create_tables(C.byref(d), C.byref(plan)))
+        return cls._wrap(plan, device, {"rows_h": np.asarray(T.h_ref).shape[0],
+                                        "rows_p": [len(ms.idx_f_to_global) for ms in T.meas]})
+
+    @classmethod
+    def from_sequence(cls, spec, device: int = 0, precision: str = "fp32") -> "OperatorBorn":
+        """Plan from a SequenceSpec: h_ref and every p_incident are built on
+        the device (FP64 evaluation, FP32 storage)."""
+        L = _lib.lib()
+        keeper = _DescKeeper()
+        meas = []
+        for m in range(spec.N_measurements):
+            fidx = spec.measurement_frequencies(m)
+            mixes = []
+            for k, mix in enumerate(spec.mixes[m]):
+                fk = np.arange(len(fidx)) if mix.indices_f is None else np.asarray(mix.indices_f)
+                mixes.append(_Mix(fk + 1, np.asarray(mix.indices_active) + 1, physics.prefactors_mix(spec, m, k)))
+            meas.append(_Meas(fidx + 1, mixes))
+        M = _measurement_descs(keeper, meas, False)
+        f = keeper.keep(_f64(spec.frequencies))
+        k = keeper.keep(np.ascontiguousarray(physics.wavenumbers(spec.frequencies, spec.medium)).view(np.float64))
+        nd = spec.fov.N_dimensions
+        npa = list(spec.fov.N_points_axis) + [0] * (3 - nd)
+        dla = list(spec.fov.delta_axis) + [0.0] * (3 - nd)
+        nea = list(spec.array.N_elements_axis) + [1] * (2 - spec.array.N_dimensions)
+        pitch = list(spec.array.pitch_axis) + [0.0] * (2 - spec.array.N_dimensions)
+        nfp = list(spec.array.N_face_points_axis) + [1] * (2 - spec.array.N_dimensions)
+        fdl = [w / n for w, n in zip(spec.array.width_axis, spec.array.N_face_points_axis)] + [0.0] * (2 - spec.array.N_dimensions)
+        ntx = (C.c_int32 * spec.N_measurements)()
+        txi = (C.POINTER(C.c_double) * spec.N_measurements)()
+        vd = (C.POINTER(C.c_double) * spec.N_measurements)()
+        for m in range(spec.N_measurements):
+            idx = keeper.keep(_f64(np.asarray(spec.waves[m].indices_active) + 1))
+            v = keeper.keep(_c128_colmajor(physics.normal_velocities(spec, m)))
+            ntx[m] = len(idx)
+            txi[m] = dptr(idx)
+            vd[m] = dptr(v)
+        d = _lib.GeometryDesc(
+            nd, (C.c_int32 * 3)(*npa), (C.c_double * 3)(*dla), spec.fov.z0,
+            (C.c_int32 * 2)(*nea), (C.c_double * 2)(*pitch), (C.c_int32 * 2)(*nfp), (C.c_double * 2)(*fdl),
+            None, len(f), dptr(f), dptr(k),
+            _lib.FILTER_CODES[spec.filter_rx.kind], _lib.FILTER_CODES[spec.filter_tx.kind],
+            spec.filter_rx.param, spec.filter_tx.param,
+            spec.N_measurements, M, ntx, txi, vd, device, _lib.PRECISIONS[precision])
+        plan = C.c_void_p()
+        check(L.born_plan_create_geometry(C.byref(d), C.byref(plan)))
+        return cls._wrap(plan, device, {"spec": spec.name, "rows_h": len(spec.frequencies),
+                                        "rows_p": [len(spec.measurement_frequencies(m))
+                                                   for m in range(spec.N_measurements)]})
+
+    @classmethod
+    def _wrap(cls, plan, device, meta=None):
+        L = _lib.lib()
+        n_obs, n_pts = C.c_int64(), C.c_int64()
+        check(L.born_plan_size(plan, C.byref(n_obs), C.byref(n_pts)))
+        return cls(plan.value, n_obs.value, n_pts.value, device, meta)
+
+    def close(self):
+        if self._plan is not None and self._plan.value:
+            check(_lib.lib().born_plan_destroy(self._plan))
+        self._plan = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------------------------------------------------- queries
+    @property
+    def size(self):
+        """[N_observations, N_points] (op_A mode 0, operator.m:1035-1046)."""
+        return (self.N_observations, self.N_points)
+
+    @property
+    def path(self) -> str:
+        return {0: "structured", 1: "general"}.get(_lib.lib().born_plan_path(self._plan), "invalid")
+
+    def force_general(self, on: bool = True):
+        check(_lib.lib().born_plan_force_general(self._plan, int(on)))
+
+    def set_precision(self, precision: str):
+        check(_lib.lib().born_plan_set_precision(self._plan, _lib.PRECISIONS[precision]))
+
+    def table(self, which: str, index: int = 0) -> np.ndarray:
+        """Device copy (complex64) of h_ref ('h_ref', N_f_unique x N_pts) or of
+        p_incident of measurement `index` ('p_incident', N_f_meas x N_pts)."""
+        w = {"h_ref": 0, "p_incident": 1}[which]
+        rows = self.meta["rows_h"] if w == 0 else self.meta["rows_p"][index]
+        cap = rows * self.N_points
+        buf = np.empty(cap, dtype=np.complex64)
+        check(_lib.lib().born_plan_get_table(self._plan, w, index, buf.ctypes.data_as(C.c_void_p), cap))
+        return buf.reshape(self.N_points, rows).T.copy()
+
+    # ---------------------------------------------------------- host FP64 path
+    def forward(self, gamma) -> np.ndarray:
+        """u_M = A gamma with MEX semantics (occupancy rule, zero output when
+        nothing is occupied).  gamma: (N_pts,) or (N_pts, N_rhs)."""
+        g = np.asarray(gamma)
+        squeeze = g.ndim == 1
+        g2 = g[:, None] if squeeze else g
+        if g2.shape[0] != self.N_points:
+            raise BornError(7, "combined_quick_gpu:NoValidDoubleMatrix",
+                            f"input must be a double matrix with {self.N_points} rows!")
+        nrhs = g2.shape[1]
+        gin = _c128_colmajor(g2)
+        out = np.empty((nrhs, self.N_observations), dtype=np.complex128)
+        check(_lib.lib().born_forward(self._plan, dptr(gin), nrhs, dptr(out.view(np.float64))))
+        res = out.T
+        return res[:, 0].copy() if squeeze else np.ascontiguousarray(res)
+
+    def adjoint(self, u) -> np.ndarray:
+        """gamma_hat = A^H u_M over all N_pts points."""
+        uu = np.asarray(u)
+        squeeze = uu.ndim == 1
+        u2 = uu[:, None] if squeeze else uu
+        if u2.shape[0] != self.N_observations:
+            raise BornError(7, "combined_quick_gpu:NoValidDoubleMatrix",
+                            f"input must be a double matrix with {self.N_observations} rows!")
+        nrhs = u2.shape[1]
+        uin = _c128_colmajor(u2)
+        out = np.empty((nrhs, self.N_points), dtype=np.complex128)
+        check(_lib.lib().born_adjoint(self._plan, dptr(uin), nrhs, dptr(out.view(np.float64))))
+        res = out.T
+        return res[:, 0].copy() if squeeze else np.ascontiguousarray(res)
+
+    def __matmul__(self, gamma):
+        return self.forward(gamma)
+
+    mtimes = forward
+
+    # ---------------------------------------------------------- device path
+    def forward_dev(self, gamma, out=None, stream=None):
+        """Device forward on torch complex64 tensors: gamma (N_pts,) or
+        (N_rhs, N_pts) contiguous -> u (N_obs,) or (N_rhs, N_obs)."""
+        import torch
+        squeeze = gamma.dim() == 1
+        g = gamma.reshape(1, -1) if squeeze else gamma
+        self._check_dev(g, self.N_points)
+        nrhs = g.shape[0]
+        if out is None:
+            out = torch.empty((nrhs, self.N_observations), dtype=torch.complex64, device=g.device)
+        s = torch.cuda.current_stream(g.device).cuda_stream if stream is None else stream
+        check(_lib.lib().born_forward_dev(self._plan, C.c_void_p(g.data_ptr()), nrhs,
+                                          C.c_void_p(out.data_ptr()), C.c_void_p(s)))
+        return out.reshape(-1) if squeeze else out
+
+    def adjoint_dev(self, u, out=None, stream=None):
+        """Device adjoint: u (N_obs,) or (N_rhs, N_obs) complex64 -> gamma_hat."""
+        import torch
+        squeeze = u.dim() == 1
+        uu = u.reshape(1, -1) if squeeze else u
+        self._check_dev(uu, self.N_observations)
+        nrhs = uu.shape[0]
+        if out is None:
+            out = torch.empty((nrhs, self.N_points), dtype=torch.complex64, device=uu.device)
+        s = torch.cuda.current_stream(uu.device).cuda_stream if stream is None else stream
+        check(_lib.lib().born_adjoint_dev(self._plan, C.c_void_p(uu.data_ptr()), nrhs,
+                                          C.c_void_p(out.data_ptr()), C.c_void_p(s)))
+        return out.reshape(-1) if squeeze else out
+
+    @staticmethod
+    def _check_dev(t, rows):
+        import torch
+        if t.dtype != torch.complex64 or not t.is_cuda or not t.is_contiguous() or t.shape[-1] != rows:
+            raise BornError(7, "combined_quick_gpu:NoValidDoubleMatrix",
+                            f"expected a contiguous complex64 device tensor with last dimension {rows}")
+
+    # ---------------------------------------------------------- solver contract
+    def op_A(self, x, mode: int):
+        """operator.combined_scalar: mode 0 size, 1 forward, 2 adjoint
+        (operator.m:1019-1072)."""
+        if mode == 0:
+            return list(self.size)
+        if mode == 1:
+            return self.forward(x)
+        if mode == 2:
+            return self.adjoint(x)
+        raise BornError(6, "combined_scalar:InvalidMode", "mode must be 0, 1 or 2")
+
+
+def combined_quick_gpu(operator_born, mode, input_matrix):
+    """output = combined_quick_gpu(operator_born, mode, input_matrix)
+    (combined_quick_gpu.m:11-23): mode 1 forward, any other value adjoint;
+    the input must be a nonempty 2-D double matrix with N_points (forward)
+    or N_observations (adjoint) rows; real input is promoted to complex."""
+    if not isinstance(operator_born, OperatorBorn):
+        raise BornError(5, "combined_quick_gpu:NoOperatorBorn", "operator_born must be a single scattering.operator_born!")
+    m = np.asarray(mode)
+    if m.size != 1 or not np.issubdtype(m.dtype, np.number) or np.iscomplexobj(m):
+        raise BornError(6, "combined_quick_gpu:NoNumericScalarMode", "mode must be a real-valued numeric scalar!")
+    mode_i = int(m.reshape(()).item())
+    x = np.asarray(input_matrix)
+    rows = operator_born.N_points if mode_i == 1 else operator_born.N_observations
+    if x.dtype not in (np.float64, np.complex128) or x.ndim != 2 or x.shape[0] != rows or x.size == 0:
+        raise BornError(7, "combined_quick_gpu:NoValidDoubleMatrix", f"input must be a double matrix with {rows} rows!")
+    return operator_born.forward(x) if mode_i == 1 else operator_born.adjoint(x)

@@ -1,0 +1,190 @@
+/*
+ * born.h -- C ABI of the MI355X Born scattering operator pair.
+ *
+ * Replaces the CUDA MEX gateway `combined_quick_gpu`
+ * (+scattering/combined_quick_gpu.cu:66-1094, documented in
+ * +scattering/combined_quick_gpu.m:1-41), which MATLAB calls as
+ *
+ *     output = scattering.combined_quick_gpu( operator_born, mode, input )
+ *
+ * from operator_born.forward_scalar (operator_born.m:217) and
+ * operator_born.adjoint_scalar (operator_born.m:286).  The MEX pulls its
+ * tables out of the operator object on every call and frees everything at
+ * the end (cu:201-391, :817-823, :1011-1088).  This library splits that into
+ * a persistent plan that owns device-resident tables, and apply calls.
+ *
+ * Conventions
+ *   - complex numbers are interleaved (re, im) pairs, as in MATLAB's
+ *     interleaved complex API (R2018a, combined_quick_gpu.m:31-32);
+ *   - matrices are column-major, as MATLAB stores them;
+ *   - index arrays are 1-based doubles exactly as MATLAB holds them;
+ *   - observations are ordered measurement-major, then mix, then frequency
+ *     (cu:773, :978);
+ *   - a plan is not re-entrant: calls on one plan must not overlap.  Use one
+ *     plan per stream.
+ */
+#ifndef BORN_H
+#define BORN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes; born_status_id() maps each to the reference's error
+ * identifier string (cu:201, :228, :372, :385, :397, :414, :421, :428;
+ * h:31-44 for the runtime error). */
+enum born_status {
+    BORN_OK = 0,
+    BORN_ERR_NUMBER_ARGUMENTS = 1,       /* combined_quick_gpu:NumberArguments */
+    BORN_ERR_NO_SYMMETRIC_GRIDS_SETUP = 2,/* combined_quick_gpu:NoSymmetricGridsSetup */
+    BORN_ERR_NO_GPU_ACTIVE = 3,          /* combined_quick_gpu:NoScatteringOptionsGPUActive */
+    BORN_ERR_NO_COMPLEX_DOUBLES = 4,     /* combined_quick_gpu:NoComplexDoubles */
+    BORN_ERR_NO_OPERATOR_BORN = 5,       /* combined_quick_gpu:NoOperatorBorn */
+    BORN_ERR_NO_NUMERIC_SCALAR_MODE = 6, /* combined_quick_gpu:NoNumericScalarMode */
+    BORN_ERR_NO_VALID_DOUBLE_MATRIX = 7, /* combined_quick_gpu:NoValidDoubleMatrix */
+    BORN_ERR_CONVERSION_FAILED = 8,      /* combined_quick_gpu:ConversionFailed */
+    BORN_ERR_DEVICE = 9,                 /* adjoint_quick_gpu:ErrorCUDA */
+    BORN_ERR_INVALID_ARGUMENT = 10,      /* born:InvalidArgument (bad sizes/indices) */
+    BORN_ERR_UNSUPPORTED = 11,           /* born:Unsupported */
+    BORN_ERR_OUT_OF_MEMORY = 12          /* born:OutOfMemory */
+};
+
+/* Numeric form of the grid contraction. */
+enum born_precision {
+    BORN_PRECISION_FP32 = 0,     /* plain FP32 operands, FP32 MFMA products      */
+    BORN_PRECISION_SPLIT_BF16 = 1,/* hi+lo BF16 operands, FP32 accumulation       */
+    BORN_PRECISION_SPLIT_FP16 = 2 /* hi+lo FP16 operands (scaled), FP32 accum.   */
+};
+
+typedef struct born_plan_s* born_plan_t;
+
+/* One mixed voltage signal k of measurement m: settings(m).rx(k) and
+ * prefactors{m}(k) (cu:298, :309-333, :352). */
+typedef struct {
+    int32_t N_f;                         /* frequencies of the mix (cu:298)                 */
+    const double* indices_f_to_unique;   /* N_f, 1-based into the measurement's unique freqs
+                                            (settings(m).indices_f_to_unique{k}, cu:317)    */
+    int32_t N_active;                    /* active elements (cu:309-310)                    */
+    const double* indices_active;        /* N_active, 1-based element indices               */
+    const double* prefactors;            /* complex N_f x N_active, col-major (cu:352)      */
+} born_mix_desc;
+
+/* One selected sequential pulse-echo measurement m. */
+typedef struct {
+    int32_t N_f_unique;                  /* unique frequencies of the measurement           */
+    const double* indices_f_to_unique;   /* N_f_unique, 1-based into the global unique freqs
+                                            (sequence.indices_f_to_unique{m}, cu:278-281)   */
+    const double* p_incident;            /* complex N_f_unique x N_points, col-major
+                                            (incident_waves(m).p_incident, cu:817-823);
+                                            NULL when the plan synthesizes it on the device */
+    int32_t N_mix;
+    const born_mix_desc* mixes;
+} born_measurement_desc;
+
+/* The tables the MEX reads from a scattering.operator_born object
+ * (cu:201-391), plus the FOV grid shape (FOV.shape.grid.N_points_axis) and
+ * the array shape, which the plan uses to verify the lateral structure of
+ * the shift map. */
+typedef struct {
+    int32_t N_dimensions;                /* FOV grid dimensions: 2 or 3                     */
+    int32_t N_points_axis[3];            /* (N_x, N_z, -) or (N_x, N_y, N_z)               */
+    int32_t N_elements_axis[2];          /* (N_ex, 1) or (N_ex, N_ey)                       */
+    int32_t N_points;                    /* sequence.size(2) (cu:222)                       */
+    int32_t N_elements;                  /* xdc_array.N_elements (cu:234)                   */
+    int32_t N_f_unique;                  /* rows of h_ref (cu:391)                          */
+    const double* h_ref;                 /* complex N_f_unique x N_points (cu:382-391)      */
+    const double* indices_grid_FOV_shift;/* N_points x N_elements, 1-based (cu:231)         */
+    int32_t N_measurements;              /* numel(indices_measurement_sel) (cu:241)         */
+    const born_measurement_desc* measurements;
+    int32_t device;                      /* options.momentary.gpu.index (cu:378)            */
+    int32_t precision;                   /* enum born_precision                             */
+} born_tables_desc;
+
+/* Geometry for the device-side table build: processing.greens_function
+ * (greens_function.m:36-122) integrated over the reference element's face
+ * (setup.m:1216-1281), the anti-aliasing taper (setup.m:916-1032), and the
+ * incident field p_in = sum_e h_tx(shift_e) v_d(e) (sequence.m:768-861). */
+typedef struct {
+    int32_t N_dimensions;                /* 2 or 3                                          */
+    int32_t N_points_axis[3];
+    double delta_axis[3];                /* grid spacings                                   */
+    double z0;                           /* first axial grid position                       */
+    int32_t N_elements_axis[2];
+    double pitch_axis[2];
+    int32_t N_face_points_axis[2];       /* face integration points (grid_numbers)          */
+    double face_delta_axis[2];           /* face grid spacings                              */
+    const double* face_weights;          /* complex N_face x N_f_unique or NULL (= 1)
+                                            (face.m:359-376)                                */
+    int32_t N_f_unique;
+    const double* frequencies;           /* Hz, global unique axis                          */
+    const double* wavenumbers;           /* complex N_f_unique k-tilde (time_causal.m:99-155)*/
+    int32_t filter_rx_kind, filter_tx_kind; /* 0 off, 1 raised cosine, 2 logistic, 3 boxcar */
+    double filter_rx_param, filter_tx_param;
+    int32_t N_measurements;
+    const born_measurement_desc* measurements; /* p_incident must be NULL                    */
+    const int32_t* N_tx_active;          /* per measurement                                 */
+    const double* const* tx_indices_active; /* per measurement, 1-based                     */
+    const double* const* v_d;            /* per measurement, complex N_f_unique(m) x N_tx_active
+                                            (settings(m).v_d_unique, tx.m:264-286)          */
+    int32_t device;
+    int32_t precision;
+} born_geometry_desc;
+
+/* Plan creation from raw operator tables (the MEX path).  Converts the
+ * FP64 tables to the device layout once (the reference re-converts and
+ * re-uploads on every call, cu:548-593, :817-847). */
+int born_plan_create_tables(const born_tables_desc* desc, born_plan_t* plan);
+
+/* Plan creation from geometry: builds h_ref (FP64 evaluation, FP32 store)
+ * and p_incident on the device. */
+int born_plan_create_geometry(const born_geometry_desc* desc, born_plan_t* plan);
+
+int born_plan_destroy(born_plan_t plan);
+
+/* op_A( x, 0 ): size of the operator [N_observations, N_points]
+ * (operator.m:1035-1046). */
+int born_plan_size(born_plan_t plan, int64_t* N_observations, int64_t* N_points);
+
+/* Select the numeric form used by later apply calls (enum born_precision). */
+int born_plan_set_precision(born_plan_t plan, int32_t precision);
+
+/* combined_quick_gpu( op, 1, gamma ): host FP64 in/out, MEX semantics --
+ * forward occupancy rule (cu:466-488), zero output when nothing is occupied
+ * (cu:496).  gamma: complex N_points x N_rhs; u: complex N_obs x N_rhs. */
+int born_forward(born_plan_t plan, const double* gamma, int32_t N_rhs, double* u);
+
+/* combined_quick_gpu( op, 2, u ): host FP64 in/out; all N_points rows
+ * (cu:460, :999-1003). */
+int born_adjoint(born_plan_t plan, const double* u, int32_t N_rhs, double* gamma_hat);
+
+/* Device variants for PyTorch callers: complex64 device pointers in the same
+ * column-major layout, enqueued on `stream` (a hipStream_t; NULL = default
+ * stream).  Asynchronous with respect to the host. */
+int born_forward_dev(born_plan_t plan, const void* gamma_c64, int32_t N_rhs, void* u_c64, void* stream);
+int born_adjoint_dev(born_plan_t plan, const void* u_c64, int32_t N_rhs, void* gamma_hat_c64, void* stream);
+
+/* Copies of the device tables, complex64, for checking the device table
+ * build: which = 0 h_ref (N_f_unique x N_points), 1 p_incident of
+ * measurement `index` (N_f_unique(m) x N_points).  Column-major like the
+ * reference matrices. */
+int born_plan_get_table(born_plan_t plan, int32_t which, int32_t index, void* out_c64, int64_t capacity);
+
+/* Execution path chosen at plan creation: 0 structured (shift-invariant
+ * lateral Toeplitz tiles), 1 general gather (any shift map). */
+int born_plan_path(born_plan_t plan);
+
+/* Force the general gather path (for cross-checking the structured one). */
+int born_plan_force_general(born_plan_t plan, int32_t on);
+
+/* Thread-local message of the last failure on this thread. */
+const char* born_last_error(void);
+
+/* Reference error identifier for a status code. */
+const char* born_status_id(int status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BORN_H */

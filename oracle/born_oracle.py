@@ -1,0 +1,453 @@
+"""FP64 CPU restatement of the Born operator pair -- TEST INFRASTRUCTURE ONLY.
+
+This module is the oracle: tests, `__graft_entry__.smoke()` and the
+`cpu_baseline` leg of `bench.py` use it as the checker.  The product path
+(`comp_ui_toolbox_amd`) never imports it.
+
+It restates, in NumPy/SciPy double precision:
+
+* the table producers the operator consumes -- complex wavenumbers
+  (time_causal.m:99-155), the Green's function (greens_function.m:67, :88,
+  :103), the face-integrated spatial transfer function (setup.m:1216-1281),
+  the anti-aliasing taper (setup.m:916-1032, on.m:49-72, raised_cosine.m:92-121,
+  logistic.m:92-106, boxcar.m:72-86), the lateral shift/mirror map
+  (setup_grid_symmetric.m:95-241), the prefactors (setup.m:709-768,
+  sequence.m:590-634), the normal velocities and the incident field
+  (setup.m:531-653, tx.m:264-286, sequence.m:768-861);
+* the operator itself -- forward_quick_cpu / adjoint_quick_cpu
+  (operator_born.m:332-596) with the MEX's occupancy rule
+  (combined_quick_gpu.cu:454-499) and observation ordering (cu:953-991);
+* a dense assembly of A for tiny cases (brute force).
+
+Every function is pinned by a check in tests/test_oracle.py (closed forms,
+scipy/mpmath special functions, brute force, invariants).
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import List, Optional, Sequence
+
+import numpy as np
+import scipy.special
+
+DBL_EPS = np.finfo(np.float64).eps
+T_CLK = 1.0 / 80e6
+
+
+# ---------------------------------------------------------------------------
+# physics
+# ---------------------------------------------------------------------------
+def wavenumbers(f: np.ndarray, medium) -> np.ndarray:
+    """Complex wavenumbers of the time-causal power-law model
+    (time_causal.m:82-83 for alpha_0/alpha_1, :118-153 for k-tilde)."""
+    f = np.asarray(f, dtype=np.float64)
+    y = float(medium.exponent)
+    alpha_0 = medium.absorption_constant * math.log(10) / (20 * 0.01)
+    alpha_1 = medium.absorption * math.log(10) / (20 * 0.01 * (1e6) ** y)
+    k_ref = 2 * math.pi * f / medium.c_ref
+    k_imag = -alpha_0 - alpha_1 * np.abs(f) ** y
+    if medium.dispersion:
+        if (y % 2 == 0) or (math.floor(y) != y):
+            with np.errstate(divide="ignore", invalid="ignore"):
+                k_real = k_ref + alpha_1 * math.tan(y * math.pi / 2) * f * (
+                    np.abs(f) ** (y - 1) - abs(medium.f_ref) ** (y - 1))
+        else:
+            with np.errstate(divide="ignore", invalid="ignore"):
+                k_real = k_ref - 2 * alpha_1 * f ** y * np.log(np.abs(f / medium.f_ref)) / math.pi
+    else:
+        k_real = k_ref.copy()
+    k_real = np.where(np.abs(f) < DBL_EPS, 0.0, k_real)
+    return k_real + 1j * k_imag
+
+
+def greens_function(k: np.ndarray, D: np.ndarray, n_dims: int) -> np.ndarray:
+    """Free-space Green's function (greens_function.m:67, :88, :103):
+    2-D (j/4) H0^(2)(k D), 3-D -exp(-j k D)/(4 pi D).  Broadcasts k vs D."""
+    kD = k * D
+    if n_dims == 2:
+        return 0.25j * scipy.special.hankel2(0, kD)
+    if n_dims == 3:
+        return -np.exp(-1j * kD) / (4 * math.pi * D)
+    raise ValueError("greens_function:UnknownDimensions")
+
+
+def hankel2_0_series(z: np.ndarray) -> np.ndarray:
+    """H0^(2)(z) written out from its published expansions: the ascending
+    series for |z| < 12 (DLMF 10.2.2, 10.8.2) and the Hankel asymptotic
+    expansion truncated at its smallest term for |z| >= 12 (DLMF 10.17.4,
+    10.17.6).  The device table build evaluates the same formulas; this
+    CPU copy exists so tests can pin them against scipy and mpmath."""
+    z = np.asarray(z, dtype=np.complex128)
+    out = np.empty_like(z)
+    flat_z = z.reshape(-1)
+    flat_o = out.reshape(-1)
+    for i, zz in enumerate(flat_z):
+        flat_o[i] = _h20_scalar(complex(zz))
+    return out
+
+
+def _h20_scalar(z: complex) -> complex:
+    az = abs(z)
+    if az >= 12.0:
+        # sum_k (-i)^k a_k(0) / z^k with a_0 = 1, a_k = a_{k-1} * (-(2k-1)^2) / (8k)
+        s = 1.0 + 0.0j
+        term = 1.0 + 0.0j
+        prev = math.inf
+        for k in range(1, 60):
+            term = term * (-(2 * k - 1) ** 2) / (8.0 * k) * (-1j) / z
+            at = abs(term)
+            if at >= prev:
+                break
+            s += term
+            prev = at
+            if at < 1e-17 * abs(s):
+                break
+        # sqrt(2/(pi z)) * exp(-i (z - pi/4))
+        return np.sqrt(2.0 / (math.pi * z)) * np.exp(-1j * (z - math.pi / 4)) * s
+    q = 0.25 * z * z
+    term = 1.0 + 0.0j
+    j0 = 1.0 + 0.0j
+    ysum = 0.0 + 0.0j
+    hk = 0.0
+    for k in range(1, 80):
+        term = term * (-q) / (k * k)
+        hk += 1.0 / k
+        j0 += term
+        ysum += -term * hk          # (-1)^{k+1} H_k (z^2/4)^k / (k!)^2
+        if abs(term) < 1e-18 * max(1.0, abs(j0)):
+            break
+    y0 = (2.0 / math.pi) * ((np.log(z / 2.0) + np.euler_gamma) * j0 + ysum)
+    return j0 - 1j * y0
+
+
+# ---------------------------------------------------------------------------
+# anti-aliasing taper
+# ---------------------------------------------------------------------------
+def filter_samples(flags: np.ndarray, filt) -> np.ndarray:
+    """Taper of the local phase shift per pitch, product over the last axis
+    (raised_cosine.m:92-121, logistic.m:92-106, boxcar.m:72-86, off = 1)."""
+    kind = filt.kind
+    if kind == "off":
+        return np.ones(flags.shape[:-1])
+    if kind == "raised_cosine":
+        lb = math.pi * (1 - filt.param)
+        ub = math.pi
+        out = np.where(flags <= lb, 1.0, 0.0)
+        taper = (flags > lb) & (flags < ub)
+        out = np.where(taper, 0.5 * (1 + np.cos(math.pi * (flags - lb) / (ub - lb))), out)
+        return np.prod(out, axis=-1)
+    if kind == "logistic":
+        return np.prod(1.0 / (1.0 + np.exp(filt.param * (flags - math.pi))), axis=-1)
+    if kind == "boxcar":
+        return np.all(flags < math.pi, axis=-1).astype(np.float64)
+    raise ValueError(kind)
+
+
+def filter_flags(spec, f: np.ndarray, element: int, positions: np.ndarray) -> np.ndarray:
+    """Local phase shift per pitch real(k) * pitch * |e_lateral| for the unit
+    vectors from the element centre to the grid points, lateral dimensions
+    with more than one element only (setup.m:980-1017).  Shape
+    (N_f, N_pts, N_lat_relevant)."""
+    k = wavenumbers(f, spec.medium)
+    ctr = spec.array.positions_ctr()[element]
+    nd = spec.fov.N_dimensions
+    ctr_full = np.concatenate([ctr, [0.0]])
+    vec = positions - ctr_full[None, :]
+    dist = np.linalg.norm(vec, axis=1)
+    e = np.abs(vec[:, : nd - 1] / dist[:, None])
+    rel = [d for d in range(nd - 1) if spec.array.N_elements_axis[d] > 1]
+    e = e[:, rel]
+    pitch = np.array([spec.array.pitch_axis[d] for d in rel])
+    return np.real(k)[:, None, None] * pitch[None, None, :] * e[None, :, :]
+
+
+# ---------------------------------------------------------------------------
+# spatial transfer function, shift map
+# ---------------------------------------------------------------------------
+def transfer_function(spec, f: np.ndarray, element: int = 0, filt=None,
+                      positions: Optional[np.ndarray] = None,
+                      face_weights: Optional[np.ndarray] = None) -> np.ndarray:
+    """h[f, p] = -2 dA sum_face w G(k_f, |r_p - r_face|) (setup.m:1216-1281),
+    then the anti-aliasing taper of the same element (setup.m:902,
+    on.m:49-72).  `element` is 0-based; `positions` defaults to the FOV grid."""
+    if positions is None:
+        positions = spec.fov.positions()
+    nd = spec.fov.N_dimensions
+    k = wavenumbers(f, spec.medium)
+    offs, dA = spec.array.face_offsets()
+    ctr = spec.array.positions_ctr()[element]
+    h = np.zeros((len(f), positions.shape[0]), dtype=np.complex128)
+    for q in range(offs.shape[0]):
+        fp = np.concatenate([ctr + offs[q], [0.0]])
+        D = np.linalg.norm(positions - fp[None, :], axis=1)
+        G = greens_function(k[:, None], D[None, :], nd)
+        w = 1.0 if face_weights is None else face_weights[q][:, None]
+        h -= 2.0 * dA * w * G
+    if filt is not None and filt.kind != "off":
+        flags = filter_flags(spec, f, element, positions)
+        h *= filter_samples(flags, filt)
+    return h
+
+
+def shift_lateral(spec) -> np.ndarray:
+    """1-based N_pts x N_elements map: element e's transfer function at p is
+    h_ref[:, shift[p, e]] (setup_grid_symmetric.m:95-241)."""
+    npp = spec.N_points_per_pitch_axis
+    N_axis = spec.fov.N_points_axis
+    nd = len(N_axis)
+    nlat = nd - 1
+    sub = np.stack(np.unravel_index(np.arange(spec.fov.N_points), N_axis, order="F"), axis=1) + 1
+    e_sub = np.stack(np.unravel_index(np.arange(spec.array.N_elements),
+                                      spec.array.N_elements_axis, order="F"), axis=1) + 1
+    out = np.empty((spec.fov.N_points, spec.array.N_elements), dtype=np.int64)
+    for e in range(spec.array.N_elements):
+        s = sub.copy()
+        for d in range(nlat):
+            s[:, d] = sub[:, d] - (e_sub[e, d] - 1) * npp[d]
+            off = N_axis[d] - (spec.array.N_elements_axis[d] - 1) * npp[d] + 1
+            neg = s[:, d] <= 0
+            s[neg, d] = off - s[neg, d]
+        out[:, e] = np.ravel_multi_index(tuple((s - 1).T), N_axis, order="F") + 1
+    return out
+
+
+# ---------------------------------------------------------------------------
+# operator tables (what operator_born holds and the MEX reads)
+# ---------------------------------------------------------------------------
+@dataclasses.dataclass
+class MixTables:
+    idx_f_to_meas: np.ndarray      # 1-based into the measurement's unique freqs
+    idx_active: np.ndarray         # 1-based element indices
+    prefactors: np.ndarray         # (N_f_mix, N_active) complex
+
+
+@dataclasses.dataclass
+class MeasTables:
+    idx_f_to_global: np.ndarray    # 1-based into global unique freqs
+    p_incident: np.ndarray         # (N_f_meas, N_pts) complex
+    mixes: List[MixTables]
+
+
+@dataclasses.dataclass
+class BornTables:
+    h_ref: np.ndarray              # (N_f_unique, N_pts) complex
+    shift: np.ndarray              # (N_pts, N_e) 1-based
+    meas: List[MeasTables]
+    N_points_axis: tuple
+    N_elements_axis: tuple
+    N_points_per_pitch_axis: tuple
+
+    @property
+    def N_points(self):
+        return self.shift.shape[0]
+
+    @property
+    def N_elements(self):
+        return self.shift.shape[1]
+
+    def N_observations(self):
+        return sum(len(mx.idx_f_to_meas) for ms in self.meas for mx in ms.mixes)
+
+
+def normal_velocities(spec, m: int) -> np.ndarray:
+    """v_d[f, a] = apod_a U(f) exp(-j 2 pi f tau_a) H_tx(f) with tau quantized
+    to T_clk (setup.m:625, :635-646; tx.m:264-286); H_tx = 1 (delta response).
+    Rows follow the measurement's unique frequencies."""
+    wave = spec.waves[m]
+    f = spec.frequencies[spec.measurement_frequencies(m)]
+    tau_q = np.round(wave.delays / T_CLK) * T_CLK
+    U = spec.excitation_spectrum(f)
+    return U[:, None] * wave.apodization[None, :] * np.exp(-2j * math.pi * f[:, None] * tau_q[None, :])
+
+
+def prefactors_unique(spec) -> np.ndarray:
+    """-dV k(f)^2 over the global unique frequencies (setup.m:759)."""
+    k = wavenumbers(spec.frequencies, spec.medium)
+    return -spec.fov.cell_volume * k ** 2
+
+
+def build_tables(spec, h_ref: Optional[np.ndarray] = None, h_tx: Optional[np.ndarray] = None,
+                 with_p_inc: bool = True) -> BornTables:
+    """Assemble what the operator object holds: h_ref with the rx filter
+    (sequence.m:529-585), the shift map, p_inc per measurement from h_ref with
+    the tx filter (sequence.m:768-861), prefactors times rx weights
+    (sequence.m:590-634) and the 1-based frequency maps."""
+    if h_ref is None:
+        h_ref = transfer_function(spec, spec.frequencies, 0, spec.filter_rx)
+    shift = shift_lateral(spec)
+    if with_p_inc and h_tx is None:
+        if (spec.filter_tx.kind, spec.filter_tx.param) == (spec.filter_rx.kind, spec.filter_rx.param):
+            h_tx = h_ref
+        else:
+            h_tx = transfer_function(spec, spec.frequencies, 0, spec.filter_tx)
+    pref_u = prefactors_unique(spec)
+    meas = []
+    for m in range(spec.N_measurements):
+        fidx = spec.measurement_frequencies(m)
+        if with_p_inc:
+            vd = normal_velocities(spec, m)
+            p = incident_field(h_tx[fidx], shift, spec.waves[m].indices_active, vd)
+        else:
+            p = np.zeros((len(fidx), 0), dtype=np.complex128)
+        mixes = []
+        for mix in spec.mixes[m]:
+            fk = np.arange(len(fidx)) if mix.indices_f is None else np.asarray(mix.indices_f)
+            pref = pref_u[fidx[fk]][:, None] * np.asarray(mix.weights)[None, :]
+            mixes.append(MixTables(fk + 1, np.asarray(mix.indices_active) + 1, pref.astype(np.complex128)))
+        meas.append(MeasTables(fidx + 1, p, mixes))
+    return BornTables(h_ref, shift, meas, tuple(spec.fov.N_points_axis),
+                      tuple(spec.array.N_elements_axis), tuple(spec.N_points_per_pitch_axis))
+
+
+def incident_field(h_tx_meas: np.ndarray, shift: np.ndarray, indices_active: np.ndarray,
+                   v_d: np.ndarray) -> np.ndarray:
+    """p_in[f, p] = sum_a h_tx[f, shift[p, e_a]] v_d[f, a] (sequence.m:825-840).
+    `indices_active` 0-based."""
+    p = np.zeros((h_tx_meas.shape[0], shift.shape[0]), dtype=np.complex128)
+    for a, e in enumerate(indices_active):
+        p += h_tx_meas[:, shift[:, e] - 1] * v_d[:, a][:, None]
+    return p
+
+
+# ---------------------------------------------------------------------------
+# the operator pair
+# ---------------------------------------------------------------------------
+def occupied_points(gamma: np.ndarray, rule: str = "mex") -> np.ndarray:
+    """Occupied grid points: 'mex' = any |Re| or |Im| above DBL_EPSILON over
+    the columns (cu:466-488); 'cpu' = sum(|gamma|, 2) > eps
+    (operator_born.m:361)."""
+    g = np.asarray(gamma)
+    if g.ndim == 1:
+        g = g[:, None]
+    if rule == "mex":
+        occ = np.any((np.abs(g.real) > DBL_EPS) | (np.abs(g.imag) > DBL_EPS), axis=1)
+    else:
+        occ = np.sum(np.abs(g), axis=1) > DBL_EPS
+    return np.nonzero(occ)[0]
+
+
+def forward(T: BornTables, gamma: np.ndarray, rule: str = "mex",
+            rows: Optional[np.ndarray] = None) -> np.ndarray:
+    """u_M = A gamma, the loop nest measurement -> mix -> active element of
+    forward_quick_cpu (operator_born.m:353-459): Phi = h(fmap, shift) .*
+    p_inc(fmap, occ) .* pref(:, a); u += Phi * gamma(occ).  Frequencies always
+    go through the maps, as in the MEX (cu:329-333, :1114-1120).  `rows`
+    optionally restricts the output to those observation indices (0-based).
+    Returns (N_obs, N_rhs) (or (len(rows), N_rhs))."""
+    g = np.asarray(gamma, dtype=np.complex128)
+    squeeze = g.ndim == 1
+    if squeeze:
+        g = g[:, None]
+    occ = occupied_points(g, rule)
+    N_obs = T.N_observations()
+    out = np.zeros((N_obs, g.shape[1]), dtype=np.complex128)
+    want = None if rows is None else np.zeros(N_obs, dtype=bool)
+    if want is not None:
+        want[np.asarray(rows)] = True
+    obs = 0
+    for ms in T.meas:
+        fg_meas = ms.idx_f_to_global - 1
+        p_occ = ms.p_incident[:, occ]
+        for mx in ms.mixes:
+            fm = mx.idx_f_to_meas - 1
+            nf = len(fm)
+            sel = np.arange(nf) if want is None else np.nonzero(want[obs:obs + nf])[0]
+            if len(sel) and len(occ):
+                acc = np.zeros((len(sel), g.shape[1]), dtype=np.complex128)
+                for a, e1 in enumerate(mx.idx_active):
+                    sh = T.shift[occ, e1 - 1] - 1
+                    h = T.h_ref[np.ix_(fg_meas[fm[sel]], sh)]
+                    Phi = h * p_occ[fm[sel]] * mx.prefactors[sel, a][:, None]
+                    acc += Phi @ g[occ]
+                out[obs + sel] = acc
+            obs += nf
+    if rows is not None:
+        out = out[np.asarray(rows)]
+    return out[:, 0] if squeeze else out
+
+
+def adjoint(T: BornTables, u: np.ndarray, points: Optional[np.ndarray] = None) -> np.ndarray:
+    """gamma_hat = A^H u_M over all grid points (or the 0-based subset
+    `points`), the loop nest of adjoint_quick_cpu (operator_born.m:511-593):
+    gamma_hat += Phi' * u_mk."""
+    uu = np.asarray(u, dtype=np.complex128)
+    squeeze = uu.ndim == 1
+    if squeeze:
+        uu = uu[:, None]
+    pts = np.arange(T.N_points) if points is None else np.asarray(points)
+    out = np.zeros((len(pts), uu.shape[1]), dtype=np.complex128)
+    obs = 0
+    for ms in T.meas:
+        fg_meas = ms.idx_f_to_global - 1
+        p_pts = ms.p_incident[:, pts]
+        for mx in ms.mixes:
+            fm = mx.idx_f_to_meas - 1
+            nf = len(fm)
+            u_mk = uu[obs:obs + nf]
+            for a, e1 in enumerate(mx.idx_active):
+                sh = T.shift[pts, e1 - 1] - 1
+                h = T.h_ref[np.ix_(fg_meas[fm], sh)]
+                Phi = h * p_pts[fm] * mx.prefactors[:, a][:, None]
+                out += Phi.conj().T @ u_mk
+            obs += nf
+    return out[:, 0] if squeeze else out
+
+
+def assemble_matrix(T: BornTables) -> np.ndarray:
+    """Dense A (N_obs x N_pts) entry by entry from its definition; for tiny
+    tables only."""
+    A = np.zeros((T.N_observations(), T.N_points), dtype=np.complex128)
+    obs = 0
+    for ms in T.meas:
+        for mx in ms.mixes:
+            for fi, fm1 in enumerate(mx.idx_f_to_meas):
+                fg = ms.idx_f_to_global[fm1 - 1] - 1
+                for p in range(T.N_points):
+                    s = 0j
+                    for a, e1 in enumerate(mx.idx_active):
+                        s += (mx.prefactors[fi, a] * T.h_ref[fg, T.shift[p, e1 - 1] - 1]
+                              * ms.p_incident[fm1 - 1, p])
+                    A[obs, p] = s
+                obs += 1
+    return A
+
+
+def rel_l2(x: np.ndarray, ref: np.ndarray) -> float:
+    x = np.asarray(x)
+    ref = np.asarray(ref)
+    den = np.linalg.norm(ref.ravel())
+    return float(np.linalg.norm((x - ref).ravel()) / (den if den > 0 else 1.0))
+
+
+def restricted_tables(spec, points: np.ndarray) -> BornTables:
+    """Exact tables of the operator restricted to the grid points `points`
+    (0-based): the forward of a gamma supported on `points`, and the adjoint
+    at `points`, equal those of the full operator.  h_ref is evaluated only
+    where the shift map sends `points` (transfer_function at those
+    positions), p_inc only at `points` (sequence.m:825-840).  Makes
+    full-size checks affordable on the CPU."""
+    points = np.asarray(points)
+    shift = shift_lateral(spec)
+    need = np.unique(shift[points, :] - 1)
+    pos = spec.fov.positions()[need]
+    h_sub = transfer_function(spec, spec.frequencies, 0, spec.filter_rx, positions=pos)
+    same = (spec.filter_tx.kind, spec.filter_tx.param) == (spec.filter_rx.kind, spec.filter_rx.param)
+    htx_sub = h_sub if same else transfer_function(spec, spec.frequencies, 0, spec.filter_tx, positions=pos)
+    remap = np.full(spec.fov.N_points, -1, dtype=np.int64)
+    remap[need] = np.arange(len(need))
+    shift_sub = remap[shift[points, :] - 1] + 1
+    pref_u = prefactors_unique(spec)
+    meas = []
+    for m in range(spec.N_measurements):
+        fidx = spec.measurement_frequencies(m)
+        vd = normal_velocities(spec, m)
+        p = incident_field(htx_sub[fidx], shift_sub, spec.waves[m].indices_active, vd)
+        mixes = []
+        for mix in spec.mixes[m]:
+            fk = np.arange(len(fidx)) if mix.indices_f is None else np.asarray(mix.indices_f)
+            pref = pref_u[fidx[fk]][:, None] * np.asarray(mix.weights)[None, :]
+            mixes.append(MixTables(fk + 1, np.asarray(mix.indices_active) + 1, pref.astype(np.complex128)))
+        meas.append(MeasTables(fidx + 1, p, mixes))
+    return BornTables(h_sub, shift_sub, meas, (len(points), 1), tuple(spec.array.N_elements_axis),
+                      tuple(spec.N_points_per_pitch_axis))

@@ -1,0 +1,270 @@
+"""Parity of the HIP operator pair with the FP64 oracle, through the C ABI.
+
+Tolerance of the plain FP32 path: relative L2 error <= 1e-5 against FP64
+(the operator's stated bar); device-built tables <= 1e-6 (FP32 storage of
+an FP64 evaluation)."""
+import math
+
+import numpy as np
+import pytest
+
+from comp_ui_toolbox_amd import configs as C
+from comp_ui_toolbox_amd import OperatorBorn, combined_quick_gpu, BornError
+from oracle import born_oracle as O
+from tests.test_oracle import GOLD, mixed_tiny_tables, tiny_spec
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP32 = 1e-5
+TOL_TABLE = 1e-6
+
+
+def rnd(shape, seed, cplx=True):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    x = rng.standard_normal(shape)
+    if cplx:
+        x = x + 1j * rng.standard_normal(shape)
+    return x
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+# ------------------------------------------------------------- tiny tables
+@pytest.mark.parametrize("general", [False, True])
+def test_tiny_mixed_tables_forward_adjoint(torch_cuda, general):
+    spec, T = mixed_tiny_tables()
+    op = OperatorBorn.from_tables(T)
+    log("tiny mixed tables, path", op.path, "general" if general else "")
+    assert op.path == "structured"
+    if general:
+        op.force_general(True)
+        assert op.path == "general"
+    g = rnd((T.N_points, 3), 1)
+    u = rnd((T.N_observations(), 3), 2)
+    assert op.size == (T.N_observations(), T.N_points)
+    assert O.rel_l2(op.forward(g), O.forward(T, g)) < TOL_FP32
+    assert O.rel_l2(op.adjoint(u), O.adjoint(T, u)) < TOL_FP32
+    assert O.rel_l2(op.forward(g[:, 1]), O.forward(T, g[:, 1])) < TOL_FP32
+
+
+@pytest.mark.parametrize("N_e,Nx,Nz,npp", [(5, 17, 7, 3), (2, 9, 3, 2), (19, 80, 6, 4), (33, 131, 3, 4)])
+def test_tiny_odd_geometries(torch_cuda, N_e, Nx, Nz, npp):
+    spec = tiny_spec(N_e=N_e, Nx=Nx, Nz=Nz, npp=npp)
+    T = O.build_tables(spec)
+    op = OperatorBorn.from_tables(T)
+    log("geometry", N_e, Nx, Nz, npp, "path", op.path)
+    assert op.path == "structured"
+    g = rnd((T.N_points, 2), 3)
+    u = rnd((T.N_observations(), 2), 4)
+    assert O.rel_l2(op.forward(g), O.forward(T, g)) < TOL_FP32
+    assert O.rel_l2(op.adjoint(u), O.adjoint(T, u)) < TOL_FP32
+
+
+def test_unstructured_shift_map_uses_general_path(torch_cuda):
+    spec, T = mixed_tiny_tables()
+    rng = np.random.Generator(np.random.PCG64(9))
+    T.shift = rng.integers(1, T.N_points + 1, size=T.shift.shape)
+    op = OperatorBorn.from_tables(T)
+    assert op.path == "general"
+    g = rnd(T.N_points, 5)
+    u = rnd(T.N_observations(), 6)
+    assert O.rel_l2(op.forward(g), O.forward(T, g)) < TOL_FP32
+    assert O.rel_l2(op.adjoint(u), O.adjoint(T, u)) < TOL_FP32
+
+
+def test_3d_tables_general_path(torch_cuda):
+    arr = C.ArraySpec((3, 2), (3e-4, 3e-4), (2.5e-4, 2.5e-4), (2, 2))
+    fov = C.FovSpec((8, 6, 4), (1.5e-4, 1.5e-4, 1.5e-4), 2e-3)
+    spec = C.SequenceSpec("t3", arr, C.MediumSpec(), fov, np.array([2.5e6, 4e6, 5e6]),
+                          [C.qsw_wave(arr, np.array([0, 0, -1e-2]), 1500.0),
+                           C.qsw_wave(arr, np.array([3e-4, 0, -1e-2]), 1500.0)],
+                          [C.identity_mixes(6), C.identity_mixes(6)])
+    T = O.build_tables(spec)
+    op = OperatorBorn.from_tables(T)
+    g = rnd((T.N_points, 2), 7)
+    u = rnd((T.N_observations(), 2), 8)
+    assert O.rel_l2(op.forward(g), O.forward(T, g)) < TOL_FP32
+    assert O.rel_l2(op.adjoint(u), O.adjoint(T, u)) < TOL_FP32
+
+
+# -------------------------------------------------------------- config 0
+@pytest.fixture(scope="module")
+def c0_tables():
+    return O.build_tables(C.config0())
+
+
+def test_config0_golden(torch_cuda, c0_tables):
+    d = np.load(GOLD / "config0.npz")
+    op = OperatorBorn.from_tables(c0_tables)
+    assert op.path == "structured"
+    u = op.forward(d["gamma"])
+    e_f = O.rel_l2(u, d["u"])
+    gh = op.adjoint(d["u"])
+    e_a = O.rel_l2(gh[d["points"]], d["gamma_hat"])
+    log("config0 golden: forward rel-L2 %.2e adjoint rel-L2 %.2e" % (e_f, e_a))
+    assert e_f < TOL_FP32 and e_a < TOL_FP32
+
+
+def test_config0_dense_input_full_adjoint(torch_cuda, c0_tables):
+    T = c0_tables
+    op = OperatorBorn.from_tables(T)
+    spec = C.config0()
+    g = C.phantom_speckle(spec, n_disks=2, n_bright=5)
+    rows = np.arange(0, T.N_observations(), 37)
+    e_f = O.rel_l2(op.forward(g)[rows], O.forward(T, g, rows=rows))
+    u = rnd(T.N_observations(), 10)
+    e_a = O.rel_l2(op.adjoint(u), O.adjoint(T, u))
+    log("config0 dense: forward rel-L2 %.2e adjoint rel-L2 %.2e" % (e_f, e_a))
+    assert e_f < TOL_FP32 and e_a < TOL_FP32
+
+
+def test_config0_device_tables(torch_cuda, c0_tables):
+    spec = C.config0()
+    op = OperatorBorn.from_sequence(spec)
+    assert op.path == "structured"
+    h = op.table("h_ref")
+    p = op.table("p_incident", 0)
+    e_h = O.rel_l2(h, c0_tables.h_ref)
+    e_p = O.rel_l2(p, c0_tables.meas[0].p_incident)
+    log("config0 device tables: h_ref rel-L2 %.2e p_inc rel-L2 %.2e" % (e_h, e_p))
+    assert e_h < TOL_TABLE and e_p < TOL_TABLE
+    d = np.load(GOLD / "config0.npz")
+    assert O.rel_l2(op.forward(d["gamma"]), d["u"]) < TOL_FP32
+
+
+@pytest.mark.parametrize("filt", ["off", "logistic", "boxcar"])
+def test_device_tables_filters_tiny(torch_cuda, filt):
+    spec = tiny_spec(N_e=5, Nx=20, Nz=6, npp=4, filt=filt)
+    spec.filter_tx = C.FilterSpec("raised_cosine", 0.3)
+    if filt == "logistic":
+        spec.filter_rx = C.FilterSpec("logistic", 4.0)
+    T = O.build_tables(spec)
+    op = OperatorBorn.from_sequence(spec)
+    assert O.rel_l2(op.table("h_ref"), T.h_ref) < TOL_TABLE
+    for m in range(spec.N_measurements):
+        assert O.rel_l2(op.table("p_incident", m), T.meas[m].p_incident) < TOL_TABLE
+
+
+def test_device_tables_3d_tiny(torch_cuda):
+    arr = C.ArraySpec((3, 2), (3e-4, 3e-4), (2.5e-4, 2.5e-4), (2, 2))
+    fov = C.FovSpec((8, 6, 4), (1.5e-4, 1.5e-4, 1.5e-4), 2e-3)
+    spec = C.SequenceSpec("t3", arr, C.MediumSpec(), fov, np.array([2.5e6, 4e6, 5e6]),
+                          [C.qsw_wave(arr, np.array([0, 0, -1e-2]), 1500.0)], [C.identity_mixes(6)])
+    T = O.build_tables(spec)
+    op = OperatorBorn.from_sequence(spec)
+    assert O.rel_l2(op.table("h_ref"), T.h_ref) < TOL_TABLE
+    assert O.rel_l2(op.table("p_incident", 0), T.meas[0].p_incident) < TOL_TABLE
+
+
+# ----------------------------------------------------------- full sizes
+@pytest.fixture(scope="module")
+def c1():
+    spec = C.config1()
+    op = OperatorBorn.from_sequence(spec)
+    return spec, op
+
+
+def test_config1_forward_sparse_against_oracle(torch_cuda, c1):
+    spec, op = c1
+    rng = np.random.Generator(np.random.PCG64(21))
+    pts = np.sort(rng.choice(spec.fov.N_points, 24, replace=False))
+    R = O.restricted_tables(spec, pts)
+    g = np.zeros(spec.fov.N_points)
+    g[pts] = rng.standard_normal(len(pts))
+    u = op.forward(g)
+    e = O.rel_l2(u, O.forward(R, g[pts]))
+    log("config1 forward (24 points) rel-L2 %.2e" % e)
+    assert e < TOL_FP32
+
+
+def test_config1_adjoint_points_against_oracle(torch_cuda, c1):
+    spec, op = c1
+    rng = np.random.Generator(np.random.PCG64(22))
+    pts = np.sort(rng.choice(spec.fov.N_points, 24, replace=False))
+    R = O.restricted_tables(spec, pts)
+    u = rnd(op.N_observations, 23)
+    gh = op.adjoint(u)
+    e = O.rel_l2(gh[pts], O.adjoint(R, u))
+    log("config1 adjoint (24 points) rel-L2 %.2e" % e)
+    assert e < TOL_FP32
+
+
+def test_config1_adjoint_identity_and_linearity(torch_cuda, c1):
+    torch = torch_cuda
+    spec, op = c1
+    x = torch.tensor(C.phantom_speckle(spec).astype(np.complex64), device="cuda")
+    y = torch.tensor(rnd(op.N_observations, 31).astype(np.complex64), device="cuda")
+    Ax = op.forward_dev(x).to(torch.complex128)
+    Ahy = op.adjoint_dev(y).to(torch.complex128)
+    lhs = torch.vdot(y.to(torch.complex128), Ax).item()
+    rhs = torch.vdot(Ahy, x.to(torch.complex128)).item()
+    e = abs(lhs - rhs) / abs(lhs)
+    log("config1 <Ax,y> vs <x,A^H y> rel diff %.2e" % e)
+    assert e < TOL_FP32
+    x2 = torch.tensor(rnd(op.N_points, 32).astype(np.complex64), device="cuda")
+    A12 = op.forward_dev(x + 2 * x2).to(torch.complex128)
+    lin = (A12 - Ax - 2 * op.forward_dev(x2).to(torch.complex128)).norm() / A12.norm()
+    assert lin.item() < TOL_FP32
+
+
+def test_config1_multi_rhs_matches_single(torch_cuda, c1):
+    torch = torch_cuda
+    spec, op = c1
+    X = torch.tensor(rnd((3, op.N_points), 41).astype(np.complex64), device="cuda")
+    U = op.forward_dev(X)
+    for r in range(3):
+        ur = op.forward_dev(X[r].contiguous())
+        assert ((U[r] - ur).norm() / ur.norm()).item() < 1e-6
+    Y = torch.tensor(rnd((2, op.N_observations), 42).astype(np.complex64), device="cuda")
+    G = op.adjoint_dev(Y)
+    for r in range(2):
+        gr = op.adjoint_dev(Y[r].contiguous())
+        assert ((G[r] - gr).norm() / gr.norm()).item() < 1e-6
+
+
+def test_repeatability_bitwise(torch_cuda, c1):
+    torch = torch_cuda
+    spec, op = c1
+    x = torch.tensor(rnd(op.N_points, 51).astype(np.complex64), device="cuda")
+    y = torch.tensor(rnd(op.N_observations, 52).astype(np.complex64), device="cuda")
+    a1, a2 = op.forward_dev(x).clone(), op.forward_dev(x).clone()
+    b1, b2 = op.adjoint_dev(y).clone(), op.adjoint_dev(y).clone()
+    assert torch.equal(a1, a2) and torch.equal(b1, b2)
+
+
+# ---------------------------------------------------------- MEX semantics
+def test_zero_and_subnormal_input(torch_cuda):
+    spec, T = mixed_tiny_tables()
+    op = OperatorBorn.from_tables(T)
+    assert np.all(op.forward(np.zeros(T.N_points)) == 0)
+    g = np.zeros(T.N_points)
+    g[4] = 1e-17           # below DBL_EPSILON: dropped as in the MEX
+    assert np.all(op.forward(g) == 0)
+    g[4] = 1.0
+    g[7] = 1e-17
+    ref = np.zeros(T.N_points)
+    ref[4] = 1.0
+    assert O.rel_l2(op.forward(g), O.forward(T, ref)) < TOL_FP32
+
+
+def test_combined_quick_gpu_contract(torch_cuda):
+    spec, T = mixed_tiny_tables()
+    op = OperatorBorn.from_tables(T)
+    g = rnd((T.N_points, 2), 61, cplx=False)           # real input is promoted
+    u = combined_quick_gpu(op, 1, g)
+    assert u.shape == (T.N_observations(), 2) and u.dtype == np.complex128
+    assert O.rel_l2(u, O.forward(T, g)) < TOL_FP32
+    gh = combined_quick_gpu(op, 2.0, u)
+    assert gh.shape == (T.N_points, 2)
+    assert O.rel_l2(gh, O.adjoint(T, u)) < TOL_FP32
+    with pytest.raises(BornError, match="NoValidDoubleMatrix"):
+        combined_quick_gpu(op, 1, g[:-1])
+    with pytest.raises(BornError, match="NoValidDoubleMatrix"):
+        combined_quick_gpu(op, 1, g.astype(np.float32))
+    with pytest.raises(BornError, match="NoNumericScalarMode"):
+        combined_quick_gpu(op, [1, 2], g)
+    with pytest.raises(BornError, match="NoOperatorBorn"):
+        combined_quick_gpu("op", 1, g)
+    assert op.op_A(None, 0) == [T.N_observations(), T.N_points]
