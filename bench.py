@@ -1,0 +1,216 @@
+"""Benchmark of the Born operator pair on one MI355X.
+
+A step is one forward + adjoint pair of the operator on the config-1
+workload (128-element array, 512x512 FOV at pitch/4, 257 frequencies,
+11 plane waves in one launch, identity receive mixing, one right-hand
+side).  The tables are built on the device before the timed region; inputs
+are resident in HBM.  Prints one JSON line (see the keys below).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp32] [--no-cpu]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "fwd+adj operator pairs/s vs plain FP32 at ≤1e-4 rel L2 err vs FP64 oracle"
+PEAK = {"fp32": (157.3, "TFLOP/s"), "split_bf16": (2500.0, "TFLOP/s"), "split_fp16": (2500.0, "TFLOP/s")}
+
+
+def algorithmic_flops(spec, n_rhs: int) -> float:
+    """F_fwd = 8 N_meas N_f N_e N_pts N_rhs + 6 N_meas N_f N_pts N_rhs
+    (+ 8 N_meas N_f N_mix N_act N_rhs for non-identity mixing); F_adj = F_fwd."""
+    n_meas = spec.N_measurements
+    n_f = len(spec.frequencies)
+    n_e = spec.array.N_elements
+    n_pts = spec.fov.N_points
+    f = 8.0 * n_meas * n_f * n_e * n_pts * n_rhs + 6.0 * n_meas * n_f * n_pts * n_rhs
+    mixing_identity = all(len(mx.indices_active) == 1 for mm in spec.mixes for mx in mm)
+    if not mixing_identity:
+        f += sum(8.0 * n_f * len(mx.indices_active) * n_rhs for mm in spec.mixes for mx in mm)
+    return f
+
+
+def algorithmic_bytes(spec, n_rhs: int) -> float:
+    """B = 8 N_f N_pts (table) + 8 N_meas N_f N_pts (p_inc) + |gamma| + |u|
+    in FP32 complex."""
+    n_f = len(spec.frequencies)
+    n_pts = spec.fov.N_points
+    return 8.0 * n_f * n_pts + 8.0 * spec.N_measurements * n_f * n_pts + 8.0 * n_pts * n_rhs + \
+        8.0 * spec.N_observations() * n_rhs
+
+
+def cpu_baseline(op, spec, n_mixes: int, threads: int):
+    """The oracle's C restatement of forward_quick_cpu / adjoint_quick_cpu
+    (Phi materialized per active element, then a GEMV) on measurement 1 and
+    the first `n_mixes` receive channels, scaled to the full pair."""
+    from oracle import born_oracle as O
+    from oracle import ref_loop as R
+
+    class _T:
+        pass
+
+    class _M:
+        pass
+
+    T = _T()
+    T.h_ref = op.table("h_ref").astype(np.complex128)
+    T.shift = O.shift_lateral(spec)
+    ms = _M()
+    ms.idx_f_to_global = spec.measurement_frequencies(0) + 1
+    ms.p_incident = op.table("p_incident", 0).astype(np.complex128)
+    fidx = spec.measurement_frequencies(0)
+    pref_u = O.prefactors_unique(spec)
+    ms.mixes = []
+    for mix in spec.mixes[0][:n_mixes]:
+        mt = _M()
+        mt.idx_f_to_meas = np.arange(len(fidx)) + 1
+        mt.idx_active = np.asarray(mix.indices_active) + 1
+        mt.prefactors = pref_u[fidx][:, None] * np.asarray(mix.weights)[None, :]
+        ms.mixes.append(mt)
+    T.meas = [ms]
+    P = R.Prepared(T, 0)
+    g = np.ascontiguousarray(spec_gamma(spec), dtype=np.complex128)
+    t0 = time.perf_counter()
+    u = R.apply_measurement(P, 1, g, fused=False, threads=threads)
+    R.apply_measurement(P, 2, u, fused=False, threads=threads)
+    dt = time.perf_counter() - t0
+    n_units = sum(len(mm) for mm in spec.mixes)          # (measurement, mix) pairs in a full pair
+    t_pair = dt / n_mixes * n_units
+    return {
+        "value": 1.0 / t_pair, "unit": "pairs/s", "cores": threads, "kind": "port",
+        "sample": f"C FP64 loop nest (operator_born.m:332-596, Phi materialized) on measurement 1, "
+                  f"{n_mixes} of {len(spec.mixes[0])} receive channels, forward+adjoint in {dt:.1f} s, "
+                  f"scaled x{n_units / n_mixes:.0f} to the full pair",
+    }
+
+
+_GAMMA = {}
+
+
+def spec_gamma(spec):
+    from comp_ui_toolbox_amd import configs as C
+    if spec.name not in _GAMMA:
+        _GAMMA[spec.name] = C.phantom_speckle(spec)
+    return _GAMMA[spec.name]
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", default="fp32", choices=list(PEAK))
+    ap.add_argument("--config", default="config1")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-mixes", type=int, default=128)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+              f"(the operator pair runs on a single MI355X).", file=sys.stderr, flush=True)
+        sys.exit(2)
+
+    import torch
+    from comp_ui_toolbox_amd import OperatorBorn
+    from comp_ui_toolbox_amd import configs as C
+
+    spec = C.CONFIGS[args.config]()
+    t0 = time.perf_counter()
+    op = OperatorBorn.from_sequence(spec, precision=args.precision)
+    torch.cuda.synchronize()
+    t_tables = time.perf_counter() - t0
+    print(f"[bench] tables built on device in {t_tables:.2f} s, path {op.path}", file=sys.stderr, flush=True)
+
+    x = torch.tensor(spec_gamma(spec).astype(np.complex64), device="cuda")
+    u = torch.empty(op.N_observations, dtype=torch.complex64, device="cuda")
+    g = torch.empty(op.N_points, dtype=torch.complex64, device="cuda")
+    op.forward_dev(x, out=u.view(1, -1))
+    for _ in range(args.warmup):
+        op.forward_dev(x, out=u.view(1, -1))
+        op.adjoint_dev(u, out=g.view(1, -1))
+    torch.cuda.synchronize()
+    op.get_timing()
+    op.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        op.forward_dev(x, out=u.view(1, -1))
+        op.adjoint_dev(u, out=g.view(1, -1))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    op.set_timing(False)
+    timing = op.get_timing()
+    print(f"[bench] {args.steps} pairs in {elapsed * 1e3:.1f} ms", file=sys.stderr, flush=True)
+
+    pairs_per_s = args.steps / elapsed
+    F = algorithmic_flops(spec, 1)
+    B = algorithmic_bytes(spec, 1)
+    kern = {}
+    for name in ("forward", "adjoint"):
+        ms, n = timing[name]
+        kern[name] = ms / max(n, 1)
+    dom = max(kern, key=kern.get)
+    peak, unit = PEAK[args.precision]
+    achieved = F / (kern[dom] * 1e-3) / 1e12
+    traffic = None
+    tfile = ROOT / "profiles" / f"traffic_{args.config}_{dom}.json"
+    if tfile.exists():
+        try:
+            traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
+        except (ValueError, OSError):
+            traffic = None
+    result = {
+        "metric": METRIC,
+        "value": pairs_per_s,
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32" if args.precision == "fp32" else args.precision,
+        "data": "synthetic (config-1 speckle phantom; tables built on the device from the config-1 geometry)",
+        "config": {
+            "workload": f"{args.config}: {spec.array.N_elements} el, {spec.fov.N_points_axis[0]}x"
+                        f"{spec.fov.N_points_axis[-1]} FOV, {len(spec.frequencies)} f, "
+                        f"{spec.N_measurements} waves, {spec.N_observations()} obs, 1 rhs",
+            "precision": args.precision,
+            "path": op.path,
+            "table_build_s": round(t_tables, 3),
+            "kernel_ms": {k: round(v, 4) for k, v in kern.items()},
+            "flops_per_pass": F,
+            "bytes_per_pass": B,
+        },
+        "roofline": {
+            "bound": "mfma",
+            "kernel": f"{dom} contraction",
+            "achieved": achieved,
+            "peak": peak,
+            "unit": unit,
+            "frac": achieved / peak,
+            "traffic": traffic,
+        },
+    }
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        try:
+            result["cpu_baseline"] = cpu_baseline(op, spec, args.cpu_mixes, threads)
+        except Exception as e:  # report, do not hide
+            result["cpu_baseline"] = {"value": None, "error": repr(e)}
+    else:
+        result["cpu_baseline"] = None
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

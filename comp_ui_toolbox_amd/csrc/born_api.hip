@@ -258,13 +258,12 @@ static void detect_structure(Plan& P) {
       }
   P.Np = Np;
   P.N_ext = N_ext;
-  P.Ne_pad = ((P.N_e + 15) / 16) * 16;
-  // forward LDS: polyphase row + column tile (and the cross-wave reduction)
-  const int S = (Nx + 15) / 16, XP = 16 * S;
-  const int QL = (XP - 1) / Np + P.Ne_pad;
-  const size_t body = ((size_t)Np * QL + (size_t)XP * (NC + 1)) * sizeof(float2);
-  const size_t red = (size_t)4 * (P.Ne_pad / 16) * 8 * 64 * sizeof(float);
-  if (std::max(body, red) > 160 * 1024) return;
+  {
+    int mt = (P.N_e + 15) / 16;  // element tiles, rounded up to 1, 2, 4 or 8
+    mt = mt <= 1 ? 1 : (mt <= 2 ? 2 : (mt <= 4 ? 4 : 8));
+    P.Ne_pad = 16 * mt;
+  }
+  if (!fwd_structured_fits(Nx, P.Ne_pad, Np) || !adj_structured_fits(Nx, P.Ne_pad, Np)) return;
   P.d_ext = dupload(ext, P.stream);
   P.structured = true;
 }
@@ -352,6 +351,9 @@ static void common_dims(Plan& P, int nd, const int32_t* Npa, const int32_t* Nea,
 }
 
 static void finish_plan(Plan& P) {
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, P.device) == hipSuccess && ncu > 0)
+    P.n_cu = ncu;
   detect_structure(P);
   if (!P.structured) ensure_shift_on_device(P);
   set_max_lds(160 * 1024);
@@ -571,12 +573,34 @@ static Plan* create_geometry(const born_geometry_desc* d) {
 }
 
 // ----------------------------------------------------------- apply paths
+struct TimedLaunch {
+  Plan& P;
+  hipStream_t s;
+  int kind;
+  hipEvent_t a = nullptr, b = nullptr;
+  TimedLaunch(Plan& P_, hipStream_t s_, int k) : P(P_), s(s_), kind(k) {
+    if (!P.timing) return;
+    HIP_CHECK(hipEventCreate(&a));
+    HIP_CHECK(hipEventCreate(&b));
+    HIP_CHECK(hipEventRecord(a, s));
+  }
+  ~TimedLaunch() {
+    if (!P.timing || !a) return;
+    hipEventRecord(b, s);
+    P.timed.push_back({a, b, kind});
+  }
+};
+
 static void forward_device(Plan& P, const float2* gm, int N_rhs, void* u, bool out_f64, hipStream_t s) {
   if (use_structured(P)) {
-    launch_fwd_structured(P, gm, N_rhs, s);
+    {
+      TimedLaunch t(P, s, 0);
+      launch_fwd_structured(P, gm, N_rhs, s);
+    }
     launch_fwd_reduce(P, N_rhs, s);
   } else {
     ensure_shift_on_device(P);
+    TimedLaunch t(P, s, 0);
     launch_fwd_general(P, gm, N_rhs, s);
   }
   launch_fwd_mix(P, N_rhs, u, out_f64, s);
@@ -585,6 +609,7 @@ static void forward_device(Plan& P, const float2* gm, int N_rhs, void* u, bool o
 
 static void adjoint_device(Plan& P, const float2* u, int N_rhs, float2* g, hipStream_t s) {
   launch_adj_unmix(P, u, N_rhs, s);
+  TimedLaunch t(P, s, 1);
   if (use_structured(P)) {
     launch_adj_structured(P, N_rhs, g, s);
   } else {
@@ -789,6 +814,33 @@ int born_plan_force_general(born_plan_t plan, int32_t on) {
     P.force_general = on != 0;
     if (P.force_general) ensure_shift_on_device(P);
     HIP_CHECK(hipStreamSynchronize(P.stream));
+  });
+}
+
+int born_plan_set_timing(born_plan_t plan, int32_t on) {
+  return guarded([&] {
+    if (!plan) throw Error(BORN_ERR_INVALID_ARGUMENT, "plan is NULL");
+    reinterpret_cast<Plan*>(plan)->timing = on != 0;
+  });
+}
+
+int born_plan_get_timing(born_plan_t plan, double* ms, int64_t* launches) {
+  return guarded([&] {
+    if (!plan || !ms || !launches) throw Error(BORN_ERR_INVALID_ARGUMENT, "plan and outputs are required");
+    Plan& P = *reinterpret_cast<Plan*>(plan);
+    HIP_CHECK(hipSetDevice(P.device));
+    ms[0] = ms[1] = 0.0;
+    launches[0] = launches[1] = 0;
+    for (auto& t : P.timed) {
+      HIP_CHECK(hipEventSynchronize(t.b));
+      float e = 0.f;
+      HIP_CHECK(hipEventElapsedTime(&e, t.a, t.b));
+      ms[t.kind] += e;
+      launches[t.kind] += 1;
+      hipEventDestroy(t.a);
+      hipEventDestroy(t.b);
+    }
+    P.timed.clear();
   });
 }
 

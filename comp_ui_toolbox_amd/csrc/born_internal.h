@@ -21,6 +21,7 @@ struct Chunk {
 // Device tables and index maps.  All device arrays are owned by the plan.
 struct Plan {
   int device = 0;
+  int n_cu = 256;           // compute units of the device (persistent grids)
   int precision = 0;
   bool force_general = false;
   bool structured = false;  // 2-D lateral Toeplitz path usable
@@ -90,6 +91,14 @@ struct Plan {
   void* d_io_out = nullptr;
   hipStream_t stream = nullptr;   // host-path stream
 
+  // optional event timing of the contraction kernels: (start, stop, kind)
+  bool timing = false;
+  struct Timed {
+    hipEvent_t a, b;
+    int kind;
+  };
+  std::vector<Timed> timed;
+
   ~Plan();
 };
 
@@ -123,5 +132,7 @@ void launch_build_h(const GeomDev& g, int N_fu, const double2* k, const double2*
 void launch_synth_pinc(const Plan& P, const double2* h64, const int* d_meas_Nfm, const int* d_tx_off,
                        const int* d_tx_el, const double2* d_vd, const int* d_vd_off, hipStream_t s);
 void set_max_lds(size_t bytes);
+bool fwd_structured_fits(int Nx, int Ne_pad, int Np);
+bool adj_structured_fits(int Nx, int Ne_pad, int Np);
 
 }  // namespace born

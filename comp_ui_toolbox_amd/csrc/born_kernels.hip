@@ -80,146 +80,6 @@ void launch_mask_gamma(const float2* g, int N_pts, int N_rhs, float2* gm, hipStr
   hipLaunchKernelGGL(k_mask_gamma, dim3((N_pts + 255) / 256), dim3(256), 0, s, g, N_pts, N_rhs, gm);
 }
 
-// ------------------------------------------------- structured forward (2-D)
-// One workgroup = (chunk, z-block).  Four waves split each depth row's
-// points; each wave keeps all 16*MT elements x 16 columns in MFMA
-// accumulators (A = elements x 4 points read from a polyphase LDS copy of the
-// extended transfer-function row, B = 4 points x 16 columns of
-// p_inc*gamma).  Row sums are folded into a second accumulator level, the
-// four waves are summed through LDS, and the partial for the z-block is
-// written out; k_fwd_reduce sums z-blocks in FP64.
-template <int MT>
-__global__ __launch_bounds__(256) void k_fwd_s2d(
-    const float2* __restrict__ h, const float2* __restrict__ P, const float2* __restrict__ gm,
-    const int* __restrict__ ext, const Chunk* __restrict__ chunks, const int* __restrict__ col_pair,
-    int Nx, int Nz, int N_pts, int N_e, int Np, int N_ext, int NZB, int ZR, int S, int Ne_pad,
-    float2* __restrict__ vpart) {
-  extern __shared__ float2 lds[];
-  const int XP = 16 * S;                       // padded row length (4 waves x S steps x 4 points)
-  const int QL = (XP - 1) / Np + 16 * MT;      // polyphase row length
-  float2* Hp = lds;                            // [Np][QL]
-  float2* Bs = lds + Np * QL;                  // [XP][NCS]
-
-  const int ci = blockIdx.x / NZB, zb = blockIdx.x - ci * NZB;
-  const Chunk ch = chunks[ci];
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int Qoff = 16 * MT - N_e;
-  const int z_begin = zb * ZR, z_end = min(Nz, z_begin + ZR);
-  const float2* hf = h + (size_t)ch.fg * N_pts;
-  const float2* gr = gm + (size_t)ch.r * N_pts;
-
-  v4f accr[MT], acci[MT], sr[MT], si[MT];
-#pragma unroll
-  for (int mt = 0; mt < MT; ++mt) {
-    accr[mt] = v4f{0.f, 0.f, 0.f, 0.f};
-    acci[mt] = accr[mt];
-    sr[mt] = accr[mt];
-    si[mt] = accr[mt];
-  }
-
-  for (int z = z_begin; z < z_end; ++z) {
-    __syncthreads();
-    const float2* hrow = hf + (size_t)z * Nx;
-    for (int idx = tid; idx < Np * QL; idx += 256) {
-      const int ph = idx / QL, q = idx - ph * QL;
-      const int t = (q - Qoff) * Np + ph;
-      Hp[idx] = (t >= 0 && t < N_ext) ? hrow[ext[t]] : make_float2(0.f, 0.f);
-    }
-    for (int idx = tid; idx < XP * NC; idx += 256) {
-      const int c = idx / XP, i = idx - c * XP;
-      float2 v = make_float2(0.f, 0.f);
-      if (c < ch.ncols && i < Nx) {
-        const int pair = col_pair[ch.col_start + c];
-        const size_t p = (size_t)z * Nx + i;
-        const float2 a = P[(size_t)pair * N_pts + p];
-        const float2 b = gr[p];
-        v = make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-      }
-      Bs[i * NCS + c] = v;
-    }
-    __syncthreads();
-
-    const int kk = lane >> 4, col = lane & 15;
-    for (int s = 0; s < S; ++s) {
-      const int i = (w * S + s) * 4 + kk;
-      const int ph = i % Np;
-      const int qb = ph * QL + i / Np + 16 * MT - 1 - col;
-      const float2 b = Bs[i * NCS + col];
-      const float nby = -b.y;
-#pragma unroll
-      for (int mt = 0; mt < MT; ++mt) {
-        const float2 a = Hp[qb - 16 * mt];
-        accr[mt] = mfma16(a.x, b.x, accr[mt]);
-        accr[mt] = mfma16(a.y, nby, accr[mt]);
-        acci[mt] = mfma16(a.x, b.y, acci[mt]);
-        acci[mt] = mfma16(a.y, b.x, acci[mt]);
-      }
-    }
-#pragma unroll
-    for (int mt = 0; mt < MT; ++mt) {
-      sr[mt] += accr[mt];
-      si[mt] += acci[mt];
-      accr[mt] = v4f{0.f, 0.f, 0.f, 0.f};
-      acci[mt] = accr[mt];
-    }
-  }
-
-  // cross-wave reduction through LDS: [w][mt][j][lane]
-  __syncthreads();
-  float* red = reinterpret_cast<float*>(lds);
-#pragma unroll
-  for (int mt = 0; mt < MT; ++mt) {
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      red[((w * MT + mt) * 8 + v) * 64 + lane] = sr[mt][v];
-      red[((w * MT + mt) * 8 + 4 + v) * 64 + lane] = si[mt][v];
-    }
-  }
-  __syncthreads();
-  const int per_wave = MT * 8 * 64;
-  float* out = reinterpret_cast<float*>(vpart + ((size_t)ci * NZB + zb) * Ne_pad * NC);
-  for (int idx = tid; idx < per_wave; idx += 256) {
-    const float sum = red[idx] + red[idx + per_wave] + red[idx + 2 * per_wave] + red[idx + 3 * per_wave];
-    const int ln = idx & 63, j = (idx >> 6) & 7, mt = idx >> 9;
-    const int e = 16 * mt + 4 * (ln >> 4) + (j & 3);
-    const int c = ln & 15;
-    out[((size_t)e * NC + c) * 2 + (j >> 2)] = sum;
-  }
-}
-
-static size_t fwd_s2d_lds_bytes(int MT, int S, int Np) {
-  const int XP = 16 * S;
-  const int QL = (XP - 1) / Np + 16 * MT;
-  const size_t body = ((size_t)Np * QL + (size_t)XP * NCS) * sizeof(float2);
-  const size_t red = (size_t)4 * MT * 8 * 64 * sizeof(float);
-  return std::max(body, red);
-}
-
-void launch_fwd_structured(const Plan& P, const float2* gm, int N_rhs, hipStream_t s) {
-  const int MT = P.Ne_pad / 16;
-  const int S = (P.Nx + 15) / 16;
-  const size_t lds = fwd_s2d_lds_bytes(MT, S, P.Np);
-  const dim3 grid((unsigned)(P.chunks.size() * P.NZB)), block(256);
-#define BORN_FWD_CASE(M)                                                                          \
-  case M:                                                                                         \
-    hipLaunchKernelGGL(k_fwd_s2d<M>, grid, block, lds, s, P.d_h, P.d_p, gm, P.d_ext, P.d_chunks, \
-                       P.d_col_pair, P.Nx, P.Nz, P.N_pts, P.N_e, P.Np, P.N_ext, P.NZB, P.ZR, S,   \
-                       P.Ne_pad, P.d_vpart);                                                      \
-    break;
-  switch (MT) {
-    BORN_FWD_CASE(1)
-    BORN_FWD_CASE(2)
-    BORN_FWD_CASE(3)
-    BORN_FWD_CASE(4)
-    BORN_FWD_CASE(5)
-    BORN_FWD_CASE(6)
-    BORN_FWD_CASE(7)
-    BORN_FWD_CASE(8)
-    default: break;
-  }
-#undef BORN_FWD_CASE
-}
-
 // z-block partials -> V[pair][e][r] (FP64 sum)
 __global__ void k_fwd_reduce(const float2* __restrict__ vpart, const int* __restrict__ pr_chunk,
                              const int* __restrict__ pr_pos, int N_pairs, int N_e, int N_rhs, int NZB,
@@ -364,156 +224,6 @@ void launch_adj_unmix(const Plan& P, const float2* u, int N_rhs, hipStream_t s) 
   hipLaunchKernelGGL(k_adj_unmix, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, u, P.d_pair_list_off,
                      P.d_pair_list, P.d_slot, P.d_mk_meta, P.d_mk_off, P.d_pref, P.N_pairs, P.N_e, N_rhs,
                      P.N_obs, P.d_W);
-}
-
-// ------------------------------------------------- structured adjoint (2-D)
-// One workgroup = (depth row z, lateral segment of 64*MW points); it loops
-// over every chunk (rhs-major, then frequency) in a fixed order and keeps its
-// output in registers, so no atomics are used and results are bitwise
-// repeatable.  Per chunk: Y[i][c] = sum_e conj(h_e[i]) W[e][c] on MFMA
-// (A = 16 points x 4 elements from the extended row in LDS, B = 4 elements x
-// 16 columns), then gamma_hat[i] += sum_c conj(p_inc[c][i]) Y[i][c].
-// Sum the per-lane partials over the 16 column lanes of each point group and
-// write gamma_hat for rhs r.
-template <int MW>
-__device__ __forceinline__ void adj_flush(float (&gx)[MW][4], float (&gy)[MW][4], int r, int ib, int kk, int col,
-                                          int Nx, int N_pts, int z, float2* __restrict__ g_out) {
-#pragma unroll
-  for (int mw = 0; mw < MW; ++mw) {
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      float x = gx[mw][v], y = gy[mw][v];
-#pragma unroll
-      for (int o = 1; o < 16; o <<= 1) {
-        x += __shfl_xor(x, o, 64);
-        y += __shfl_xor(y, o, 64);
-      }
-      gx[mw][v] = x;
-      gy[mw][v] = y;
-    }
-  }
-  if (col < 4) {
-#pragma unroll
-    for (int mw = 0; mw < MW; ++mw) {
-      const int i = ib + 16 * mw + 4 * kk + col;
-      float x = gx[mw][0], y = gy[mw][0];
-#pragma unroll
-      for (int v = 1; v < 4; ++v) {
-        if (col == v) {
-          x = gx[mw][v];
-          y = gy[mw][v];
-        }
-      }
-      if (i < Nx) g_out[(size_t)r * N_pts + (size_t)z * Nx + i] = make_float2(x, y);
-    }
-  }
-}
-
-template <int MW>
-__global__ __launch_bounds__(256) void k_adj_s2d(
-    const float2* __restrict__ h, const float2* __restrict__ P, const float2* __restrict__ W,
-    const int* __restrict__ ext, const Chunk* __restrict__ chunks, int n_chunks,
-    const int* __restrict__ col_pair, int Nx, int N_pts, int N_e, int Ne_pad, int Np, int N_ext, int N_rhs,
-    float2* __restrict__ g_out) {
-  extern __shared__ float2 lds[];
-  constexpr int SEG = 64 * MW;
-  const int L = SEG + (Ne_pad - 1) * Np;
-  float2* He = lds;        // [L]
-  float2* Ws = lds + L;    // [Ne_pad][NCS]
-
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int z = blockIdx.y;
-  const int i_s = blockIdx.x * SEG;
-  const int ib = i_s + w * 16 * MW;
-  const int kk = lane >> 4, col = lane & 15;
-  const int t_shift = i_s - (Ne_pad - N_e) * Np;
-
-  float gx[MW][4], gy[MW][4];
-  int cur_r = -1;
-
-  for (int ci = 0; ci < n_chunks; ++ci) {
-    const Chunk ch = chunks[ci];
-    if (ch.r != cur_r) {
-      if (cur_r >= 0) adj_flush<MW>(gx, gy, cur_r, ib, kk, col, Nx, N_pts, z, g_out);
-      cur_r = ch.r;
-#pragma unroll
-      for (int mw = 0; mw < MW; ++mw)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) gx[mw][v] = gy[mw][v] = 0.f;
-    }
-    __syncthreads();
-    const float2* hrow = h + (size_t)ch.fg * N_pts + (size_t)z * Nx;
-    for (int u = tid; u < L; u += 256) {
-      const int t = u + t_shift;
-      He[u] = (t >= 0 && t < N_ext) ? hrow[ext[t]] : make_float2(0.f, 0.f);
-    }
-    for (int idx = tid; idx < Ne_pad * NC; idx += 256) {
-      const int k = idx / NC, c = idx - k * NC;
-      float2 v = make_float2(0.f, 0.f);
-      if (c < ch.ncols && k < N_e) v = W[((size_t)col_pair[ch.col_start + c] * N_e + k) * N_rhs + ch.r];
-      Ws[k * NCS + c] = v;
-    }
-    __syncthreads();
-
-    v4f ar[MW], ai[MW];
-#pragma unroll
-    for (int mw = 0; mw < MW; ++mw) {
-      ar[mw] = v4f{0.f, 0.f, 0.f, 0.f};
-      ai[mw] = ar[mw];
-    }
-    const int ubase = ib - i_s + col + (Ne_pad - 1) * Np;
-    for (int k0 = 0; k0 < Ne_pad; k0 += 4) {
-      const int k = k0 + kk;
-      const float2 b = Ws[k * NCS + col];
-      const float nbx = -b.x;
-      const int u0 = ubase - k * Np;
-#pragma unroll
-      for (int mw = 0; mw < MW; ++mw) {
-        const float2 a = He[u0 + 16 * mw];
-        ar[mw] = mfma16(a.x, b.x, ar[mw]);
-        ar[mw] = mfma16(a.y, b.y, ar[mw]);
-        ai[mw] = mfma16(a.x, b.y, ai[mw]);
-        ai[mw] = mfma16(a.y, nbx, ai[mw]);
-      }
-    }
-    if (col < ch.ncols) {
-      const float2* pr = P + (size_t)col_pair[ch.col_start + col] * N_pts + (size_t)z * Nx;
-#pragma unroll
-      for (int mw = 0; mw < MW; ++mw) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int i = ib + 16 * mw + 4 * kk + v;
-          const float2 p = (i < Nx) ? pr[i] : make_float2(0.f, 0.f);
-          gx[mw][v] += p.x * ar[mw][v] + p.y * ai[mw][v];
-          gy[mw][v] += p.x * ai[mw][v] - p.y * ar[mw][v];
-        }
-      }
-    }
-  }
-  if (cur_r >= 0) adj_flush<MW>(gx, gy, cur_r, ib, kk, col, Nx, N_pts, z, g_out);
-}
-
-void launch_adj_structured(const Plan& P, int N_rhs, float2* g_out, hipStream_t s) {
-  int MW = 8;
-  if (P.Nx <= 128) MW = 2;
-  else if (P.Nx <= 256) MW = 4;
-  const int SEG = 64 * MW;
-  const int L = SEG + (P.Ne_pad - 1) * P.Np;
-  const size_t lds = ((size_t)L + (size_t)P.Ne_pad * NCS) * sizeof(float2);
-  const dim3 grid((P.Nx + SEG - 1) / SEG, P.Nz), block(256);
-  const int nch = (int)P.chunks.size();
-#define BORN_ADJ_CASE(M)                                                                               \
-  case M:                                                                                              \
-    hipLaunchKernelGGL(k_adj_s2d<M>, grid, block, lds, s, P.d_h, P.d_p, P.d_W, P.d_ext, P.d_chunks, nch, \
-                       P.d_col_pair, P.Nx, P.N_pts, P.N_e, P.Ne_pad, P.Np, P.N_ext, N_rhs, g_out);     \
-    break;
-  switch (MW) {
-    BORN_ADJ_CASE(2)
-    BORN_ADJ_CASE(4)
-    BORN_ADJ_CASE(8)
-    default: break;
-  }
-#undef BORN_ADJ_CASE
 }
 
 // ------------------------------------------------------ general adjoint
@@ -724,19 +434,4 @@ void launch_synth_pinc(const Plan& P, const double2* h64, const int* d_meas_Nfm,
                      P.N_pts, P.d_p);
 }
 
-// Allow dynamic LDS beyond the 64 KiB default for the structured kernels.
-void set_max_lds(size_t bytes) {
-  const int b = (int)bytes;
-  hipFuncSetAttribute((const void*)k_fwd_s2d<1>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
-  hipFuncSetAttribute((const void*)k_fwd_s2d<2>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
-  hipFuncSetAttribute((const void*)k_fwd_s2d<3>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
-  hipFuncSetAttribute((const void*)k_fwd_s2d<4>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
-  hipFuncSetAttribute((const void*)k_fwd_s2d<5>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
-  hipFuncSetAttribute((const void*)k_fwd_s2d<6>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
-  hipFuncSetAttribute((const void*)k_fwd_s2d<7>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
-  hipFuncSetAttribute((const void*)k_fwd_s2d<8>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
-  hipFuncSetAttribute((const void*)k_adj_s2d<2>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
-  hipFuncSetAttribute((const void*)k_adj_s2d<4>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
-  hipFuncSetAttribute((const void*)k_adj_s2d<8>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
-}
 }  // namespace born

@@ -1,0 +1,569 @@
+// Structured 2-D grid contractions of the Born operator pair (gfx950).
+//
+// setup_grid_symmetric.shift_lateral (setup_grid_symmetric.m:95-241) makes
+// element e see grid point (i, z) at h_ref point (ext[i - e*Np + (N_e-1)*Np], z):
+// for one (frequency, depth row) every element's transfer function is a
+// window of one extended lateral row.  Per row the forward is therefore the
+// Toeplitz-in-x complex GEMM
+//     V[e][c] += sum_i Hext[i - e*Np + off] * B[i][c],  B[i][c] = p_inc[pair_c][i] gamma_r[i]
+// and the adjoint its conjugate transpose.  Both run on 16x16x4 FP32 MFMA with
+// the three-multiplication complex product (Re = P1 - P2 or P1 + P2, Im from
+// the product of the sums), so each complex multiply-add costs three real
+// MFMA products instead of four.
+//
+// Data movement: the next row (forward) or next chunk (adjoint) is loaded
+// from HBM into registers while the MFMAs of the current one run, then
+// written to LDS between two barriers.  Two 4-wave workgroups per CU; the
+// MFMA accumulators stay in architectural VGPRs.
+#include "born_internal.h"
+
+namespace born {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+namespace {
+constexpr int NCS = NC + 1;   // padded LDS stride of a 16-column row
+constexpr int HPT_MAX = 8;    // per-thread entries of the extended-row copy
+constexpr int WPT_MAX = 8;    // per-thread entries of the W tile (Ne_pad <= 128)
+
+__device__ __forceinline__ v4f mfma16(float a, float b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ v4f vzero() { return v4f{0.f, 0.f, 0.f, 0.f}; }
+__device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+}  // namespace
+
+// ------------------------------------------------------------------ forward
+struct FwdArgs {
+  const float2* h;
+  const float2* P;
+  const float2* gm;
+  const int* ext;
+  const Chunk* chunks;
+  const int* col_pair;
+  int Nx, Nz, N_pts, N_e, Np, N_ext, NZB, ZR, S, MT, n_items, Ne_pad;
+  float2* vpart;
+};
+
+// Work item = (chunk, z-block of ZR rows); workgroups are persistent and take
+// items blockIdx.x, blockIdx.x + gridDim.x, ...  Within a row the four waves
+// split the work as EW element groups x PW point groups (2 x 2 when MT >= 2):
+// wave w holds element tiles [eg*MTW, (eg+1)*MTW) x 16 columns and walks the
+// K-steps [pg*SW, (pg+1)*SW) of the row.  NP = points per pitch (4; 0 =
+// runtime), PPT = row points per thread in the B-tile fill.
+template <int MT, int NP, int PPT>
+__global__ __launch_bounds__(256, 2) void k_fwd_s2d(FwdArgs A) {
+  extern __shared__ float2 lds[];
+  const int Np = NP ? NP : A.Np;
+  // entries of the polyphase copy per thread: XP <= 256*PPT, QL = (XP-1)/Np + 16*MT
+  constexpr int HPT = NP ? PPT + (16 * MT * NP + 255) / 256 : HPT_MAX;
+  const int XP = 16 * A.S;
+  const int QL = (XP - 1) / Np + 16 * MT;
+  const int HN = Np * QL;
+  float2* Hp = lds;            // [Np][QL] polyphase extended row
+  float2* Bs = lds + HN;       // [XP][NCS]
+  constexpr int EW = MT >= 2 ? 2 : 1;        // element groups
+  constexpr int PW = 4 / EW;                 // point groups
+  constexpr int MTW = MT / EW;               // element tiles per wave
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int eg = w % EW, pg = w / EW;
+  const int SW = 4 * A.S / PW;               // K-steps per wave and row
+  const int kk = lane >> 4, col = lane & 15;
+  const int Qoff = 16 * MT - A.N_e;
+  const int n_my = (A.n_items - (int)blockIdx.x + (int)gridDim.x - 1) / (int)gridDim.x;
+  if (n_my <= 0) return;
+
+  // row-invariant gather indices of the polyphase copy
+  int eidx[HPT];
+#pragma unroll
+  for (int j = 0; j < HPT; ++j) {
+    const int idx = tid + 256 * j;
+    eidx[j] = -1;
+    if (idx < HN) {
+      const int ph = idx / QL, q = idx - ph * QL;
+      const int t = (q - Qoff) * Np + ph;
+      if (t >= 0 && t < A.N_ext) eidx[j] = A.ext[t];
+    }
+  }
+
+  // current step
+  int it = 0;
+  int item = blockIdx.x;
+  Chunk ch = A.chunks[item / A.NZB];
+  int z = (item % A.NZB) * A.ZR;
+  int z_end = min(A.Nz, z + A.ZR);
+
+  float2 hreg[HPT];
+  float2 praw[PPT][NC];
+  float2 greg[PPT];
+
+  // issue the loads of step (c, zz) into registers
+#define BORN_FWD_PREFETCH(c, zz)                                                        \
+  {                                                                                     \
+    const float2* hrow_ = A.h + (size_t)(c).fg * A.N_pts + (size_t)(zz) * A.Nx;         \
+    _Pragma("unroll") for (int j = 0; j < HPT; ++j) {                                   \
+      hreg[j] = eidx[j] >= 0 ? hrow_[eidx[j]] : make_float2(0.f, 0.f);                  \
+    }                                                                                   \
+    const float2* grow_ = A.gm + (size_t)(c).r * A.N_pts + (size_t)(zz) * A.Nx;         \
+    _Pragma("unroll") for (int j = 0; j < PPT; ++j) {                                   \
+      const int i_ = tid + 256 * j;                                                     \
+      const bool ok_ = i_ < A.Nx;                                                       \
+      greg[j] = ok_ ? grow_[i_] : make_float2(0.f, 0.f);                                \
+      _Pragma("unroll") for (int c_ = 0; c_ < NC; ++c_) {                               \
+        float2 v_ = make_float2(0.f, 0.f);                                              \
+        if (ok_ && c_ < (c).ncols)                                                      \
+          v_ = A.P[(size_t)A.col_pair[(c).col_start + c_] * A.N_pts + (size_t)(zz) * A.Nx + i_]; \
+        praw[j][c_] = v_;                                                               \
+      }                                                                                 \
+    }                                                                                   \
+  }
+
+  BORN_FWD_PREFETCH(ch, z)
+
+  v4f SR[MTW], SI[MTW];
+#pragma unroll
+  for (int mt = 0; mt < MTW; ++mt) {
+    SR[mt] = vzero();
+    SI[mt] = vzero();
+  }
+
+  // per-lane LDS bases of the MFMA operands
+  int a_base, a_step;
+  if (NP == 4 || NP == 2 || NP == 1) {
+    a_base = (kk % Np) * QL + kk / Np + 16 * MT - 1 - col - 16 * MTW * eg;
+    a_step = 4 / Np;
+  } else {
+    a_base = 0;
+    a_step = 0;
+  }
+  const int b_base = kk * NCS + col;
+
+  while (true) {
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < HPT; ++j) {
+      const int idx = tid + 256 * j;
+      if (idx < HN) Hp[idx] = hreg[j];
+    }
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+      const int i = tid + 256 * j;
+      if (i < XP) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) Bs[i * NCS + c] = cmulf(praw[j][c], greg[j]);
+      }
+    }
+    __syncthreads();
+
+    // next step and its prefetch
+    bool has_next = true;
+    int n_item = item, n_z = z + 1;
+    Chunk n_ch = ch;
+    if (n_z >= z_end) {
+      if (it + 1 >= n_my) {
+        has_next = false;
+      } else {
+        n_item = blockIdx.x + (it + 1) * gridDim.x;
+        n_ch = A.chunks[n_item / A.NZB];
+        n_z = (n_item % A.NZB) * A.ZR;
+      }
+    }
+    if (has_next) BORN_FWD_PREFETCH(n_ch, n_z)
+
+    // MFMA over this wave's points of the row
+    v4f P1[MTW], P2[MTW], P3[MTW];
+#pragma unroll
+    for (int mt = 0; mt < MTW; ++mt) {
+      P1[mt] = vzero();
+      P2[mt] = vzero();
+      P3[mt] = vzero();
+    }
+    for (int s = 0; s < SW; ++s) {
+      const int ks = pg * SW + s;
+      const float2 b = Bs[b_base + 4 * ks * NCS];
+      const float bsum = b.x + b.y;
+      int ai;
+      if (NP == 4 || NP == 2 || NP == 1) {
+        ai = a_base + a_step * ks;
+      } else {
+        const int i = 4 * ks + kk;
+        ai = (i % Np) * QL + i / Np + 16 * MT - 1 - col - 16 * MTW * eg;
+      }
+#pragma unroll
+      for (int mt = 0; mt < MTW; ++mt) {
+        const float2 a = Hp[ai - 16 * mt];
+        const float asum = a.x + a.y;
+        P1[mt] = mfma16(a.x, b.x, P1[mt]);
+        P2[mt] = mfma16(a.y, b.y, P2[mt]);
+        P3[mt] = mfma16(asum, bsum, P3[mt]);
+      }
+    }
+    // row done: Re = P1 - P2, Im = P3 - P1 - P2, folded into the item sums
+#pragma unroll
+    for (int mt = 0; mt < MTW; ++mt) {
+      SR[mt] += P1[mt] - P2[mt];
+      SI[mt] += P3[mt] - P1[mt] - P2[mt];
+    }
+
+    if (z + 1 >= z_end) {
+      // item done: sum the point groups through LDS and write the partial
+      __syncthreads();
+      float* red = reinterpret_cast<float*>(lds);   // [w][mtw][j][lane]
+#pragma unroll
+      for (int mt = 0; mt < MTW; ++mt) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          red[((w * MTW + mt) * 8 + v) * 64 + lane] = SR[mt][v];
+          red[((w * MTW + mt) * 8 + 4 + v) * 64 + lane] = SI[mt][v];
+        }
+        SR[mt] = vzero();
+        SI[mt] = vzero();
+      }
+      __syncthreads();
+      const int ci = item / A.NZB, zb = item - ci * A.NZB;
+      float* out = reinterpret_cast<float*>(A.vpart + ((size_t)ci * A.NZB + zb) * A.Ne_pad * NC);
+      for (int idx = tid; idx < MT * 512; idx += 256) {
+        const int mt = idx >> 9, e_g = mt / MTW, mtw = mt - e_g * MTW, rest = idx & 511;
+        float sum = 0.f;
+#pragma unroll
+        for (int p_g = 0; p_g < PW; ++p_g) sum += red[((p_g * EW + e_g) * MTW + mtw) * 512 + rest];
+        const int ln = idx & 63, j = (idx >> 6) & 7;
+        const int e = 16 * mt + 4 * (ln >> 4) + (j & 3);
+        out[((size_t)e * NC + (ln & 15)) * 2 + (j >> 2)] = sum;
+      }
+      ++it;
+    }
+    if (!has_next) break;
+    item = n_item;
+    ch = n_ch;
+    z = n_z;
+    z_end = min(A.Nz, (item % A.NZB) * A.ZR + A.ZR);
+  }
+#undef BORN_FWD_PREFETCH
+}
+
+static size_t fwd_lds_bytes(int MT, int S, int Np) {
+  const int XP = 16 * S;
+  const int QL = (XP - 1) / Np + 16 * MT;
+  const size_t body = ((size_t)Np * QL + (size_t)XP * NCS) * sizeof(float2);
+  const size_t red = (size_t)4 * MT * 8 * 64 * sizeof(float);
+  return std::max(body, red);
+}
+
+bool fwd_structured_fits(int Nx, int Ne_pad, int Np) {
+  const int S = (Nx + 15) / 16;
+  const int XP = 16 * S;
+  const int QL = (XP - 1) / Np + Ne_pad;
+  return XP <= 1024 && Np * QL <= 256 * HPT_MAX && fwd_lds_bytes(Ne_pad / 16, S, Np) <= 160 * 1024 &&
+         Ne_pad <= 128;
+}
+
+void launch_fwd_structured(const Plan& P, const float2* gm, int N_rhs, hipStream_t s) {
+  (void)N_rhs;
+  FwdArgs a;
+  a.h = P.d_h;
+  a.P = P.d_p;
+  a.gm = gm;
+  a.ext = P.d_ext;
+  a.chunks = P.d_chunks;
+  a.col_pair = P.d_col_pair;
+  a.Nx = P.Nx;
+  a.Nz = P.Nz;
+  a.N_pts = P.N_pts;
+  a.N_e = P.N_e;
+  a.Np = P.Np;
+  a.N_ext = P.N_ext;
+  a.NZB = P.NZB;
+  a.ZR = P.ZR;
+  a.S = (P.Nx + 15) / 16;
+  a.MT = P.Ne_pad / 16;
+  a.Ne_pad = P.Ne_pad;
+  a.n_items = (int)P.chunks.size() * P.NZB;
+  a.vpart = P.d_vpart;
+  const int XP = 16 * a.S;
+  const int PPT = (XP + 255) / 256;
+  const size_t lds = fwd_lds_bytes(a.MT, a.S, a.Np);
+  const int grid = std::min(a.n_items, 2 * P.n_cu);   // two resident workgroups per CU
+#define BORN_FWD_LAUNCH(MTv, NPv, PPTv) \
+  hipLaunchKernelGGL((k_fwd_s2d<MTv, NPv, PPTv>), dim3(grid), dim3(256), lds, s, a)
+#define BORN_FWD_PPT(MTv, NPv)                    \
+  if (PPT <= 1) BORN_FWD_LAUNCH(MTv, NPv, 1);     \
+  else if (PPT <= 2) BORN_FWD_LAUNCH(MTv, NPv, 2); \
+  else BORN_FWD_LAUNCH(MTv, NPv, 4);
+#define BORN_FWD_NP(MTv)       \
+  if (a.Np == 4) {             \
+    BORN_FWD_PPT(MTv, 4)       \
+  } else {                     \
+    BORN_FWD_PPT(MTv, 0)       \
+  }
+  switch (a.MT) {
+    case 1: BORN_FWD_NP(1) break;
+    case 2: BORN_FWD_NP(2) break;
+    case 4: BORN_FWD_NP(4) break;
+    default: BORN_FWD_NP(8) break;
+  }
+#undef BORN_FWD_NP
+#undef BORN_FWD_PPT
+#undef BORN_FWD_LAUNCH
+}
+
+// ------------------------------------------------------------------ adjoint
+struct AdjArgs {
+  const float2* h;
+  const float2* P;
+  const float2* W;
+  const int* ext;
+  const Chunk* chunks;
+  const int* col_pair;
+  int n_chunks, Nx, N_pts, N_e, Ne_pad, Np, N_ext, N_rhs;
+  float2* g_out;
+};
+
+// One workgroup = (lateral segment of 64*MW points, depth row z).  It walks
+// every chunk (rhs-major, then frequency) in a fixed order and keeps its
+// output in registers: no atomics, bitwise repeatable.  Per chunk
+//   Y[i][c] = sum_e conj(Hext[i - e*Np + off]) W[e][c]       (MFMA, A = points x elements)
+//   gamma_hat[i] += sum_c conj(p_inc[pair_c][i]) Y[i][c]     (registers)
+template <int MT, int MW, int NP>
+__global__ __launch_bounds__(256, 2) void k_adj_s2d(AdjArgs A) {
+  extern __shared__ float2 lds[];
+  constexpr int SEG = 64 * MW;
+  constexpr int PS = SEG + 4;                 // padded stride of the p_inc tile
+  constexpr int PPT = NC * SEG / 256;         // p_inc tile entries per thread
+  constexpr int NEP = 16 * MT;                // padded element count
+  constexpr int HPT = NP ? (SEG + (NEP - 1) * NP + 255) / 256 : HPT_MAX;
+  constexpr int WPT = MT;                     // W tile entries per thread
+  const int Np = NP ? NP : A.Np;
+  const int L = SEG + (NEP - 1) * Np;
+  float2* He = lds;                           // [L]
+  float2* Ws = He + L;                        // [NEP][NCS]
+  float2* Ps = Ws + NEP * NCS;                // [NC][PS]
+
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int kk = lane >> 4, col = lane & 15;
+  const int z = blockIdx.y;
+  const int i_s = blockIdx.x * SEG;
+  const int ib = i_s + w * 16 * MW;
+  const int t_shift = i_s - (NEP - A.N_e) * Np;
+
+  // row-invariant gather indices of the extended-row window
+  int eidx[HPT];
+#pragma unroll
+  for (int j = 0; j < HPT; ++j) {
+    const int u = tid + 256 * j;
+    const int t = u + t_shift;
+    eidx[j] = (u < L && t >= 0 && t < A.N_ext) ? A.ext[t] : -1;
+  }
+
+  float2 hreg[HPT], wreg[WPT], preg[PPT];
+#define BORN_ADJ_PREFETCH(c)                                                                     \
+  {                                                                                              \
+    const float2* hrow_ = A.h + (size_t)(c).fg * A.N_pts + (size_t)z * A.Nx;                     \
+    _Pragma("unroll") for (int j = 0; j < HPT; ++j) hreg[j] =                                    \
+        eidx[j] >= 0 ? hrow_[eidx[j]] : make_float2(0.f, 0.f);                                   \
+    const int wc_ = tid & 15;                                                                    \
+    const bool wok_ = wc_ < (c).ncols;                                                           \
+    const int wpair_ = wok_ ? A.col_pair[(c).col_start + wc_] : 0;                               \
+    _Pragma("unroll") for (int j = 0; j < WPT; ++j) {                                            \
+      const int k_ = (tid >> 4) + 16 * j;                                                        \
+      wreg[j] = (wok_ && k_ < A.N_e)                                                             \
+                    ? A.W[((size_t)wpair_ * A.N_e + k_) * A.N_rhs + (c).r] : make_float2(0.f, 0.f); \
+    }                                                                                            \
+    _Pragma("unroll") for (int j = 0; j < PPT; ++j) {                                            \
+      const int idx_ = tid + 256 * j;                                                            \
+      const int pc_ = idx_ / SEG, pi_ = idx_ - pc_ * SEG;                                        \
+      const int gi_ = i_s + pi_;                                                                 \
+      preg[j] = (pc_ < (c).ncols && gi_ < A.Nx)                                                  \
+                    ? A.P[(size_t)A.col_pair[(c).col_start + pc_] * A.N_pts + (size_t)z * A.Nx + gi_] \
+                    : make_float2(0.f, 0.f);                                                     \
+    }                                                                                            \
+  }
+
+  Chunk ch = A.chunks[0];
+  BORN_ADJ_PREFETCH(ch)
+
+  float gx[MW][4], gy[MW][4];
+#pragma unroll
+  for (int mw = 0; mw < MW; ++mw)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) gx[mw][v] = gy[mw][v] = 0.f;
+
+  const int u_base = ib - i_s + col + (NEP - 1) * Np;
+
+  for (int ci = 0; ci < A.n_chunks; ++ci) {
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < HPT; ++j) {
+      const int u = tid + 256 * j;
+      if (u < L) He[u] = hreg[j];
+    }
+#pragma unroll
+    for (int j = 0; j < WPT; ++j) Ws[((tid >> 4) + 16 * j) * NCS + (tid & 15)] = wreg[j];
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+      const int idx = tid + 256 * j;
+      const int pc = idx / SEG, pi = idx - pc * SEG;
+      Ps[pc * PS + pi] = preg[j];
+    }
+    __syncthreads();
+    const int r = ch.r;
+    const bool has_next = ci + 1 < A.n_chunks;
+    Chunk n_ch = ch;
+    if (has_next) {
+      n_ch = A.chunks[ci + 1];
+      BORN_ADJ_PREFETCH(n_ch)
+    }
+
+    v4f P1[MW], P2[MW], P3[MW];
+#pragma unroll
+    for (int mw = 0; mw < MW; ++mw) {
+      P1[mw] = vzero();
+      P2[mw] = vzero();
+      P3[mw] = vzero();
+    }
+#pragma unroll 4
+    for (int k0 = 0; k0 < NEP; k0 += 4) {
+      const int k = k0 + kk;
+      const float2 b = Ws[k * NCS + col];
+      const float bsum = b.x + b.y;
+      const int u0 = u_base - k * Np;
+#pragma unroll
+      for (int mw = 0; mw < MW; ++mw) {
+        const float2 a = He[u0 + 16 * mw];
+        const float adiff = a.x - a.y;
+        P1[mw] = mfma16(a.x, b.x, P1[mw]);
+        P2[mw] = mfma16(a.y, b.y, P2[mw]);
+        P3[mw] = mfma16(adiff, bsum, P3[mw]);
+      }
+    }
+    // conj(A) W: Re = P1 + P2, Im = P3 - P1 + P2; then gamma_hat += conj(p) * Y
+#pragma unroll
+    for (int mw = 0; mw < MW; ++mw) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const float yr = P1[mw][v] + P2[mw][v];
+        const float yi = P3[mw][v] - P1[mw][v] + P2[mw][v];
+        const float2 p = Ps[col * PS + (ib - i_s) + 16 * mw + 4 * kk + v];
+        gx[mw][v] += p.x * yr + p.y * yi;
+        gy[mw][v] += p.x * yi - p.y * yr;
+      }
+    }
+    if (!has_next || n_ch.r != r) {
+      // sum over the 16 column lanes and write gamma_hat[:, r]
+#pragma unroll
+      for (int mw = 0; mw < MW; ++mw) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          float x = gx[mw][v], y = gy[mw][v];
+#pragma unroll
+          for (int o = 1; o < 16; o <<= 1) {
+            x += __shfl_xor(x, o, 64);
+            y += __shfl_xor(y, o, 64);
+          }
+          gx[mw][v] = x;
+          gy[mw][v] = y;
+        }
+      }
+      if (col < 4) {
+#pragma unroll
+        for (int mw = 0; mw < MW; ++mw) {
+          const int i = ib + 16 * mw + 4 * kk + col;
+          float x = gx[mw][0], y = gy[mw][0];
+#pragma unroll
+          for (int v = 1; v < 4; ++v) {
+            if (col == v) {
+              x = gx[mw][v];
+              y = gy[mw][v];
+            }
+          }
+          if (i < A.Nx) A.g_out[(size_t)r * A.N_pts + (size_t)z * A.Nx + i] = make_float2(x, y);
+        }
+      }
+#pragma unroll
+      for (int mw = 0; mw < MW; ++mw)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) gx[mw][v] = gy[mw][v] = 0.f;
+    }
+    ch = n_ch;
+  }
+#undef BORN_ADJ_PREFETCH
+}
+
+static int adj_mw(int Nx) { return Nx <= 128 ? 2 : 4; }
+
+static size_t adj_lds_bytes(int MW, int Ne_pad, int Np) {
+  const int SEG = 64 * MW;
+  const int L = SEG + (Ne_pad - 1) * Np;
+  return ((size_t)L + (size_t)Ne_pad * NCS + (size_t)NC * (SEG + 4)) * sizeof(float2);
+}
+
+bool adj_structured_fits(int Nx, int Ne_pad, int Np) {
+  const int MW = adj_mw(Nx);
+  const int L = 64 * MW + (Ne_pad - 1) * Np;
+  return L <= 256 * HPT_MAX && Ne_pad <= 16 * WPT_MAX && adj_lds_bytes(MW, Ne_pad, Np) <= 160 * 1024;
+}
+
+void launch_adj_structured(const Plan& P, int N_rhs, float2* g_out, hipStream_t s) {
+  AdjArgs a;
+  a.h = P.d_h;
+  a.P = P.d_p;
+  a.W = P.d_W;
+  a.ext = P.d_ext;
+  a.chunks = P.d_chunks;
+  a.col_pair = P.d_col_pair;
+  a.n_chunks = (int)P.chunks.size();
+  a.Nx = P.Nx;
+  a.N_pts = P.N_pts;
+  a.N_e = P.N_e;
+  a.Ne_pad = P.Ne_pad;
+  a.Np = P.Np;
+  a.N_ext = P.N_ext;
+  a.N_rhs = N_rhs;
+  a.g_out = g_out;
+  const int MW = adj_mw(P.Nx);
+  const int SEG = 64 * MW;
+  const size_t lds = adj_lds_bytes(MW, P.Ne_pad, P.Np);
+  const dim3 grid((P.Nx + SEG - 1) / SEG, P.Nz);
+#define BORN_ADJ_LAUNCH(MTv, MWv, NPv) hipLaunchKernelGGL((k_adj_s2d<MTv, MWv, NPv>), grid, dim3(256), lds, s, a)
+#define BORN_ADJ_MW(MTv, NPv)                     \
+  if (MW == 2) BORN_ADJ_LAUNCH(MTv, 2, NPv);      \
+  else BORN_ADJ_LAUNCH(MTv, 4, NPv);
+#define BORN_ADJ_NP(MTv)        \
+  if (P.Np == 4) {              \
+    BORN_ADJ_MW(MTv, 4)         \
+  } else {                      \
+    BORN_ADJ_MW(MTv, 0)         \
+  }
+  switch (P.Ne_pad / 16) {
+    case 1: BORN_ADJ_NP(1) break;
+    case 2: BORN_ADJ_NP(2) break;
+    case 4: BORN_ADJ_NP(4) break;
+    default: BORN_ADJ_NP(8) break;
+  }
+#undef BORN_ADJ_NP
+#undef BORN_ADJ_MW
+#undef BORN_ADJ_LAUNCH
+}
+
+// Allow dynamic LDS up to the CU's 160 KiB for the structured kernels.
+template <int MT>
+static void set_max_lds_mt(int b) {
+#define BORN_SET(K) hipFuncSetAttribute((const void*)(K), hipFuncAttributeMaxDynamicSharedMemorySize, b)
+  BORN_SET((k_fwd_s2d<MT, 4, 1>)); BORN_SET((k_fwd_s2d<MT, 4, 2>)); BORN_SET((k_fwd_s2d<MT, 4, 4>));
+  BORN_SET((k_fwd_s2d<MT, 0, 1>)); BORN_SET((k_fwd_s2d<MT, 0, 2>)); BORN_SET((k_fwd_s2d<MT, 0, 4>));
+  BORN_SET((k_adj_s2d<MT, 2, 4>)); BORN_SET((k_adj_s2d<MT, 4, 4>));
+  BORN_SET((k_adj_s2d<MT, 2, 0>)); BORN_SET((k_adj_s2d<MT, 4, 0>));
+#undef BORN_SET
+}
+
+void set_max_lds(size_t bytes) {
+  const int b = (int)bytes;
+  set_max_lds_mt<1>(b);
+  set_max_lds_mt<2>(b);
+  set_max_lds_mt<4>(b);
+  set_max_lds_mt<8>(b);
+}
+
+}  // namespace born

@@ -187,6 +187,18 @@ class OperatorBorn:
     def force_general(self, on: bool = True):
         check(_lib.lib().born_plan_force_general(self._plan, int(on)))
 
+    def set_timing(self, on: bool = True):
+        """Record HIP events around every contraction launch (on its stream)."""
+        check(_lib.lib().born_plan_set_timing(self._plan, int(on)))
+
+    def get_timing(self):
+        """(ms, launches) summed per contraction kernel since the last call:
+        {'forward': (ms, n), 'adjoint': (ms, n)}."""
+        ms = (C.c_double * 2)()
+        n = (C.c_int64 * 2)()
+        check(_lib.lib().born_plan_get_timing(self._plan, ms, n))
+        return {"forward": (ms[0], n[0]), "adjoint": (ms[1], n[1])}
+
     def set_precision(self, precision: str):
         check(_lib.lib().born_plan_set_precision(self._plan, _lib.PRECISIONS[precision]))
 

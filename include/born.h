@@ -178,6 +178,15 @@ int born_plan_path(born_plan_t plan);
 /* Force the general gather path (for cross-checking the structured one). */
 int born_plan_force_general(born_plan_t plan, int32_t on);
 
+/* Kernel timing with HIP events recorded on the launch stream around the
+ * grid-contraction kernels (no counterpart in the reference; used by the
+ * benchmark).  While enabled, every forward/adjoint apply records one event
+ * pair per contraction launch.  born_plan_get_timing synchronizes and returns
+ * the summed milliseconds and launch counts of [0] the forward contraction
+ * and [1] the adjoint contraction, then clears them. */
+int born_plan_set_timing(born_plan_t plan, int32_t on);
+int born_plan_get_timing(born_plan_t plan, double* ms, int64_t* launches);
+
 /* Thread-local message of the last failure on this thread. */
 const char* born_last_error(void);
 
