@@ -109,6 +109,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", default="fp32", choices=list(PEAK))
+    ap.add_argument("--compare", action="store_true",
+                    help="also time the other form and report the split/plain ratio")
     ap.add_argument("--config", default="config1")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-mixes", type=int, default=128)

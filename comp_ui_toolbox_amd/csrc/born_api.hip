@@ -357,6 +357,7 @@ static void finish_plan(Plan& P) {
   detect_structure(P);
   if (!P.structured) ensure_shift_on_device(P);
   set_max_lds(160 * 1024);
+  set_max_lds_split(160 * 1024);
   HIP_CHECK(hipStreamSynchronize(P.stream));
 }
 
@@ -595,7 +596,10 @@ static void forward_device(Plan& P, const float2* gm, int N_rhs, void* u, bool o
   if (use_structured(P)) {
     {
       TimedLaunch t(P, s, 0);
-      launch_fwd_structured(P, gm, N_rhs, s);
+      if (P.precision == BORN_PRECISION_FP32)
+        launch_fwd_structured(P, gm, N_rhs, s);
+      else
+        launch_fwd_split(P, gm, P.precision, 1.0f, 1.0f, s);
     }
     launch_fwd_reduce(P, N_rhs, s);
   } else {
@@ -611,7 +615,10 @@ static void adjoint_device(Plan& P, const float2* u, int N_rhs, float2* g, hipSt
   launch_adj_unmix(P, u, N_rhs, s);
   TimedLaunch t(P, s, 1);
   if (use_structured(P)) {
-    launch_adj_structured(P, N_rhs, g, s);
+    if (P.precision == BORN_PRECISION_FP32)
+      launch_adj_structured(P, N_rhs, g, s);
+    else
+      launch_adj_split(P, N_rhs, g, s);
   } else {
     ensure_shift_on_device(P);
     launch_adj_general(P, N_rhs, g, s);
@@ -620,8 +627,11 @@ static void adjoint_device(Plan& P, const float2* u, int N_rhs, float2* g, hipSt
 }
 
 static void check_precision(const Plan& P) {
-  if (P.precision != BORN_PRECISION_FP32)
-    throw Error(BORN_ERR_UNSUPPORTED, "split-precision contraction is not available in this build");
+  if (P.precision == BORN_PRECISION_SPLIT_FP16)
+    throw Error(BORN_ERR_UNSUPPORTED, "FP16 split precision is not available in this build");
+  if (P.precision != BORN_PRECISION_FP32 && !(fwd_split_supported(P) && !P.force_general))
+    throw Error(BORN_ERR_UNSUPPORTED,
+                "split precision needs the structured 2-D path with 4 grid points per pitch and 17 to 128 elements");
 }
 
 static void host_forward(Plan& P, const double* gamma, int N_rhs, double* u) {

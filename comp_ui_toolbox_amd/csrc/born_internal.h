@@ -133,6 +133,10 @@ void launch_synth_pinc(const Plan& P, const double2* h64, const int* d_meas_Nfm,
                        const int* d_tx_el, const double2* d_vd, const int* d_vd_off, hipStream_t s);
 void set_max_lds(size_t bytes);
 bool fwd_structured_fits(int Nx, int Ne_pad, int Np);
+bool fwd_split_supported(const Plan& P);
+void launch_fwd_split(const Plan& P, const float2* gm, int precision, float a_scale, float b_scale, hipStream_t s);
+void launch_adj_split(const Plan& P, int N_rhs, float2* g_out, hipStream_t s);
+void set_max_lds_split(int bytes);
 bool adj_structured_fits(int Nx, int Ne_pad, int Np);
 
 }  // namespace born

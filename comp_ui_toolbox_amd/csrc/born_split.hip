@@ -1,0 +1,612 @@
+// Split-precision structured contractions (gfx950).
+//
+// Same contractions as k_fwd_s2d / k_adj_s2d (born_structured.hip), with every
+// FP32 operand x written as x_hi + x_lo (hi = x rounded to BF16, lo = the
+// rounded remainder) and each real product taken as
+//     x*y ~ x_hi*y_hi + x_hi*y_lo + x_lo*y_hi
+// on 16x16x32 MFMA with FP32 accumulation.  The dropped x_lo*y_lo term and
+// the roundings leave a relative error of about 2^-16 per product.  A complex
+// multiply-add is four real products (Re = Ar Br - Ai Bi, Im = Ar Bi + Ai Br),
+// i.e. 12 MFMAs of K = 32 with 4 hi/lo vectors per operand: fewer LDS bytes
+// per MFMA than the three-product form, which matters because the operands
+// stream from LDS.
+#include "born_internal.h"
+
+namespace born {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int SEG = 256;          // points per step (forward) / per workgroup (adjoint)
+
+__device__ __forceinline__ unsigned short bf_bits(float x) {
+  return __builtin_bit_cast(unsigned short, (__bf16)x);  // round to nearest even
+}
+__device__ __forceinline__ float bf_val(unsigned short b) { return __uint_as_float(((unsigned int)b) << 16); }
+
+__device__ __forceinline__ void split2(float x, unsigned short& hi, unsigned short& lo) {
+  hi = bf_bits(x);
+  lo = bf_bits(x - bf_val(hi));
+}
+
+__device__ __forceinline__ v4f mfma32(uint4 a, uint4 b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a), __builtin_bit_cast(v8bf, b), c, 0, 0, 0);
+}
+__device__ __forceinline__ uint4 neg8(uint4 v) {
+  return make_uint4(v.x ^ 0x80008000u, v.y ^ 0x80008000u, v.z ^ 0x80008000u, v.w ^ 0x80008000u);
+}
+__device__ __forceinline__ uint4 join(uint2 a, uint2 b) { return make_uint4(a.x, a.y, b.x, b.y); }
+
+__device__ __forceinline__ v4f vz() { return v4f{0.f, 0.f, 0.f, 0.f}; }
+
+}  // namespace
+
+// ======================================================================
+// forward
+// ======================================================================
+struct FwdSplitArgs {
+  const float2* h;
+  const float2* P;
+  const float2* gm;
+  const int* ext;
+  const Chunk* chunks;
+  const int* col_pair;
+  int Nx, Nz, N_pts, N_e, N_ext, NZB, ZR, nseg, n_items, Ne_pad;
+  float2* vpart;
+};
+
+// Work item = (chunk, z-block); a step = (row z, 256-point segment xs).
+// Waves: 2 element groups x 2 point groups; wave (eg, pg) holds element
+// tiles [eg*MTW, (eg+1)*MTW) and K-blocks pg*4 .. pg*4+3 of the segment.
+// Points per pitch Np = 4.
+//
+// LDS (u16 units):
+//   Aw[comp][copy][LAP]  extended-row window of the segment (comp = re_hi,
+//                        re_lo, im_hi, im_lo); copy 1 is copy 0 shifted by 4
+//                        entries so every lane's 8-entry read is 16-byte aligned
+//   Bt[comp][c][SEGP]    p_inc*gamma of the segment, points contiguous
+template <int MT>
+__global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(FwdSplitArgs A) {
+  extern __shared__ unsigned short ldsu[];
+  constexpr int NP = 4;
+  constexpr int SEGP = SEG + 8;
+  constexpr int EW = MT >= 2 ? 2 : 1;
+  constexpr int PW = 4 / EW;
+  constexpr int MTW = MT / EW;
+  constexpr int KBW = (SEG / 32) / PW;
+  constexpr int LA = SEG + NP * (16 * MT - 1);
+  constexpr int LAP = ((LA + 4 + 7) / 8) * 8 + 8;
+  constexpr int APT = (LA + 255) / 256;
+  unsigned short* Aw = ldsu;                 // [4][2][LAP]
+  unsigned short* Bt = ldsu + 4 * 2 * LAP;   // [4][NC][SEGP]
+
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int eg = w % EW, pg = w / EW;
+  const int kk = lane >> 4, col = lane & 15;
+  const int n_my = (A.n_items - (int)blockIdx.x + (int)gridDim.x - 1) / (int)gridDim.x;
+  if (n_my <= 0) return;
+  const int tpad = (16 * MT - A.N_e) * NP;
+
+  int it = 0, item = blockIdx.x;
+  Chunk ch = A.chunks[item / A.NZB];
+  int z = (item % A.NZB) * A.ZR;
+  int z_end = min(A.Nz, z + A.ZR);
+  int xs = 0;
+
+  float2 hreg[APT];
+  float2 preg[8][2];
+  float2 greg[2];
+  const int bc0 = (tid >> 7) * 8;
+  const int bi = 2 * (tid & 127);
+
+#define BORN_SPLIT_PREFETCH(c, zz, xx)                                                          \
+  {                                                                                              \
+    const float2* hrow_ = A.h + (size_t)(c).fg * A.N_pts + (size_t)(zz) * A.Nx;                 \
+    _Pragma("unroll") for (int j = 0; j < APT; ++j) {                                           \
+      const int u_ = tid + 256 * j;                                                              \
+      const int t_ = u_ + (xx) * SEG - tpad;                                                     \
+      hreg[j] = (u_ < LA && t_ >= 0 && t_ < A.N_ext) ? hrow_[A.ext[t_]] : make_float2(0.f, 0.f); \
+    }                                                                                            \
+    const size_t rowo_ = (size_t)(zz) * A.Nx;                                                    \
+    const int gi_ = (xx) * SEG + bi;                                                             \
+    const float2* grow_ = A.gm + (size_t)(c).r * A.N_pts + rowo_;                                \
+    greg[0] = gi_ < A.Nx ? grow_[gi_] : make_float2(0.f, 0.f);                                   \
+    greg[1] = gi_ + 1 < A.Nx ? grow_[gi_ + 1] : make_float2(0.f, 0.f);                           \
+    _Pragma("unroll") for (int q = 0; q < 8; ++q) {                                             \
+      const int cc_ = bc0 + q;                                                                   \
+      const float2* prow_ = (cc_ < (c).ncols)                                                    \
+          ? A.P + (size_t)A.col_pair[(c).col_start + cc_] * A.N_pts + rowo_ : nullptr;          \
+      preg[q][0] = (prow_ && gi_ < A.Nx) ? prow_[gi_] : make_float2(0.f, 0.f);                   \
+      preg[q][1] = (prow_ && gi_ + 1 < A.Nx) ? prow_[gi_ + 1] : make_float2(0.f, 0.f);           \
+    }                                                                                            \
+  }
+
+  BORN_SPLIT_PREFETCH(ch, z, xs)
+
+  v4f CR[MTW], CI[MTW];
+#pragma unroll
+  for (int m = 0; m < MTW; ++m) {
+    CR[m] = vz();
+    CI[m] = vz();
+  }
+
+  // lane A position: e = 16*(eg*MTW + m) + col, u = 32*k + 8*kk + NP*(16*MT - 1 - e);
+  // u mod 8 = 4 exactly when col is even
+  const int e0 = 16 * (eg * MTW) + col;
+  const int u00 = 8 * kk + NP * (16 * MT - 1 - e0);
+  const int acopy = (u00 & 7) ? 1 : 0;
+  const int abase = acopy * LAP + (u00 - 4 * acopy);
+  const int bbase = col * SEGP + 8 * kk;
+
+  while (true) {
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < APT; ++j) {
+      const int u = tid + 256 * j;
+      if (u < LA) {
+        unsigned short v[4];
+        split2(hreg[j].x, v[0], v[1]);
+        split2(hreg[j].y, v[2], v[3]);
+#pragma unroll
+        for (int cp = 0; cp < 4; ++cp) {
+          Aw[(cp * 2 + 0) * LAP + u] = v[cp];
+          if (u >= 4) Aw[(cp * 2 + 1) * LAP + u - 4] = v[cp];
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      unsigned int packed[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+      for (int pp = 0; pp < 2; ++pp) {
+        const float2 p = preg[q][pp], g = greg[pp];
+        unsigned short v[4];
+        split2(p.x * g.x - p.y * g.y, v[0], v[1]);
+        split2(p.x * g.y + p.y * g.x, v[2], v[3]);
+#pragma unroll
+        for (int cp = 0; cp < 4; ++cp) packed[cp] |= ((unsigned int)v[cp]) << (16 * pp);
+      }
+#pragma unroll
+      for (int cp = 0; cp < 4; ++cp)
+        *reinterpret_cast<unsigned int*>(&Bt[(cp * NC + bc0 + q) * SEGP + bi]) = packed[cp];
+    }
+    __syncthreads();
+
+    bool has_next = true, item_done = false;
+    int n_item = item, n_z = z, n_xs = xs + 1;
+    Chunk n_ch = ch;
+    if (n_xs >= A.nseg) {
+      n_xs = 0;
+      n_z = z + 1;
+      if (n_z >= z_end) {
+        item_done = true;
+        if (it + 1 >= n_my) {
+          has_next = false;
+        } else {
+          n_item = blockIdx.x + (it + 1) * gridDim.x;
+          n_ch = A.chunks[n_item / A.NZB];
+          n_z = (n_item % A.NZB) * A.ZR;
+        }
+      }
+    }
+    if (has_next) BORN_SPLIT_PREFETCH(n_ch, n_z, n_xs)
+
+    // Tile m at K-block kb reads the window at 32*(pg*KBW + kb) - 64*m: the
+    // same data as tile 0 at K-block kb - 2m.  Walk the diagonals
+    // d = kb - 2m, holding the wave's B blocks in registers and streaming one
+    // A set per diagonal (double-buffered).
+    uint4 B4[KBW][4], NB[KBW][2];
+#pragma unroll
+    for (int kb = 0; kb < KBW; ++kb) {
+      const int k = pg * KBW + kb;
+#pragma unroll
+      for (int cp = 0; cp < 4; ++cp)
+        B4[kb][cp] = *reinterpret_cast<const uint4*>(&Bt[cp * NC * SEGP + bbase + 32 * k]);
+      NB[kb][0] = neg8(B4[kb][2]);
+      NB[kb][1] = neg8(B4[kb][3]);
+    }
+    constexpr int D0 = -2 * (MTW - 1);
+    uint4 Acur[4], Anext[4];
+    {
+      const int ao = abase + 32 * (pg * KBW + D0);
+#pragma unroll
+      for (int cp = 0; cp < 4; ++cp) Acur[cp] = *reinterpret_cast<const uint4*>(&Aw[cp * 2 * LAP + ao]);
+    }
+#pragma unroll
+    for (int d = D0; d < KBW; ++d) {
+      if (d + 1 < KBW) {
+        const int ao = abase + 32 * (pg * KBW + d + 1);
+#pragma unroll
+        for (int cp = 0; cp < 4; ++cp) Anext[cp] = *reinterpret_cast<const uint4*>(&Aw[cp * 2 * LAP + ao]);
+      }
+#pragma unroll
+      for (int m = 0; m < MTW; ++m) {
+        const int kb = d + 2 * m;
+        if (kb >= 0 && kb < KBW) {
+          CR[m] = mfma32(Acur[0], B4[kb][0], CR[m]);
+          CR[m] = mfma32(Acur[0], B4[kb][1], CR[m]);
+          CR[m] = mfma32(Acur[1], B4[kb][0], CR[m]);
+          CR[m] = mfma32(Acur[2], NB[kb][0], CR[m]);
+          CR[m] = mfma32(Acur[2], NB[kb][1], CR[m]);
+          CR[m] = mfma32(Acur[3], NB[kb][0], CR[m]);
+          CI[m] = mfma32(Acur[0], B4[kb][2], CI[m]);
+          CI[m] = mfma32(Acur[0], B4[kb][3], CI[m]);
+          CI[m] = mfma32(Acur[1], B4[kb][2], CI[m]);
+          CI[m] = mfma32(Acur[2], B4[kb][0], CI[m]);
+          CI[m] = mfma32(Acur[2], B4[kb][1], CI[m]);
+          CI[m] = mfma32(Acur[3], B4[kb][0], CI[m]);
+        }
+      }
+      if (d + 1 < KBW) {
+#pragma unroll
+        for (int cp = 0; cp < 4; ++cp) Acur[cp] = Anext[cp];
+      }
+    }
+
+    if (item_done) {
+      __syncthreads();
+      float* red = reinterpret_cast<float*>(ldsu);
+#pragma unroll
+      for (int m = 0; m < MTW; ++m) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          red[((w * MTW + m) * 8 + v) * 64 + lane] = CR[m][v];
+          red[((w * MTW + m) * 8 + 4 + v) * 64 + lane] = CI[m][v];
+        }
+        CR[m] = vz();
+        CI[m] = vz();
+      }
+      __syncthreads();
+      const int ci = item / A.NZB, zb = item - ci * A.NZB;
+      float* out = reinterpret_cast<float*>(A.vpart + ((size_t)ci * A.NZB + zb) * A.Ne_pad * NC);
+      for (int idx = tid; idx < MT * 512; idx += 256) {
+        const int mt = idx >> 9, e_g = mt / MTW, mtw = mt - e_g * MTW, rest = idx & 511;
+        float sum = 0.f;
+#pragma unroll
+        for (int p_g = 0; p_g < PW; ++p_g) sum += red[((p_g * EW + e_g) * MTW + mtw) * 512 + rest];
+        const int ln = idx & 63, j = (idx >> 6) & 7;
+        const int e = 16 * mt + 4 * (ln >> 4) + (j & 3);
+        out[((size_t)e * NC + (ln & 15)) * 2 + (j >> 2)] = sum;
+      }
+      ++it;
+    }
+    if (!has_next) break;
+    item = n_item;
+    ch = n_ch;
+    z = n_z;
+    xs = n_xs;
+    if (item_done) z_end = min(A.Nz, (item % A.NZB) * A.ZR + A.ZR);
+  }
+#undef BORN_SPLIT_PREFETCH
+}
+
+template <int MT>
+static size_t fwd_split_lds_bytes() {
+  constexpr int SEGP = SEG + 8;
+  constexpr int LA = SEG + 4 * (16 * MT - 1);
+  constexpr int LAP = ((LA + 4 + 7) / 8) * 8 + 8;
+  const size_t body = ((size_t)4 * 2 * LAP + (size_t)4 * NC * SEGP) * sizeof(unsigned short);
+  const size_t red = (size_t)4 * (MT >= 2 ? MT / 2 : MT) * 8 * 64 * sizeof(float);
+  return std::max(body, red);
+}
+
+// ======================================================================
+// adjoint
+// ======================================================================
+struct AdjSplitArgs {
+  const float2* h;
+  const float2* P;
+  const float2* W;
+  const int* ext;
+  const Chunk* chunks;
+  const int* col_pair;
+  int n_chunks, Nx, N_pts, N_e, N_ext, N_rhs;
+  float2* g_out;
+};
+
+// One workgroup = (256-point segment, depth row z); chunks are walked in a
+// fixed order with gamma_hat kept in registers (no atomics).
+// For Np = 4 the segment's points split into 4 phases ph = i mod 4 of 64
+// points s; for one phase the adjoint is the Toeplitz product
+//     Y[s][c] = sum_e conj(H(4 s + ph - 4 e + off)) W[e][c]
+// whose table index moves by one entry per element and per point.  Each
+// phase row is stored reversed (ascending in e) in four copies shifted by
+// 0..3 entries; an MFMA tile takes the 16 points s = s0 + 4 r (r = 0..15), so
+// all its lanes share one copy and every 8-byte read is aligned.
+// Wave w owns phase w (4 tiles, s0 = 0..3); K = elements in blocks of 32.
+//
+// LDS: Ar[comp][ph][copy][LR] (u16), Wt[comp][c][NEP + 8] (u16), Ps[c][SEG + 4] (float2)
+template <int MT>
+__global__ __launch_bounds__(256, 2) void k_adj_s2d_split(AdjSplitArgs A) {
+  extern __shared__ unsigned short ldsu[];
+  constexpr int NP = 4;
+  constexpr int NEP = 16 * MT;
+  constexpr int SPH = SEG / NP;                 // points per phase
+  constexpr int LQ = SPH + NEP;                 // phase-row entries used
+  constexpr int LR = ((LQ + 3 + 3) / 4) * 4 + 4; // padded copy length
+  constexpr int WS = NEP + 8;                   // padded W row
+  constexpr int PSS = SEG + 4;                  // padded p_inc row (float2)
+  constexpr int L = SEG + NP * (NEP - 1);       // extended-row window entries
+  constexpr int HPT = (L + 255) / 256;
+  constexpr int WPT = NEP * NC / 256;           // W entries per thread
+  constexpr int PPT = NC * SEG / 256;           // p_inc entries per thread
+  unsigned short* Ar = ldsu;                                    // [4][NP][4][LR]
+  unsigned short* Wt = ldsu + 4 * NP * 4 * LR;                  // [4][NC][WS]
+  float2* Ps = reinterpret_cast<float2*>(ldsu + ((4 * NP * 4 * LR + 4 * NC * WS + 7) / 8) * 8);  // [NC][PSS]
+
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int kk = lane >> 4, col = lane & 15;
+  const int z = blockIdx.y;
+  const int i_s = blockIdx.x * SEG;
+  const int t_shift = i_s - (NEP - A.N_e) * NP;  // window entry u holds Hext[u + t_shift]
+
+  // gather indices of the window (chunk-invariant)
+  int eidx[HPT];
+#pragma unroll
+  for (int j = 0; j < HPT; ++j) {
+    const int u = tid + 256 * j;
+    const int t = u + t_shift;
+    eidx[j] = (u < L && t >= 0 && t < A.N_ext) ? A.ext[t] : -1;
+  }
+  float2 hreg[HPT], wreg[WPT], preg[PPT];
+#define BORN_ADJS_PREFETCH(c)                                                                    \
+  {                                                                                              \
+    const float2* hrow_ = A.h + (size_t)(c).fg * A.N_pts + (size_t)z * A.Nx;                     \
+    _Pragma("unroll") for (int j = 0; j < HPT; ++j) hreg[j] =                                    \
+        eidx[j] >= 0 ? hrow_[eidx[j]] : make_float2(0.f, 0.f);                                   \
+    _Pragma("unroll") for (int j = 0; j < WPT; ++j) {                                            \
+      const int idx_ = tid + 256 * j;                                                            \
+      const int wc_ = idx_ / NEP, k_ = idx_ - wc_ * NEP;                                         \
+      wreg[j] = (wc_ < (c).ncols && k_ < A.N_e)                                                  \
+                    ? A.W[((size_t)A.col_pair[(c).col_start + wc_] * A.N_e + k_) * A.N_rhs + (c).r] \
+                    : make_float2(0.f, 0.f);                                                     \
+    }                                                                                            \
+    _Pragma("unroll") for (int j = 0; j < PPT; ++j) {                                            \
+      const int idx_ = tid + 256 * j;                                                            \
+      const int pc_ = idx_ / SEG, pi_ = idx_ - pc_ * SEG;                                        \
+      const int gi_ = i_s + pi_;                                                                 \
+      preg[j] = (pc_ < (c).ncols && gi_ < A.Nx)                                                  \
+                    ? A.P[(size_t)A.col_pair[(c).col_start + pc_] * A.N_pts + (size_t)z * A.Nx + gi_] \
+                    : make_float2(0.f, 0.f);                                                     \
+    }                                                                                            \
+  }
+
+  Chunk ch = A.chunks[0];
+  BORN_ADJS_PREFETCH(ch)
+
+  // gamma_hat partials: tile s0 (0..3), v (0..3) -> point s = s0 + 4*(4*kk + v), i = 4 s + ph
+  float gx[4][4], gy[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) gx[a][v] = gy[a][v] = 0.f;
+
+  const int ph = w;
+  for (int ci = 0; ci < A.n_chunks; ++ci) {
+    __syncthreads();
+    // window entry u (point offset i = u - NP*(NEP-1) + NP*e ... ) -> phase rows:
+    // u = (i - i_s) + NP*(NEP - 1 - e); with i - i_s = NP*s + ph:
+    //   u = NP*(s + NEP - 1 - e) + ph  ->  phase ph, index q = s + NEP - 1 - e
+    // reversed: m = (LQ - 1) - q = (SPH - s) + e; copy d holds m + d at position m
+#pragma unroll
+    for (int j = 0; j < HPT; ++j) {
+      const int u = tid + 256 * j;
+      if (u < L) {
+        const int p = u & 3, q = u >> 2;
+        const int m = (LQ - 1) - q;
+        unsigned short v[4];
+        split2(hreg[j].x, v[0], v[1]);
+        split2(-hreg[j].y, v[2], v[3]);        // conj
+#pragma unroll
+        for (int cp = 0; cp < 4; ++cp) {
+#pragma unroll
+          for (int d = 0; d < 4; ++d) {
+            const int mm = m - d;
+            if (mm >= 0) Ar[((cp * NP + p) * 4 + d) * LR + mm] = v[cp];
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < WPT; ++j) {
+      const int idx = tid + 256 * j;
+      const int wc = idx / NEP, k = idx - wc * NEP;
+      unsigned short v[4];
+      split2(wreg[j].x, v[0], v[1]);
+      split2(wreg[j].y, v[2], v[3]);
+#pragma unroll
+      for (int cp = 0; cp < 4; ++cp) Wt[(cp * NC + wc) * WS + k] = v[cp];
+    }
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+      const int idx = tid + 256 * j;
+      const int pc = idx / SEG, pi = idx - pc * SEG;
+      Ps[pc * PSS + pi] = preg[j];
+    }
+    __syncthreads();
+    const int r = ch.r;
+    const bool has_next = ci + 1 < A.n_chunks;
+    Chunk n_ch = ch;
+    if (has_next) {
+      n_ch = A.chunks[ci + 1];
+      BORN_ADJS_PREFETCH(n_ch)
+    }
+
+    // Y for the 4 tiles of this phase: conj(H) W with conj folded into the
+    // table (imag part stored negated): Re = Ar Wr - Ai' Wi, Im = Ar Wi + Ai' Wr
+    v4f YR[4], YI[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      YR[a] = vz();
+      YI[a] = vz();
+    }
+#pragma unroll 1
+    for (int kb = 0; kb < MT / 2; ++kb) {
+      const int eoff = 32 * kb + 8 * kk;
+      const uint4 wrh = *reinterpret_cast<const uint4*>(&Wt[(0 * NC + col) * WS + eoff]);
+      const uint4 wrl = *reinterpret_cast<const uint4*>(&Wt[(1 * NC + col) * WS + eoff]);
+      const uint4 wih = *reinterpret_cast<const uint4*>(&Wt[(2 * NC + col) * WS + eoff]);
+      const uint4 wil = *reinterpret_cast<const uint4*>(&Wt[(3 * NC + col) * WS + eoff]);
+      const uint4 nwih = neg8(wih), nwil = neg8(wil);
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        // tile s0 = a, row r = col (lane's A row): s = a + 4*col; m0 = SPH - s + e,
+        // e = eoff + j; copy d = m0 & 3 is the same for all lanes of the tile
+        const int m0 = SPH - (a + 4 * col) + eoff;
+        const int d = m0 & 3;
+        const int base = ((0 * NP + ph) * 4 + d) * LR + (m0 - d);
+        const int cstride = NP * 4 * LR;
+        const uint4 arh = join(*reinterpret_cast<const uint2*>(&Ar[base]),
+                               *reinterpret_cast<const uint2*>(&Ar[base + 4]));
+        const uint4 arl = join(*reinterpret_cast<const uint2*>(&Ar[base + cstride]),
+                               *reinterpret_cast<const uint2*>(&Ar[base + cstride + 4]));
+        const uint4 aih = join(*reinterpret_cast<const uint2*>(&Ar[base + 2 * cstride]),
+                               *reinterpret_cast<const uint2*>(&Ar[base + 2 * cstride + 4]));
+        const uint4 ail = join(*reinterpret_cast<const uint2*>(&Ar[base + 3 * cstride]),
+                               *reinterpret_cast<const uint2*>(&Ar[base + 3 * cstride + 4]));
+        YR[a] = mfma32(arh, wrh, YR[a]);
+        YR[a] = mfma32(arh, wrl, YR[a]);
+        YR[a] = mfma32(arl, wrh, YR[a]);
+        YR[a] = mfma32(aih, nwih, YR[a]);
+        YR[a] = mfma32(aih, nwil, YR[a]);
+        YR[a] = mfma32(ail, nwih, YR[a]);
+        YI[a] = mfma32(arh, wih, YI[a]);
+        YI[a] = mfma32(arh, wil, YI[a]);
+        YI[a] = mfma32(arl, wih, YI[a]);
+        YI[a] = mfma32(aih, wrh, YI[a]);
+        YI[a] = mfma32(aih, wrl, YI[a]);
+        YI[a] = mfma32(ail, wrh, YI[a]);
+      }
+    }
+    // gamma_hat += conj(p) Y; lane holds rows 4*kk + v (points s = a + 4*(4*kk + v)), column col
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int s = a + 4 * (4 * kk + v);
+        const float2 p = Ps[col * PSS + NP * s + ph];
+        gx[a][v] += p.x * YR[a][v] + p.y * YI[a][v];
+        gy[a][v] += p.x * YI[a][v] - p.y * YR[a][v];
+      }
+    }
+    if (!has_next || n_ch.r != r) {
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          float x = gx[a][v], y = gy[a][v];
+#pragma unroll
+          for (int o = 1; o < 16; o <<= 1) {
+            x += __shfl_xor(x, o, 64);
+            y += __shfl_xor(y, o, 64);
+          }
+          gx[a][v] = x;
+          gy[a][v] = y;
+        }
+      }
+      // lane col < 4 writes v = col for every tile
+      if (col < 4) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          float x = gx[a][0], y = gy[a][0];
+#pragma unroll
+          for (int v = 1; v < 4; ++v) {
+            if (col == v) {
+              x = gx[a][v];
+              y = gy[a][v];
+            }
+          }
+          const int s = a + 4 * (4 * kk + col);
+          const int i = i_s + NP * s + ph;
+          if (i < A.Nx) A.g_out[(size_t)r * A.N_pts + (size_t)z * A.Nx + i] = make_float2(x, y);
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) gx[a][v] = gy[a][v] = 0.f;
+    }
+    ch = n_ch;
+  }
+#undef BORN_ADJS_PREFETCH
+}
+
+template <int MT>
+static size_t adj_split_lds_bytes() {
+  constexpr int NEP = 16 * MT;
+  constexpr int LQ = SEG / 4 + NEP;
+  constexpr int LR = ((LQ + 3 + 3) / 4) * 4 + 4;
+  constexpr int WS = NEP + 8;
+  const size_t u16s = ((size_t)(4 * 4 * 4 * LR + 4 * NC * WS) + 7) / 8 * 8;
+  return u16s * sizeof(unsigned short) + (size_t)NC * (SEG + 4) * sizeof(float2);
+}
+
+// ======================================================================
+// launchers
+// ======================================================================
+bool fwd_split_supported(const Plan& P) {
+  return P.structured && P.Np == 4 && P.Ne_pad >= 32 && P.Ne_pad <= 128;
+}
+
+void launch_fwd_split(const Plan& P, const float2* gm, int precision, float, float, hipStream_t s) {
+  (void)precision;
+  FwdSplitArgs a;
+  a.h = P.d_h;
+  a.P = P.d_p;
+  a.gm = gm;
+  a.ext = P.d_ext;
+  a.chunks = P.d_chunks;
+  a.col_pair = P.d_col_pair;
+  a.Nx = P.Nx;
+  a.Nz = P.Nz;
+  a.N_pts = P.N_pts;
+  a.N_e = P.N_e;
+  a.N_ext = P.N_ext;
+  a.NZB = P.NZB;
+  a.ZR = P.ZR;
+  a.nseg = (P.Nx + SEG - 1) / SEG;
+  a.n_items = (int)P.chunks.size() * P.NZB;
+  a.Ne_pad = P.Ne_pad;
+  a.vpart = P.d_vpart;
+  const int grid = std::min(a.n_items, 2 * P.n_cu);
+  switch (P.Ne_pad / 16) {
+    case 2: hipLaunchKernelGGL((k_fwd_s2d_split<2>), dim3(grid), dim3(256), fwd_split_lds_bytes<2>(), s, a); break;
+    case 4: hipLaunchKernelGGL((k_fwd_s2d_split<4>), dim3(grid), dim3(256), fwd_split_lds_bytes<4>(), s, a); break;
+    default: hipLaunchKernelGGL((k_fwd_s2d_split<8>), dim3(grid), dim3(256), fwd_split_lds_bytes<8>(), s, a); break;
+  }
+}
+
+void launch_adj_split(const Plan& P, int N_rhs, float2* g_out, hipStream_t s) {
+  AdjSplitArgs a;
+  a.h = P.d_h;
+  a.P = P.d_p;
+  a.W = P.d_W;
+  a.ext = P.d_ext;
+  a.chunks = P.d_chunks;
+  a.col_pair = P.d_col_pair;
+  a.n_chunks = (int)P.chunks.size();
+  a.Nx = P.Nx;
+  a.N_pts = P.N_pts;
+  a.N_e = P.N_e;
+  a.N_ext = P.N_ext;
+  a.N_rhs = N_rhs;
+  a.g_out = g_out;
+  const dim3 grid((P.Nx + SEG - 1) / SEG, P.Nz);
+  switch (P.Ne_pad / 16) {
+    case 2: hipLaunchKernelGGL((k_adj_s2d_split<2>), grid, dim3(256), adj_split_lds_bytes<2>(), s, a); break;
+    case 4: hipLaunchKernelGGL((k_adj_s2d_split<4>), grid, dim3(256), adj_split_lds_bytes<4>(), s, a); break;
+    default: hipLaunchKernelGGL((k_adj_s2d_split<8>), grid, dim3(256), adj_split_lds_bytes<8>(), s, a); break;
+  }
+}
+
+void set_max_lds_split(int bytes) {
+#define BORN_SET(K) hipFuncSetAttribute((const void*)(K), hipFuncAttributeMaxDynamicSharedMemorySize, bytes)
+  BORN_SET((k_fwd_s2d_split<2>)); BORN_SET((k_fwd_s2d_split<4>)); BORN_SET((k_fwd_s2d_split<8>));
+  BORN_SET((k_adj_s2d_split<2>)); BORN_SET((k_adj_s2d_split<4>)); BORN_SET((k_adj_s2d_split<8>));
+#undef BORN_SET
+}
+
+}  // namespace born

@@ -268,3 +268,45 @@ def test_combined_quick_gpu_contract(torch_cuda):
     with pytest.raises(BornError, match="NoOperatorBorn"):
         combined_quick_gpu("op", 1, g)
     assert op.op_A(None, 0) == [T.N_observations(), T.N_points]
+
+
+# ------------------------------------------------------- split precision
+TOL_SPLIT = 1e-4
+
+
+@pytest.mark.parametrize("N_e,Nx,Nz", [(20, 80, 4), (40, 300, 3), (128, 520, 2)])
+def test_split_bf16_small_tables(torch_cuda, N_e, Nx, Nz):
+    spec = tiny_spec(N_e=N_e, Nx=Nx, Nz=Nz, npp=4)
+    T = O.build_tables(spec)
+    op = OperatorBorn.from_tables(T, precision="split_bf16")
+    g = rnd((T.N_points, 2), 71)
+    u = rnd((T.N_observations(), 2), 72)
+    e_f = O.rel_l2(op.forward(g), O.forward(T, g))
+    e_a = O.rel_l2(op.adjoint(u), O.adjoint(T, u))
+    assert op.path == "structured"
+    log("split bf16 N_e=%d Nx=%d: forward rel-L2 %.2e adjoint rel-L2 %.2e" % (N_e, Nx, e_f, e_a))
+    assert e_f < TOL_SPLIT and e_a < TOL_SPLIT
+
+
+def test_split_bf16_config1(torch_cuda, c1):
+    spec, op = c1
+    rng = np.random.Generator(np.random.PCG64(81))
+    pts = np.sort(rng.choice(spec.fov.N_points, 24, replace=False))
+    R = O.restricted_tables(spec, pts)
+    g = np.zeros(spec.fov.N_points)
+    g[pts] = rng.standard_normal(len(pts))
+    u = rnd(op.N_observations, 82)
+    op.set_precision("split_bf16")
+    try:
+        e_f = O.rel_l2(op.forward(g), O.forward(R, g[pts]))
+        e_a = O.rel_l2(op.adjoint(u)[pts], O.adjoint(R, u))
+        torch = torch_cuda
+        x = torch.tensor(C.phantom_speckle(spec).astype(np.complex64), device="cuda")
+        us = op.forward_dev(x)
+        op.set_precision("fp32")
+        u32 = op.forward_dev(x)
+        e_d = ((us - u32).norm() / u32.norm()).item()
+    finally:
+        op.set_precision("fp32")
+    log("split bf16 config1: forward rel-L2 %.2e adjoint rel-L2 %.2e dense vs fp32 %.2e" % (e_f, e_a, e_d))
+    assert e_f < TOL_SPLIT and e_a < TOL_SPLIT and e_d < TOL_SPLIT
