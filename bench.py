@@ -6,7 +6,12 @@ workload (128-element array, 512x512 FOV at pitch/4, 257 frequencies,
 side).  The tables are built on the device before the timed region; inputs
 are resident in HBM.  Prints one JSON line (see the keys below).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp32] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision split_bf16]
+                    [--no-compare] [--no-cpu]
+
+`value` is pairs/s of the selected form (split BF16 by default); the plain
+FP32 form is timed beside it on the same plan and input, and the ratio and
+the relative L2 difference between the two forms' outputs are reported.
 """
 import argparse
 import json
@@ -103,18 +108,93 @@ def spec_gamma(spec):
     return _GAMMA[spec.name]
 
 
+def time_form(op, precision, x, u, g, steps, warmup):
+    """Warm up, then time `steps` forward+adjoint pairs of one numeric form.
+    Returns wall seconds and the per-launch kernel milliseconds."""
+    import torch
+    op.set_precision(precision)
+    for _ in range(warmup):
+        op.forward_dev(x, out=u.view(1, -1))
+        op.adjoint_dev(u, out=g.view(1, -1))
+    torch.cuda.synchronize()
+    op.get_timing()
+    op.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        op.forward_dev(x, out=u.view(1, -1))
+        op.adjoint_dev(u, out=g.view(1, -1))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    op.set_timing(False)
+    timing = op.get_timing()
+    kern = {name: timing[name][0] / max(timing[name][1], 1) for name in ("forward", "adjoint")}
+    print(f"[bench] {precision}: {steps} pairs in {elapsed * 1e3:.1f} ms, kernels "
+          f"fwd {kern['forward']:.3f} ms adj {kern['adjoint']:.3f} ms", file=sys.stderr, flush=True)
+    return elapsed, kern
+
+
+def roofline(F, kern, precision, config):
+    dom = max(kern, key=kern.get)
+    peak, unit = PEAK[precision]
+    achieved = F / (kern[dom] * 1e-3) / 1e12
+    traffic = None
+    tfile = ROOT / "profiles" / f"{config}_hbm_traffic.md"
+    if tfile.exists():
+        # table rows: | precision | kernel | launch | fetch | write | bytes_per_launch | ratio |
+        for line in tfile.read_text().splitlines():
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) == 7 and cells[0] == precision and cells[1] == dom:
+                try:
+                    traffic = float(cells[5])
+                except ValueError:
+                    traffic = None
+    r = {"bound": "mfma", "kernel": f"{dom} contraction ({precision})", "achieved": achieved, "peak": peak,
+         "unit": unit, "frac": achieved / peak, "traffic": traffic}
+    if precision != "fp32":
+        # hi*hi + hi*lo + lo*hi: three MFMA products per algorithmic product
+        r["mfma_work_frac"] = 3.0 * achieved / peak
+    return r
+
+
+def rel_l2(a, b):
+    import torch
+    a = a.to(torch.complex128)
+    b = b.to(torch.complex128)
+    return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+
+
+def copy_probe(mib: int):
+    """A plain copy kernel of known size: its FETCH_SIZE / WRITE_SIZE
+    counters give the factor that turns counter readings into bytes."""
+    import torch
+    n = mib * 2**20
+    src = torch.ones(n, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"[bench] copy probe: 5 copies of {n} bytes", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--precision", default="fp32", choices=list(PEAK))
-    ap.add_argument("--compare", action="store_true",
-                    help="also time the other form and report the split/plain ratio")
+    ap.add_argument("--precision", default="split_bf16", choices=list(PEAK),
+                    help="the form whose pairs/s is reported as value")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="do not time the plain FP32 form beside the selected one")
     ap.add_argument("--config", default="config1")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-mixes", type=int, default=128)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
               f"(the operator pair runs on a single MI355X).", file=sys.stderr, flush=True)
@@ -126,7 +206,7 @@ def main():
 
     spec = C.CONFIGS[args.config]()
     t0 = time.perf_counter()
-    op = OperatorBorn.from_sequence(spec, precision=args.precision)
+    op = OperatorBorn.from_sequence(spec, precision="fp32")
     torch.cuda.synchronize()
     t_tables = time.perf_counter() - t0
     print(f"[bench] tables built on device in {t_tables:.2f} s, path {op.path}", file=sys.stderr, flush=True)
@@ -134,41 +214,25 @@ def main():
     x = torch.tensor(spec_gamma(spec).astype(np.complex64), device="cuda")
     u = torch.empty(op.N_observations, dtype=torch.complex64, device="cuda")
     g = torch.empty(op.N_points, dtype=torch.complex64, device="cuda")
-    op.forward_dev(x, out=u.view(1, -1))
-    for _ in range(args.warmup):
-        op.forward_dev(x, out=u.view(1, -1))
-        op.adjoint_dev(u, out=g.view(1, -1))
-    torch.cuda.synchronize()
-    op.get_timing()
-    op.set_timing(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        op.forward_dev(x, out=u.view(1, -1))
-        op.adjoint_dev(u, out=g.view(1, -1))
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    op.set_timing(False)
-    timing = op.get_timing()
-    print(f"[bench] {args.steps} pairs in {elapsed * 1e3:.1f} ms", file=sys.stderr, flush=True)
 
-    pairs_per_s = args.steps / elapsed
+    forms = [args.precision]
+    if not args.no_compare and args.precision != "fp32":
+        forms.append("fp32")
+    res, outs = {}, {}
+    for form in forms:
+        res[form] = time_form(op, form, x, u, g, args.steps, args.warmup)
+        # one pair from the same input, kept for the cross-form comparison
+        uf = torch.empty_like(u)
+        gf = torch.empty_like(g)
+        op.forward_dev(x, out=uf.view(1, -1))
+        op.adjoint_dev(uf, out=gf.view(1, -1))
+        outs[form] = (uf, gf)
+    torch.cuda.synchronize()
+
     F = algorithmic_flops(spec, 1)
     B = algorithmic_bytes(spec, 1)
-    kern = {}
-    for name in ("forward", "adjoint"):
-        ms, n = timing[name]
-        kern[name] = ms / max(n, 1)
-    dom = max(kern, key=kern.get)
-    peak, unit = PEAK[args.precision]
-    achieved = F / (kern[dom] * 1e-3) / 1e12
-    traffic = None
-    tfile = ROOT / "profiles" / f"traffic_{args.config}_{dom}.json"
-    if tfile.exists():
-        try:
-            traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
-        except (ValueError, OSError):
-            traffic = None
+    elapsed, kern = res[args.precision]
+    pairs_per_s = args.steps / elapsed
     result = {
         "metric": METRIC,
         "value": pairs_per_s,
@@ -193,16 +257,18 @@ def main():
             "flops_per_pass": F,
             "bytes_per_pass": B,
         },
-        "roofline": {
-            "bound": "mfma",
-            "kernel": f"{dom} contraction",
-            "achieved": achieved,
-            "peak": peak,
-            "unit": unit,
-            "frac": achieved / peak,
-            "traffic": traffic,
-        },
+        "roofline": roofline(F, kern, args.precision, args.config),
     }
+    if "fp32" in res and args.precision != "fp32":
+        e32, k32 = res["fp32"]
+        result["config"]["fp32_pairs_per_s"] = args.steps / e32
+        result["config"]["fp32_kernel_ms"] = {k: round(v, 4) for k, v in k32.items()}
+        result["config"]["ratio_vs_fp32"] = e32 / elapsed
+        result["config"]["rel_l2_vs_fp32"] = {
+            "forward": rel_l2(outs[args.precision][0], outs["fp32"][0]),
+            "adjoint": rel_l2(outs[args.precision][1], outs["fp32"][1]),
+        }
+        result["roofline_fp32"] = roofline(F, k32, "fp32", args.config)
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         try:

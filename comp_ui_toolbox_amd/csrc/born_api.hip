@@ -63,6 +63,7 @@ static T* dupload(const std::vector<T>& v, hipStream_t) {
 static void free_workspace(Plan& P) {
   dfree(P.d_chunks);
   dfree(P.d_col_pair);
+  dfree(P.d_chunk_pairs);
   dfree(P.d_pr_chunk);
   dfree(P.d_pr_pos);
   dfree(P.d_vpart);
@@ -301,15 +302,26 @@ static void ensure_workspace(Plan& P, int N_rhs) {
       }
     }
   const int n_chunks = (int)P.chunks.size();
+  std::vector<int> chunk_pairs((size_t)n_chunks * NC, -1);
+  for (int c = 0; c < n_chunks; ++c)
+    for (int q = 0; q < P.chunks[c].ncols; ++q) chunk_pairs[(size_t)c * NC + q] = P.col_pair[P.chunks[c].col_start + q];
+  // about 4096 work items per forward launch: (chunk, z-block) for the plain
+  // kernel, (chunk, z-block, 256-point segment) for the split kernel
   const int target = 4096;
   int NZB = std::max(1, std::min(P.Nz, target / std::max(1, n_chunks)));
   P.ZR = (P.Nz + NZB - 1) / NZB;
   P.NZB = (P.Nz + P.ZR - 1) / P.ZR;
+  P.nseg_s = (P.Nx + 255) / 256;
+  NZB = std::max(1, std::min(P.Nz, target / std::max(1, n_chunks * P.nseg_s)));
+  P.ZR_s = (P.Nz + NZB - 1) / NZB;
+  P.NZB_s = (P.Nz + P.ZR_s - 1) / P.ZR_s;
+  P.vblk = std::max(P.NZB, P.NZB_s * P.nseg_s);
   P.d_chunks = dupload(P.chunks, P.stream);
   P.d_col_pair = dupload(P.col_pair, P.stream);
+  P.d_chunk_pairs = dupload(chunk_pairs, P.stream);
   P.d_pr_chunk = dupload(pr_chunk, P.stream);
   P.d_pr_pos = dupload(pr_pos, P.stream);
-  if (P.structured) P.d_vpart = dalloc<float2>((size_t)n_chunks * P.NZB * P.Ne_pad * NC);
+  if (P.structured) P.d_vpart = dalloc<float2>((size_t)n_chunks * P.vblk * P.Ne_pad * NC);
   P.d_V = dalloc<double2>((size_t)P.N_pairs * P.N_e * N_rhs);
   P.d_W = dalloc<float2>((size_t)P.N_pairs * P.N_e * N_rhs);
   P.d_gm = dalloc<float2>((size_t)N_rhs * P.N_pts);
@@ -601,7 +613,7 @@ static void forward_device(Plan& P, const float2* gm, int N_rhs, void* u, bool o
       else
         launch_fwd_split(P, gm, P.precision, 1.0f, 1.0f, s);
     }
-    launch_fwd_reduce(P, N_rhs, s);
+    launch_fwd_reduce(P, N_rhs, P.precision == BORN_PRECISION_FP32 ? P.NZB : P.NZB_s * P.nseg_s, s);
   } else {
     ensure_shift_on_device(P);
     TimedLaunch t(P, s, 0);

@@ -78,11 +78,14 @@ struct Plan {
   std::vector<int> col_pair;
   Chunk* d_chunks = nullptr;
   int* d_col_pair = nullptr;
+  int* d_chunk_pairs = nullptr;   // [n_chunks][NC]: pair of each column, -1 past ncols
   int* d_pr_chunk = nullptr;      // [N_pairs*N_rhs]: chunk of (pair, r)
   int* d_pr_pos = nullptr;        // [N_pairs*N_rhs]: column position in chunk
-  int NZB = 1, ZR = 1;            // forward z-blocking
+  int NZB = 1, ZR = 1;            // forward z-blocking (plain FP32 kernel: whole rows)
+  int NZB_s = 1, ZR_s = 1, nseg_s = 1;  // split forward: z-blocks x 256-point segments
+  int vblk = 1;                   // partial-sum blocks per chunk allocated in vpart
   int Ne_pad = 16;
-  float2* d_vpart = nullptr;      // [n_chunks][NZB][Ne_pad][NC]
+  float2* d_vpart = nullptr;      // [n_chunks][vblk][Ne_pad][NC]
   double2* d_V = nullptr;         // [N_pairs][N_e][N_rhs]
   float2* d_W = nullptr;          // [N_pairs][N_e][N_rhs]
   float2* d_gm = nullptr;         // masked gamma [N_rhs][N_pts]
@@ -109,7 +112,7 @@ void launch_mask_gamma(const float2* g, int N_pts, int N_rhs, float2* gm, hipStr
 
 void launch_fwd_structured(const Plan& P, const float2* gm, int N_rhs, hipStream_t s);
 void launch_fwd_general(const Plan& P, const float2* gm, int N_rhs, hipStream_t s);
-void launch_fwd_reduce(const Plan& P, int N_rhs, hipStream_t s);
+void launch_fwd_reduce(const Plan& P, int N_rhs, int nblk, hipStream_t s);
 void launch_fwd_mix(const Plan& P, int N_rhs, void* u, bool out_f64, hipStream_t s);
 
 void launch_adj_unmix(const Plan& P, const float2* u, int N_rhs, hipStream_t s);

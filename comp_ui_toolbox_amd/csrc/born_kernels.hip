@@ -101,10 +101,10 @@ __global__ void k_fwd_reduce(const float2* __restrict__ vpart, const int* __rest
   V[idx] = make_double2(sx, sy);
 }
 
-void launch_fwd_reduce(const Plan& P, int N_rhs, hipStream_t s) {
+void launch_fwd_reduce(const Plan& P, int N_rhs, int nblk, hipStream_t s) {
   const size_t total = (size_t)P.N_pairs * P.N_e * N_rhs;
   hipLaunchKernelGGL(k_fwd_reduce, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, P.d_vpart,
-                     P.d_pr_chunk, P.d_pr_pos, P.N_pairs, P.N_e, N_rhs, P.NZB, P.Ne_pad, P.d_V);
+                     P.d_pr_chunk, P.d_pr_pos, P.N_pairs, P.N_e, N_rhs, nblk, P.Ne_pad, P.d_V);
 }
 
 // ------------------------------------------------------ general forward

@@ -53,10 +53,26 @@ struct FwdSplitArgs {
   const float2* gm;
   const int* ext;
   const Chunk* chunks;
-  const int* col_pair;
+  const int* chunk_pairs;  // [n_chunks][NC]
   int Nx, Nz, N_pts, N_e, N_ext, NZB, ZR, nseg, n_items, Ne_pad;
-  float2* vpart;
+  float2* vpart;           // [n_chunks][NZB * nseg][Ne_pad][NC]
 };
+
+// Two adjacent complex entries i, i+1 of a row of length n, from clamped
+// (always valid) addresses: one 16-byte load when n is even (i even, row
+// start 16-byte aligned), else two 8-byte loads.  Entries past the end are
+// zeroed by the caller.  No lane-dependent branch: a load into registers
+// that another path also loads into would force a wait on the first.
+__device__ __forceinline__ void load_pair(const float2* row, int i, int n, bool even, float2& a, float2& b) {
+  if (even) {
+    const float4 v = *reinterpret_cast<const float4*>(row + min(i, n - 2));
+    a = make_float2(v.x, v.y);
+    b = make_float2(v.z, v.w);
+  } else {
+    a = row[min(i, n - 1)];
+    b = row[min(i + 1, n - 1)];
+  }
+}
 
 // Work item = (chunk, z-block); a step = (row z, 256-point segment xs).
 // Waves: 2 element groups x 2 point groups; wave (eg, pg) holds element
@@ -89,42 +105,61 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(FwdSplitArgs A) {
   const int n_my = (A.n_items - (int)blockIdx.x + (int)gridDim.x - 1) / (int)gridDim.x;
   if (n_my <= 0) return;
   const int tpad = (16 * MT - A.N_e) * NP;
+  const int NBLK = A.NZB * A.nseg;   // partial-sum blocks per chunk
+  const bool even = (A.Nx & 1) == 0;
+  const int bc0 = (tid >> 7) * 8;
+  const int bi = 2 * (tid & 127);
 
+  // Work item = (chunk, z-block, segment) = ((ci * NZB + zb) * nseg + xs); a
+  // step is one row z of the item.  Everything that only depends on the
+  // item (gather indices of the window, p_inc rows of the columns) is held
+  // in registers, so the loads of a step depend on nothing loaded before.
   int it = 0, item = blockIdx.x;
-  Chunk ch = A.chunks[item / A.NZB];
-  int z = (item % A.NZB) * A.ZR;
-  int z_end = min(A.Nz, z + A.ZR);
-  int xs = 0;
+  int ci, xs, z, z_end, fg, rr;
+  int hidx[APT];
+  int prow[8];
+  bool gv0, gv1;
+#define BORN_SPLIT_ITEM(itm)                                                              \
+  {                                                                                       \
+    ci = (itm) / NBLK;                                                                    \
+    const int rem_ = (itm) - ci * NBLK;                                                   \
+    const int zb_ = rem_ / A.nseg;                                                        \
+    xs = rem_ - zb_ * A.nseg;                                                             \
+    z = zb_ * A.ZR;                                                                       \
+    z_end = min(A.Nz, z + A.ZR);                                                          \
+    const Chunk c_ = A.chunks[ci];                                                        \
+    fg = c_.fg;                                                                           \
+    rr = c_.r;                                                                            \
+    _Pragma("unroll") for (int j = 0; j < APT; ++j) {                                    \
+      const int u_ = tid + 256 * j;                                                       \
+      const int t_ = u_ + xs * SEG - tpad;                                                \
+      hidx[j] = (u_ < LA && t_ >= 0 && t_ < A.N_ext) ? A.ext[t_] : -1;                   \
+    }                                                                                     \
+    _Pragma("unroll") for (int q = 0; q < 8; ++q) prow[q] = A.chunk_pairs[ci * NC + bc0 + q]; \
+    gv0 = xs * SEG + bi < A.Nx;                                                           \
+    gv1 = xs * SEG + bi + 1 < A.Nx;                                                       \
+    /* settle the index loads here, so the per-step prefetch waits on none */             \
+    _Pragma("unroll") for (int j = 0; j < APT; ++j) asm volatile("" ::"v"(hidx[j]));     \
+    _Pragma("unroll") for (int q = 0; q < 8; ++q) asm volatile("" ::"v"(prow[q]));       \
+  }
 
   float2 hreg[APT];
   float2 preg[8][2];
   float2 greg[2];
-  const int bc0 = (tid >> 7) * 8;
-  const int bi = 2 * (tid & 127);
 
-#define BORN_SPLIT_PREFETCH(c, zz, xx)                                                          \
+#define BORN_SPLIT_PREFETCH(zz)                                                                  \
   {                                                                                              \
-    const float2* hrow_ = A.h + (size_t)(c).fg * A.N_pts + (size_t)(zz) * A.Nx;                 \
-    _Pragma("unroll") for (int j = 0; j < APT; ++j) {                                           \
-      const int u_ = tid + 256 * j;                                                              \
-      const int t_ = u_ + (xx) * SEG - tpad;                                                     \
-      hreg[j] = (u_ < LA && t_ >= 0 && t_ < A.N_ext) ? hrow_[A.ext[t_]] : make_float2(0.f, 0.f); \
-    }                                                                                            \
     const size_t rowo_ = (size_t)(zz) * A.Nx;                                                    \
-    const int gi_ = (xx) * SEG + bi;                                                             \
-    const float2* grow_ = A.gm + (size_t)(c).r * A.N_pts + rowo_;                                \
-    greg[0] = gi_ < A.Nx ? grow_[gi_] : make_float2(0.f, 0.f);                                   \
-    greg[1] = gi_ + 1 < A.Nx ? grow_[gi_ + 1] : make_float2(0.f, 0.f);                           \
-    _Pragma("unroll") for (int q = 0; q < 8; ++q) {                                             \
-      const int cc_ = bc0 + q;                                                                   \
-      const float2* prow_ = (cc_ < (c).ncols)                                                    \
-          ? A.P + (size_t)A.col_pair[(c).col_start + cc_] * A.N_pts + rowo_ : nullptr;          \
-      preg[q][0] = (prow_ && gi_ < A.Nx) ? prow_[gi_] : make_float2(0.f, 0.f);                   \
-      preg[q][1] = (prow_ && gi_ + 1 < A.Nx) ? prow_[gi_ + 1] : make_float2(0.f, 0.f);           \
-    }                                                                                            \
+    const float2* hrow_ = A.h + (size_t)fg * A.N_pts + rowo_;                                    \
+    _Pragma("unroll") for (int j = 0; j < APT; ++j) hreg[j] = hrow_[max(hidx[j], 0)];          \
+    const int gi_ = xs * SEG + bi;                                                               \
+    load_pair(A.gm + (size_t)rr * A.N_pts + rowo_, gi_, A.Nx, even, greg[0], greg[1]);           \
+    _Pragma("unroll") for (int q = 0; q < 8; ++q)                                               \
+      load_pair(A.P + (size_t)max(prow[q], 0) * A.N_pts + rowo_, gi_, A.Nx, even, preg[q][0], preg[q][1]); \
   }
 
-  BORN_SPLIT_PREFETCH(ch, z, xs)
+  BORN_SPLIT_ITEM(item)
+  BORN_SPLIT_PREFETCH(z)
 
   v4f CR[MTW], CI[MTW];
 #pragma unroll
@@ -147,9 +182,10 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(FwdSplitArgs A) {
     for (int j = 0; j < APT; ++j) {
       const int u = tid + 256 * j;
       if (u < LA) {
+        const float2 hv = hidx[j] >= 0 ? hreg[j] : make_float2(0.f, 0.f);
         unsigned short v[4];
-        split2(hreg[j].x, v[0], v[1]);
-        split2(hreg[j].y, v[2], v[3]);
+        split2(hv.x, v[0], v[1]);
+        split2(hv.y, v[2], v[3]);
 #pragma unroll
         for (int cp = 0; cp < 4; ++cp) {
           Aw[(cp * 2 + 0) * LAP + u] = v[cp];
@@ -162,7 +198,8 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(FwdSplitArgs A) {
       unsigned int packed[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
       for (int pp = 0; pp < 2; ++pp) {
-        const float2 p = preg[q][pp], g = greg[pp];
+        const float2 p = prow[q] >= 0 ? preg[q][pp] : make_float2(0.f, 0.f);
+        const float2 g = (pp == 0 ? gv0 : gv1) ? greg[pp] : make_float2(0.f, 0.f);
         unsigned short v[4];
         split2(p.x * g.x - p.y * g.y, v[0], v[1]);
         split2(p.x * g.y + p.y * g.x, v[2], v[3]);
@@ -175,24 +212,19 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(FwdSplitArgs A) {
     }
     __syncthreads();
 
-    bool has_next = true, item_done = false;
-    int n_item = item, n_z = z, n_xs = xs + 1;
-    Chunk n_ch = ch;
-    if (n_xs >= A.nseg) {
-      n_xs = 0;
-      n_z = z + 1;
-      if (n_z >= z_end) {
-        item_done = true;
-        if (it + 1 >= n_my) {
-          has_next = false;
-        } else {
-          n_item = blockIdx.x + (it + 1) * gridDim.x;
-          n_ch = A.chunks[n_item / A.NZB];
-          n_z = (n_item % A.NZB) * A.ZR;
-        }
-      }
+    // next step: the next row of this item, else the first row of the next item
+    const int ci_done = ci, xs_done = xs, item_done_id = item;
+    const bool item_done = z + 1 >= z_end;
+    bool has_next = true;
+    if (!item_done) {
+      ++z;
+    } else if (it + 1 >= n_my) {
+      has_next = false;
+    } else {
+      item = blockIdx.x + (it + 1) * gridDim.x;
+      BORN_SPLIT_ITEM(item)
     }
-    if (has_next) BORN_SPLIT_PREFETCH(n_ch, n_z, n_xs)
+    if (has_next) BORN_SPLIT_PREFETCH(z)
 
     // Tile m at K-block kb reads the window at 32*(pg*KBW + kb) - 64*m: the
     // same data as tile 0 at K-block kb - 2m.  Walk the diagonals
@@ -260,8 +292,9 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(FwdSplitArgs A) {
         CI[m] = vz();
       }
       __syncthreads();
-      const int ci = item / A.NZB, zb = item - ci * A.NZB;
-      float* out = reinterpret_cast<float*>(A.vpart + ((size_t)ci * A.NZB + zb) * A.Ne_pad * NC);
+      const int blk = (item_done_id - ci_done * NBLK);  // = zb * nseg + xs
+      (void)xs_done;
+      float* out = reinterpret_cast<float*>(A.vpart + ((size_t)ci_done * NBLK + blk) * A.Ne_pad * NC);
       for (int idx = tid; idx < MT * 512; idx += 256) {
         const int mt = idx >> 9, e_g = mt / MTW, mtw = mt - e_g * MTW, rest = idx & 511;
         float sum = 0.f;
@@ -274,13 +307,9 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(FwdSplitArgs A) {
       ++it;
     }
     if (!has_next) break;
-    item = n_item;
-    ch = n_ch;
-    z = n_z;
-    xs = n_xs;
-    if (item_done) z_end = min(A.Nz, (item % A.NZB) * A.ZR + A.ZR);
   }
 #undef BORN_SPLIT_PREFETCH
+#undef BORN_SPLIT_ITEM
 }
 
 template <int MT>
@@ -559,16 +588,16 @@ void launch_fwd_split(const Plan& P, const float2* gm, int precision, float, flo
   a.gm = gm;
   a.ext = P.d_ext;
   a.chunks = P.d_chunks;
-  a.col_pair = P.d_col_pair;
+  a.chunk_pairs = P.d_chunk_pairs;
   a.Nx = P.Nx;
   a.Nz = P.Nz;
   a.N_pts = P.N_pts;
   a.N_e = P.N_e;
   a.N_ext = P.N_ext;
-  a.NZB = P.NZB;
-  a.ZR = P.ZR;
-  a.nseg = (P.Nx + SEG - 1) / SEG;
-  a.n_items = (int)P.chunks.size() * P.NZB;
+  a.NZB = P.NZB_s;
+  a.ZR = P.ZR_s;
+  a.nseg = P.nseg_s;
+  a.n_items = (int)P.chunks.size() * P.NZB_s * P.nseg_s;
   a.Ne_pad = P.Ne_pad;
   a.vpart = P.d_vpart;
   const int grid = std::min(a.n_items, 2 * P.n_cu);
