@@ -69,6 +69,7 @@ static void free_workspace(Plan& P) {
   dfree(P.d_vpart);
   dfree(P.d_V);
   dfree(P.d_W);
+  dfree(P.d_Wc);
   dfree(P.d_gm);
   P.ws_rhs = -1;
   P.chunks.clear();
@@ -324,6 +325,11 @@ static void ensure_workspace(Plan& P, int N_rhs) {
   if (P.structured) P.d_vpart = dalloc<float2>((size_t)n_chunks * P.vblk * P.Ne_pad * NC);
   P.d_V = dalloc<double2>((size_t)P.N_pairs * P.N_e * N_rhs);
   P.d_W = dalloc<float2>((size_t)P.N_pairs * P.N_e * N_rhs);
+  if (P.structured) {
+    const size_t n = (size_t)n_chunks * P.Ne_pad * NC;
+    P.d_Wc = dalloc<float2>(n);
+    HIP_CHECK(hipMemset(P.d_Wc, 0, n * sizeof(float2)));  // padding columns/elements stay zero
+  }
   P.d_gm = dalloc<float2>((size_t)N_rhs * P.N_pts);
   HIP_CHECK(hipDeviceSynchronize());  // workspace ready before work on any stream
   P.ws_rhs = N_rhs;
@@ -624,7 +630,7 @@ static void forward_device(Plan& P, const float2* gm, int N_rhs, void* u, bool o
 }
 
 static void adjoint_device(Plan& P, const float2* u, int N_rhs, float2* g, hipStream_t s) {
-  launch_adj_unmix(P, u, N_rhs, s);
+  launch_adj_unmix(P, u, N_rhs, use_structured(P), s);
   TimedLaunch t(P, s, 1);
   if (use_structured(P)) {
     if (P.precision == BORN_PRECISION_FP32)

@@ -87,7 +87,8 @@ struct Plan {
   int Ne_pad = 16;
   float2* d_vpart = nullptr;      // [n_chunks][vblk][Ne_pad][NC]
   double2* d_V = nullptr;         // [N_pairs][N_e][N_rhs]
-  float2* d_W = nullptr;          // [N_pairs][N_e][N_rhs]
+  float2* d_W = nullptr;          // [N_pairs][N_e][N_rhs] (general path)
+  float2* d_Wc = nullptr;         // [n_chunks][Ne_pad][NC] (structured path; unused slots stay zero)
   float2* d_gm = nullptr;         // masked gamma [N_rhs][N_pts]
   size_t io_bytes = 0;
   void* d_io_in = nullptr;        // host-path staging
@@ -115,7 +116,7 @@ void launch_fwd_general(const Plan& P, const float2* gm, int N_rhs, hipStream_t 
 void launch_fwd_reduce(const Plan& P, int N_rhs, int nblk, hipStream_t s);
 void launch_fwd_mix(const Plan& P, int N_rhs, void* u, bool out_f64, hipStream_t s);
 
-void launch_adj_unmix(const Plan& P, const float2* u, int N_rhs, hipStream_t s);
+void launch_adj_unmix(const Plan& P, const float2* u, int N_rhs, bool chunked, hipStream_t s);
 void launch_adj_structured(const Plan& P, int N_rhs, float2* g_out, hipStream_t s);
 void launch_adj_general(const Plan& P, int N_rhs, float2* g_out, hipStream_t s);
 

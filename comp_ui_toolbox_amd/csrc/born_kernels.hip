@@ -193,12 +193,15 @@ void launch_fwd_mix(const Plan& P, int N_rhs, void* u, bool out_f64, hipStream_t
 // ------------------------------------------------------------- un-mixing
 // W[pair][e][r] = sum over the (mk, f) reading this pair and the active
 // slot a of element e: conj(pref[mk][f][a]) * u[r][obs(mk, f)].  A gather,
-// so the result is deterministic.
+// so the result is deterministic.  With Wc set, the result goes to the
+// chunk layout Wc[chunk(pair, r)][e][column(pair, r)] of the structured
+// kernels instead.
 __global__ void k_adj_unmix(const float2* __restrict__ u, const int* __restrict__ pair_list_off,
                             const int2* __restrict__ pair_list, const int* __restrict__ slot,
                             const int* __restrict__ mk_meta, const int64_t* __restrict__ mk_off,
                             const double2* __restrict__ pref, int N_pairs, int N_e, int N_rhs, int64_t N_obs,
-                            float2* __restrict__ W) {
+                            float2* __restrict__ W, float2* __restrict__ Wc, const int* __restrict__ pr_chunk,
+                            const int* __restrict__ pr_pos, int Ne_pad) {
   const size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (idx >= (size_t)N_pairs * N_e * N_rhs) return;
   const int r = (int)(idx % N_rhs);
@@ -216,14 +219,20 @@ __global__ void k_adj_unmix(const float2* __restrict__ u, const int* __restrict_
     sx += c.x * v.x + c.y * v.y;
     sy += c.x * v.y - c.y * v.x;
   }
-  W[idx] = make_float2((float)sx, (float)sy);
+  const float2 res = make_float2((float)sx, (float)sy);
+  if (Wc) {
+    const int ch = pr_chunk[(size_t)pair * N_rhs + r], pos = pr_pos[(size_t)pair * N_rhs + r];
+    Wc[((size_t)ch * Ne_pad + e) * NC + pos] = res;
+  } else {
+    W[idx] = res;
+  }
 }
 
-void launch_adj_unmix(const Plan& P, const float2* u, int N_rhs, hipStream_t s) {
+void launch_adj_unmix(const Plan& P, const float2* u, int N_rhs, bool chunked, hipStream_t s) {
   const size_t total = (size_t)P.N_pairs * P.N_e * N_rhs;
   hipLaunchKernelGGL(k_adj_unmix, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, u, P.d_pair_list_off,
                      P.d_pair_list, P.d_slot, P.d_mk_meta, P.d_mk_off, P.d_pref, P.N_pairs, P.N_e, N_rhs,
-                     P.N_obs, P.d_W);
+                     P.N_obs, P.d_W, chunked ? P.d_Wc : nullptr, P.d_pr_chunk, P.d_pr_pos, P.Ne_pad);
 }
 
 // ------------------------------------------------------ general adjoint

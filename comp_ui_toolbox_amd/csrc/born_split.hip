@@ -328,10 +328,10 @@ static size_t fwd_split_lds_bytes() {
 struct AdjSplitArgs {
   const float2* h;
   const float2* P;
-  const float2* W;
+  const float2* Wc;        // [n_chunks][NEP][NC], zero in unused columns and padded elements
   const int* ext;
   const Chunk* chunks;
-  const int* col_pair;
+  const int* chunk_pairs;  // [n_chunks][NC], -1 past ncols
   int n_chunks, Nx, N_pts, N_e, N_ext, N_rhs;
   float2* g_out;
 };
@@ -349,7 +349,9 @@ struct AdjSplitArgs {
 //
 // LDS: Ar[comp][ph][copy][LR] (u16), Wt[comp][c][NEP + 8] (u16), Ps[c][SEG + 4] (float2)
 template <int MT>
-__global__ __launch_bounds__(256, 2) void k_adj_s2d_split(AdjSplitArgs A) {
+__global__ __launch_bounds__(256, 2) void k_adj_s2d_split(AdjSplitArgs A, const Chunk* __restrict__ chunks,
+                                                          const int* __restrict__ chunk_pairs,
+                                                          float2* __restrict__ g_out) {
   extern __shared__ unsigned short ldsu[];
   constexpr int NP = 4;
   constexpr int NEP = 16 * MT;
@@ -381,30 +383,37 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d_split(AdjSplitArgs A) {
     eidx[j] = (u < L && t >= 0 && t < A.N_ext) ? A.ext[t] : -1;
   }
   float2 hreg[HPT], wreg[WPT], preg[PPT];
-#define BORN_ADJS_PREFETCH(c)                                                                    \
+  // Loads of chunk `cidx` from its descriptor (frequency FG, column pairs PR),
+  // all from valid addresses: window entries outside the table are zeroed
+  // when written to LDS; p_inc rows of unused columns are read from row 0 and
+  // meet zero weights; points past the row end are never written out.
+#define BORN_ADJS_PREFETCH(cidx, FG, PR)                                                         \
   {                                                                                              \
-    const float2* hrow_ = A.h + (size_t)(c).fg * A.N_pts + (size_t)z * A.Nx;                     \
-    _Pragma("unroll") for (int j = 0; j < HPT; ++j) hreg[j] =                                    \
-        eidx[j] >= 0 ? hrow_[eidx[j]] : make_float2(0.f, 0.f);                                   \
-    _Pragma("unroll") for (int j = 0; j < WPT; ++j) {                                            \
-      const int idx_ = tid + 256 * j;                                                            \
-      const int wc_ = idx_ / NEP, k_ = idx_ - wc_ * NEP;                                         \
-      wreg[j] = (wc_ < (c).ncols && k_ < A.N_e)                                                  \
-                    ? A.W[((size_t)A.col_pair[(c).col_start + wc_] * A.N_e + k_) * A.N_rhs + (c).r] \
-                    : make_float2(0.f, 0.f);                                                     \
-    }                                                                                            \
-    _Pragma("unroll") for (int j = 0; j < PPT; ++j) {                                            \
-      const int idx_ = tid + 256 * j;                                                            \
-      const int pc_ = idx_ / SEG, pi_ = idx_ - pc_ * SEG;                                        \
-      const int gi_ = i_s + pi_;                                                                 \
-      preg[j] = (pc_ < (c).ncols && gi_ < A.Nx)                                                  \
-                    ? A.P[(size_t)A.col_pair[(c).col_start + pc_] * A.N_pts + (size_t)z * A.Nx + gi_] \
-                    : make_float2(0.f, 0.f);                                                     \
-    }                                                                                            \
+    const float2* hrow_ = A.h + (size_t)(FG) * A.N_pts + (size_t)z * A.Nx;                       \
+    _Pragma("unroll") for (int j = 0; j < HPT; ++j) hreg[j] = hrow_[max(eidx[j], 0)];           \
+    const float2* wblk_ = A.Wc + (size_t)(cidx) * NEP * NC;                                      \
+    _Pragma("unroll") for (int j = 0; j < WPT; ++j) wreg[j] = wblk_[tid + 256 * j];             \
+    const int gi_ = min(i_s + tid, A.Nx - 1);                                                    \
+    _Pragma("unroll") for (int j = 0; j < PPT; ++j)                                             \
+      preg[j] = A.P[(size_t)(PR)[j] * A.N_pts + (size_t)z * A.Nx + gi_];                         \
+  }
+  // descriptor of a chunk: uniform, so it lives in scalar registers
+#define BORN_ADJS_DESC(cidx, FG, R, PR)                                                          \
+  {                                                                                              \
+    const Chunk c_ = chunks[cidx];                                                               \
+    FG = c_.fg;                                                                                  \
+    R = c_.r;                                                                                    \
+    _Pragma("unroll") for (int q = 0; q < NC; ++q) (PR)[q] = max(chunk_pairs[(cidx) * NC + q], 0); \
   }
 
-  Chunk ch = A.chunks[0];
-  BORN_ADJS_PREFETCH(ch)
+  int fg0, r0, pr0[NC];
+  BORN_ADJS_DESC(0, fg0, r0, pr0)
+  BORN_ADJS_PREFETCH(0, fg0, pr0)
+  int nfg = fg0, nr = r0, npr[NC];
+#pragma unroll
+  for (int q = 0; q < NC; ++q) npr[q] = pr0[q];
+  if (A.n_chunks > 1) BORN_ADJS_DESC(1, nfg, nr, npr)
+  int cur_r = r0;
 
   // gamma_hat partials: tile s0 (0..3), v (0..3) -> point s = s0 + 4*(4*kk + v), i = 4 s + ph
   float gx[4][4], gy[4][4];
@@ -426,9 +435,10 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d_split(AdjSplitArgs A) {
       if (u < L) {
         const int p = u & 3, q = u >> 2;
         const int m = (LQ - 1) - q;
+        const float2 hv = eidx[j] >= 0 ? hreg[j] : make_float2(0.f, 0.f);
         unsigned short v[4];
-        split2(hreg[j].x, v[0], v[1]);
-        split2(-hreg[j].y, v[2], v[3]);        // conj
+        split2(hv.x, v[0], v[1]);
+        split2(-hv.y, v[2], v[3]);             // conj
 #pragma unroll
         for (int cp = 0; cp < 4; ++cp) {
 #pragma unroll
@@ -442,7 +452,7 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d_split(AdjSplitArgs A) {
 #pragma unroll
     for (int j = 0; j < WPT; ++j) {
       const int idx = tid + 256 * j;
-      const int wc = idx / NEP, k = idx - wc * NEP;
+      const int k = idx / NC, wc = idx - k * NC;
       unsigned short v[4];
       split2(wreg[j].x, v[0], v[1]);
       split2(wreg[j].y, v[2], v[3]);
@@ -456,13 +466,11 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d_split(AdjSplitArgs A) {
       Ps[pc * PSS + pi] = preg[j];
     }
     __syncthreads();
-    const int r = ch.r;
+    const int r = cur_r;
     const bool has_next = ci + 1 < A.n_chunks;
-    Chunk n_ch = ch;
-    if (has_next) {
-      n_ch = A.chunks[ci + 1];
-      BORN_ADJS_PREFETCH(n_ch)
-    }
+    const int next_r = nr;
+    if (has_next) BORN_ADJS_PREFETCH(ci + 1, nfg, npr)
+    if (ci + 2 < A.n_chunks) BORN_ADJS_DESC(ci + 2, nfg, nr, npr)
 
     // Y for the 4 tiles of this phase: conj(H) W with conj folded into the
     // table (imag part stored negated): Re = Ar Wr - Ai' Wi, Im = Ar Wi + Ai' Wr
@@ -521,7 +529,7 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d_split(AdjSplitArgs A) {
         gy[a][v] += p.x * YI[a][v] - p.y * YR[a][v];
       }
     }
-    if (!has_next || n_ch.r != r) {
+    if (!has_next || next_r != r) {
 #pragma unroll
       for (int a = 0; a < 4; ++a) {
 #pragma unroll
@@ -550,7 +558,7 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d_split(AdjSplitArgs A) {
           }
           const int s = a + 4 * (4 * kk + col);
           const int i = i_s + NP * s + ph;
-          if (i < A.Nx) A.g_out[(size_t)r * A.N_pts + (size_t)z * A.Nx + i] = make_float2(x, y);
+          if (i < A.Nx) g_out[(size_t)r * A.N_pts + (size_t)z * A.Nx + i] = make_float2(x, y);
         }
       }
 #pragma unroll
@@ -558,9 +566,10 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d_split(AdjSplitArgs A) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) gx[a][v] = gy[a][v] = 0.f;
     }
-    ch = n_ch;
+    cur_r = next_r;
   }
 #undef BORN_ADJS_PREFETCH
+#undef BORN_ADJS_DESC
 }
 
 template <int MT>
@@ -612,10 +621,10 @@ void launch_adj_split(const Plan& P, int N_rhs, float2* g_out, hipStream_t s) {
   AdjSplitArgs a;
   a.h = P.d_h;
   a.P = P.d_p;
-  a.W = P.d_W;
+  a.Wc = P.d_Wc;
   a.ext = P.d_ext;
   a.chunks = P.d_chunks;
-  a.col_pair = P.d_col_pair;
+  a.chunk_pairs = P.d_chunk_pairs;
   a.n_chunks = (int)P.chunks.size();
   a.Nx = P.Nx;
   a.N_pts = P.N_pts;
@@ -625,9 +634,9 @@ void launch_adj_split(const Plan& P, int N_rhs, float2* g_out, hipStream_t s) {
   a.g_out = g_out;
   const dim3 grid((P.Nx + SEG - 1) / SEG, P.Nz);
   switch (P.Ne_pad / 16) {
-    case 2: hipLaunchKernelGGL((k_adj_s2d_split<2>), grid, dim3(256), adj_split_lds_bytes<2>(), s, a); break;
-    case 4: hipLaunchKernelGGL((k_adj_s2d_split<4>), grid, dim3(256), adj_split_lds_bytes<4>(), s, a); break;
-    default: hipLaunchKernelGGL((k_adj_s2d_split<8>), grid, dim3(256), adj_split_lds_bytes<8>(), s, a); break;
+    case 2: hipLaunchKernelGGL((k_adj_s2d_split<2>), grid, dim3(256), adj_split_lds_bytes<2>(), s, a, a.chunks, a.chunk_pairs, a.g_out); break;
+    case 4: hipLaunchKernelGGL((k_adj_s2d_split<4>), grid, dim3(256), adj_split_lds_bytes<4>(), s, a, a.chunks, a.chunk_pairs, a.g_out); break;
+    default: hipLaunchKernelGGL((k_adj_s2d_split<8>), grid, dim3(256), adj_split_lds_bytes<8>(), s, a, a.chunks, a.chunk_pairs, a.g_out); break;
   }
 }
 

@@ -41,8 +41,6 @@ struct FwdArgs {
   const float2* P;
   const float2* gm;
   const int* ext;
-  const Chunk* chunks;
-  const int* col_pair;
   int Nx, Nz, N_pts, N_e, Np, N_ext, NZB, ZR, S, MT, n_items, Ne_pad;
   float2* vpart;
 };
@@ -54,7 +52,9 @@ struct FwdArgs {
 // K-steps [pg*SW, (pg+1)*SW) of the row.  NP = points per pitch (4; 0 =
 // runtime), PPT = row points per thread in the B-tile fill.
 template <int MT, int NP, int PPT>
-__global__ __launch_bounds__(256, 2) void k_fwd_s2d(FwdArgs A) {
+__global__ __launch_bounds__(256, 2) void k_fwd_s2d(FwdArgs A, const Chunk* __restrict__ chunks,
+                                                    const int* __restrict__ chunk_pairs,
+                                                    float2* __restrict__ vpart) {
   extern __shared__ float2 lds[];
   const int Np = NP ? NP : A.Np;
   // entries of the polyphase copy per thread: XP <= 256*PPT, QL = (XP-1)/Np + 16*MT
@@ -91,36 +91,43 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(FwdArgs A) {
   // current step
   int it = 0;
   int item = blockIdx.x;
-  Chunk ch = A.chunks[item / A.NZB];
   int z = (item % A.NZB) * A.ZR;
   int z_end = min(A.Nz, z + A.ZR);
+  // descriptor of the item's chunk (uniform: scalar registers); unused
+  // columns read p_inc row 0 and are zeroed in the B tile
+  int fg, rr, ncols, pr[NC];
+#define BORN_FWD_DESC(itm)                                                                \
+  {                                                                                       \
+    const int ci_ = (itm) / A.NZB;                                                        \
+    const Chunk c_ = chunks[ci_];                                                         \
+    fg = c_.fg;                                                                           \
+    rr = c_.r;                                                                            \
+    ncols = c_.ncols;                                                                     \
+    _Pragma("unroll") for (int q = 0; q < NC; ++q) pr[q] = max(chunk_pairs[ci_ * NC + q], 0); \
+  }
+  BORN_FWD_DESC(item)
 
   float2 hreg[HPT];
   float2 praw[PPT][NC];
   float2 greg[PPT];
 
-  // issue the loads of step (c, zz) into registers
-#define BORN_FWD_PREFETCH(c, zz)                                                        \
+  // issue the loads of row zz of the current chunk into registers; every
+  // address is valid (clamped), out-of-range entries are zeroed in the fill
+#define BORN_FWD_PREFETCH(zz)                                                           \
   {                                                                                     \
-    const float2* hrow_ = A.h + (size_t)(c).fg * A.N_pts + (size_t)(zz) * A.Nx;         \
-    _Pragma("unroll") for (int j = 0; j < HPT; ++j) {                                   \
-      hreg[j] = eidx[j] >= 0 ? hrow_[eidx[j]] : make_float2(0.f, 0.f);                  \
-    }                                                                                   \
-    const float2* grow_ = A.gm + (size_t)(c).r * A.N_pts + (size_t)(zz) * A.Nx;         \
+    const float2* hrow_ = A.h + (size_t)fg * A.N_pts + (size_t)(zz) * A.Nx;             \
+    _Pragma("unroll") for (int j = 0; j < HPT; ++j) hreg[j] = hrow_[max(eidx[j], 0)];  \
+    const float2* grow_ = A.gm + (size_t)rr * A.N_pts + (size_t)(zz) * A.Nx;            \
     _Pragma("unroll") for (int j = 0; j < PPT; ++j) {                                   \
-      const int i_ = tid + 256 * j;                                                     \
-      const bool ok_ = i_ < A.Nx;                                                       \
-      greg[j] = ok_ ? grow_[i_] : make_float2(0.f, 0.f);                                \
-      _Pragma("unroll") for (int c_ = 0; c_ < NC; ++c_) {                               \
-        float2 v_ = make_float2(0.f, 0.f);                                              \
-        if (ok_ && c_ < (c).ncols)                                                      \
-          v_ = A.P[(size_t)A.col_pair[(c).col_start + c_] * A.N_pts + (size_t)(zz) * A.Nx + i_]; \
-        praw[j][c_] = v_;                                                               \
-      }                                                                                 \
+      const int i_ = min(tid + 256 * j, A.Nx - 1);                                      \
+      greg[j] = grow_[i_];                                                              \
+      _Pragma("unroll") for (int c_ = 0; c_ < NC; ++c_)                                 \
+        praw[j][c_] = A.P[(size_t)pr[c_] * A.N_pts + (size_t)(zz) * A.Nx + i_];         \
     }                                                                                   \
   }
 
-  BORN_FWD_PREFETCH(ch, z)
+  BORN_FWD_PREFETCH(z)
+  int cur_nc = ncols;   // columns of the data held in registers
 
   v4f SR[MTW], SI[MTW];
 #pragma unroll
@@ -145,32 +152,34 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(FwdArgs A) {
 #pragma unroll
     for (int j = 0; j < HPT; ++j) {
       const int idx = tid + 256 * j;
-      if (idx < HN) Hp[idx] = hreg[j];
+      if (idx < HN) Hp[idx] = eidx[j] >= 0 ? hreg[j] : make_float2(0.f, 0.f);
     }
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
       const int i = tid + 256 * j;
       if (i < XP) {
+        const float2 g = i < A.Nx ? greg[j] : make_float2(0.f, 0.f);
 #pragma unroll
-        for (int c = 0; c < NC; ++c) Bs[i * NCS + c] = cmulf(praw[j][c], greg[j]);
+        for (int c = 0; c < NC; ++c) Bs[i * NCS + c] = c < cur_nc ? cmulf(praw[j][c], g) : make_float2(0.f, 0.f);
       }
     }
     __syncthreads();
 
     // next step and its prefetch
     bool has_next = true;
-    int n_item = item, n_z = z + 1;
-    Chunk n_ch = ch;
-    if (n_z >= z_end) {
-      if (it + 1 >= n_my) {
-        has_next = false;
-      } else {
-        n_item = blockIdx.x + (it + 1) * gridDim.x;
-        n_ch = A.chunks[n_item / A.NZB];
-        n_z = (n_item % A.NZB) * A.ZR;
-      }
+    const int item_cur = item;
+    const bool item_done = z + 1 >= z_end;
+    if (!item_done) {
+      ++z;
+    } else if (it + 1 >= n_my) {
+      has_next = false;
+    } else {
+      item = blockIdx.x + (it + 1) * gridDim.x;
+      BORN_FWD_DESC(item)
+      z = (item % A.NZB) * A.ZR;
+      z_end = min(A.Nz, z + A.ZR);
     }
-    if (has_next) BORN_FWD_PREFETCH(n_ch, n_z)
+    if (has_next) BORN_FWD_PREFETCH(z)
 
     // MFMA over this wave's points of the row
     v4f P1[MTW], P2[MTW], P3[MTW];
@@ -207,7 +216,7 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(FwdArgs A) {
       SI[mt] += P3[mt] - P1[mt] - P2[mt];
     }
 
-    if (z + 1 >= z_end) {
+    if (item_done) {
       // item done: sum the point groups through LDS and write the partial
       __syncthreads();
       float* red = reinterpret_cast<float*>(lds);   // [w][mtw][j][lane]
@@ -222,8 +231,8 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(FwdArgs A) {
         SI[mt] = vzero();
       }
       __syncthreads();
-      const int ci = item / A.NZB, zb = item - ci * A.NZB;
-      float* out = reinterpret_cast<float*>(A.vpart + ((size_t)ci * A.NZB + zb) * A.Ne_pad * NC);
+      const int ci = item_cur / A.NZB, zb = item_cur - ci * A.NZB;
+      float* out = reinterpret_cast<float*>(vpart + ((size_t)ci * A.NZB + zb) * A.Ne_pad * NC);
       for (int idx = tid; idx < MT * 512; idx += 256) {
         const int mt = idx >> 9, e_g = mt / MTW, mtw = mt - e_g * MTW, rest = idx & 511;
         float sum = 0.f;
@@ -236,12 +245,10 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(FwdArgs A) {
       ++it;
     }
     if (!has_next) break;
-    item = n_item;
-    ch = n_ch;
-    z = n_z;
-    z_end = min(A.Nz, (item % A.NZB) * A.ZR + A.ZR);
+    cur_nc = ncols;
   }
 #undef BORN_FWD_PREFETCH
+#undef BORN_FWD_DESC
 }
 
 static size_t fwd_lds_bytes(int MT, int S, int Np) {
@@ -267,8 +274,6 @@ void launch_fwd_structured(const Plan& P, const float2* gm, int N_rhs, hipStream
   a.P = P.d_p;
   a.gm = gm;
   a.ext = P.d_ext;
-  a.chunks = P.d_chunks;
-  a.col_pair = P.d_col_pair;
   a.Nx = P.Nx;
   a.Nz = P.Nz;
   a.N_pts = P.N_pts;
@@ -287,7 +292,8 @@ void launch_fwd_structured(const Plan& P, const float2* gm, int N_rhs, hipStream
   const size_t lds = fwd_lds_bytes(a.MT, a.S, a.Np);
   const int grid = std::min(a.n_items, 2 * P.n_cu);   // two resident workgroups per CU
 #define BORN_FWD_LAUNCH(MTv, NPv, PPTv) \
-  hipLaunchKernelGGL((k_fwd_s2d<MTv, NPv, PPTv>), dim3(grid), dim3(256), lds, s, a)
+  hipLaunchKernelGGL((k_fwd_s2d<MTv, NPv, PPTv>), dim3(grid), dim3(256), lds, s, a, P.d_chunks, P.d_chunk_pairs, \
+                     P.d_vpart)
 #define BORN_FWD_PPT(MTv, NPv)                    \
   if (PPT <= 1) BORN_FWD_LAUNCH(MTv, NPv, 1);     \
   else if (PPT <= 2) BORN_FWD_LAUNCH(MTv, NPv, 2); \
@@ -313,12 +319,9 @@ void launch_fwd_structured(const Plan& P, const float2* gm, int N_rhs, hipStream
 struct AdjArgs {
   const float2* h;
   const float2* P;
-  const float2* W;
+  const float2* Wc;   // [n_chunks][Ne_pad][NC], zero in unused columns and padded elements
   const int* ext;
-  const Chunk* chunks;
-  const int* col_pair;
   int n_chunks, Nx, N_pts, N_e, Ne_pad, Np, N_ext, N_rhs;
-  float2* g_out;
 };
 
 // One workgroup = (lateral segment of 64*MW points, depth row z).  It walks
@@ -327,7 +330,9 @@ struct AdjArgs {
 //   Y[i][c] = sum_e conj(Hext[i - e*Np + off]) W[e][c]       (MFMA, A = points x elements)
 //   gamma_hat[i] += sum_c conj(p_inc[pair_c][i]) Y[i][c]     (registers)
 template <int MT, int MW, int NP>
-__global__ __launch_bounds__(256, 2) void k_adj_s2d(AdjArgs A) {
+__global__ __launch_bounds__(256, 2) void k_adj_s2d(AdjArgs A, const Chunk* __restrict__ chunks,
+                                                    const int* __restrict__ chunk_pairs,
+                                                    float2* __restrict__ g_out) {
   extern __shared__ float2 lds[];
   constexpr int SEG = 64 * MW;
   constexpr int PS = SEG + 4;                 // padded stride of the p_inc tile
@@ -358,31 +363,39 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d(AdjArgs A) {
   }
 
   float2 hreg[HPT], wreg[WPT], preg[PPT];
-#define BORN_ADJ_PREFETCH(c)                                                                     \
+  // Loads of chunk `cidx` (frequency FG, p_inc rows PV of this thread's
+  // entries), all from valid addresses: window entries outside the table are
+  // zeroed when written to LDS, unused columns meet zero weights, points past
+  // the row end are never written out.
+#define BORN_ADJ_PREFETCH(cidx, FG, PV)                                                          \
   {                                                                                              \
-    const float2* hrow_ = A.h + (size_t)(c).fg * A.N_pts + (size_t)z * A.Nx;                     \
-    _Pragma("unroll") for (int j = 0; j < HPT; ++j) hreg[j] =                                    \
-        eidx[j] >= 0 ? hrow_[eidx[j]] : make_float2(0.f, 0.f);                                   \
-    const int wc_ = tid & 15;                                                                    \
-    const bool wok_ = wc_ < (c).ncols;                                                           \
-    const int wpair_ = wok_ ? A.col_pair[(c).col_start + wc_] : 0;                               \
-    _Pragma("unroll") for (int j = 0; j < WPT; ++j) {                                            \
-      const int k_ = (tid >> 4) + 16 * j;                                                        \
-      wreg[j] = (wok_ && k_ < A.N_e)                                                             \
-                    ? A.W[((size_t)wpair_ * A.N_e + k_) * A.N_rhs + (c).r] : make_float2(0.f, 0.f); \
-    }                                                                                            \
-    _Pragma("unroll") for (int j = 0; j < PPT; ++j) {                                            \
+    const float2* hrow_ = A.h + (size_t)(FG) * A.N_pts + (size_t)z * A.Nx;                       \
+    _Pragma("unroll") for (int j = 0; j < HPT; ++j) hreg[j] = hrow_[max(eidx[j], 0)];           \
+    const float2* wblk_ = A.Wc + (size_t)(cidx) * NEP * NC;                                      \
+    _Pragma("unroll") for (int j = 0; j < WPT; ++j) wreg[j] = wblk_[tid + 256 * j];             \
+    _Pragma("unroll") for (int j = 0; j < PPT; ++j) {                                           \
       const int idx_ = tid + 256 * j;                                                            \
-      const int pc_ = idx_ / SEG, pi_ = idx_ - pc_ * SEG;                                        \
-      const int gi_ = i_s + pi_;                                                                 \
-      preg[j] = (pc_ < (c).ncols && gi_ < A.Nx)                                                  \
-                    ? A.P[(size_t)A.col_pair[(c).col_start + pc_] * A.N_pts + (size_t)z * A.Nx + gi_] \
-                    : make_float2(0.f, 0.f);                                                     \
+      const int gi_ = min(i_s + idx_ % SEG, A.Nx - 1);                                           \
+      preg[j] = A.P[(size_t)(PV)[j] * A.N_pts + (size_t)z * A.Nx + gi_];                         \
     }                                                                                            \
   }
+#define BORN_ADJ_DESC(cidx, FG, R, PV)                                                           \
+  {                                                                                              \
+    const Chunk c_ = chunks[cidx];                                                               \
+    FG = c_.fg;                                                                                  \
+    R = c_.r;                                                                                    \
+    _Pragma("unroll") for (int j = 0; j < PPT; ++j)                                             \
+      (PV)[j] = max(chunk_pairs[(cidx) * NC + (tid + 256 * j) / SEG], 0);                        \
+  }
 
-  Chunk ch = A.chunks[0];
-  BORN_ADJ_PREFETCH(ch)
+  int fg0, r0, pv0[PPT];
+  BORN_ADJ_DESC(0, fg0, r0, pv0)
+  BORN_ADJ_PREFETCH(0, fg0, pv0)
+  int nfg = fg0, nr = r0, npv[PPT];
+#pragma unroll
+  for (int j = 0; j < PPT; ++j) npv[j] = pv0[j];
+  if (A.n_chunks > 1) BORN_ADJ_DESC(1, nfg, nr, npv)
+  int cur_r = r0;
 
   float gx[MW][4], gy[MW][4];
 #pragma unroll
@@ -397,7 +410,7 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d(AdjArgs A) {
 #pragma unroll
     for (int j = 0; j < HPT; ++j) {
       const int u = tid + 256 * j;
-      if (u < L) He[u] = hreg[j];
+      if (u < L) He[u] = eidx[j] >= 0 ? hreg[j] : make_float2(0.f, 0.f);
     }
 #pragma unroll
     for (int j = 0; j < WPT; ++j) Ws[((tid >> 4) + 16 * j) * NCS + (tid & 15)] = wreg[j];
@@ -408,13 +421,11 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d(AdjArgs A) {
       Ps[pc * PS + pi] = preg[j];
     }
     __syncthreads();
-    const int r = ch.r;
+    const int r = cur_r;
     const bool has_next = ci + 1 < A.n_chunks;
-    Chunk n_ch = ch;
-    if (has_next) {
-      n_ch = A.chunks[ci + 1];
-      BORN_ADJ_PREFETCH(n_ch)
-    }
+    const int next_r = nr;
+    if (has_next) BORN_ADJ_PREFETCH(ci + 1, nfg, npv)
+    if (ci + 2 < A.n_chunks) BORN_ADJ_DESC(ci + 2, nfg, nr, npv)
 
     v4f P1[MW], P2[MW], P3[MW];
 #pragma unroll
@@ -450,7 +461,7 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d(AdjArgs A) {
         gy[mw][v] += p.x * yi - p.y * yr;
       }
     }
-    if (!has_next || n_ch.r != r) {
+    if (!has_next || next_r != r) {
       // sum over the 16 column lanes and write gamma_hat[:, r]
 #pragma unroll
       for (int mw = 0; mw < MW; ++mw) {
@@ -478,7 +489,7 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d(AdjArgs A) {
               y = gy[mw][v];
             }
           }
-          if (i < A.Nx) A.g_out[(size_t)r * A.N_pts + (size_t)z * A.Nx + i] = make_float2(x, y);
+          if (i < A.Nx) g_out[(size_t)r * A.N_pts + (size_t)z * A.Nx + i] = make_float2(x, y);
         }
       }
 #pragma unroll
@@ -486,9 +497,10 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d(AdjArgs A) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) gx[mw][v] = gy[mw][v] = 0.f;
     }
-    ch = n_ch;
+    cur_r = next_r;
   }
 #undef BORN_ADJ_PREFETCH
+#undef BORN_ADJ_DESC
 }
 
 static int adj_mw(int Nx) { return Nx <= 128 ? 2 : 4; }
@@ -509,10 +521,8 @@ void launch_adj_structured(const Plan& P, int N_rhs, float2* g_out, hipStream_t 
   AdjArgs a;
   a.h = P.d_h;
   a.P = P.d_p;
-  a.W = P.d_W;
+  a.Wc = P.d_Wc;
   a.ext = P.d_ext;
-  a.chunks = P.d_chunks;
-  a.col_pair = P.d_col_pair;
   a.n_chunks = (int)P.chunks.size();
   a.Nx = P.Nx;
   a.N_pts = P.N_pts;
@@ -521,12 +531,12 @@ void launch_adj_structured(const Plan& P, int N_rhs, float2* g_out, hipStream_t 
   a.Np = P.Np;
   a.N_ext = P.N_ext;
   a.N_rhs = N_rhs;
-  a.g_out = g_out;
   const int MW = adj_mw(P.Nx);
   const int SEG = 64 * MW;
   const size_t lds = adj_lds_bytes(MW, P.Ne_pad, P.Np);
   const dim3 grid((P.Nx + SEG - 1) / SEG, P.Nz);
-#define BORN_ADJ_LAUNCH(MTv, MWv, NPv) hipLaunchKernelGGL((k_adj_s2d<MTv, MWv, NPv>), grid, dim3(256), lds, s, a)
+#define BORN_ADJ_LAUNCH(MTv, MWv, NPv) \
+  hipLaunchKernelGGL((k_adj_s2d<MTv, MWv, NPv>), grid, dim3(256), lds, s, a, P.d_chunks, P.d_chunk_pairs, g_out)
 #define BORN_ADJ_MW(MTv, NPv)                     \
   if (MW == 2) BORN_ADJ_LAUNCH(MTv, 2, NPv);      \
   else BORN_ADJ_LAUNCH(MTv, 4, NPv);
