@@ -60,6 +60,14 @@ static T* dupload(const std::vector<T>& v, hipStream_t) {
   return p;
 }
 
+// p_inc rows of all pairs plus one zero row (index N_pairs) that unused
+// columns of the structured kernels read.
+static float2* alloc_p_table(const Plan& P) {
+  float2* d = dalloc<float2>((size_t)(P.N_pairs + 1) * P.N_pts);
+  HIP_CHECK(hipMemset(d + (size_t)P.N_pairs * P.N_pts, 0, (size_t)P.N_pts * sizeof(float2)));
+  return d;
+}
+
 static void free_workspace(Plan& P) {
   dfree(P.d_chunks);
   dfree(P.d_col_pair);
@@ -303,7 +311,7 @@ static void ensure_workspace(Plan& P, int N_rhs) {
       }
     }
   const int n_chunks = (int)P.chunks.size();
-  std::vector<int> chunk_pairs((size_t)n_chunks * NC, -1);
+  std::vector<int> chunk_pairs((size_t)n_chunks * NC, P.N_pairs);  // unused: the zero row
   for (int c = 0; c < n_chunks; ++c)
     for (int q = 0; q < P.chunks[c].ncols; ++q) chunk_pairs[(size_t)c * NC + q] = P.col_pair[P.chunks[c].col_start + q];
   // about 4096 work items per forward launch: (chunk, z-block) for the plain
@@ -420,7 +428,7 @@ static Plan* create_tables(const born_tables_desc* d) {
     HIP_CHECK(hipMemcpyAsync(stage, d->h_ref, sizeof(double2) * (size_t)P->N_fu * P->N_pts, hipMemcpyHostToDevice,
                              P->stream));
     launch_convert_transpose(stage, P->N_fu, P->N_pts, P->d_h, P->stream);
-    P->d_p = dalloc<float2>((size_t)P->N_pairs * P->N_pts);
+    P->d_p = alloc_p_table(*P);
     for (int m = 0; m < d->N_measurements; ++m) {
       const int nf = d->measurements[m].N_f_unique;
       HIP_CHECK(hipMemcpyAsync(stage, d->measurements[m].p_incident, sizeof(double2) * (size_t)nf * P->N_pts,
@@ -574,7 +582,7 @@ static Plan* create_geometry(const born_geometry_desc* d) {
     d_vd = dupload(vd, s);
     d_vd_off = dupload(vd_off, s);
     ensure_shift_on_device(*P);
-    P->d_p = dalloc<float2>((size_t)P->N_pairs * P->N_pts);
+    P->d_p = alloc_p_table(*P);
     launch_synth_pinc(*P, htx, d_meas_Nfm, d_tx_off, d_tx_el, d_vd, d_vd_off, s);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipStreamSynchronize(s));

@@ -53,7 +53,7 @@ struct Plan {
 
   // device copies
   float2* d_h = nullptr;          // [N_fu][N_pts]
-  float2* d_p = nullptr;          // [N_pairs][N_pts]
+  float2* d_p = nullptr;          // [N_pairs + 1][N_pts], last row zero
   int* d_shift = nullptr;         // [N_e][N_pts], 0-based (general path)
   int* d_ext = nullptr;           // [N_ext] lateral map (structured path)
   int* d_pair_fg = nullptr;
@@ -78,7 +78,7 @@ struct Plan {
   std::vector<int> col_pair;
   Chunk* d_chunks = nullptr;
   int* d_col_pair = nullptr;
-  int* d_chunk_pairs = nullptr;   // [n_chunks][NC]: pair of each column, -1 past ncols
+  int* d_chunk_pairs = nullptr;   // [n_chunks][NC]: pair of each column, N_pairs (zero row) past ncols
   int* d_pr_chunk = nullptr;      // [N_pairs*N_rhs]: chunk of (pair, r)
   int* d_pr_pos = nullptr;        // [N_pairs*N_rhs]: column position in chunk
   int NZB = 1, ZR = 1;            // forward z-blocking (plain FP32 kernel: whole rows)

@@ -49,43 +49,51 @@ __device__ __forceinline__ v4f vz() { return v4f{0.f, 0.f, 0.f, 0.f}; }
 // ======================================================================
 struct FwdSplitArgs {
   const float2* h;
-  const float2* P;
+  const float2* P;         // N_pairs + 1 rows; the last row is zero
   const float2* gm;
   const int* ext;
-  const Chunk* chunks;
-  const int* chunk_pairs;  // [n_chunks][NC]
   int Nx, Nz, N_pts, N_e, N_ext, NZB, ZR, nseg, n_items, Ne_pad;
-  float2* vpart;           // [n_chunks][NZB * nseg][Ne_pad][NC]
 };
 
-// Two adjacent complex entries i, i+1 of a row of length n, from clamped
-// (always valid) addresses: one 16-byte load when n is even (i even, row
-// start 16-byte aligned), else two 8-byte loads.  Entries past the end are
-// zeroed by the caller.  No lane-dependent branch: a load into registers
-// that another path also loads into would force a wait on the first.
-__device__ __forceinline__ void load_pair(const float2* row, int i, int n, bool even, float2& a, float2& b) {
-  if (even) {
-    const float4 v = *reinterpret_cast<const float4*>(row + min(i, n - 2));
-    a = make_float2(v.x, v.y);
-    b = make_float2(v.z, v.w);
-  } else {
-    a = row[min(i, n - 1)];
-    b = row[min(i + 1, n - 1)];
-  }
+// Round two floats to BF16 (nearest even) in one conversion; returns the
+// packed pair (x0 in the low half) and the rounded values as floats.
+__device__ __forceinline__ unsigned int bf_pack(float x0, float x1, float& r0, float& r1) {
+  typedef float v2f __attribute__((ext_vector_type(2)));
+  typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+  const unsigned int p = __builtin_bit_cast(unsigned int, __builtin_convertvector((v2f{x0, x1}), v2bf));
+  r0 = __uint_as_float(p << 16);
+  r1 = __uint_as_float(p & 0xffff0000u);
+  return p;
+}
+// hi/lo split of a pair: hi = bf16(x), lo = bf16(x - hi), each packed (x0 low)
+__device__ __forceinline__ void split_pair(float x0, float x1, unsigned int& hi, unsigned int& lo) {
+  float h0, h1, d0, d1;
+  hi = bf_pack(x0, x1, h0, h1);
+  lo = bf_pack(x0 - h0, x1 - h1, d0, d1);
+}
+__device__ __forceinline__ const void* byte_offset(const void* base, unsigned int off) {
+  return static_cast<const char*>(base) + off;
 }
 
-// Work item = (chunk, z-block); a step = (row z, 256-point segment xs).
-// Waves: 2 element groups x 2 point groups; wave (eg, pg) holds element
-// tiles [eg*MTW, (eg+1)*MTW) and K-blocks pg*4 .. pg*4+3 of the segment.
-// Points per pitch Np = 4.
+// Work item = (chunk, z-block, 256-point segment xs), taken persistently;
+// a step is one row z of the item.  Waves: 2 element groups x 2 point
+// groups; wave (eg, pg) holds element tiles [eg*MTW, (eg+1)*MTW) and the
+// K-blocks pg*KBW .. pg*KBW + KBW-1 of the segment.  Points per pitch 4.
+//
+// Loads of a step depend on nothing loaded in the step before: per item the
+// window gather offsets (registers) and the column rows (scalar registers,
+// one p_inc row per column; unused columns read the zero row) are fixed, and
+// per step only the row base moves.
 //
 // LDS (u16 units):
-//   Aw[comp][copy][LAP]  extended-row window of the segment (comp = re_hi,
-//                        re_lo, im_hi, im_lo); copy 1 is copy 0 shifted by 4
-//                        entries so every lane's 8-entry read is 16-byte aligned
+//   Aw[comp][copy][LAP]  window of the extended row (comp = re_hi, re_lo,
+//                        im_hi, im_lo); copy 1 is copy 0 shifted by 4 entries
+//                        so every lane's 8-entry read is 16-byte aligned
 //   Bt[comp][c][SEGP]    p_inc*gamma of the segment, points contiguous
-template <int MT>
-__global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(FwdSplitArgs A) {
+template <int MT, bool EVEN>
+__global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(FwdSplitArgs A, const Chunk* __restrict__ chunks,
+                                                          const int* __restrict__ chunk_pairs,
+                                                          float2* __restrict__ vpart) {
   extern __shared__ unsigned short ldsu[];
   constexpr int NP = 4;
   constexpr int SEGP = SEG + 8;
@@ -98,6 +106,7 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(FwdSplitArgs A) {
   constexpr int APT = (LA + 255) / 256;
   unsigned short* Aw = ldsu;                 // [4][2][LAP]
   unsigned short* Bt = ldsu + 4 * 2 * LAP;   // [4][NC][SEGP]
+  unsigned int* Bt32 = reinterpret_cast<unsigned int*>(Bt);
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int eg = w % EW, pg = w / EW;
@@ -106,19 +115,16 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(FwdSplitArgs A) {
   if (n_my <= 0) return;
   const int tpad = (16 * MT - A.N_e) * NP;
   const int NBLK = A.NZB * A.nseg;   // partial-sum blocks per chunk
-  const bool even = (A.Nx & 1) == 0;
-  const int bc0 = (tid >> 7) * 8;
+  const int bc0 = (tid >> 7) * 8;    // columns of this thread's B loads: uniform per wave
   const int bi = 2 * (tid & 127);
 
-  // Work item = (chunk, z-block, segment) = ((ci * NZB + zb) * nseg + xs); a
-  // step is one row z of the item.  Everything that only depends on the
-  // item (gather indices of the window, p_inc rows of the columns) is held
-  // in registers, so the loads of a step depend on nothing loaded before.
   int it = 0, item = blockIdx.x;
   int ci, xs, z, z_end, fg, rr;
-  int hidx[APT];
-  int prow[8];
-  bool gv0, gv1;
+  bool hfull, gfull;                 // window inside the table / segment inside the row
+  unsigned int hoff[APT];            // window gather byte offsets within a row
+  unsigned int hval = 0;             // bit j: window entry j inside the table
+  unsigned int goff0, goff1;         // byte offsets of points bi, bi+1 (clamped)
+  int prow[8];                       // p_inc rows of the columns bc0 .. bc0+7
 #define BORN_SPLIT_ITEM(itm)                                                              \
   {                                                                                       \
     ci = (itm) / NBLK;                                                                    \
@@ -127,20 +133,27 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(FwdSplitArgs A) {
     xs = rem_ - zb_ * A.nseg;                                                             \
     z = zb_ * A.ZR;                                                                       \
     z_end = min(A.Nz, z + A.ZR);                                                          \
-    const Chunk c_ = A.chunks[ci];                                                        \
+    const Chunk c_ = chunks[ci];                                                          \
     fg = c_.fg;                                                                           \
     rr = c_.r;                                                                            \
+    const int t0_ = xs * SEG - tpad;                                                      \
+    hfull = t0_ >= 0 && t0_ + LA <= A.N_ext;                                              \
+    gfull = xs * SEG + SEG <= A.Nx;                                                       \
+    hval = 0;                                                                             \
     _Pragma("unroll") for (int j = 0; j < APT; ++j) {                                    \
       const int u_ = tid + 256 * j;                                                       \
-      const int t_ = u_ + xs * SEG - tpad;                                                \
-      hidx[j] = (u_ < LA && t_ >= 0 && t_ < A.N_ext) ? A.ext[t_] : -1;                   \
+      const int t_ = u_ + t0_;                                                            \
+      const bool ok_ = u_ < LA && t_ >= 0 && t_ < A.N_ext;                                \
+      hoff[j] = ok_ ? 8u * (unsigned int)A.ext[t_] : 0u;                                  \
+      hval |= ok_ ? (1u << j) : 0u;                                                       \
     }                                                                                     \
-    _Pragma("unroll") for (int q = 0; q < 8; ++q) prow[q] = A.chunk_pairs[ci * NC + bc0 + q]; \
-    gv0 = xs * SEG + bi < A.Nx;                                                           \
-    gv1 = xs * SEG + bi + 1 < A.Nx;                                                       \
-    /* settle the index loads here, so the per-step prefetch waits on none */             \
-    _Pragma("unroll") for (int j = 0; j < APT; ++j) asm volatile("" ::"v"(hidx[j]));     \
-    _Pragma("unroll") for (int q = 0; q < 8; ++q) asm volatile("" ::"v"(prow[q]));       \
+    const int gi_ = xs * SEG + bi;                                                        \
+    goff0 = 8u * (unsigned int)(EVEN ? min(gi_, A.Nx - 2) : min(gi_, A.Nx - 1));          \
+    goff1 = 8u * (unsigned int)min(gi_ + 1, A.Nx - 1);                                    \
+    _Pragma("unroll") for (int q = 0; q < 8; ++q)                                        \
+      prow[q] = __builtin_amdgcn_readfirstlane(chunk_pairs[ci * NC + bc0 + q]);           \
+    /* settle the per-item loads here, so the per-step prefetch waits on none */          \
+    _Pragma("unroll") for (int j = 0; j < APT; ++j) asm volatile("" ::"v"(hoff[j]));     \
   }
 
   float2 hreg[APT];
@@ -151,11 +164,28 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(FwdSplitArgs A) {
   {                                                                                              \
     const size_t rowo_ = (size_t)(zz) * A.Nx;                                                    \
     const float2* hrow_ = A.h + (size_t)fg * A.N_pts + rowo_;                                    \
-    _Pragma("unroll") for (int j = 0; j < APT; ++j) hreg[j] = hrow_[max(hidx[j], 0)];          \
-    const int gi_ = xs * SEG + bi;                                                               \
-    load_pair(A.gm + (size_t)rr * A.N_pts + rowo_, gi_, A.Nx, even, greg[0], greg[1]);           \
-    _Pragma("unroll") for (int q = 0; q < 8; ++q)                                               \
-      load_pair(A.P + (size_t)max(prow[q], 0) * A.N_pts + rowo_, gi_, A.Nx, even, preg[q][0], preg[q][1]); \
+    _Pragma("unroll") for (int j = 0; j < APT; ++j)                                             \
+      hreg[j] = *static_cast<const float2*>(byte_offset(hrow_, hoff[j]));                        \
+    const float2* grow_ = A.gm + (size_t)rr * A.N_pts + rowo_;                                   \
+    if (EVEN) {                                                                                  \
+      const float4 v_ = *static_cast<const float4*>(byte_offset(grow_, goff0));                  \
+      greg[0] = make_float2(v_.x, v_.y);                                                         \
+      greg[1] = make_float2(v_.z, v_.w);                                                         \
+    } else {                                                                                     \
+      greg[0] = *static_cast<const float2*>(byte_offset(grow_, goff0));                          \
+      greg[1] = *static_cast<const float2*>(byte_offset(grow_, goff1));                          \
+    }                                                                                            \
+    _Pragma("unroll") for (int q = 0; q < 8; ++q) {                                             \
+      const float2* prow_ = A.P + (size_t)prow[q] * A.N_pts + rowo_;                             \
+      if (EVEN) {                                                                                \
+        const float4 v_ = *static_cast<const float4*>(byte_offset(prow_, goff0));                \
+        preg[q][0] = make_float2(v_.x, v_.y);                                                    \
+        preg[q][1] = make_float2(v_.z, v_.w);                                                    \
+      } else {                                                                                   \
+        preg[q][0] = *static_cast<const float2*>(byte_offset(prow_, goff0));                     \
+        preg[q][1] = *static_cast<const float2*>(byte_offset(prow_, goff1));                     \
+      }                                                                                          \
+    }                                                                                            \
   }
 
   BORN_SPLIT_ITEM(item)
@@ -178,14 +208,17 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(FwdSplitArgs A) {
 
   while (true) {
     __syncthreads();
+    // window -> hi/lo planes, two copies
 #pragma unroll
     for (int j = 0; j < APT; ++j) {
       const int u = tid + 256 * j;
       if (u < LA) {
-        const float2 hv = hidx[j] >= 0 ? hreg[j] : make_float2(0.f, 0.f);
-        unsigned short v[4];
-        split2(hv.x, v[0], v[1]);
-        split2(hv.y, v[2], v[3]);
+        float2 hv = hreg[j];
+        if (!hfull && !((hval >> j) & 1u)) hv = make_float2(0.f, 0.f);
+        unsigned int hi, lo;   // (re, im) packed
+        split_pair(hv.x, hv.y, hi, lo);
+        const unsigned short v[4] = {(unsigned short)hi, (unsigned short)lo, (unsigned short)(hi >> 16),
+                                     (unsigned short)(lo >> 16)};
 #pragma unroll
         for (int cp = 0; cp < 4; ++cp) {
           Aw[(cp * 2 + 0) * LAP + u] = v[cp];
@@ -193,27 +226,32 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(FwdSplitArgs A) {
         }
       }
     }
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      unsigned int packed[4] = {0u, 0u, 0u, 0u};
-#pragma unroll
-      for (int pp = 0; pp < 2; ++pp) {
-        const float2 p = prow[q] >= 0 ? preg[q][pp] : make_float2(0.f, 0.f);
-        const float2 g = (pp == 0 ? gv0 : gv1) ? greg[pp] : make_float2(0.f, 0.f);
-        unsigned short v[4];
-        split2(p.x * g.x - p.y * g.y, v[0], v[1]);
-        split2(p.x * g.y + p.y * g.x, v[2], v[3]);
-#pragma unroll
-        for (int cp = 0; cp < 4; ++cp) packed[cp] |= ((unsigned int)v[cp]) << (16 * pp);
+    // B = p_inc * gamma -> hi/lo planes; a thread writes points bi, bi+1 of 8 columns
+    {
+      float2 g0 = greg[0], g1 = greg[1];
+      if (!gfull) {
+        if (xs * SEG + bi >= A.Nx) g0 = make_float2(0.f, 0.f);
+        if (xs * SEG + bi + 1 >= A.Nx) g1 = make_float2(0.f, 0.f);
       }
 #pragma unroll
-      for (int cp = 0; cp < 4; ++cp)
-        *reinterpret_cast<unsigned int*>(&Bt[(cp * NC + bc0 + q) * SEGP + bi]) = packed[cp];
+      for (int q = 0; q < 8; ++q) {
+        const float2 p0 = preg[q][0], p1 = preg[q][1];
+        const float re0 = p0.x * g0.x - p0.y * g0.y, im0 = p0.x * g0.y + p0.y * g0.x;
+        const float re1 = p1.x * g1.x - p1.y * g1.y, im1 = p1.x * g1.y + p1.y * g1.x;
+        unsigned int rh, rl, ih, il;
+        split_pair(re0, re1, rh, rl);
+        split_pair(im0, im1, ih, il);
+        const int o = ((bc0 + q) * SEGP + bi) >> 1;
+        Bt32[0 * (NC * SEGP / 2) + o] = rh;
+        Bt32[1 * (NC * SEGP / 2) + o] = rl;
+        Bt32[2 * (NC * SEGP / 2) + o] = ih;
+        Bt32[3 * (NC * SEGP / 2) + o] = il;
+      }
     }
     __syncthreads();
 
     // next step: the next row of this item, else the first row of the next item
-    const int ci_done = ci, xs_done = xs, item_done_id = item;
+    const int ci_done = ci, item_done_id = item;
     const bool item_done = z + 1 >= z_end;
     bool has_next = true;
     if (!item_done) {
@@ -292,9 +330,8 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(FwdSplitArgs A) {
         CI[m] = vz();
       }
       __syncthreads();
-      const int blk = (item_done_id - ci_done * NBLK);  // = zb * nseg + xs
-      (void)xs_done;
-      float* out = reinterpret_cast<float*>(A.vpart + ((size_t)ci_done * NBLK + blk) * A.Ne_pad * NC);
+      const int blk = item_done_id - ci_done * NBLK;  // = zb * nseg + xs
+      float* out = reinterpret_cast<float*>(vpart + ((size_t)ci_done * NBLK + blk) * A.Ne_pad * NC);
       for (int idx = tid; idx < MT * 512; idx += 256) {
         const int mt = idx >> 9, e_g = mt / MTW, mtw = mt - e_g * MTW, rest = idx & 511;
         float sum = 0.f;
@@ -331,7 +368,7 @@ struct AdjSplitArgs {
   const float2* Wc;        // [n_chunks][NEP][NC], zero in unused columns and padded elements
   const int* ext;
   const Chunk* chunks;
-  const int* chunk_pairs;  // [n_chunks][NC], -1 past ncols
+  const int* chunk_pairs;  // [n_chunks][NC], zero row past ncols
   int n_chunks, Nx, N_pts, N_e, N_ext, N_rhs;
   float2* g_out;
 };
@@ -403,7 +440,7 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d_split(AdjSplitArgs A, const 
     const Chunk c_ = chunks[cidx];                                                               \
     FG = c_.fg;                                                                                  \
     R = c_.r;                                                                                    \
-    _Pragma("unroll") for (int q = 0; q < NC; ++q) (PR)[q] = max(chunk_pairs[(cidx) * NC + q], 0); \
+    _Pragma("unroll") for (int q = 0; q < NC; ++q) (PR)[q] = chunk_pairs[(cidx) * NC + q]; \
   }
 
   int fg0, r0, pr0[NC];
@@ -596,8 +633,6 @@ void launch_fwd_split(const Plan& P, const float2* gm, int precision, float, flo
   a.P = P.d_p;
   a.gm = gm;
   a.ext = P.d_ext;
-  a.chunks = P.d_chunks;
-  a.chunk_pairs = P.d_chunk_pairs;
   a.Nx = P.Nx;
   a.Nz = P.Nz;
   a.N_pts = P.N_pts;
@@ -608,13 +643,21 @@ void launch_fwd_split(const Plan& P, const float2* gm, int precision, float, flo
   a.nseg = P.nseg_s;
   a.n_items = (int)P.chunks.size() * P.NZB_s * P.nseg_s;
   a.Ne_pad = P.Ne_pad;
-  a.vpart = P.d_vpart;
   const int grid = std::min(a.n_items, 2 * P.n_cu);
+  const bool even = (P.Nx % 2) == 0;
+#define BORN_FWDS(MTv)                                                                               \
+  if (even)                                                                                          \
+    hipLaunchKernelGGL((k_fwd_s2d_split<MTv, true>), dim3(grid), dim3(256), fwd_split_lds_bytes<MTv>(), s, a, \
+                       P.d_chunks, P.d_chunk_pairs, P.d_vpart);                                        \
+  else                                                                                               \
+    hipLaunchKernelGGL((k_fwd_s2d_split<MTv, false>), dim3(grid), dim3(256), fwd_split_lds_bytes<MTv>(), s, a, \
+                       P.d_chunks, P.d_chunk_pairs, P.d_vpart);
   switch (P.Ne_pad / 16) {
-    case 2: hipLaunchKernelGGL((k_fwd_s2d_split<2>), dim3(grid), dim3(256), fwd_split_lds_bytes<2>(), s, a); break;
-    case 4: hipLaunchKernelGGL((k_fwd_s2d_split<4>), dim3(grid), dim3(256), fwd_split_lds_bytes<4>(), s, a); break;
-    default: hipLaunchKernelGGL((k_fwd_s2d_split<8>), dim3(grid), dim3(256), fwd_split_lds_bytes<8>(), s, a); break;
+    case 2: BORN_FWDS(2) break;
+    case 4: BORN_FWDS(4) break;
+    default: BORN_FWDS(8) break;
   }
+#undef BORN_FWDS
 }
 
 void launch_adj_split(const Plan& P, int N_rhs, float2* g_out, hipStream_t s) {
@@ -642,7 +685,8 @@ void launch_adj_split(const Plan& P, int N_rhs, float2* g_out, hipStream_t s) {
 
 void set_max_lds_split(int bytes) {
 #define BORN_SET(K) hipFuncSetAttribute((const void*)(K), hipFuncAttributeMaxDynamicSharedMemorySize, bytes)
-  BORN_SET((k_fwd_s2d_split<2>)); BORN_SET((k_fwd_s2d_split<4>)); BORN_SET((k_fwd_s2d_split<8>));
+  BORN_SET((k_fwd_s2d_split<2, true>)); BORN_SET((k_fwd_s2d_split<4, true>)); BORN_SET((k_fwd_s2d_split<8, true>));
+  BORN_SET((k_fwd_s2d_split<2, false>)); BORN_SET((k_fwd_s2d_split<4, false>)); BORN_SET((k_fwd_s2d_split<8, false>));
   BORN_SET((k_adj_s2d_split<2>)); BORN_SET((k_adj_s2d_split<4>)); BORN_SET((k_adj_s2d_split<8>));
 #undef BORN_SET
 }

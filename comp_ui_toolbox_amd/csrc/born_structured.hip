@@ -94,16 +94,15 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(FwdArgs A, const Chunk* __re
   int z = (item % A.NZB) * A.ZR;
   int z_end = min(A.Nz, z + A.ZR);
   // descriptor of the item's chunk (uniform: scalar registers); unused
-  // columns read p_inc row 0 and are zeroed in the B tile
-  int fg, rr, ncols, pr[NC];
+  // columns read the zero row of the p_inc table
+  int fg, rr, pr[NC];
 #define BORN_FWD_DESC(itm)                                                                \
   {                                                                                       \
     const int ci_ = (itm) / A.NZB;                                                        \
     const Chunk c_ = chunks[ci_];                                                         \
     fg = c_.fg;                                                                           \
     rr = c_.r;                                                                            \
-    ncols = c_.ncols;                                                                     \
-    _Pragma("unroll") for (int q = 0; q < NC; ++q) pr[q] = max(chunk_pairs[ci_ * NC + q], 0); \
+    _Pragma("unroll") for (int q = 0; q < NC; ++q) pr[q] = chunk_pairs[ci_ * NC + q]; \
   }
   BORN_FWD_DESC(item)
 
@@ -127,7 +126,7 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(FwdArgs A, const Chunk* __re
   }
 
   BORN_FWD_PREFETCH(z)
-  int cur_nc = ncols;   // columns of the data held in registers
+  const bool xfull = XP == A.Nx;   // no padded points in the B tile
 
   v4f SR[MTW], SI[MTW];
 #pragma unroll
@@ -158,9 +157,9 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(FwdArgs A, const Chunk* __re
     for (int j = 0; j < PPT; ++j) {
       const int i = tid + 256 * j;
       if (i < XP) {
-        const float2 g = i < A.Nx ? greg[j] : make_float2(0.f, 0.f);
+        const float2 g = (xfull || i < A.Nx) ? greg[j] : make_float2(0.f, 0.f);
 #pragma unroll
-        for (int c = 0; c < NC; ++c) Bs[i * NCS + c] = c < cur_nc ? cmulf(praw[j][c], g) : make_float2(0.f, 0.f);
+        for (int c = 0; c < NC; ++c) Bs[i * NCS + c] = cmulf(praw[j][c], g);
       }
     }
     __syncthreads();
@@ -245,7 +244,6 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(FwdArgs A, const Chunk* __re
       ++it;
     }
     if (!has_next) break;
-    cur_nc = ncols;
   }
 #undef BORN_FWD_PREFETCH
 #undef BORN_FWD_DESC
@@ -385,7 +383,7 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d(AdjArgs A, const Chunk* __re
     FG = c_.fg;                                                                                  \
     R = c_.r;                                                                                    \
     _Pragma("unroll") for (int j = 0; j < PPT; ++j)                                             \
-      (PV)[j] = max(chunk_pairs[(cidx) * NC + (tid + 256 * j) / SEG], 0);                        \
+      (PV)[j] = chunk_pairs[(cidx) * NC + (tid + 256 * j) / SEG];                        \
   }
 
   int fg0, r0, pv0[PPT];
