@@ -411,7 +411,8 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d_split(AdjSplitArgs A, const 
   const int i_s = blockIdx.x * SEG;
   const int t_shift = i_s - (NEP - A.N_e) * NP;  // window entry u holds Hext[u + t_shift]
 
-  // gather indices of the window (chunk-invariant)
+  // gather indices of the window (chunk-invariant); hfull: all inside the table
+  const bool hfull = t_shift >= 0 && t_shift + L <= A.N_ext;
   int eidx[HPT];
 #pragma unroll
   for (int j = 0; j < HPT; ++j) {
@@ -472,10 +473,12 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d_split(AdjSplitArgs A, const 
       if (u < L) {
         const int p = u & 3, q = u >> 2;
         const int m = (LQ - 1) - q;
-        const float2 hv = eidx[j] >= 0 ? hreg[j] : make_float2(0.f, 0.f);
-        unsigned short v[4];
-        split2(hv.x, v[0], v[1]);
-        split2(-hv.y, v[2], v[3]);             // conj
+        float2 hv = hreg[j];
+        if (!hfull && eidx[j] < 0) hv = make_float2(0.f, 0.f);
+        unsigned int hi, lo;
+        split_pair(hv.x, -hv.y, hi, lo);   // conj
+        const unsigned short v[4] = {(unsigned short)hi, (unsigned short)lo, (unsigned short)(hi >> 16),
+                                     (unsigned short)(lo >> 16)};
 #pragma unroll
         for (int cp = 0; cp < 4; ++cp) {
 #pragma unroll
@@ -486,15 +489,23 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d_split(AdjSplitArgs A, const 
         }
       }
     }
+    // weights: entries j and j + 1 of a thread are elements k and k + 16 of one column
 #pragma unroll
-    for (int j = 0; j < WPT; ++j) {
+    for (int j = 0; j < WPT; j += 2) {
       const int idx = tid + 256 * j;
       const int k = idx / NC, wc = idx - k * NC;
-      unsigned short v[4];
-      split2(wreg[j].x, v[0], v[1]);
-      split2(wreg[j].y, v[2], v[3]);
-#pragma unroll
-      for (int cp = 0; cp < 4; ++cp) Wt[(cp * NC + wc) * WS + k] = v[cp];
+      unsigned int rh, rl, ih, il;
+      split_pair(wreg[j].x, wreg[j + 1].x, rh, rl);
+      split_pair(wreg[j].y, wreg[j + 1].y, ih, il);
+      unsigned short* wd = &Wt[wc * WS + k];
+      wd[0 * NC * WS] = (unsigned short)rh;
+      wd[0 * NC * WS + 16] = (unsigned short)(rh >> 16);
+      wd[1 * NC * WS] = (unsigned short)rl;
+      wd[1 * NC * WS + 16] = (unsigned short)(rl >> 16);
+      wd[2 * NC * WS] = (unsigned short)ih;
+      wd[2 * NC * WS + 16] = (unsigned short)(ih >> 16);
+      wd[3 * NC * WS] = (unsigned short)il;
+      wd[3 * NC * WS + 16] = (unsigned short)(il >> 16);
     }
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
