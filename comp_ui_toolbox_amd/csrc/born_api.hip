@@ -242,11 +242,13 @@ static void detect_structure(Plan& P) {
   if (P.nd != 2 || P.Nex != P.N_e || P.N_e > 128) return;
   const int Nx = P.Nx, Nz = P.Nz;
   const int* sh = P.h_shift.data();
+  const int HR = P.HR;   // h_ref entries per depth row
   int Np = 1;
   if (P.N_e >= 2) {
-    const int p = P.N_pts - 1;
-    Np = p - sh[(size_t)1 * P.N_pts + p];
-    if (Np < 1 || Np > Nx - 1) return;
+    // the last point of the first row: element 1 sees it Np points closer
+    const int p = Nx - 1;
+    Np = sh[p] - sh[(size_t)1 * P.N_pts + p];
+    if (Np < 1 || Np > HR - 1) return;
   }
   const int N_ext = Nx + (P.N_e - 1) * Np;
   std::vector<int> ext(N_ext, -1);
@@ -254,7 +256,7 @@ static void detect_structure(Plan& P) {
     for (int i = 0; i < Nx; ++i) {
       const int t = i - e * Np + (P.N_e - 1) * Np;
       const int v = sh[(size_t)e * P.N_pts + i];
-      if (v < 0 || v >= Nx) return;
+      if (v < 0 || v >= HR) return;
       if (ext[t] < 0) ext[t] = v;
       else if (ext[t] != v) return;
     }
@@ -264,7 +266,7 @@ static void detect_structure(Plan& P) {
     for (int z = 0; z < Nz; ++z)
       for (int i = 0; i < Nx; ++i) {
         const int t = i - e * Np + (P.N_e - 1) * Np;
-        if (sh[(size_t)e * P.N_pts + (size_t)z * Nx + i] != z * Nx + ext[t]) return;
+        if (sh[(size_t)e * P.N_pts + (size_t)z * Nx + i] != z * HR + ext[t]) return;
       }
   P.Np = Np;
   P.N_ext = N_ext;
@@ -374,6 +376,8 @@ static void common_dims(Plan& P, int nd, const int32_t* Npa, const int32_t* Nea,
     throw Error(BORN_ERR_INVALID_ARGUMENT, "N_elements_axis does not match N_elements");
   P.N_pts = N_points;
   P.N_e = N_elements;
+  P.H_plane = N_points;
+  P.HR = N_points / P.Nz;
 }
 
 static void finish_plan(Plan& P) {
@@ -449,14 +453,31 @@ static Plan* create_tables(const born_tables_desc* d) {
 // shift_lateral restated (setup_grid_symmetric.m:171-233): subscripts shifted
 // by (element subscript - 1) * points-per-pitch, mirrored when <= 0 about
 // index_offset = N_points_axis - (N_elements_axis - 1) * Np + 1.
+// Element e sees grid point p at h_ref entry shift[e][p].  Mirrored layout
+// (setup_grid_symmetric.m:184-226): the lateral subscript moves by the
+// element offset and reflects at the FOV edge, so h_ref covers the FOV grid.
+// Extended layout (FOV narrower than the aperture, where the reflection is
+// undefined): h_ref covers the grid extended by (N_elements_axis - 1)
+// pitches before the FOV, and the lateral subscript is the offset itself.
 static void compute_shift_map(Plan& P, const int* Np_axis) {
   const int nlat = P.nd - 1;
   const int Na[3] = {P.Nx, P.Ny, P.Nz};
   const int Ne[2] = {P.Nex, P.Ney};
-  P.h_shift.resize((size_t)P.N_e * P.N_pts);
+  int Nt[3] = {P.Nx, P.Ny, P.Nz};   // table grid
+  int off[2] = {0, 0};
+  P.extended = false;
   for (int d = 0; d < nlat; ++d)
-    if (Na[d] - (Ne[d] - 1) * Np_axis[d] < 1)
-      throw Error(BORN_ERR_UNSUPPORTED, "lateral FOV narrower than the aperture: the mirror map is undefined");
+    if (Na[d] - (Ne[d] - 1) * Np_axis[d] < 1) P.extended = true;
+  if (P.extended)
+    for (int d = 0; d < nlat; ++d) {
+      off[d] = (Ne[d] - 1) * Np_axis[d];
+      Nt[d] = Na[d] + off[d];
+    }
+  const int64_t plane = (int64_t)Nt[0] * (P.nd == 3 ? Nt[1] : 1) * P.Nz;
+  if (plane > INT32_MAX) throw Error(BORN_ERR_UNSUPPORTED, "extended table too large");
+  P.H_plane = (int)plane;
+  P.HR = P.H_plane / P.Nz;
+  P.h_shift.resize((size_t)P.N_e * P.N_pts);
   for (int e = 0; e < P.N_e; ++e) {
     const int es[2] = {e % P.Nex, e / P.Nex};
     for (int p = 0; p < P.N_pts; ++p) {
@@ -474,10 +495,13 @@ static void compute_shift_map(Plan& P, const int* Np_axis) {
         int v = s[d] + 1;  // 1-based subscript
         if (d < nlat) {
           v -= es[d] * Np_axis[d];
-          if (v <= 0) v = (Na[d] - (Ne[d] - 1) * Np_axis[d] + 1) - v;
+          if (P.extended)
+            v += off[d];
+          else if (v <= 0)
+            v = (Na[d] - (Ne[d] - 1) * Np_axis[d] + 1) - v;
         }
         lin += (v - 1) * mul;
-        mul *= (P.nd == 3) ? Na[d] : (d == 0 ? P.Nx : P.Nz);
+        mul *= (P.nd == 3) ? Nt[d] : (d == 0 ? Nt[0] : P.Nz);
       }
       P.h_shift[(size_t)e * P.N_pts + p] = lin;
     }
@@ -517,6 +541,10 @@ static Plan* create_geometry(const born_geometry_desc* d) {
   GeomDev g;
   g.nd = d->N_dimensions;
   g.Nx = P->Nx; g.Ny = P->Ny; g.Nz = P->Nz;
+  g.ox = P->extended ? (P->Nex - 1) * Np_axis[0] : 0;
+  g.oy = (P->extended && g.nd == 3) ? (P->Ney - 1) * Np_axis[1] : 0;
+  g.Nx_tab = P->Nx + g.ox;
+  g.Ny_tab = P->Ny + g.oy;
   g.dx = d->delta_axis[0];
   g.dy = (g.nd == 3) ? d->delta_axis[1] : 0.0;
   g.dz = d->delta_axis[g.nd - 1];
@@ -548,15 +576,16 @@ static Plan* create_geometry(const born_geometry_desc* d) {
   double2* d_vd = nullptr;
   int* d_vd_off = nullptr;
   try {
-    h64 = dalloc<double2>((size_t)P->N_fu * P->N_pts);
+    const size_t nh = (size_t)P->N_fu * P->H_plane;
+    h64 = dalloc<double2>(nh);
     g.filt_kind = d->filter_rx_kind;
     g.filt_param = d->filter_rx_param;
     launch_build_h(g, P->N_fu, dk, dfw, h64, s);
-    P->d_h = dalloc<float2>((size_t)P->N_fu * P->N_pts);
-    launch_f64_to_f32(h64, (size_t)P->N_fu * P->N_pts, P->d_h, s);
+    P->d_h = dalloc<float2>(nh);
+    launch_f64_to_f32(h64, nh, P->d_h, s);
     const double2* htx = h64;
     if (d->filter_tx_kind != d->filter_rx_kind || d->filter_tx_param != d->filter_rx_param) {
-      h64tx = dalloc<double2>((size_t)P->N_fu * P->N_pts);
+      h64tx = dalloc<double2>(nh);
       g.filt_kind = d->filter_tx_kind;
       g.filt_param = d->filter_tx_param;
       launch_build_h(g, P->N_fu, dk, dfw, h64tx, s);
@@ -810,30 +839,41 @@ int born_adjoint_dev(born_plan_t plan, const void* u_c64, int32_t N_rhs, void* g
   });
 }
 
+static void table_shape(const Plan& P, int which, int index, int64_t& rows, int64_t& cols) {
+  if (which == 0) {
+    rows = P.N_fu;
+    cols = P.H_plane;
+  } else if (which == 1) {
+    if (index < 0 || index >= (int)P.meas_Nfm.size()) throw Error(BORN_ERR_INVALID_ARGUMENT, "measurement index");
+    rows = P.meas_Nfm[index];
+    cols = P.N_pts;
+  } else {
+    throw Error(BORN_ERR_INVALID_ARGUMENT, "unknown table");
+  }
+}
+
+int born_plan_table_shape(born_plan_t plan, int32_t which, int32_t index, int64_t* rows, int64_t* cols) {
+  return guarded([&] {
+    if (!plan || !rows || !cols) throw Error(BORN_ERR_INVALID_ARGUMENT, "plan and outputs are required");
+    table_shape(*reinterpret_cast<Plan*>(plan), which, index, *rows, *cols);
+  });
+}
+
 int born_plan_get_table(born_plan_t plan, int32_t which, int32_t index, void* out_c64, int64_t capacity) {
   return guarded([&] {
     if (!plan || !out_c64) throw Error(BORN_ERR_INVALID_ARGUMENT, "plan and output are required");
     Plan& P = *reinterpret_cast<Plan*>(plan);
     HIP_CHECK(hipSetDevice(P.device));
-    const float2* src = nullptr;
-    int rows = 0;
-    if (which == 0) {
-      src = P.d_h;
-      rows = P.N_fu;
-    } else if (which == 1) {
-      if (index < 0 || index >= (int)P.meas_Nfm.size()) throw Error(BORN_ERR_INVALID_ARGUMENT, "measurement index");
-      src = P.d_p + (size_t)P.meas_pair_base[index] * P.N_pts;
-      rows = P.meas_Nfm[index];
-    } else {
-      throw Error(BORN_ERR_INVALID_ARGUMENT, "unknown table");
-    }
-    const size_t n = (size_t)rows * P.N_pts;
+    int64_t rows = 0, cols = 0;
+    table_shape(P, which, index, rows, cols);
+    const float2* src = which == 0 ? P.d_h : P.d_p + (size_t)P.meas_pair_base[index] * P.N_pts;
+    const size_t n = (size_t)rows * cols;
     if ((int64_t)n > capacity) throw Error(BORN_ERR_INVALID_ARGUMENT, "output capacity too small");
     std::vector<float2> tmp(n);
     HIP_CHECK(hipMemcpy(tmp.data(), src, n * sizeof(float2), hipMemcpyDeviceToHost));
     float2* o = (float2*)out_c64;
-    for (int r = 0; r < rows; ++r)
-      for (int p = 0; p < P.N_pts; ++p) o[(size_t)p * rows + r] = tmp[(size_t)r * P.N_pts + p];
+    for (int64_t r = 0; r < rows; ++r)
+      for (int64_t p = 0; p < cols; ++p) o[(size_t)p * rows + r] = tmp[(size_t)r * cols + p];
   });
 }
 

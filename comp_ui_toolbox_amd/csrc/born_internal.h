@@ -31,6 +31,12 @@ struct Plan {
   int N_pts = 0;
   int N_e = 0, Nex = 0, Ney = 1;
   int N_fu = 0;
+  // h_ref holds H_plane entries per frequency: the FOV grid (N_pts, the
+  // reference's mirrored layout) or, for a FOV narrower than the aperture,
+  // the grid extended laterally by (N_elements_axis - 1) pitches, indexed by
+  // lateral offset.  HR = H_plane / Nz entries per depth row.
+  int H_plane = 0, HR = 0;
+  bool extended = false;
   int Np = 1;       // grid points per pitch (lateral, 2-D structured path)
   int N_ext = 0;    // extended lateral length Nx + (Nex-1)*Np
 
@@ -52,7 +58,7 @@ struct Plan {
   std::vector<double> pref; // complex (re, im) per (mk, f, a), f fastest
 
   // device copies
-  float2* d_h = nullptr;          // [N_fu][N_pts]
+  float2* d_h = nullptr;          // [N_fu][H_plane]
   float2* d_p = nullptr;          // [N_pairs + 1][N_pts], last row zero
   int* d_shift = nullptr;         // [N_e][N_pts], 0-based (general path)
   int* d_ext = nullptr;           // [N_ext] lateral map (structured path)
@@ -122,7 +128,9 @@ void launch_adj_general(const Plan& P, int N_rhs, float2* g_out, hipStream_t s);
 
 struct GeomDev {
   int nd;
-  int Nx, Ny, Nz;
+  int Nx, Ny, Nz;          // FOV grid
+  int Nx_tab, Ny_tab;      // lateral size of the table grid (FOV grid plus ox, oy points before it)
+  int ox, oy;
   double dx, dy, dz, z0;
   int Nex, Ney;
   double pitch_x, pitch_y;

@@ -112,10 +112,10 @@ void launch_fwd_reduce(const Plan& P, int N_rhs, int nblk, hipStream_t s) {
 // map (the reference's Phi entries, cu:1114-1120, never materialized).
 __global__ __launch_bounds__(256) void k_fwd_general(
     const float2* __restrict__ h, const float2* __restrict__ P, const float2* __restrict__ gm,
-    const int* __restrict__ shift, const int* __restrict__ pair_fg, int N_pts, int N_e, int N_rhs,
+    const int* __restrict__ shift, const int* __restrict__ pair_fg, int N_pts, int H_plane, int N_e, int N_rhs,
     double2* __restrict__ V) {
   const int pair = blockIdx.x, e = blockIdx.y, r = blockIdx.z;
-  const float2* hr = h + (size_t)pair_fg[pair] * N_pts;
+  const float2* hr = h + (size_t)pair_fg[pair] * H_plane;
   const int* sh = shift + (size_t)e * N_pts;
   const float2* pr = P + (size_t)pair * N_pts;
   const float2* g = gm + (size_t)r * N_pts;
@@ -144,7 +144,7 @@ __global__ __launch_bounds__(256) void k_fwd_general(
 
 void launch_fwd_general(const Plan& P, const float2* gm, int N_rhs, hipStream_t s) {
   hipLaunchKernelGGL(k_fwd_general, dim3(P.N_pairs, P.N_e, N_rhs), dim3(256), 0, s, P.d_h, P.d_p, gm,
-                     P.d_shift, P.d_pair_fg, P.N_pts, P.N_e, N_rhs, P.d_V);
+                     P.d_shift, P.d_pair_fg, P.N_pts, P.H_plane, P.N_e, N_rhs, P.d_V);
 }
 
 // ------------------------------------------------------------- mixing
@@ -238,14 +238,14 @@ void launch_adj_unmix(const Plan& P, const float2* u, int N_rhs, bool chunked, h
 // ------------------------------------------------------ general adjoint
 __global__ __launch_bounds__(256) void k_adj_general(
     const float2* __restrict__ h, const float2* __restrict__ P, const float2* __restrict__ W,
-    const int* __restrict__ shift, const int* __restrict__ pair_fg, int N_pts, int N_e, int N_pairs, int N_rhs,
-    float2* __restrict__ g_out) {
+    const int* __restrict__ shift, const int* __restrict__ pair_fg, int N_pts, int H_plane, int N_e, int N_pairs,
+    int N_rhs, float2* __restrict__ g_out) {
   const int p = blockIdx.x * 256 + threadIdx.x;
   const int r = blockIdx.y;
   if (p >= N_pts) return;
   float gxs = 0.f, gys = 0.f;
   for (int pair = 0; pair < N_pairs; ++pair) {
-    const float2* hr = h + (size_t)pair_fg[pair] * N_pts;
+    const float2* hr = h + (size_t)pair_fg[pair] * H_plane;
     float yx = 0.f, yy = 0.f;
     for (int e = 0; e < N_e; ++e) {
       const float2 a = hr[shift[(size_t)e * N_pts + p]];
@@ -262,7 +262,7 @@ __global__ __launch_bounds__(256) void k_adj_general(
 
 void launch_adj_general(const Plan& P, int N_rhs, float2* g_out, hipStream_t s) {
   hipLaunchKernelGGL(k_adj_general, dim3((P.N_pts + 255) / 256, N_rhs), dim3(256), 0, s, P.d_h, P.d_p, P.d_W,
-                     P.d_shift, P.d_pair_fg, P.N_pts, P.N_e, P.N_pairs, N_rhs, g_out);
+                     P.d_shift, P.d_pair_fg, P.N_pts, P.H_plane, P.N_e, P.N_pairs, N_rhs, g_out);
 }
 
 // --------------------------------------------------- table build (FP64)
@@ -344,14 +344,14 @@ __device__ __forceinline__ double taper(int kind, double param, double flag) {
 // reference element (setup.m:1216-1281, :916-1032; greens_function.m:88, :103)
 __global__ __launch_bounds__(256) void k_build_h(GeomDev g, int N_fu, const double2* __restrict__ kk,
                                                  const double2* __restrict__ face_w, double2* __restrict__ h64) {
-  const int N_pts = g.Nx * g.Ny * g.Nz;
+  const int N_tab = g.Nx_tab * g.Ny_tab * g.Nz;
   const int p = blockIdx.x * 256 + threadIdx.x;
   const int fg = blockIdx.y;
-  if (p >= N_pts) return;
-  const int ix = p % g.Nx;
-  const int rest = p / g.Nx;
-  const int iy = (g.nd == 3) ? rest % g.Ny : 0;
-  const int iz = (g.nd == 3) ? rest / g.Ny : rest;
+  if (p >= N_tab) return;
+  const int ix = p % g.Nx_tab - g.ox;   // FOV subscript (negative before the FOV)
+  const int rest = p / g.Nx_tab;
+  const int iy = (g.nd == 3) ? rest % g.Ny_tab - g.oy : 0;
+  const int iz = (g.nd == 3) ? rest / g.Ny_tab : rest;
   const double x = (ix - 0.5 * (g.Nx - 1)) * g.dx;
   const double y = (g.nd == 3) ? (iy - 0.5 * (g.Ny - 1)) * g.dy : 0.0;
   const double z = g.z0 + iz * g.dz;
@@ -397,13 +397,13 @@ __global__ __launch_bounds__(256) void k_build_h(GeomDev g, int N_fu, const doub
     }
     acc = cscale(acc, t);
   }
-  h64[(size_t)fg * N_pts + p] = make_double2(acc.re, acc.im);
+  h64[(size_t)fg * N_tab + p] = make_double2(acc.re, acc.im);
 }
 
 void launch_build_h(const GeomDev& g, int N_fu, const double2* k, const double2* face_w, double2* h64,
                     hipStream_t s) {
-  const int N_pts = g.Nx * g.Ny * g.Nz;
-  hipLaunchKernelGGL(k_build_h, dim3((N_pts + 255) / 256, N_fu), dim3(256), 0, s, g, N_fu, k, face_w, h64);
+  const int N_tab = g.Nx_tab * g.Ny_tab * g.Nz;
+  hipLaunchKernelGGL(k_build_h, dim3((N_tab + 255) / 256, N_fu), dim3(256), 0, s, g, N_fu, k, face_w, h64);
 }
 
 // p_in[pair][p] = sum_a h_tx[fg][shift(p, e_a)] v_d[a] in FP64, stored FP32
@@ -414,7 +414,7 @@ __global__ __launch_bounds__(256) void k_synth_pinc(const double2* __restrict__ 
                                                     const int* __restrict__ meas_pair_base,
                                                     const int* __restrict__ meas_Nfm, const int* __restrict__ tx_off,
                                                     const int* __restrict__ tx_el, const double2* __restrict__ vd,
-                                                    const int* __restrict__ vd_off, int N_pts,
+                                                    const int* __restrict__ vd_off, int N_pts, int H_plane,
                                                     float2* __restrict__ P) {
   const int p = blockIdx.x * 256 + threadIdx.x;
   const int pair = blockIdx.y;
@@ -422,7 +422,7 @@ __global__ __launch_bounds__(256) void k_synth_pinc(const double2* __restrict__ 
   const int m = pair_meas[pair];
   const int fm = pair - meas_pair_base[m];
   const int Nfm = meas_Nfm[m];
-  const double2* hr = h64 + (size_t)pair_fg[pair] * N_pts;
+  const double2* hr = h64 + (size_t)pair_fg[pair] * H_plane;
   double sx = 0.0, sy = 0.0;
   for (int a = tx_off[m]; a < tx_off[m + 1]; ++a) {
     const double2 hv = hr[shift[(size_t)tx_el[a] * N_pts + p]];
@@ -440,7 +440,7 @@ void launch_synth_pinc(const Plan& P, const double2* h64, const int* d_meas_Nfm,
                        const int* d_tx_el, const double2* d_vd, const int* d_vd_off, hipStream_t s) {
   hipLaunchKernelGGL(k_synth_pinc, dim3((P.N_pts + 255) / 256, P.N_pairs), dim3(256), 0, s, h64, P.d_shift,
                      P.d_pair_fg, P.d_pair_meas, P.d_meas_pair_base, d_meas_Nfm, d_tx_off, d_tx_el, d_vd, d_vd_off,
-                     P.N_pts, P.d_p);
+                     P.N_pts, P.H_plane, P.d_p);
 }
 
 }  // namespace born

@@ -53,6 +53,7 @@ struct FwdSplitArgs {
   const float2* gm;
   const int* ext;
   int Nx, Nz, N_pts, N_e, N_ext, NZB, ZR, nseg, n_items, Ne_pad;
+  int Hp, HR;   // h_ref entries per frequency and per depth row
 };
 
 // Round two floats to BF16 (nearest even) in one conversion; returns the
@@ -163,7 +164,7 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(FwdSplitArgs A, const 
 #define BORN_SPLIT_PREFETCH(zz)                                                                  \
   {                                                                                              \
     const size_t rowo_ = (size_t)(zz) * A.Nx;                                                    \
-    const float2* hrow_ = A.h + (size_t)fg * A.N_pts + rowo_;                                    \
+    const float2* hrow_ = A.h + (size_t)fg * A.Hp + (size_t)(zz) * A.HR;                         \
     _Pragma("unroll") for (int j = 0; j < APT; ++j)                                             \
       hreg[j] = *static_cast<const float2*>(byte_offset(hrow_, hoff[j]));                        \
     const float2* grow_ = A.gm + (size_t)rr * A.N_pts + rowo_;                                   \
@@ -370,6 +371,7 @@ struct AdjSplitArgs {
   const Chunk* chunks;
   const int* chunk_pairs;  // [n_chunks][NC], zero row past ncols
   int n_chunks, Nx, N_pts, N_e, N_ext, N_rhs;
+  int Hp, HR;              // h_ref entries per frequency and per depth row
   float2* g_out;
 };
 
@@ -427,7 +429,7 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d_split(AdjSplitArgs A, const 
   // meet zero weights; points past the row end are never written out.
 #define BORN_ADJS_PREFETCH(cidx, FG, PR)                                                         \
   {                                                                                              \
-    const float2* hrow_ = A.h + (size_t)(FG) * A.N_pts + (size_t)z * A.Nx;                       \
+    const float2* hrow_ = A.h + (size_t)(FG) * A.Hp + (size_t)z * A.HR;                          \
     _Pragma("unroll") for (int j = 0; j < HPT; ++j) hreg[j] = hrow_[max(eidx[j], 0)];           \
     const float2* wblk_ = A.Wc + (size_t)(cidx) * NEP * NC;                                      \
     _Pragma("unroll") for (int j = 0; j < WPT; ++j) wreg[j] = wblk_[tid + 256 * j];             \
@@ -647,6 +649,8 @@ void launch_fwd_split(const Plan& P, const float2* gm, int precision, float, flo
   a.Nx = P.Nx;
   a.Nz = P.Nz;
   a.N_pts = P.N_pts;
+  a.Hp = P.H_plane;
+  a.HR = P.HR;
   a.N_e = P.N_e;
   a.N_ext = P.N_ext;
   a.NZB = P.NZB_s;
@@ -682,6 +686,8 @@ void launch_adj_split(const Plan& P, int N_rhs, float2* g_out, hipStream_t s) {
   a.n_chunks = (int)P.chunks.size();
   a.Nx = P.Nx;
   a.N_pts = P.N_pts;
+  a.Hp = P.H_plane;
+  a.HR = P.HR;
   a.N_e = P.N_e;
   a.N_ext = P.N_ext;
   a.N_rhs = N_rhs;

@@ -42,6 +42,7 @@ struct FwdArgs {
   const float2* gm;
   const int* ext;
   int Nx, Nz, N_pts, N_e, Np, N_ext, NZB, ZR, S, MT, n_items, Ne_pad;
+  int Hp, HR;   // h_ref entries per frequency and per depth row
   float2* vpart;
 };
 
@@ -114,7 +115,7 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(FwdArgs A, const Chunk* __re
   // address is valid (clamped), out-of-range entries are zeroed in the fill
 #define BORN_FWD_PREFETCH(zz)                                                           \
   {                                                                                     \
-    const float2* hrow_ = A.h + (size_t)fg * A.N_pts + (size_t)(zz) * A.Nx;             \
+    const float2* hrow_ = A.h + (size_t)fg * A.Hp + (size_t)(zz) * A.HR;                \
     _Pragma("unroll") for (int j = 0; j < HPT; ++j) hreg[j] = hrow_[max(eidx[j], 0)];  \
     const float2* grow_ = A.gm + (size_t)rr * A.N_pts + (size_t)(zz) * A.Nx;            \
     _Pragma("unroll") for (int j = 0; j < PPT; ++j) {                                   \
@@ -275,6 +276,8 @@ void launch_fwd_structured(const Plan& P, const float2* gm, int N_rhs, hipStream
   a.Nx = P.Nx;
   a.Nz = P.Nz;
   a.N_pts = P.N_pts;
+  a.Hp = P.H_plane;
+  a.HR = P.HR;
   a.N_e = P.N_e;
   a.Np = P.Np;
   a.N_ext = P.N_ext;
@@ -320,6 +323,7 @@ struct AdjArgs {
   const float2* Wc;   // [n_chunks][Ne_pad][NC], zero in unused columns and padded elements
   const int* ext;
   int n_chunks, Nx, N_pts, N_e, Ne_pad, Np, N_ext, N_rhs;
+  int Hp, HR;   // h_ref entries per frequency and per depth row
 };
 
 // One workgroup = (lateral segment of 64*MW points, depth row z).  It walks
@@ -367,7 +371,7 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d(AdjArgs A, const Chunk* __re
   // the row end are never written out.
 #define BORN_ADJ_PREFETCH(cidx, FG, PV)                                                          \
   {                                                                                              \
-    const float2* hrow_ = A.h + (size_t)(FG) * A.N_pts + (size_t)z * A.Nx;                       \
+    const float2* hrow_ = A.h + (size_t)(FG) * A.Hp + (size_t)z * A.HR;                          \
     _Pragma("unroll") for (int j = 0; j < HPT; ++j) hreg[j] = hrow_[max(eidx[j], 0)];           \
     const float2* wblk_ = A.Wc + (size_t)(cidx) * NEP * NC;                                      \
     _Pragma("unroll") for (int j = 0; j < WPT; ++j) wreg[j] = wblk_[tid + 256 * j];             \
@@ -524,6 +528,8 @@ void launch_adj_structured(const Plan& P, int N_rhs, float2* g_out, hipStream_t 
   a.n_chunks = (int)P.chunks.size();
   a.Nx = P.Nx;
   a.N_pts = P.N_pts;
+  a.Hp = P.H_plane;
+  a.HR = P.HR;
   a.N_e = P.N_e;
   a.Ne_pad = P.Ne_pad;
   a.Np = P.Np;

@@ -203,14 +203,17 @@ class OperatorBorn:
         check(_lib.lib().born_plan_set_precision(self._plan, _lib.PRECISIONS[precision]))
 
     def table(self, which: str, index: int = 0) -> np.ndarray:
-        """Device copy (complex64) of h_ref ('h_ref', N_f_unique x N_pts) or of
-        p_incident of measurement `index` ('p_incident', N_f_meas x N_pts)."""
+        """Device copy (complex64) of h_ref ('h_ref', N_f_unique x N_h) or of
+        p_incident of measurement `index` ('p_incident', N_f_meas x N_pts).
+        N_h = N_pts, or the laterally extended table grid when the FOV is
+        narrower than the aperture."""
         w = {"h_ref": 0, "p_incident": 1}[which]
-        rows = self.meta["rows_h"] if w == 0 else self.meta["rows_p"][index]
-        cap = rows * self.N_points
+        rows, cols = C.c_int64(), C.c_int64()
+        check(_lib.lib().born_plan_table_shape(self._plan, w, index, C.byref(rows), C.byref(cols)))
+        cap = rows.value * cols.value
         buf = np.empty(cap, dtype=np.complex64)
         check(_lib.lib().born_plan_get_table(self._plan, w, index, buf.ctypes.data_as(C.c_void_p), cap))
-        return buf.reshape(self.N_points, rows).T.copy()
+        return buf.reshape(cols.value, rows.value).T.copy()
 
     # ---------------------------------------------------------- host FP64 path
     def forward(self, gamma) -> np.ndarray:

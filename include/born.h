@@ -166,9 +166,15 @@ int born_forward_dev(born_plan_t plan, const void* gamma_c64, int32_t N_rhs, voi
 int born_adjoint_dev(born_plan_t plan, const void* u_c64, int32_t N_rhs, void* gamma_hat_c64, void* stream);
 
 /* Copies of the device tables, complex64, for checking the device table
- * build: which = 0 h_ref (N_f_unique x N_points), 1 p_incident of
- * measurement `index` (N_f_unique(m) x N_points).  Column-major like the
- * reference matrices. */
+ * build: which = 0 h_ref (N_f_unique x N_h), 1 p_incident of measurement
+ * `index` (N_f_unique(m) x N_points).  Column-major like the reference
+ * matrices.  N_h = N_points for the reference's mirrored layout
+ * (setup_grid_symmetric.m:95-241).  A plan built from geometry whose FOV is
+ * narrower than the aperture (where that mirror map is undefined) keeps h_ref
+ * on the grid extended laterally by (N_elements_axis - 1) pitches before the
+ * FOV, indexed by lateral offset; then N_h is the size of that grid.
+ * born_plan_table_shape returns the rows and columns of a table. */
+int born_plan_table_shape(born_plan_t plan, int32_t which, int32_t index, int64_t* rows, int64_t* cols);
 int born_plan_get_table(born_plan_t plan, int32_t which, int32_t index, void* out_c64, int64_t capacity);
 
 /* Execution path chosen at plan creation: 0 structured (shift-invariant

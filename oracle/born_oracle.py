@@ -451,3 +451,43 @@ def restricted_tables(spec, points: np.ndarray) -> BornTables:
         meas.append(MeasTables(fidx + 1, p, mixes))
     return BornTables(h_sub, shift_sub, meas, (len(points), 1), tuple(spec.array.N_elements_axis),
                       tuple(spec.N_points_per_pitch_axis))
+
+
+def direct_tables(spec, points: Optional[np.ndarray] = None) -> BornTables:
+    """Tables of the operator restricted to the grid points `points` (0-based;
+    all by default) built from every element's own transfer function at each
+    point (setup.m:1216-1281 evaluated at the element's centre, with its own
+    taper, setup.m:902), i.e. without any lateral shift or mirror map: column
+    p*N_e + e of h_ref is element e at point p.  p_inc is the direct sum over
+    the transmitting elements (sequence.m:825-840).  Valid for any FOV,
+    including one narrower than the aperture."""
+    N_e = spec.array.N_elements
+    if points is None:
+        points = np.arange(spec.fov.N_points)
+    points = np.asarray(points)
+    pos = spec.fov.positions()[points]
+    f = spec.frequencies
+    n = len(points)
+    h_rx = np.empty((len(f), n * N_e), dtype=np.complex128)
+    same = (spec.filter_tx.kind, spec.filter_tx.param) == (spec.filter_rx.kind, spec.filter_rx.param)
+    h_tx = h_rx if same else np.empty_like(h_rx)
+    for e in range(N_e):
+        h_rx[:, e::N_e] = transfer_function(spec, f, e, spec.filter_rx, positions=pos)
+        if not same:
+            h_tx[:, e::N_e] = transfer_function(spec, f, e, spec.filter_tx, positions=pos)
+    shift = (np.arange(n)[:, None] * N_e + np.arange(N_e)[None, :] + 1).astype(np.int64)
+    pref_u = prefactors_unique(spec)
+    meas = []
+    for m in range(spec.N_measurements):
+        fidx = spec.measurement_frequencies(m)
+        vd = normal_velocities(spec, m)
+        p = incident_field(h_tx[fidx], shift, spec.waves[m].indices_active, vd)
+        mixes = []
+        for mix in spec.mixes[m]:
+            fk = np.arange(len(fidx)) if mix.indices_f is None else np.asarray(mix.indices_f)
+            pref = pref_u[fidx[fk]][:, None] * np.asarray(mix.weights)[None, :]
+            mixes.append(MixTables(fk + 1, np.asarray(mix.indices_active) + 1, pref.astype(np.complex128)))
+        meas.append(MeasTables(fidx + 1, p, mixes))
+    return BornTables(h_rx, shift, meas, (n, 1), tuple(spec.array.N_elements_axis),
+                      tuple(spec.N_points_per_pitch_axis))
+

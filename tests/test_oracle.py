@@ -268,3 +268,22 @@ def test_config0_golden_regression():
     assert O.rel_l2(u, d["u"]) < 1e-12
     gh = O.adjoint(T, d["u"], points=d["points"])
     assert O.rel_l2(gh, d["gamma_hat"]) < 1e-12
+
+
+def test_direct_tables_equal_mirrored_tables():
+    """Every element's own transfer function (no shift map) gives the same
+    operator as h_ref of the first element through the mirror map, on a FOV
+    as wide as the aperture."""
+    spec = tiny_spec(N_e=4, Nx=14, Nz=3, npp=4, filt="raised_cosine")
+    A_mirror = O.assemble_matrix(O.build_tables(spec))
+    A_direct = O.assemble_matrix(O.direct_tables(spec))
+    assert O.rel_l2(A_direct, A_mirror) < 1e-12
+
+
+def test_direct_tables_restrict_to_points():
+    """Restricting direct_tables to a point subset keeps those columns of A."""
+    spec = tiny_spec(N_e=5, Nx=9, Nz=3, npp=2, filt="off")
+    A = O.assemble_matrix(O.direct_tables(spec))
+    pts = np.array([0, 4, 11, 26])
+    A_sub = O.assemble_matrix(O.direct_tables(spec, pts))
+    assert O.rel_l2(A_sub, A[:, pts]) < 1e-14
