@@ -88,6 +88,7 @@ Plan::~Plan() {
   hipSetDevice(device);
   if (stream) hipStreamSynchronize(stream);
   free_workspace(*this);
+  dfree(ls.d_inst);
   dfree(d_h);
   dfree(d_p);
   dfree(d_shift);
@@ -287,6 +288,75 @@ static void ensure_shift_on_device(Plan& P) {
   P.d_shift = dupload(P.h_shift, P.stream);
 }
 
+// Forward blocking: about items_target work items per launch, (chunk,
+// z-block) for the plain kernel and (chunk, z-block, 256-point segment) for
+// the split kernel.
+static void set_blocking(Plan& P) {
+  const int n_chunks = (int)P.chunks.size();
+  const int target = P.items_target;
+  int NZB = std::max(1, std::min(P.Nz, target / std::max(1, n_chunks)));
+  P.ZR = (P.Nz + NZB - 1) / NZB;
+  P.NZB = (P.Nz + P.ZR - 1) / P.ZR;
+  P.nseg_s = (P.Nx + 255) / 256;
+  NZB = std::max(1, std::min(P.Nz, target / std::max(1, n_chunks * P.nseg_s)));
+  P.ZR_s = (P.Nz + NZB - 1) / NZB;
+  P.NZB_s = (P.Nz + P.ZR_s - 1) / P.ZR_s;
+  P.vblk = std::max(P.NZB, P.NZB_s * P.nseg_s);
+}
+
+// The record of a plan inside a structured launch (workspace must exist).
+static Inst make_inst(const Plan& P) {
+  Inst I;
+  std::memset(&I, 0, sizeof(I));
+  I.h = (decltype(I.h))P.d_h;
+  I.P = (decltype(I.P))P.d_p;
+  I.gm = (decltype(I.gm))P.d_gm;
+  I.ext = (decltype(I.ext))P.d_ext;
+  I.Wc = (decltype(I.Wc))P.d_Wc;
+  I.vpart = (decltype(I.vpart))P.d_vpart;
+  I.Nx = P.Nx;
+  I.Nz = P.Nz;
+  I.N_pts = P.N_pts;
+  I.N_e = P.N_e;
+  I.Np = P.Np;
+  I.N_ext = P.N_ext;
+  I.Hp = P.H_plane;
+  I.HR = P.HR;
+  I.n_chunks = (int)P.chunks.size();
+  I.NZB = P.NZB;
+  I.ZR = P.ZR;
+  I.S = (P.Nx + 15) / 16;
+  I.NZB_s = P.NZB_s;
+  I.ZR_s = P.ZR_s;
+  I.nseg_s = P.nseg_s;
+  return I;
+}
+
+// Work item and workgroup ranges of every instance, totals, device copy.
+static void finalize_launchset(LaunchSet& L) {
+  int it = 0, its = 0, wg = 0, wgs = 0;
+  L.S_max = 1;
+  for (Inst& I : L.inst) {
+    I.item_begin = it;
+    it += I.n_chunks * I.NZB;
+    I.item_begin_s = its;
+    its += I.n_chunks * I.NZB_s * I.nseg_s;
+    I.adj_segs = (I.Nx + 64 * L.adj_mw - 1) / (64 * L.adj_mw);
+    I.wg_begin = wg;
+    wg += I.adj_segs * I.Nz;
+    I.adj_segs_s = (I.Nx + 255) / 256;
+    I.wg_begin_s = wgs;
+    wgs += I.adj_segs_s * I.Nz;
+    L.S_max = std::max(L.S_max, I.S);
+  }
+  L.n_items = it;
+  L.n_items_s = its;
+  L.n_wg = wg;
+  L.n_wg_s = wgs;
+  if (!L.d_inst) L.d_inst = dalloc<Inst>(MAX_BATCH);
+  HIP_CHECK(hipMemcpy(L.d_inst, L.inst.data(), L.inst.size() * sizeof(Inst), hipMemcpyHostToDevice));
+}
+
 // Chunks of <= NC (pair, rhs) columns per frequency, rhs-major.
 static void ensure_workspace(Plan& P, int N_rhs) {
   if (P.ws_rhs == N_rhs) return;
@@ -316,17 +386,7 @@ static void ensure_workspace(Plan& P, int N_rhs) {
   std::vector<int> chunk_pairs((size_t)n_chunks * NC, P.N_pairs);  // unused: the zero row
   for (int c = 0; c < n_chunks; ++c)
     for (int q = 0; q < P.chunks[c].ncols; ++q) chunk_pairs[(size_t)c * NC + q] = P.col_pair[P.chunks[c].col_start + q];
-  // about 4096 work items per forward launch: (chunk, z-block) for the plain
-  // kernel, (chunk, z-block, 256-point segment) for the split kernel
-  const int target = 4096;
-  int NZB = std::max(1, std::min(P.Nz, target / std::max(1, n_chunks)));
-  P.ZR = (P.Nz + NZB - 1) / NZB;
-  P.NZB = (P.Nz + P.ZR - 1) / P.ZR;
-  P.nseg_s = (P.Nx + 255) / 256;
-  NZB = std::max(1, std::min(P.Nz, target / std::max(1, n_chunks * P.nseg_s)));
-  P.ZR_s = (P.Nz + NZB - 1) / NZB;
-  P.NZB_s = (P.Nz + P.ZR_s - 1) / P.ZR_s;
-  P.vblk = std::max(P.NZB, P.NZB_s * P.nseg_s);
+  set_blocking(P);
   P.d_chunks = dupload(P.chunks, P.stream);
   P.d_col_pair = dupload(P.col_pair, P.stream);
   P.d_chunk_pairs = dupload(chunk_pairs, P.stream);
@@ -341,8 +401,17 @@ static void ensure_workspace(Plan& P, int N_rhs) {
     HIP_CHECK(hipMemset(P.d_Wc, 0, n * sizeof(float2)));  // padding columns/elements stay zero
   }
   P.d_gm = dalloc<float2>((size_t)N_rhs * P.N_pts);
-  HIP_CHECK(hipDeviceSynchronize());  // workspace ready before work on any stream
   P.ws_rhs = N_rhs;
+  if (P.structured) {
+    P.ls.inst.assign(1, make_inst(P));
+    P.ls.MT = P.Ne_pad / 16;
+    P.ls.Np = P.Np;
+    P.ls.adj_mw = adj_mw_default(P.Nx);
+    P.ls.d_chunks = P.d_chunks;
+    P.ls.d_chunk_pairs = P.d_chunk_pairs;
+    finalize_launchset(P.ls);
+  }
+  HIP_CHECK(hipDeviceSynchronize());  // workspace ready before work on any stream
 }
 
 static void ensure_io(Plan& P, size_t in_bytes, size_t out_bytes) {
@@ -649,12 +718,13 @@ struct TimedLaunch {
 
 static void forward_device(Plan& P, const float2* gm, int N_rhs, void* u, bool out_f64, hipStream_t s) {
   if (use_structured(P)) {
+    (void)gm;   // the launch record points at the masked input P.d_gm
     {
       TimedLaunch t(P, s, 0);
       if (P.precision == BORN_PRECISION_FP32)
-        launch_fwd_structured(P, gm, N_rhs, s);
+        launch_fwd_structured(P.ls, P.n_cu, s);
       else
-        launch_fwd_split(P, gm, P.precision, 1.0f, 1.0f, s);
+        launch_fwd_split(P.ls, P.n_cu, s);
     }
     launch_fwd_reduce(P, N_rhs, P.precision == BORN_PRECISION_FP32 ? P.NZB : P.NZB_s * P.nseg_s, s);
   } else {
@@ -670,10 +740,12 @@ static void adjoint_device(Plan& P, const float2* u, int N_rhs, float2* g, hipSt
   launch_adj_unmix(P, u, N_rhs, use_structured(P), s);
   TimedLaunch t(P, s, 1);
   if (use_structured(P)) {
+    OutPtrs outs;
+    outs.p[0] = g;
     if (P.precision == BORN_PRECISION_FP32)
-      launch_adj_structured(P, N_rhs, g, s);
+      launch_adj_structured(P.ls, outs, s);
     else
-      launch_adj_split(P, N_rhs, g, s);
+      launch_adj_split(P.ls, outs, s);
   } else {
     ensure_shift_on_device(P);
     launch_adj_general(P, N_rhs, g, s);
@@ -717,8 +789,9 @@ static void host_forward(Plan& P, const double* gamma, int N_rhs, double* u) {
   }
   ensure_workspace(P, N_rhs);
   ensure_io(P, npr * sizeof(float2), nout * sizeof(double2));
-  HIP_CHECK(hipMemcpyAsync(P.d_io_in, buf.data(), npr * sizeof(float2), hipMemcpyHostToDevice, P.stream));
-  forward_device(P, (const float2*)P.d_io_in, N_rhs, P.d_io_out, true, P.stream);
+  // the occupancy mask is applied on the host: upload straight to the masked-input buffer
+  HIP_CHECK(hipMemcpyAsync(P.d_gm, buf.data(), npr * sizeof(float2), hipMemcpyHostToDevice, P.stream));
+  forward_device(P, P.d_gm, N_rhs, P.d_io_out, true, P.stream);
   HIP_CHECK(hipMemcpyAsync(u, P.d_io_out, nout * sizeof(double2), hipMemcpyDeviceToHost, P.stream));
   HIP_CHECK(hipStreamSynchronize(P.stream));
 }

@@ -18,6 +18,56 @@ struct Chunk {
   int ncols;      // <= NC
 };
 
+// Pointers that kernels read out of device memory are marked as global
+// memory in device code, so loads through them are global (not flat) loads.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define BORN_GLOBAL __attribute__((address_space(1)))
+#else
+#define BORN_GLOBAL
+#endif
+
+// One operator instance of a structured launch.  A launch covers one plan
+// or a batch of plans that share the element padding and the points per
+// pitch; every per-instance size and pointer comes from this record.
+struct Inst {
+  BORN_GLOBAL const float2* h;
+  BORN_GLOBAL const float2* P;
+  BORN_GLOBAL const float2* gm;
+  BORN_GLOBAL const int* ext;
+  BORN_GLOBAL const float2* Wc;
+  BORN_GLOBAL float2* vpart;
+  int Nx, Nz, N_pts, N_e, Np, N_ext, Hp, HR;
+  int n_chunks;
+  int chunk_base;           // first chunk of the instance in the launch's chunk tables
+  int NZB, ZR, S;           // plain forward: z-blocks of ZR rows, S = ceil(Nx / 16)
+  int NZB_s, ZR_s, nseg_s;  // split forward: z-blocks x 256-point segments
+  int item_begin, item_begin_s;  // first forward work item of the instance (plain, split)
+  int wg_begin, wg_begin_s;      // first adjoint workgroup of the instance (plain, split)
+  int adj_segs, adj_segs_s;      // adjoint segments per row (plain, split)
+};
+
+constexpr int MAX_BATCH = 64;
+struct OutPtrs {  // adjoint outputs, passed by value
+  float2* p[MAX_BATCH];
+};
+
+// Launch parameters of one structured launch over instances.
+struct LaunchSet {
+  Inst* d_inst = nullptr;         // device copy of inst
+  std::vector<Inst> inst;
+  // chunk descriptors and column rows of all instances, concatenated
+  // (kernel arguments, so uniform reads of them are scalar loads)
+  const Chunk* d_chunks = nullptr;
+  const int* d_chunk_pairs = nullptr;
+  bool owns_chunks = false;
+  int MT = 1, Np = 4;
+  int S_max = 1;                  // largest S (plain forward LDS and registers)
+  int adj_mw = 4;                 // plain adjoint segment = 64 * adj_mw points
+  int n_items = 0, n_items_s = 0;
+  int n_wg = 0, n_wg_s = 0;
+  int grid_mult = 2;              // persistent forward workgroups per CU
+};
+
 // Device tables and index maps.  All device arrays are owned by the plan.
 struct Plan {
   int device = 0;
@@ -96,6 +146,8 @@ struct Plan {
   float2* d_W = nullptr;          // [N_pairs][N_e][N_rhs] (general path)
   float2* d_Wc = nullptr;         // [n_chunks][Ne_pad][NC] (structured path; unused slots stay zero)
   float2* d_gm = nullptr;         // masked gamma [N_rhs][N_pts]
+  int items_target = 4096;        // forward work items per launch (tunable)
+  LaunchSet ls;                   // structured launch of this plan alone
   size_t io_bytes = 0;
   void* d_io_in = nullptr;        // host-path staging
   void* d_io_out = nullptr;
@@ -117,13 +169,13 @@ void launch_convert_transpose(const double2* src, int rows, int cols, float2* ds
 void launch_f64_to_f32(const double2* src, size_t n, float2* dst, hipStream_t s);
 void launch_mask_gamma(const float2* g, int N_pts, int N_rhs, float2* gm, hipStream_t s);
 
-void launch_fwd_structured(const Plan& P, const float2* gm, int N_rhs, hipStream_t s);
+void launch_fwd_structured(const LaunchSet& L, int n_cu, hipStream_t s);
 void launch_fwd_general(const Plan& P, const float2* gm, int N_rhs, hipStream_t s);
 void launch_fwd_reduce(const Plan& P, int N_rhs, int nblk, hipStream_t s);
 void launch_fwd_mix(const Plan& P, int N_rhs, void* u, bool out_f64, hipStream_t s);
 
 void launch_adj_unmix(const Plan& P, const float2* u, int N_rhs, bool chunked, hipStream_t s);
-void launch_adj_structured(const Plan& P, int N_rhs, float2* g_out, hipStream_t s);
+void launch_adj_structured(const LaunchSet& L, const OutPtrs& outs, hipStream_t s);
 void launch_adj_general(const Plan& P, int N_rhs, float2* g_out, hipStream_t s);
 
 struct GeomDev {
@@ -146,8 +198,10 @@ void launch_synth_pinc(const Plan& P, const double2* h64, const int* d_meas_Nfm,
 void set_max_lds(size_t bytes);
 bool fwd_structured_fits(int Nx, int Ne_pad, int Np);
 bool fwd_split_supported(const Plan& P);
-void launch_fwd_split(const Plan& P, const float2* gm, int precision, float a_scale, float b_scale, hipStream_t s);
-void launch_adj_split(const Plan& P, int N_rhs, float2* g_out, hipStream_t s);
+void launch_fwd_split(const LaunchSet& L, int n_cu, hipStream_t s);
+void launch_adj_split(const LaunchSet& L, const OutPtrs& outs, hipStream_t s);
+int fwd_ppt(int S);              // B-tile points per thread of the plain forward
+int adj_mw_default(int Nx);
 void set_max_lds_split(int bytes);
 bool adj_structured_fits(int Nx, int Ne_pad, int Np);
 

@@ -47,15 +47,6 @@ __device__ __forceinline__ v4f vz() { return v4f{0.f, 0.f, 0.f, 0.f}; }
 // ======================================================================
 // forward
 // ======================================================================
-struct FwdSplitArgs {
-  const float2* h;
-  const float2* P;         // N_pairs + 1 rows; the last row is zero
-  const float2* gm;
-  const int* ext;
-  int Nx, Nz, N_pts, N_e, N_ext, NZB, ZR, nseg, n_items, Ne_pad;
-  int Hp, HR;   // h_ref entries per frequency and per depth row
-};
-
 // Round two floats to BF16 (nearest even) in one conversion; returns the
 // packed pair (x0 in the low half) and the rounded values as floats.
 __device__ __forceinline__ unsigned int bf_pack(float x0, float x1, float& r0, float& r1) {
@@ -76,8 +67,10 @@ __device__ __forceinline__ const void* byte_offset(const void* base, unsigned in
   return static_cast<const char*>(base) + off;
 }
 
-// Work item = (chunk, z-block, 256-point segment xs), taken persistently;
-// a step is one row z of the item.  Waves: 2 element groups x 2 point
+// Work item = (instance, chunk, z-block, 256-point segment xs), taken
+// persistently (the items of instance k are [item_begin_s_k,
+// item_begin_s_{k+1})); a step is one row z of the item.  The p_inc table of
+// every instance has a zero row after its N_pairs rows.  Waves: 2 element groups x 2 point
 // groups; wave (eg, pg) holds element tiles [eg*MTW, (eg+1)*MTW) and the
 // K-blocks pg*KBW .. pg*KBW + KBW-1 of the segment.  Points per pitch 4.
 //
@@ -92,9 +85,9 @@ __device__ __forceinline__ const void* byte_offset(const void* base, unsigned in
 //                        so every lane's 8-entry read is 16-byte aligned
 //   Bt[comp][c][SEGP]    p_inc*gamma of the segment, points contiguous
 template <int MT, bool EVEN>
-__global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(FwdSplitArgs A, const Chunk* __restrict__ chunks,
-                                                          const int* __restrict__ chunk_pairs,
-                                                          float2* __restrict__ vpart) {
+__global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict__ inst, int n_inst, int n_items,
+                                                          int Ne_pad, const Chunk* __restrict__ chunks,
+                                                          const int* __restrict__ chunk_pairs) {
   extern __shared__ unsigned short ldsu[];
   constexpr int NP = 4;
   constexpr int SEGP = SEG + 8;
@@ -112,10 +105,11 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(FwdSplitArgs A, const 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int eg = w % EW, pg = w / EW;
   const int kk = lane >> 4, col = lane & 15;
-  const int n_my = (A.n_items - (int)blockIdx.x + (int)gridDim.x - 1) / (int)gridDim.x;
+  const int n_my = (n_items - (int)blockIdx.x + (int)gridDim.x - 1) / (int)gridDim.x;
   if (n_my <= 0) return;
-  const int tpad = (16 * MT - A.N_e) * NP;
-  const int NBLK = A.NZB * A.nseg;   // partial-sum blocks per chunk
+  int ki = 0;
+  Inst Iv = inst[0];            // record of the current instance, in scalar registers
+  const Inst* I = &Iv;
   const int bc0 = (tid >> 7) * 8;    // columns of this thread's B loads: uniform per wave
   const int bi = 2 * (tid & 127);
 
@@ -128,31 +122,38 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(FwdSplitArgs A, const 
   int prow[8];                       // p_inc rows of the columns bc0 .. bc0+7
 #define BORN_SPLIT_ITEM(itm)                                                              \
   {                                                                                       \
-    ci = (itm) / NBLK;                                                                    \
-    const int rem_ = (itm) - ci * NBLK;                                                   \
-    const int zb_ = rem_ / A.nseg;                                                        \
-    xs = rem_ - zb_ * A.nseg;                                                             \
-    z = zb_ * A.ZR;                                                                       \
-    z_end = min(A.Nz, z + A.ZR);                                                          \
-    const Chunk c_ = chunks[ci];                                                          \
+    if (ki + 1 < n_inst && (itm) >= inst[ki + 1].item_begin_s) {                          \
+      while (ki + 1 < n_inst && (itm) >= inst[ki + 1].item_begin_s) ++ki;                 \
+      Iv = inst[ki];                                                                      \
+    }                                                                                     \
+    const int tpad = (16 * MT - I->N_e) * NP;                                             \
+    const int NBLK = I->NZB_s * I->nseg_s;   /* partial-sum blocks per chunk */           \
+    const int li_ = (itm) - I->item_begin_s;                                              \
+    ci = li_ / NBLK;                                                                      \
+    const int rem_ = li_ - ci * NBLK;                                                     \
+    const int zb_ = rem_ / I->nseg_s;                                                     \
+    xs = rem_ - zb_ * I->nseg_s;                                                          \
+    z = zb_ * I->ZR_s;                                                                    \
+    z_end = min(I->Nz, z + I->ZR_s);                                                      \
+    const Chunk c_ = chunks[I->chunk_base + ci];                                          \
     fg = c_.fg;                                                                           \
     rr = c_.r;                                                                            \
     const int t0_ = xs * SEG - tpad;                                                      \
-    hfull = t0_ >= 0 && t0_ + LA <= A.N_ext;                                              \
-    gfull = xs * SEG + SEG <= A.Nx;                                                       \
+    hfull = t0_ >= 0 && t0_ + LA <= I->N_ext;                                              \
+    gfull = xs * SEG + SEG <= I->Nx;                                                       \
     hval = 0;                                                                             \
     _Pragma("unroll") for (int j = 0; j < APT; ++j) {                                    \
       const int u_ = tid + 256 * j;                                                       \
       const int t_ = u_ + t0_;                                                            \
-      const bool ok_ = u_ < LA && t_ >= 0 && t_ < A.N_ext;                                \
-      hoff[j] = ok_ ? 8u * (unsigned int)A.ext[t_] : 0u;                                  \
+      const bool ok_ = u_ < LA && t_ >= 0 && t_ < I->N_ext;                                \
+      hoff[j] = ok_ ? 8u * (unsigned int)I->ext[t_] : 0u;                                  \
       hval |= ok_ ? (1u << j) : 0u;                                                       \
     }                                                                                     \
     const int gi_ = xs * SEG + bi;                                                        \
-    goff0 = 8u * (unsigned int)(EVEN ? min(gi_, A.Nx - 2) : min(gi_, A.Nx - 1));          \
-    goff1 = 8u * (unsigned int)min(gi_ + 1, A.Nx - 1);                                    \
+    goff0 = 8u * (unsigned int)(EVEN ? min(gi_, I->Nx - 2) : min(gi_, I->Nx - 1));          \
+    goff1 = 8u * (unsigned int)min(gi_ + 1, I->Nx - 1);                                    \
     _Pragma("unroll") for (int q = 0; q < 8; ++q)                                        \
-      prow[q] = __builtin_amdgcn_readfirstlane(chunk_pairs[ci * NC + bc0 + q]);           \
+      prow[q] = __builtin_amdgcn_readfirstlane(chunk_pairs[(I->chunk_base + ci) * NC + bc0 + q]); \
     /* settle the per-item loads here, so the per-step prefetch waits on none */          \
     _Pragma("unroll") for (int j = 0; j < APT; ++j) asm volatile("" ::"v"(hoff[j]));     \
   }
@@ -163,11 +164,11 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(FwdSplitArgs A, const 
 
 #define BORN_SPLIT_PREFETCH(zz)                                                                  \
   {                                                                                              \
-    const size_t rowo_ = (size_t)(zz) * A.Nx;                                                    \
-    const float2* hrow_ = A.h + (size_t)fg * A.Hp + (size_t)(zz) * A.HR;                         \
+    const size_t rowo_ = (size_t)(zz) * I->Nx;                                                    \
+    const float2* hrow_ = I->h + (size_t)fg * I->Hp + (size_t)(zz) * I->HR;                         \
     _Pragma("unroll") for (int j = 0; j < APT; ++j)                                             \
       hreg[j] = *static_cast<const float2*>(byte_offset(hrow_, hoff[j]));                        \
-    const float2* grow_ = A.gm + (size_t)rr * A.N_pts + rowo_;                                   \
+    const float2* grow_ = I->gm + (size_t)rr * I->N_pts + rowo_;                                   \
     if (EVEN) {                                                                                  \
       const float4 v_ = *static_cast<const float4*>(byte_offset(grow_, goff0));                  \
       greg[0] = make_float2(v_.x, v_.y);                                                         \
@@ -177,7 +178,7 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(FwdSplitArgs A, const 
       greg[1] = *static_cast<const float2*>(byte_offset(grow_, goff1));                          \
     }                                                                                            \
     _Pragma("unroll") for (int q = 0; q < 8; ++q) {                                             \
-      const float2* prow_ = A.P + (size_t)prow[q] * A.N_pts + rowo_;                             \
+      const float2* prow_ = I->P + (size_t)prow[q] * I->N_pts + rowo_;                             \
       if (EVEN) {                                                                                \
         const float4 v_ = *static_cast<const float4*>(byte_offset(prow_, goff0));                \
         preg[q][0] = make_float2(v_.x, v_.y);                                                    \
@@ -231,8 +232,8 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(FwdSplitArgs A, const 
     {
       float2 g0 = greg[0], g1 = greg[1];
       if (!gfull) {
-        if (xs * SEG + bi >= A.Nx) g0 = make_float2(0.f, 0.f);
-        if (xs * SEG + bi + 1 >= A.Nx) g1 = make_float2(0.f, 0.f);
+        if (xs * SEG + bi >= I->Nx) g0 = make_float2(0.f, 0.f);
+        if (xs * SEG + bi + 1 >= I->Nx) g1 = make_float2(0.f, 0.f);
       }
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
@@ -252,7 +253,9 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(FwdSplitArgs A, const 
     __syncthreads();
 
     // next step: the next row of this item, else the first row of the next item
-    const int ci_done = ci, item_done_id = item;
+    const int item_done_id = item;
+    float2* const vpart_done = Iv.vpart;
+    const int ib_done = Iv.item_begin_s;
     const bool item_done = z + 1 >= z_end;
     bool has_next = true;
     if (!item_done) {
@@ -331,8 +334,8 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(FwdSplitArgs A, const 
         CI[m] = vz();
       }
       __syncthreads();
-      const int blk = item_done_id - ci_done * NBLK;  // = zb * nseg + xs
-      float* out = reinterpret_cast<float*>(vpart + ((size_t)ci_done * NBLK + blk) * A.Ne_pad * NC);
+      // vpart block of the item: [chunk][zb * nseg + xs] = local item index
+      float* out = reinterpret_cast<float*>(vpart_done + (size_t)(item_done_id - ib_done) * Ne_pad * NC);
       for (int idx = tid; idx < MT * 512; idx += 256) {
         const int mt = idx >> 9, e_g = mt / MTW, mtw = mt - e_g * MTW, rest = idx & 511;
         float sum = 0.f;
@@ -363,19 +366,8 @@ static size_t fwd_split_lds_bytes() {
 // ======================================================================
 // adjoint
 // ======================================================================
-struct AdjSplitArgs {
-  const float2* h;
-  const float2* P;
-  const float2* Wc;        // [n_chunks][NEP][NC], zero in unused columns and padded elements
-  const int* ext;
-  const Chunk* chunks;
-  const int* chunk_pairs;  // [n_chunks][NC], zero row past ncols
-  int n_chunks, Nx, N_pts, N_e, N_ext, N_rhs;
-  int Hp, HR;              // h_ref entries per frequency and per depth row
-  float2* g_out;
-};
-
-// One workgroup = (256-point segment, depth row z); chunks are walked in a
+// One workgroup = (instance, 256-point segment, depth row z); the workgroups
+// of instance k are [wg_begin_s_k, wg_begin_s_{k+1}).  Chunks are walked in a
 // fixed order with gamma_hat kept in registers (no atomics).
 // For Np = 4 the segment's points split into 4 phases ph = i mod 4 of 64
 // points s; for one phase the adjoint is the Toeplitz product
@@ -388,9 +380,9 @@ struct AdjSplitArgs {
 //
 // LDS: Ar[comp][ph][copy][LR] (u16), Wt[comp][c][NEP + 8] (u16), Ps[c][SEG + 4] (float2)
 template <int MT>
-__global__ __launch_bounds__(256, 2) void k_adj_s2d_split(AdjSplitArgs A, const Chunk* __restrict__ chunks,
-                                                          const int* __restrict__ chunk_pairs,
-                                                          float2* __restrict__ g_out) {
+__global__ __launch_bounds__(256, 2) void k_adj_s2d_split(const Inst* __restrict__ inst, int n_inst, OutPtrs outs,
+                                                          const Chunk* __restrict__ chunks,
+                                                          const int* __restrict__ chunk_pairs) {
   extern __shared__ unsigned short ldsu[];
   constexpr int NP = 4;
   constexpr int NEP = 16 * MT;
@@ -409,18 +401,24 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d_split(AdjSplitArgs A, const 
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int kk = lane >> 4, col = lane & 15;
-  const int z = blockIdx.y;
-  const int i_s = blockIdx.x * SEG;
-  const int t_shift = i_s - (NEP - A.N_e) * NP;  // window entry u holds Hext[u + t_shift]
+  int ki = 0;
+  while (ki + 1 < n_inst && (int)blockIdx.x >= inst[ki + 1].wg_begin_s) ++ki;
+  const Inst Iv = inst[ki];   // loop-invariant: held in scalar registers
+  const Inst* I = &Iv;
+  float2* __restrict__ g_out = outs.p[ki];
+  const int lw = (int)blockIdx.x - I->wg_begin_s;
+  const int z = lw / I->adj_segs_s;
+  const int i_s = (lw - z * I->adj_segs_s) * SEG;
+  const int t_shift = i_s - (NEP - I->N_e) * NP;  // window entry u holds Hext[u + t_shift]
 
   // gather indices of the window (chunk-invariant); hfull: all inside the table
-  const bool hfull = t_shift >= 0 && t_shift + L <= A.N_ext;
+  const bool hfull = t_shift >= 0 && t_shift + L <= I->N_ext;
   int eidx[HPT];
 #pragma unroll
   for (int j = 0; j < HPT; ++j) {
     const int u = tid + 256 * j;
     const int t = u + t_shift;
-    eidx[j] = (u < L && t >= 0 && t < A.N_ext) ? A.ext[t] : -1;
+    eidx[j] = (u < L && t >= 0 && t < I->N_ext) ? I->ext[t] : -1;
   }
   float2 hreg[HPT], wreg[WPT], preg[PPT];
   // Loads of chunk `cidx` from its descriptor (frequency FG, column pairs PR),
@@ -429,21 +427,21 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d_split(AdjSplitArgs A, const 
   // meet zero weights; points past the row end are never written out.
 #define BORN_ADJS_PREFETCH(cidx, FG, PR)                                                         \
   {                                                                                              \
-    const float2* hrow_ = A.h + (size_t)(FG) * A.Hp + (size_t)z * A.HR;                          \
+    const float2* hrow_ = I->h + (size_t)(FG) * I->Hp + (size_t)z * I->HR;                          \
     _Pragma("unroll") for (int j = 0; j < HPT; ++j) hreg[j] = hrow_[max(eidx[j], 0)];           \
-    const float2* wblk_ = A.Wc + (size_t)(cidx) * NEP * NC;                                      \
+    const float2* wblk_ = I->Wc + (size_t)(cidx) * NEP * NC;                                      \
     _Pragma("unroll") for (int j = 0; j < WPT; ++j) wreg[j] = wblk_[tid + 256 * j];             \
-    const int gi_ = min(i_s + tid, A.Nx - 1);                                                    \
+    const int gi_ = min(i_s + tid, I->Nx - 1);                                                    \
     _Pragma("unroll") for (int j = 0; j < PPT; ++j)                                             \
-      preg[j] = A.P[(size_t)(PR)[j] * A.N_pts + (size_t)z * A.Nx + gi_];                         \
+      preg[j] = I->P[(size_t)(PR)[j] * I->N_pts + (size_t)z * I->Nx + gi_];                         \
   }
   // descriptor of a chunk: uniform, so it lives in scalar registers
 #define BORN_ADJS_DESC(cidx, FG, R, PR)                                                          \
   {                                                                                              \
-    const Chunk c_ = chunks[cidx];                                                               \
+    const Chunk c_ = chunks[I->chunk_base + (cidx)];                                             \
     FG = c_.fg;                                                                                  \
     R = c_.r;                                                                                    \
-    _Pragma("unroll") for (int q = 0; q < NC; ++q) (PR)[q] = chunk_pairs[(cidx) * NC + q]; \
+    _Pragma("unroll") for (int q = 0; q < NC; ++q) (PR)[q] = chunk_pairs[(I->chunk_base + (cidx)) * NC + q]; \
   }
 
   int fg0, r0, pr0[NC];
@@ -452,7 +450,7 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d_split(AdjSplitArgs A, const 
   int nfg = fg0, nr = r0, npr[NC];
 #pragma unroll
   for (int q = 0; q < NC; ++q) npr[q] = pr0[q];
-  if (A.n_chunks > 1) BORN_ADJS_DESC(1, nfg, nr, npr)
+  if (I->n_chunks > 1) BORN_ADJS_DESC(1, nfg, nr, npr)
   int cur_r = r0;
 
   // gamma_hat partials: tile s0 (0..3), v (0..3) -> point s = s0 + 4*(4*kk + v), i = 4 s + ph
@@ -463,7 +461,7 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d_split(AdjSplitArgs A, const 
     for (int v = 0; v < 4; ++v) gx[a][v] = gy[a][v] = 0.f;
 
   const int ph = w;
-  for (int ci = 0; ci < A.n_chunks; ++ci) {
+  for (int ci = 0; ci < I->n_chunks; ++ci) {
     __syncthreads();
     // window entry u (point offset i = u - NP*(NEP-1) + NP*e ... ) -> phase rows:
     // u = (i - i_s) + NP*(NEP - 1 - e); with i - i_s = NP*s + ph:
@@ -517,10 +515,10 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d_split(AdjSplitArgs A, const 
     }
     __syncthreads();
     const int r = cur_r;
-    const bool has_next = ci + 1 < A.n_chunks;
+    const bool has_next = ci + 1 < I->n_chunks;
     const int next_r = nr;
     if (has_next) BORN_ADJS_PREFETCH(ci + 1, nfg, npr)
-    if (ci + 2 < A.n_chunks) BORN_ADJS_DESC(ci + 2, nfg, nr, npr)
+    if (ci + 2 < I->n_chunks) BORN_ADJS_DESC(ci + 2, nfg, nr, npr)
 
     // Y for the 4 tiles of this phase: conj(H) W with conj folded into the
     // table (imag part stored negated): Re = Ar Wr - Ai' Wi, Im = Ar Wi + Ai' Wr
@@ -608,7 +606,7 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d_split(AdjSplitArgs A, const 
           }
           const int s = a + 4 * (4 * kk + col);
           const int i = i_s + NP * s + ph;
-          if (i < A.Nx) g_out[(size_t)r * A.N_pts + (size_t)z * A.Nx + i] = make_float2(x, y);
+          if (i < I->Nx) g_out[(size_t)r * I->N_pts + (size_t)z * I->Nx + i] = make_float2(x, y);
         }
       }
 #pragma unroll
@@ -639,35 +637,20 @@ bool fwd_split_supported(const Plan& P) {
   return P.structured && P.Np == 4 && P.Ne_pad >= 32 && P.Ne_pad <= 128;
 }
 
-void launch_fwd_split(const Plan& P, const float2* gm, int precision, float, float, hipStream_t s) {
-  (void)precision;
-  FwdSplitArgs a;
-  a.h = P.d_h;
-  a.P = P.d_p;
-  a.gm = gm;
-  a.ext = P.d_ext;
-  a.Nx = P.Nx;
-  a.Nz = P.Nz;
-  a.N_pts = P.N_pts;
-  a.Hp = P.H_plane;
-  a.HR = P.HR;
-  a.N_e = P.N_e;
-  a.N_ext = P.N_ext;
-  a.NZB = P.NZB_s;
-  a.ZR = P.ZR_s;
-  a.nseg = P.nseg_s;
-  a.n_items = (int)P.chunks.size() * P.NZB_s * P.nseg_s;
-  a.Ne_pad = P.Ne_pad;
-  const int grid = std::min(a.n_items, 2 * P.n_cu);
-  const bool even = (P.Nx % 2) == 0;
+void launch_fwd_split(const LaunchSet& L, int n_cu, hipStream_t s) {
+  const int grid = std::max(1, std::min(L.n_items_s, L.grid_mult * n_cu));
+  bool even = true;
+  for (const Inst& I : L.inst) even = even && (I.Nx % 2) == 0;
+  const int n_inst = (int)L.inst.size();
+  const int Ne_pad = 16 * L.MT;
 #define BORN_FWDS(MTv)                                                                               \
   if (even)                                                                                          \
-    hipLaunchKernelGGL((k_fwd_s2d_split<MTv, true>), dim3(grid), dim3(256), fwd_split_lds_bytes<MTv>(), s, a, \
-                       P.d_chunks, P.d_chunk_pairs, P.d_vpart);                                        \
+    hipLaunchKernelGGL((k_fwd_s2d_split<MTv, true>), dim3(grid), dim3(256), fwd_split_lds_bytes<MTv>(), s, \
+                       L.d_inst, n_inst, L.n_items_s, Ne_pad, L.d_chunks, L.d_chunk_pairs);            \
   else                                                                                               \
-    hipLaunchKernelGGL((k_fwd_s2d_split<MTv, false>), dim3(grid), dim3(256), fwd_split_lds_bytes<MTv>(), s, a, \
-                       P.d_chunks, P.d_chunk_pairs, P.d_vpart);
-  switch (P.Ne_pad / 16) {
+    hipLaunchKernelGGL((k_fwd_s2d_split<MTv, false>), dim3(grid), dim3(256), fwd_split_lds_bytes<MTv>(), s, \
+                       L.d_inst, n_inst, L.n_items_s, Ne_pad, L.d_chunks, L.d_chunk_pairs);
+  switch (L.MT) {
     case 2: BORN_FWDS(2) break;
     case 4: BORN_FWDS(4) break;
     default: BORN_FWDS(8) break;
@@ -675,28 +658,13 @@ void launch_fwd_split(const Plan& P, const float2* gm, int precision, float, flo
 #undef BORN_FWDS
 }
 
-void launch_adj_split(const Plan& P, int N_rhs, float2* g_out, hipStream_t s) {
-  AdjSplitArgs a;
-  a.h = P.d_h;
-  a.P = P.d_p;
-  a.Wc = P.d_Wc;
-  a.ext = P.d_ext;
-  a.chunks = P.d_chunks;
-  a.chunk_pairs = P.d_chunk_pairs;
-  a.n_chunks = (int)P.chunks.size();
-  a.Nx = P.Nx;
-  a.N_pts = P.N_pts;
-  a.Hp = P.H_plane;
-  a.HR = P.HR;
-  a.N_e = P.N_e;
-  a.N_ext = P.N_ext;
-  a.N_rhs = N_rhs;
-  a.g_out = g_out;
-  const dim3 grid((P.Nx + SEG - 1) / SEG, P.Nz);
-  switch (P.Ne_pad / 16) {
-    case 2: hipLaunchKernelGGL((k_adj_s2d_split<2>), grid, dim3(256), adj_split_lds_bytes<2>(), s, a, a.chunks, a.chunk_pairs, a.g_out); break;
-    case 4: hipLaunchKernelGGL((k_adj_s2d_split<4>), grid, dim3(256), adj_split_lds_bytes<4>(), s, a, a.chunks, a.chunk_pairs, a.g_out); break;
-    default: hipLaunchKernelGGL((k_adj_s2d_split<8>), grid, dim3(256), adj_split_lds_bytes<8>(), s, a, a.chunks, a.chunk_pairs, a.g_out); break;
+void launch_adj_split(const LaunchSet& L, const OutPtrs& outs, hipStream_t s) {
+  const dim3 grid(L.n_wg_s);
+  const int n_inst = (int)L.inst.size();
+  switch (L.MT) {
+    case 2: hipLaunchKernelGGL((k_adj_s2d_split<2>), grid, dim3(256), adj_split_lds_bytes<2>(), s, L.d_inst, n_inst, outs, L.d_chunks, L.d_chunk_pairs); break;
+    case 4: hipLaunchKernelGGL((k_adj_s2d_split<4>), grid, dim3(256), adj_split_lds_bytes<4>(), s, L.d_inst, n_inst, outs, L.d_chunks, L.d_chunk_pairs); break;
+    default: hipLaunchKernelGGL((k_adj_s2d_split<8>), grid, dim3(256), adj_split_lds_bytes<8>(), s, L.d_inst, n_inst, outs, L.d_chunks, L.d_chunk_pairs); break;
   }
 }
 

@@ -36,76 +36,75 @@ __device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
 }  // namespace
 
 // ------------------------------------------------------------------ forward
-struct FwdArgs {
-  const float2* h;
-  const float2* P;
-  const float2* gm;
-  const int* ext;
-  int Nx, Nz, N_pts, N_e, Np, N_ext, NZB, ZR, S, MT, n_items, Ne_pad;
-  int Hp, HR;   // h_ref entries per frequency and per depth row
-  float2* vpart;
-};
-
-// Work item = (chunk, z-block of ZR rows); workgroups are persistent and take
-// items blockIdx.x, blockIdx.x + gridDim.x, ...  Within a row the four waves
-// split the work as EW element groups x PW point groups (2 x 2 when MT >= 2):
-// wave w holds element tiles [eg*MTW, (eg+1)*MTW) x 16 columns and walks the
-// K-steps [pg*SW, (pg+1)*SW) of the row.  NP = points per pitch (4; 0 =
-// runtime), PPT = row points per thread in the B-tile fill.
+// Work item = (instance, chunk, z-block of ZR rows); workgroups are
+// persistent and take items blockIdx.x, blockIdx.x + gridDim.x, ... (the
+// items of instance k are [item_begin_k, item_begin_{k+1})).  Within a row
+// the four waves split the work as EW element groups x PW point groups (2 x 2
+// when MT >= 2): wave w holds element tiles [eg*MTW, (eg+1)*MTW) x 16 columns
+// and walks the K-steps [pg*SW, (pg+1)*SW) of the row.  NP = points per pitch
+// (4; 0 = runtime), PPT = row points per thread in the B-tile fill.
+//
+// The LDS layout follows the instance (row length), so the layout of the
+// row held in LDS (used by the MFMA phase) and that of the prefetched row
+// (used by the next fill) are tracked separately.
 template <int MT, int NP, int PPT>
-__global__ __launch_bounds__(256, 2) void k_fwd_s2d(FwdArgs A, const Chunk* __restrict__ chunks,
-                                                    const int* __restrict__ chunk_pairs,
-                                                    float2* __restrict__ vpart) {
+__global__ __launch_bounds__(256, 2) void k_fwd_s2d(const Inst* __restrict__ inst, int n_inst, int n_items,
+                                                    int Ne_pad, const Chunk* __restrict__ chunks,
+                                                    const int* __restrict__ chunk_pairs) {
   extern __shared__ float2 lds[];
-  const int Np = NP ? NP : A.Np;
   // entries of the polyphase copy per thread: XP <= 256*PPT, QL = (XP-1)/Np + 16*MT
   constexpr int HPT = NP ? PPT + (16 * MT * NP + 255) / 256 : HPT_MAX;
-  const int XP = 16 * A.S;
-  const int QL = (XP - 1) / Np + 16 * MT;
-  const int HN = Np * QL;
-  float2* Hp = lds;            // [Np][QL] polyphase extended row
-  float2* Bs = lds + HN;       // [XP][NCS]
   constexpr int EW = MT >= 2 ? 2 : 1;        // element groups
   constexpr int PW = 4 / EW;                 // point groups
   constexpr int MTW = MT / EW;               // element tiles per wave
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int eg = w % EW, pg = w / EW;
-  const int SW = 4 * A.S / PW;               // K-steps per wave and row
   const int kk = lane >> 4, col = lane & 15;
-  const int Qoff = 16 * MT - A.N_e;
-  const int n_my = (A.n_items - (int)blockIdx.x + (int)gridDim.x - 1) / (int)gridDim.x;
+  const int n_my = (n_items - (int)blockIdx.x + (int)gridDim.x - 1) / (int)gridDim.x;
   if (n_my <= 0) return;
 
-  // row-invariant gather indices of the polyphase copy
+  // instance of the prefetched row ("n") and of the row in LDS ("c")
+  int k = 0;
+  Inst Iv = inst[0];            // record of the instance of the prefetched row
+  const Inst* I = &Iv;
+  int Np, XP, QL, HN;           // layout of the prefetched row
   int eidx[HPT];
-#pragma unroll
-  for (int j = 0; j < HPT; ++j) {
-    const int idx = tid + 256 * j;
-    eidx[j] = -1;
-    if (idx < HN) {
-      const int ph = idx / QL, q = idx - ph * QL;
-      const int t = (q - Qoff) * Np + ph;
-      if (t >= 0 && t < A.N_ext) eidx[j] = A.ext[t];
-    }
-  }
+  int fg, rr, pr[NC];           // chunk of the prefetched row
+  int z, z_end, item = blockIdx.x;
 
-  // current step
-  int it = 0;
-  int item = blockIdx.x;
-  int z = (item % A.NZB) * A.ZR;
-  int z_end = min(A.Nz, z + A.ZR);
+#define BORN_FWD_INST()                                                                   \
+  {                                                                                       \
+    while (k + 1 < n_inst && item >= inst[k + 1].item_begin) ++k;                         \
+    Iv = inst[k];                                                                         \
+    Np = NP ? NP : I->Np;                                                                 \
+    XP = 16 * I->S;                                                                       \
+    QL = (XP - 1) / Np + 16 * MT;                                                         \
+    HN = Np * QL;                                                                         \
+    const int Qoff_ = 16 * MT - I->N_e;                                                   \
+    _Pragma("unroll") for (int j = 0; j < HPT; ++j) {                                    \
+      const int idx_ = tid + 256 * j;                                                     \
+      eidx[j] = -1;                                                                       \
+      if (idx_ < HN) {                                                                    \
+        const int ph_ = idx_ / QL, q_ = idx_ - ph_ * QL;                                  \
+        const int t_ = (q_ - Qoff_) * Np + ph_;                                           \
+        if (t_ >= 0 && t_ < I->N_ext) eidx[j] = I->ext[t_];                               \
+      }                                                                                   \
+    }                                                                                     \
+    _Pragma("unroll") for (int j = 0; j < HPT; ++j) asm volatile("" ::"v"(eidx[j]));     \
+  }
   // descriptor of the item's chunk (uniform: scalar registers); unused
   // columns read the zero row of the p_inc table
-  int fg, rr, pr[NC];
-#define BORN_FWD_DESC(itm)                                                                \
+#define BORN_FWD_DESC()                                                                   \
   {                                                                                       \
-    const int ci_ = (itm) / A.NZB;                                                        \
-    const Chunk c_ = chunks[ci_];                                                         \
+    const int li_ = item - I->item_begin;                                                 \
+    const int ci_ = li_ / I->NZB;                                                         \
+    const Chunk c_ = chunks[I->chunk_base + ci_];                                         \
     fg = c_.fg;                                                                           \
     rr = c_.r;                                                                            \
-    _Pragma("unroll") for (int q = 0; q < NC; ++q) pr[q] = chunk_pairs[ci_ * NC + q]; \
+    _Pragma("unroll") for (int q = 0; q < NC; ++q) pr[q] = chunk_pairs[(I->chunk_base + ci_) * NC + q]; \
+    z = (li_ - ci_ * I->NZB) * I->ZR;                                                     \
+    z_end = min(I->Nz, z + I->ZR);                                                        \
   }
-  BORN_FWD_DESC(item)
 
   float2 hreg[HPT];
   float2 praw[PPT][NC];
@@ -115,19 +114,22 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(FwdArgs A, const Chunk* __re
   // address is valid (clamped), out-of-range entries are zeroed in the fill
 #define BORN_FWD_PREFETCH(zz)                                                           \
   {                                                                                     \
-    const float2* hrow_ = A.h + (size_t)fg * A.Hp + (size_t)(zz) * A.HR;                \
+    const float2* hrow_ = I->h + (size_t)fg * I->Hp + (size_t)(zz) * I->HR;             \
     _Pragma("unroll") for (int j = 0; j < HPT; ++j) hreg[j] = hrow_[max(eidx[j], 0)];  \
-    const float2* grow_ = A.gm + (size_t)rr * A.N_pts + (size_t)(zz) * A.Nx;            \
+    const size_t rowo_ = (size_t)(zz) * I->Nx;                                          \
+    const float2* grow_ = I->gm + (size_t)rr * I->N_pts + rowo_;                        \
     _Pragma("unroll") for (int j = 0; j < PPT; ++j) {                                   \
-      const int i_ = min(tid + 256 * j, A.Nx - 1);                                      \
+      const int i_ = min(tid + 256 * j, I->Nx - 1);                                     \
       greg[j] = grow_[i_];                                                              \
       _Pragma("unroll") for (int c_ = 0; c_ < NC; ++c_)                                 \
-        praw[j][c_] = A.P[(size_t)pr[c_] * A.N_pts + (size_t)(zz) * A.Nx + i_];         \
+        praw[j][c_] = I->P[(size_t)pr[c_] * I->N_pts + rowo_ + i_];                     \
     }                                                                                   \
   }
 
+  BORN_FWD_INST()
+  BORN_FWD_DESC()
   BORN_FWD_PREFETCH(z)
-  const bool xfull = XP == A.Nx;   // no padded points in the B tile
+  int n_Nx = I->Nx;              // row length of the prefetched row
 
   v4f SR[MTW], SI[MTW];
 #pragma unroll
@@ -135,32 +137,30 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(FwdArgs A, const Chunk* __re
     SR[mt] = vzero();
     SI[mt] = vzero();
   }
-
-  // per-lane LDS bases of the MFMA operands
-  int a_base, a_step;
-  if (NP == 4 || NP == 2 || NP == 1) {
-    a_base = (kk % Np) * QL + kk / Np + 16 * MT - 1 - col - 16 * MTW * eg;
-    a_step = 4 / Np;
-  } else {
-    a_base = 0;
-    a_step = 0;
-  }
   const int b_base = kk * NCS + col;
+  int it = 0;
 
   while (true) {
+    // layout of the row about to be written to LDS
+    const int cNp = Np, cQL = QL, cHN = HN, cXP = XP;
+    float2* Hp = lds;              // [Np][QL] polyphase extended row
+    float2* Bs = lds + cHN;        // [XP][NCS]
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < HPT; ++j) {
       const int idx = tid + 256 * j;
-      if (idx < HN) Hp[idx] = eidx[j] >= 0 ? hreg[j] : make_float2(0.f, 0.f);
+      if (idx < cHN) Hp[idx] = eidx[j] >= 0 ? hreg[j] : make_float2(0.f, 0.f);
     }
+    {
+      const bool xfull = cXP == n_Nx;   // no padded points in the B tile
 #pragma unroll
-    for (int j = 0; j < PPT; ++j) {
-      const int i = tid + 256 * j;
-      if (i < XP) {
-        const float2 g = (xfull || i < A.Nx) ? greg[j] : make_float2(0.f, 0.f);
+      for (int j = 0; j < PPT; ++j) {
+        const int i = tid + 256 * j;
+        if (i < cXP) {
+          const float2 g = (xfull || i < n_Nx) ? greg[j] : make_float2(0.f, 0.f);
 #pragma unroll
-        for (int c = 0; c < NC; ++c) Bs[i * NCS + c] = cmulf(praw[j][c], g);
+          for (int c = 0; c < NC; ++c) Bs[i * NCS + c] = cmulf(praw[j][c], g);
+        }
       }
     }
     __syncthreads();
@@ -168,6 +168,8 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(FwdArgs A, const Chunk* __re
     // next step and its prefetch
     bool has_next = true;
     const int item_cur = item;
+    float2* const vpart_cur = Iv.vpart;
+    const int ib_cur = Iv.item_begin;
     const bool item_done = z + 1 >= z_end;
     if (!item_done) {
       ++z;
@@ -175,13 +177,24 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(FwdArgs A, const Chunk* __re
       has_next = false;
     } else {
       item = blockIdx.x + (it + 1) * gridDim.x;
-      BORN_FWD_DESC(item)
-      z = (item % A.NZB) * A.ZR;
-      z_end = min(A.Nz, z + A.ZR);
+      if (k + 1 < n_inst && item >= inst[k + 1].item_begin) BORN_FWD_INST()
+      BORN_FWD_DESC()
     }
-    if (has_next) BORN_FWD_PREFETCH(z)
+    if (has_next) {
+      BORN_FWD_PREFETCH(z)
+      n_Nx = I->Nx;
+    }
 
-    // MFMA over this wave's points of the row
+    // MFMA over this wave's points of the row in LDS
+    const int SW = cXP / (4 * PW);               // K-steps per wave and row
+    int a_base, a_step;
+    if (NP == 4 || NP == 2 || NP == 1) {
+      a_base = (kk % cNp) * cQL + kk / cNp + 16 * MT - 1 - col - 16 * MTW * eg;
+      a_step = 4 / cNp;
+    } else {
+      a_base = 0;
+      a_step = 0;
+    }
     v4f P1[MTW], P2[MTW], P3[MTW];
 #pragma unroll
     for (int mt = 0; mt < MTW; ++mt) {
@@ -198,7 +211,7 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(FwdArgs A, const Chunk* __re
         ai = a_base + a_step * ks;
       } else {
         const int i = 4 * ks + kk;
-        ai = (i % Np) * QL + i / Np + 16 * MT - 1 - col - 16 * MTW * eg;
+        ai = (i % cNp) * cQL + i / cNp + 16 * MT - 1 - col - 16 * MTW * eg;
       }
 #pragma unroll
       for (int mt = 0; mt < MTW; ++mt) {
@@ -231,8 +244,8 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(FwdArgs A, const Chunk* __re
         SI[mt] = vzero();
       }
       __syncthreads();
-      const int ci = item_cur / A.NZB, zb = item_cur - ci * A.NZB;
-      float* out = reinterpret_cast<float*>(vpart + ((size_t)ci * A.NZB + zb) * A.Ne_pad * NC);
+      const int li = item_cur - ib_cur;
+      float* out = reinterpret_cast<float*>(vpart_cur + (size_t)li * Ne_pad * NC);   // [chunk][zb] = li
       for (int idx = tid; idx < MT * 512; idx += 256) {
         const int mt = idx >> 9, e_g = mt / MTW, mtw = mt - e_g * MTW, rest = idx & 511;
         float sum = 0.f;
@@ -248,6 +261,7 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(FwdArgs A, const Chunk* __re
   }
 #undef BORN_FWD_PREFETCH
 #undef BORN_FWD_DESC
+#undef BORN_FWD_INST
 }
 
 static size_t fwd_lds_bytes(int MT, int S, int Np) {
@@ -266,46 +280,30 @@ bool fwd_structured_fits(int Nx, int Ne_pad, int Np) {
          Ne_pad <= 128;
 }
 
-void launch_fwd_structured(const Plan& P, const float2* gm, int N_rhs, hipStream_t s) {
-  (void)N_rhs;
-  FwdArgs a;
-  a.h = P.d_h;
-  a.P = P.d_p;
-  a.gm = gm;
-  a.ext = P.d_ext;
-  a.Nx = P.Nx;
-  a.Nz = P.Nz;
-  a.N_pts = P.N_pts;
-  a.Hp = P.H_plane;
-  a.HR = P.HR;
-  a.N_e = P.N_e;
-  a.Np = P.Np;
-  a.N_ext = P.N_ext;
-  a.NZB = P.NZB;
-  a.ZR = P.ZR;
-  a.S = (P.Nx + 15) / 16;
-  a.MT = P.Ne_pad / 16;
-  a.Ne_pad = P.Ne_pad;
-  a.n_items = (int)P.chunks.size() * P.NZB;
-  a.vpart = P.d_vpart;
-  const int XP = 16 * a.S;
-  const int PPT = (XP + 255) / 256;
-  const size_t lds = fwd_lds_bytes(a.MT, a.S, a.Np);
-  const int grid = std::min(a.n_items, 2 * P.n_cu);   // two resident workgroups per CU
+int fwd_ppt(int S) {
+  const int XP = 16 * S;
+  return XP <= 256 ? 1 : (XP <= 512 ? 2 : 4);
+}
+
+void launch_fwd_structured(const LaunchSet& L, int n_cu, hipStream_t s) {
+  const int PPT = fwd_ppt(L.S_max);
+  const size_t lds = fwd_lds_bytes(L.MT, L.S_max, L.Np);
+  const int grid = std::max(1, std::min(L.n_items, L.grid_mult * n_cu));
+  const int Ne_pad = 16 * L.MT;
 #define BORN_FWD_LAUNCH(MTv, NPv, PPTv) \
-  hipLaunchKernelGGL((k_fwd_s2d<MTv, NPv, PPTv>), dim3(grid), dim3(256), lds, s, a, P.d_chunks, P.d_chunk_pairs, \
-                     P.d_vpart)
+  hipLaunchKernelGGL((k_fwd_s2d<MTv, NPv, PPTv>), dim3(grid), dim3(256), lds, s, L.d_inst, (int)L.inst.size(), \
+                     L.n_items, Ne_pad, L.d_chunks, L.d_chunk_pairs)
 #define BORN_FWD_PPT(MTv, NPv)                    \
   if (PPT <= 1) BORN_FWD_LAUNCH(MTv, NPv, 1);     \
   else if (PPT <= 2) BORN_FWD_LAUNCH(MTv, NPv, 2); \
   else BORN_FWD_LAUNCH(MTv, NPv, 4);
 #define BORN_FWD_NP(MTv)       \
-  if (a.Np == 4) {             \
+  if (L.Np == 4) {             \
     BORN_FWD_PPT(MTv, 4)       \
   } else {                     \
     BORN_FWD_PPT(MTv, 0)       \
   }
-  switch (a.MT) {
+  switch (L.MT) {
     case 1: BORN_FWD_NP(1) break;
     case 2: BORN_FWD_NP(2) break;
     case 4: BORN_FWD_NP(4) break;
@@ -317,24 +315,16 @@ void launch_fwd_structured(const Plan& P, const float2* gm, int N_rhs, hipStream
 }
 
 // ------------------------------------------------------------------ adjoint
-struct AdjArgs {
-  const float2* h;
-  const float2* P;
-  const float2* Wc;   // [n_chunks][Ne_pad][NC], zero in unused columns and padded elements
-  const int* ext;
-  int n_chunks, Nx, N_pts, N_e, Ne_pad, Np, N_ext, N_rhs;
-  int Hp, HR;   // h_ref entries per frequency and per depth row
-};
-
-// One workgroup = (lateral segment of 64*MW points, depth row z).  It walks
+// One workgroup = (instance, lateral segment of 64*MW points, depth row z);
+// the workgroups of instance k are [wg_begin_k, wg_begin_{k+1}).  It walks
 // every chunk (rhs-major, then frequency) in a fixed order and keeps its
 // output in registers: no atomics, bitwise repeatable.  Per chunk
 //   Y[i][c] = sum_e conj(Hext[i - e*Np + off]) W[e][c]       (MFMA, A = points x elements)
 //   gamma_hat[i] += sum_c conj(p_inc[pair_c][i]) Y[i][c]     (registers)
 template <int MT, int MW, int NP>
-__global__ __launch_bounds__(256, 2) void k_adj_s2d(AdjArgs A, const Chunk* __restrict__ chunks,
-                                                    const int* __restrict__ chunk_pairs,
-                                                    float2* __restrict__ g_out) {
+__global__ __launch_bounds__(256, 2) void k_adj_s2d(const Inst* __restrict__ inst, int n_inst, OutPtrs outs,
+                                                    const Chunk* __restrict__ chunks,
+                                                    const int* __restrict__ chunk_pairs) {
   extern __shared__ float2 lds[];
   constexpr int SEG = 64 * MW;
   constexpr int PS = SEG + 4;                 // padded stride of the p_inc tile
@@ -342,7 +332,12 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d(AdjArgs A, const Chunk* __re
   constexpr int NEP = 16 * MT;                // padded element count
   constexpr int HPT = NP ? (SEG + (NEP - 1) * NP + 255) / 256 : HPT_MAX;
   constexpr int WPT = MT;                     // W tile entries per thread
-  const int Np = NP ? NP : A.Np;
+  int k = 0;
+  while (k + 1 < n_inst && (int)blockIdx.x >= inst[k + 1].wg_begin) ++k;
+  const Inst Iv = inst[k];    // loop-invariant: held in scalar registers
+  const Inst* I = &Iv;
+  float2* __restrict__ g_out = outs.p[k];
+  const int Np = NP ? NP : I->Np;
   const int L = SEG + (NEP - 1) * Np;
   float2* He = lds;                           // [L]
   float2* Ws = He + L;                        // [NEP][NCS]
@@ -350,10 +345,11 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d(AdjArgs A, const Chunk* __re
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int kk = lane >> 4, col = lane & 15;
-  const int z = blockIdx.y;
-  const int i_s = blockIdx.x * SEG;
+  const int lw = (int)blockIdx.x - I->wg_begin;
+  const int z = lw / I->adj_segs;
+  const int i_s = (lw - z * I->adj_segs) * SEG;
   const int ib = i_s + w * 16 * MW;
-  const int t_shift = i_s - (NEP - A.N_e) * Np;
+  const int t_shift = i_s - (NEP - I->N_e) * Np;
 
   // row-invariant gather indices of the extended-row window
   int eidx[HPT];
@@ -361,7 +357,7 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d(AdjArgs A, const Chunk* __re
   for (int j = 0; j < HPT; ++j) {
     const int u = tid + 256 * j;
     const int t = u + t_shift;
-    eidx[j] = (u < L && t >= 0 && t < A.N_ext) ? A.ext[t] : -1;
+    eidx[j] = (u < L && t >= 0 && t < I->N_ext) ? I->ext[t] : -1;
   }
 
   float2 hreg[HPT], wreg[WPT], preg[PPT];
@@ -371,23 +367,23 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d(AdjArgs A, const Chunk* __re
   // the row end are never written out.
 #define BORN_ADJ_PREFETCH(cidx, FG, PV)                                                          \
   {                                                                                              \
-    const float2* hrow_ = A.h + (size_t)(FG) * A.Hp + (size_t)z * A.HR;                          \
+    const float2* hrow_ = I->h + (size_t)(FG) * I->Hp + (size_t)z * I->HR;                          \
     _Pragma("unroll") for (int j = 0; j < HPT; ++j) hreg[j] = hrow_[max(eidx[j], 0)];           \
-    const float2* wblk_ = A.Wc + (size_t)(cidx) * NEP * NC;                                      \
+    const float2* wblk_ = I->Wc + (size_t)(cidx) * NEP * NC;                                      \
     _Pragma("unroll") for (int j = 0; j < WPT; ++j) wreg[j] = wblk_[tid + 256 * j];             \
     _Pragma("unroll") for (int j = 0; j < PPT; ++j) {                                           \
       const int idx_ = tid + 256 * j;                                                            \
-      const int gi_ = min(i_s + idx_ % SEG, A.Nx - 1);                                           \
-      preg[j] = A.P[(size_t)(PV)[j] * A.N_pts + (size_t)z * A.Nx + gi_];                         \
+      const int gi_ = min(i_s + idx_ % SEG, I->Nx - 1);                                           \
+      preg[j] = I->P[(size_t)(PV)[j] * I->N_pts + (size_t)z * I->Nx + gi_];                         \
     }                                                                                            \
   }
 #define BORN_ADJ_DESC(cidx, FG, R, PV)                                                           \
   {                                                                                              \
-    const Chunk c_ = chunks[cidx];                                                               \
+    const Chunk c_ = chunks[I->chunk_base + (cidx)];                                             \
     FG = c_.fg;                                                                                  \
     R = c_.r;                                                                                    \
     _Pragma("unroll") for (int j = 0; j < PPT; ++j)                                             \
-      (PV)[j] = chunk_pairs[(cidx) * NC + (tid + 256 * j) / SEG];                        \
+      (PV)[j] = chunk_pairs[(I->chunk_base + (cidx)) * NC + (tid + 256 * j) / SEG];              \
   }
 
   int fg0, r0, pv0[PPT];
@@ -396,7 +392,7 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d(AdjArgs A, const Chunk* __re
   int nfg = fg0, nr = r0, npv[PPT];
 #pragma unroll
   for (int j = 0; j < PPT; ++j) npv[j] = pv0[j];
-  if (A.n_chunks > 1) BORN_ADJ_DESC(1, nfg, nr, npv)
+  if (I->n_chunks > 1) BORN_ADJ_DESC(1, nfg, nr, npv)
   int cur_r = r0;
 
   float gx[MW][4], gy[MW][4];
@@ -407,7 +403,7 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d(AdjArgs A, const Chunk* __re
 
   const int u_base = ib - i_s + col + (NEP - 1) * Np;
 
-  for (int ci = 0; ci < A.n_chunks; ++ci) {
+  for (int ci = 0; ci < I->n_chunks; ++ci) {
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < HPT; ++j) {
@@ -424,10 +420,10 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d(AdjArgs A, const Chunk* __re
     }
     __syncthreads();
     const int r = cur_r;
-    const bool has_next = ci + 1 < A.n_chunks;
+    const bool has_next = ci + 1 < I->n_chunks;
     const int next_r = nr;
     if (has_next) BORN_ADJ_PREFETCH(ci + 1, nfg, npv)
-    if (ci + 2 < A.n_chunks) BORN_ADJ_DESC(ci + 2, nfg, nr, npv)
+    if (ci + 2 < I->n_chunks) BORN_ADJ_DESC(ci + 2, nfg, nr, npv)
 
     v4f P1[MW], P2[MW], P3[MW];
 #pragma unroll
@@ -491,7 +487,7 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d(AdjArgs A, const Chunk* __re
               y = gy[mw][v];
             }
           }
-          if (i < A.Nx) g_out[(size_t)r * A.N_pts + (size_t)z * A.Nx + i] = make_float2(x, y);
+          if (i < I->Nx) g_out[(size_t)r * I->N_pts + (size_t)z * I->Nx + i] = make_float2(x, y);
         }
       }
 #pragma unroll
@@ -505,7 +501,6 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d(AdjArgs A, const Chunk* __re
 #undef BORN_ADJ_DESC
 }
 
-static int adj_mw(int Nx) { return Nx <= 128 ? 2 : 4; }
 
 static size_t adj_lds_bytes(int MW, int Ne_pad, int Np) {
   const int SEG = 64 * MW;
@@ -514,43 +509,30 @@ static size_t adj_lds_bytes(int MW, int Ne_pad, int Np) {
 }
 
 bool adj_structured_fits(int Nx, int Ne_pad, int Np) {
-  const int MW = adj_mw(Nx);
-  const int L = 64 * MW + (Ne_pad - 1) * Np;
-  return L <= 256 * HPT_MAX && Ne_pad <= 16 * WPT_MAX && adj_lds_bytes(MW, Ne_pad, Np) <= 160 * 1024;
+  (void)Nx;
+  const int L = 64 * 4 + (Ne_pad - 1) * Np;   // both segment widths (MW = 2, 4) must fit
+  return L <= 256 * HPT_MAX && Ne_pad <= 16 * WPT_MAX && adj_lds_bytes(4, Ne_pad, Np) <= 160 * 1024;
 }
 
-void launch_adj_structured(const Plan& P, int N_rhs, float2* g_out, hipStream_t s) {
-  AdjArgs a;
-  a.h = P.d_h;
-  a.P = P.d_p;
-  a.Wc = P.d_Wc;
-  a.ext = P.d_ext;
-  a.n_chunks = (int)P.chunks.size();
-  a.Nx = P.Nx;
-  a.N_pts = P.N_pts;
-  a.Hp = P.H_plane;
-  a.HR = P.HR;
-  a.N_e = P.N_e;
-  a.Ne_pad = P.Ne_pad;
-  a.Np = P.Np;
-  a.N_ext = P.N_ext;
-  a.N_rhs = N_rhs;
-  const int MW = adj_mw(P.Nx);
-  const int SEG = 64 * MW;
-  const size_t lds = adj_lds_bytes(MW, P.Ne_pad, P.Np);
-  const dim3 grid((P.Nx + SEG - 1) / SEG, P.Nz);
+int adj_mw_default(int Nx) { return Nx <= 128 ? 2 : 4; }
+
+void launch_adj_structured(const LaunchSet& L, const OutPtrs& outs, hipStream_t s) {
+  const int MW = L.adj_mw;
+  const size_t lds = adj_lds_bytes(MW, 16 * L.MT, L.Np);
+  const dim3 grid(L.n_wg);
 #define BORN_ADJ_LAUNCH(MTv, MWv, NPv) \
-  hipLaunchKernelGGL((k_adj_s2d<MTv, MWv, NPv>), grid, dim3(256), lds, s, a, P.d_chunks, P.d_chunk_pairs, g_out)
+  hipLaunchKernelGGL((k_adj_s2d<MTv, MWv, NPv>), grid, dim3(256), lds, s, L.d_inst, (int)L.inst.size(), outs, \
+                     L.d_chunks, L.d_chunk_pairs)
 #define BORN_ADJ_MW(MTv, NPv)                     \
   if (MW == 2) BORN_ADJ_LAUNCH(MTv, 2, NPv);      \
   else BORN_ADJ_LAUNCH(MTv, 4, NPv);
 #define BORN_ADJ_NP(MTv)        \
-  if (P.Np == 4) {              \
+  if (L.Np == 4) {              \
     BORN_ADJ_MW(MTv, 4)         \
   } else {                      \
     BORN_ADJ_MW(MTv, 0)         \
   }
-  switch (P.Ne_pad / 16) {
+  switch (L.MT) {
     case 1: BORN_ADJ_NP(1) break;
     case 2: BORN_ADJ_NP(2) break;
     case 4: BORN_ADJ_NP(4) break;
