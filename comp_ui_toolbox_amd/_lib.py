@@ -28,6 +28,9 @@ EXPORTED = (
     "born_plan_set_precision", "born_forward", "born_adjoint", "born_forward_dev", "born_adjoint_dev",
     "born_plan_get_table", "born_plan_table_shape", "born_plan_path", "born_plan_force_general", "born_plan_set_timing",
     "born_plan_get_timing", "born_last_error", "born_status_id",
+    "born_batch_create", "born_batch_destroy", "born_batch_forward_dev", "born_batch_adjoint_dev",
+    "born_batch_set_graph", "born_batch_autotune", "born_plan_autotune", "born_batch_set_timing",
+    "born_batch_get_timing",
 )
 
 
@@ -113,6 +116,15 @@ def lib() -> C.CDLL:
     L.born_plan_force_general.argtypes = [vp, C.c_int32]
     L.born_plan_set_timing.argtypes = [vp, C.c_int32]
     L.born_plan_get_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+    L.born_batch_create.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(vp)]
+    L.born_batch_destroy.argtypes = [vp]
+    L.born_batch_forward_dev.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), vp]
+    L.born_batch_adjoint_dev.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), vp]
+    L.born_batch_set_graph.argtypes = [vp, C.c_int32]
+    L.born_batch_autotune.argtypes = [vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.born_plan_autotune.argtypes = [vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.born_batch_set_timing.argtypes = [vp, C.c_int32]
+    L.born_batch_get_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     L.born_last_error.restype = C.c_char_p
     L.born_status_id.restype = C.c_char_p
     L.born_status_id.argtypes = [C.c_int]

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -816,6 +817,288 @@ static void host_adjoint(Plan& P, const double* uin, int N_rhs, double* g) {
   }
 }
 
+// ------------------------------------------------------------- batches
+// A batch runs several plans (operator instances of uneven size) with one
+// persistent contraction launch per direction: the instance records of all
+// plans go into one launch set, the chunk tables are concatenated.  The
+// per-plan masking, reduction, mixing and un-mixing kernels are launched
+// around it.  With graph replay on, the whole sequence is captured once per
+// (stream, buffers) into a hipGraph and replayed.
+struct Batch {
+  int device = 0;
+  int n_cu = 256;
+  int precision = 0;
+  std::vector<Plan*> plans;
+  LaunchSet ls;
+  Chunk* d_chunks = nullptr;
+  int* d_chunk_pairs = nullptr;
+  bool graph = false;
+  struct Captured {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t stream = nullptr;
+    std::vector<const void*> sig;
+  } cap[2];
+  bool timing = false;
+  std::vector<Plan::Timed> timed;
+  // graphs cannot be captured on the legacy default stream: work submitted
+  // there runs on this stream, ordered by events
+  hipStream_t own = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  ~Batch();
+};
+
+static void drop_graph(Batch::Captured& c) {
+  if (c.exec) hipGraphExecDestroy(c.exec);
+  if (c.g) hipGraphDestroy(c.g);
+  c.exec = nullptr;
+  c.g = nullptr;
+  c.sig.clear();
+}
+
+Batch::~Batch() {
+  hipSetDevice(device);
+  for (auto& c : cap) drop_graph(c);
+  for (auto& t : timed) {
+    hipEventDestroy(t.a);
+    hipEventDestroy(t.b);
+  }
+  dfree(ls.d_inst);
+  dfree(d_chunks);
+  dfree(d_chunk_pairs);
+  if (ev_in) hipEventDestroy(ev_in);
+  if (ev_out) hipEventDestroy(ev_out);
+  if (own) {
+    hipStreamSynchronize(own);
+    hipStreamDestroy(own);
+  }
+}
+
+// Instance records, concatenated chunk tables and ranges of the batch.
+static void build_batch_launch(Batch& B) {
+  B.ls.inst.clear();
+  size_t n_chunks = 0;
+  for (Plan* P : B.plans) n_chunks += P->chunks.size();
+  dfree(B.d_chunks);
+  dfree(B.d_chunk_pairs);
+  B.d_chunks = dalloc<Chunk>(n_chunks);
+  B.d_chunk_pairs = dalloc<int>(n_chunks * NC);
+  size_t base = 0;
+  bool small = true;
+  for (Plan* P : B.plans) {
+    const size_t nc = P->chunks.size();
+    HIP_CHECK(hipMemcpy(B.d_chunks + base, P->d_chunks, nc * sizeof(Chunk), hipMemcpyDeviceToDevice));
+    HIP_CHECK(hipMemcpy(B.d_chunk_pairs + base * NC, P->d_chunk_pairs, nc * NC * sizeof(int),
+                        hipMemcpyDeviceToDevice));
+    Inst I = make_inst(*P);
+    I.chunk_base = (int)base;
+    B.ls.inst.push_back(I);
+    base += nc;
+    small = small && P->Nx <= 128;
+  }
+  B.ls.d_chunks = B.d_chunks;
+  B.ls.d_chunk_pairs = B.d_chunk_pairs;
+  B.ls.MT = B.plans[0]->Ne_pad / 16;
+  B.ls.Np = B.plans[0]->Np;
+  if (B.ls.adj_mw != 2 && B.ls.adj_mw != 4) B.ls.adj_mw = small ? 2 : 4;
+  finalize_launchset(B.ls);
+  for (auto& c : B.cap) drop_graph(c);
+}
+
+static void batch_forward_enqueue(Batch& B, const void* const* in, void* const* out, hipStream_t s) {
+  for (size_t i = 0; i < B.plans.size(); ++i) {
+    Plan& P = *B.plans[i];
+    launch_mask_gamma((const float2*)in[i], P.N_pts, 1, P.d_gm, s);
+  }
+  hipEvent_t a = nullptr, b = nullptr;
+  if (B.timing) {
+    HIP_CHECK(hipEventCreate(&a));
+    HIP_CHECK(hipEventCreate(&b));
+    HIP_CHECK(hipEventRecord(a, s));
+  }
+  if (B.precision == BORN_PRECISION_FP32)
+    launch_fwd_structured(B.ls, B.n_cu, s);
+  else
+    launch_fwd_split(B.ls, B.n_cu, s);
+  if (B.timing) {
+    HIP_CHECK(hipEventRecord(b, s));
+    B.timed.push_back({a, b, 0});
+  }
+  for (size_t i = 0; i < B.plans.size(); ++i) {
+    Plan& P = *B.plans[i];
+    launch_fwd_reduce(P, 1, B.precision == BORN_PRECISION_FP32 ? P.NZB : P.NZB_s * P.nseg_s, s);
+    launch_fwd_mix(P, 1, out[i], false, s);
+  }
+  HIP_CHECK(hipGetLastError());
+}
+
+static void batch_adjoint_enqueue(Batch& B, const void* const* in, void* const* out, hipStream_t s) {
+  OutPtrs outs;
+  for (size_t i = 0; i < B.plans.size(); ++i) {
+    Plan& P = *B.plans[i];
+    launch_adj_unmix(P, (const float2*)in[i], 1, true, s);
+    outs.p[i] = (float2*)out[i];
+  }
+  hipEvent_t a = nullptr, b = nullptr;
+  if (B.timing) {
+    HIP_CHECK(hipEventCreate(&a));
+    HIP_CHECK(hipEventCreate(&b));
+    HIP_CHECK(hipEventRecord(a, s));
+  }
+  if (B.precision == BORN_PRECISION_FP32)
+    launch_adj_structured(B.ls, outs, s);
+  else
+    launch_adj_split(B.ls, outs, s);
+  if (B.timing) {
+    HIP_CHECK(hipEventRecord(b, s));
+    B.timed.push_back({a, b, 1});
+  }
+  HIP_CHECK(hipGetLastError());
+}
+
+// Run one direction, replaying a captured graph when the stream and the
+// buffers are those it was captured with.
+static void batch_apply(Batch& B, int dir, const void* const* in, void* const* out, hipStream_t s) {
+  const size_t n = B.plans.size();
+  if (!B.graph || B.timing) {
+    if (dir == 0) batch_forward_enqueue(B, in, out, s);
+    else batch_adjoint_enqueue(B, in, out, s);
+    return;
+  }
+  hipStream_t user = s;
+  if (!user) {
+    HIP_CHECK(hipEventRecord(B.ev_in, nullptr));
+    HIP_CHECK(hipStreamWaitEvent(B.own, B.ev_in, 0));
+    s = B.own;
+  }
+  std::vector<const void*> sig(in, in + n);
+  sig.insert(sig.end(), out, out + n);
+  Batch::Captured& c = B.cap[dir];
+  if (!c.exec || c.stream != s || c.sig != sig) {
+    drop_graph(c);
+    HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+      if (dir == 0) batch_forward_enqueue(B, in, out, s);
+      else batch_adjoint_enqueue(B, in, out, s);
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      hipStreamEndCapture(s, &g);
+      if (g) hipGraphDestroy(g);
+      throw;
+    }
+    HIP_CHECK(hipStreamEndCapture(s, &c.g));
+    HIP_CHECK(hipGraphInstantiate(&c.exec, c.g, nullptr, nullptr, 0));
+    c.stream = s;
+    c.sig = sig;
+  }
+  HIP_CHECK(hipGraphLaunch(c.exec, s));
+  if (!user) {
+    HIP_CHECK(hipEventRecord(B.ev_out, s));
+    HIP_CHECK(hipStreamWaitEvent(nullptr, B.ev_out, 0));
+  }
+}
+
+// Autotuner: times the contraction launches of the candidate work-item
+// counts and persistent grid sizes (forward) and segment widths (plain
+// adjoint), keeps the fastest.  The timed launches read the plans' current
+// workspaces; their results are discarded.
+static float time_launches(const std::function<void()>& f, hipStream_t s, int reps) {
+  hipEvent_t a, b;
+  HIP_CHECK(hipEventCreate(&a));
+  HIP_CHECK(hipEventCreate(&b));
+  f();  // warm-up
+  HIP_CHECK(hipEventRecord(a, s));
+  for (int r = 0; r < reps; ++r) f();
+  HIP_CHECK(hipEventRecord(b, s));
+  HIP_CHECK(hipEventSynchronize(b));
+  float ms = 0.f;
+  HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  return ms / reps;
+}
+
+// New forward blocking for every plan; vpart grows when needed.
+static void set_items_target(std::vector<Plan*>& plans, int target) {
+  for (Plan* P : plans) {
+    if (P->items_target == target) continue;
+    P->items_target = target;
+    const size_t have = P->chunks.size() * (size_t)P->vblk;
+    set_blocking(*P);
+    if (P->chunks.size() * (size_t)P->vblk > have) {
+      HIP_CHECK(hipDeviceSynchronize());
+      dfree(P->d_vpart);
+      P->d_vpart = dalloc<float2>(P->chunks.size() * (size_t)P->vblk * P->Ne_pad * NC);
+    } else {
+      P->vblk = (int)(have / std::max<size_t>(1, P->chunks.size()));   // capacity of the allocation
+    }
+    P->ls.inst.assign(1, make_inst(*P));
+    finalize_launchset(P->ls);
+  }
+}
+
+struct TuneResult {
+  int items, grid_mult, adj_mw;
+  float fwd_ms, adj_ms;
+};
+
+static TuneResult autotune(std::vector<Plan*>& plans, LaunchSet& L, int precision, int n_cu,
+                           const std::function<void()>& rebuild, hipStream_t s) {
+  const int items_c[] = {1024, 2048, 4096, 8192};
+  const int grid_c[] = {1, 2};
+  TuneResult best{plans[0]->items_target, L.grid_mult, L.adj_mw, 1e30f, 1e30f};
+  OutPtrs outs;
+  std::vector<float2*> scratch;
+  try {
+    for (size_t i = 0; i < plans.size(); ++i) {
+      float2* g = dalloc<float2>(plans[i]->N_pts);
+      scratch.push_back(g);
+      outs.p[i] = g;
+    }
+    for (int items : items_c) {
+      set_items_target(plans, items);
+      rebuild();
+      for (int gm : grid_c) {
+        L.grid_mult = gm;
+        const float t = time_launches([&] {
+          if (precision == BORN_PRECISION_FP32) launch_fwd_structured(L, n_cu, s);
+          else launch_fwd_split(L, n_cu, s);
+        }, s, 3);
+        if (t < best.fwd_ms) {
+          best.fwd_ms = t;
+          best.items = items;
+          best.grid_mult = gm;
+        }
+      }
+    }
+    set_items_target(plans, best.items);
+    L.grid_mult = best.grid_mult;
+    const int mw_c[] = {2, 4};
+    for (int mw : mw_c) {
+      if (precision != BORN_PRECISION_FP32 && mw != 4) continue;   // the split adjoint has one width
+      L.adj_mw = mw;
+      rebuild();
+      const float t = time_launches([&] {
+        if (precision == BORN_PRECISION_FP32) launch_adj_structured(L, outs, s);
+        else launch_adj_split(L, outs, s);
+      }, s, 3);
+      if (t < best.adj_ms) {
+        best.adj_ms = t;
+        best.adj_mw = mw;
+      }
+    }
+    L.adj_mw = best.adj_mw;
+    rebuild();
+    HIP_CHECK(hipStreamSynchronize(s));
+  } catch (...) {
+    hipStreamSynchronize(s);
+    for (float2* g : scratch) dfree(g);
+    throw;
+  }
+  for (float2* g : scratch) dfree(g);
+  return best;
+}
+
 template <typename F>
 static int guarded(F&& f) {
   try {
@@ -990,6 +1273,137 @@ int born_plan_get_timing(born_plan_t plan, double* ms, int64_t* launches) {
       hipEventDestroy(t.b);
     }
     P.timed.clear();
+  });
+}
+
+
+int born_batch_create(const born_plan_t* plans, int32_t n_plans, born_batch_t* batch) {
+  return guarded([&] {
+    if (!plans || !batch || n_plans < 1) throw Error(BORN_ERR_INVALID_ARGUMENT, "plans and output are required");
+    if (n_plans > MAX_BATCH) throw Error(BORN_ERR_UNSUPPORTED, "too many plans in one batch");
+    *batch = nullptr;
+    std::unique_ptr<Batch> B(new Batch());
+    for (int i = 0; i < n_plans; ++i) {
+      if (!plans[i]) throw Error(BORN_ERR_NO_OPERATOR_BORN, "plan is NULL");
+      B->plans.push_back(reinterpret_cast<Plan*>(plans[i]));
+    }
+    Plan& P0 = *B->plans[0];
+    B->device = P0.device;
+    B->n_cu = P0.n_cu;
+    B->precision = P0.precision;
+    HIP_CHECK(hipSetDevice(B->device));
+    for (Plan* P : B->plans) {
+      if (P->device != B->device) throw Error(BORN_ERR_UNSUPPORTED, "plans of a batch must share the device");
+      if (!use_structured(*P)) throw Error(BORN_ERR_UNSUPPORTED, "a batch needs plans on the structured path");
+      if (P->Ne_pad != P0.Ne_pad || P->Np != P0.Np)
+        throw Error(BORN_ERR_UNSUPPORTED, "plans of a batch must share element padding and points per pitch");
+      if (P->precision != B->precision) throw Error(BORN_ERR_UNSUPPORTED, "plans of a batch must share the precision");
+      check_precision(*P);
+      ensure_workspace(*P, 1);
+    }
+    HIP_CHECK(hipStreamCreateWithFlags(&B->own, hipStreamNonBlocking));
+    HIP_CHECK(hipEventCreateWithFlags(&B->ev_in, hipEventDisableTiming));
+    HIP_CHECK(hipEventCreateWithFlags(&B->ev_out, hipEventDisableTiming));
+    build_batch_launch(*B);
+    *batch = reinterpret_cast<born_batch_t>(B.release());
+  });
+}
+
+int born_batch_destroy(born_batch_t batch) {
+  return guarded([&] { delete reinterpret_cast<Batch*>(batch); });
+}
+
+int born_batch_forward_dev(born_batch_t batch, const void* const* gamma_c64, void* const* u_c64, void* stream) {
+  return guarded([&] {
+    if (!batch || !gamma_c64 || !u_c64) throw Error(BORN_ERR_INVALID_ARGUMENT, "batch and buffers are required");
+    Batch& B = *reinterpret_cast<Batch*>(batch);
+    for (size_t i = 0; i < B.plans.size(); ++i)
+      if (!gamma_c64[i] || !u_c64[i]) throw Error(BORN_ERR_NO_VALID_DOUBLE_MATRIX, "invalid device buffers");
+    HIP_CHECK(hipSetDevice(B.device));
+    batch_apply(B, 0, gamma_c64, u_c64, (hipStream_t)stream);
+  });
+}
+
+int born_batch_adjoint_dev(born_batch_t batch, const void* const* u_c64, void* const* gamma_hat_c64, void* stream) {
+  return guarded([&] {
+    if (!batch || !u_c64 || !gamma_hat_c64) throw Error(BORN_ERR_INVALID_ARGUMENT, "batch and buffers are required");
+    Batch& B = *reinterpret_cast<Batch*>(batch);
+    for (size_t i = 0; i < B.plans.size(); ++i)
+      if (!u_c64[i] || !gamma_hat_c64[i]) throw Error(BORN_ERR_NO_VALID_DOUBLE_MATRIX, "invalid device buffers");
+    HIP_CHECK(hipSetDevice(B.device));
+    batch_apply(B, 1, u_c64, gamma_hat_c64, (hipStream_t)stream);
+  });
+}
+
+int born_batch_set_graph(born_batch_t batch, int32_t on) {
+  return guarded([&] {
+    if (!batch) throw Error(BORN_ERR_INVALID_ARGUMENT, "batch is NULL");
+    Batch& B = *reinterpret_cast<Batch*>(batch);
+    B.graph = on != 0;
+    if (!B.graph) {
+      HIP_CHECK(hipSetDevice(B.device));
+      for (auto& c : B.cap) drop_graph(c);
+    }
+  });
+}
+
+int born_batch_autotune(born_batch_t batch, void* stream, double* fwd_ms, double* adj_ms) {
+  return guarded([&] {
+    if (!batch) throw Error(BORN_ERR_INVALID_ARGUMENT, "batch is NULL");
+    Batch& B = *reinterpret_cast<Batch*>(batch);
+    HIP_CHECK(hipSetDevice(B.device));
+    hipStream_t s = (hipStream_t)stream;
+    HIP_CHECK(hipDeviceSynchronize());
+    const TuneResult r = autotune(B.plans, B.ls, B.precision, B.n_cu, [&] { build_batch_launch(B); }, s);
+    if (fwd_ms) *fwd_ms = r.fwd_ms;
+    if (adj_ms) *adj_ms = r.adj_ms;
+  });
+}
+
+int born_plan_autotune(born_plan_t plan, void* stream, double* fwd_ms, double* adj_ms) {
+  return guarded([&] {
+    if (!plan) throw Error(BORN_ERR_INVALID_ARGUMENT, "plan is NULL");
+    Plan& P = *reinterpret_cast<Plan*>(plan);
+    if (!use_structured(P)) throw Error(BORN_ERR_UNSUPPORTED, "autotuning covers the structured path");
+    check_precision(P);
+    HIP_CHECK(hipSetDevice(P.device));
+    ensure_workspace(P, 1);
+    hipStream_t s = (hipStream_t)stream;
+    HIP_CHECK(hipDeviceSynchronize());
+    std::vector<Plan*> one{&P};
+    const TuneResult r = autotune(one, P.ls, P.precision, P.n_cu, [&] {
+      P.ls.inst.assign(1, make_inst(P));
+      finalize_launchset(P.ls);
+    }, s);
+    if (fwd_ms) *fwd_ms = r.fwd_ms;
+    if (adj_ms) *adj_ms = r.adj_ms;
+  });
+}
+
+int born_batch_set_timing(born_batch_t batch, int32_t on) {
+  return guarded([&] {
+    if (!batch) throw Error(BORN_ERR_INVALID_ARGUMENT, "batch is NULL");
+    reinterpret_cast<Batch*>(batch)->timing = on != 0;
+  });
+}
+
+int born_batch_get_timing(born_batch_t batch, double* ms, int64_t* launches) {
+  return guarded([&] {
+    if (!batch || !ms || !launches) throw Error(BORN_ERR_INVALID_ARGUMENT, "batch and outputs are required");
+    Batch& B = *reinterpret_cast<Batch*>(batch);
+    HIP_CHECK(hipSetDevice(B.device));
+    ms[0] = ms[1] = 0.0;
+    launches[0] = launches[1] = 0;
+    for (auto& t : B.timed) {
+      HIP_CHECK(hipEventSynchronize(t.b));
+      float e = 0.f;
+      HIP_CHECK(hipEventElapsedTime(&e, t.a, t.b));
+      ms[t.kind] += e;
+      launches[t.kind] += 1;
+      hipEventDestroy(t.a);
+      hipEventDestroy(t.b);
+    }
+    B.timed.clear();
   });
 }
 

@@ -282,6 +282,16 @@ class OperatorBorn:
                                           C.c_void_p(out.data_ptr()), C.c_void_p(s)))
         return out.reshape(-1) if squeeze else out
 
+    def autotune(self, stream=None):
+        """Choose the forward work-item count, persistent grid size and
+        adjoint segment width of this plan by timing (structured path).
+        Returns the best (forward, adjoint) contraction times in ms."""
+        import torch
+        s = torch.cuda.current_stream().cuda_stream if stream is None else stream
+        f, a = C.c_double(), C.c_double()
+        check(_lib.lib().born_plan_autotune(self._plan, C.c_void_p(s), C.byref(f), C.byref(a)))
+        return f.value, a.value
+
     @staticmethod
     def _check_dev(t, rows):
         import torch
@@ -318,3 +328,75 @@ def combined_quick_gpu(operator_born, mode, input_matrix):
     if x.dtype not in (np.float64, np.complex128) or x.ndim != 2 or x.shape[0] != rows or x.size == 0:
         raise BornError(7, "combined_quick_gpu:NoValidDoubleMatrix", f"input must be a double matrix with {rows} rows!")
     return operator_born.forward(x) if mode_i == 1 else operator_born.adjoint(x)
+
+
+class OperatorBatch:
+    """Several operator instances applied together, one right-hand side each:
+    one persistent contraction launch per direction covers all of them
+    (born_batch_*).  The plans must stay alive while the batch exists."""
+
+    def __init__(self, ops, graph: bool = False):
+        self.ops = list(ops)
+        n = len(self.ops)
+        arr = (C.c_void_p * n)(*[op._plan for op in self.ops])
+        b = C.c_void_p()
+        check(_lib.lib().born_batch_create(arr, n, C.byref(b)))
+        self._batch = b
+        if graph:
+            self.set_graph(True)
+
+    def close(self):
+        if getattr(self, "_batch", None):
+            _lib.lib().born_batch_destroy(self._batch)
+            self._batch = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_graph(self, on: bool = True):
+        check(_lib.lib().born_batch_set_graph(self._batch, int(bool(on))))
+
+    def set_timing(self, on: bool = True):
+        check(_lib.lib().born_batch_set_timing(self._batch, int(bool(on))))
+
+    def get_timing(self):
+        ms = (C.c_double * 2)()
+        n = (C.c_int64 * 2)()
+        check(_lib.lib().born_batch_get_timing(self._batch, ms, n))
+        return {"forward": (ms[0], n[0]), "adjoint": (ms[1], n[1])}
+
+    def autotune(self, stream=None):
+        import torch
+        s = torch.cuda.current_stream().cuda_stream if stream is None else stream
+        f, a = C.c_double(), C.c_double()
+        check(_lib.lib().born_batch_autotune(self._batch, C.c_void_p(s), C.byref(f), C.byref(a)))
+        return f.value, a.value
+
+    def _apply(self, fn, xs, rows_in, rows_out, outs, stream):
+        import torch
+        n = len(self.ops)
+        if len(xs) != n:
+            raise ValueError("one input per instance is required")
+        for x, r in zip(xs, rows_in):
+            OperatorBorn._check_dev(x.reshape(1, -1), r)
+        if outs is None:
+            outs = [torch.empty(r, dtype=torch.complex64, device=x.device) for x, r in zip(xs, rows_out)]
+        s = torch.cuda.current_stream().cuda_stream if stream is None else stream
+        pin = (C.c_void_p * n)(*[x.data_ptr() for x in xs])
+        pout = (C.c_void_p * n)(*[o.data_ptr() for o in outs])
+        check(fn(self._batch, pin, pout, C.c_void_p(s)))
+        return outs
+
+    def forward_dev(self, gammas, outs=None, stream=None):
+        """gammas[i]: complex64 (N_points(i),) -> list of u_i (N_observations(i),)."""
+        return self._apply(_lib.lib().born_batch_forward_dev, gammas, [op.N_points for op in self.ops],
+                           [op.N_observations for op in self.ops], outs, stream)
+
+    def adjoint_dev(self, us, outs=None, stream=None):
+        """us[i]: complex64 (N_observations(i),) -> list of gamma_hat_i (N_points(i),)."""
+        return self._apply(_lib.lib().born_batch_adjoint_dev, us, [op.N_observations for op in self.ops],
+                           [op.N_points for op in self.ops], outs, stream)
+

@@ -193,6 +193,32 @@ int born_plan_force_general(born_plan_t plan, int32_t on);
 int born_plan_set_timing(born_plan_t plan, int32_t on);
 int born_plan_get_timing(born_plan_t plan, double* ms, int64_t* launches);
 
+/* ---------------------------------------------------------------- batches
+ * Several operator instances (plans of uneven FOV, frequency and wave
+ * counts) applied together, one right-hand side each: one persistent
+ * contraction launch per direction covers every instance.  No counterpart in
+ * the reference (each MEX call serves one operator).  The plans must be on
+ * the structured path, share the device, the precision, the element padding
+ * and the points per pitch, and outlive the batch; at most 64 plans.
+ * gamma_c64[i] / u_c64[i] are the device buffers of plan i (complex64,
+ * N_points(i) and N_observations(i) entries). */
+typedef struct born_batch_s* born_batch_t;
+int born_batch_create(const born_plan_t* plans, int32_t n_plans, born_batch_t* batch);
+int born_batch_destroy(born_batch_t batch);
+int born_batch_forward_dev(born_batch_t batch, const void* const* gamma_c64, void* const* u_c64, void* stream);
+int born_batch_adjoint_dev(born_batch_t batch, const void* const* u_c64, void* const* gamma_hat_c64, void* stream);
+/* Replay through hipGraphs: the launch sequence of a direction is captured on
+ * first use for a (stream, buffers) combination and replayed afterwards. */
+int born_batch_set_graph(born_batch_t batch, int32_t on);
+/* Autotuner: times the candidate work-item counts and persistent grid sizes
+ * of the forward contraction and the segment widths of the adjoint on the
+ * batch's own instances and keeps the fastest; returns their times (ms). */
+int born_batch_autotune(born_batch_t batch, void* stream, double* fwd_ms, double* adj_ms);
+int born_plan_autotune(born_plan_t plan, void* stream, double* fwd_ms, double* adj_ms);
+/* Event timing of the batched contraction launches (as born_plan_set_timing). */
+int born_batch_set_timing(born_batch_t batch, int32_t on);
+int born_batch_get_timing(born_batch_t batch, double* ms, int64_t* launches);
+
 /* Thread-local message of the last failure on this thread. */
 const char* born_last_error(void);
 
