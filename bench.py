@@ -327,7 +327,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32" if args.precision == "fp32" else args.precision,
-        "data": "synthetic (config-1 speckle phantom; tables built on the device from the config-1 geometry)",
+        "data": f"synthetic ({args.config} speckle phantom; tables built on the device from the {args.config} geometry)",
         "config": {
             "workload": f"{args.config}: {spec.array.N_elements} el, {spec.fov.N_points_axis[0]}x"
                         f"{spec.fov.N_points_axis[-1]} FOV, {len(spec.frequencies)} f, "

@@ -361,3 +361,34 @@ def test_narrow_fov_split_bf16(torch_cuda):
     e_a = O.rel_l2(op.adjoint(u), O.adjoint(T, u))
     log("narrow FOV split bf16: forward rel-L2 %.2e adjoint rel-L2 %.2e" % (e_f, e_a))
     assert e_f < TOL_SPLIT and e_a < TOL_SPLIT
+
+
+# ----------------------------------------------- config 2: random waves, mixing
+@pytest.fixture(scope="module")
+def c2(torch_cuda):
+    """The plan and the oracle's answers at 16 sampled points (computed once)."""
+    spec = C.config2()
+    op = OperatorBorn.from_sequence(spec)
+    rng = np.random.Generator(np.random.PCG64(101))
+    pts = np.sort(rng.choice(spec.fov.N_points, 16, replace=False))
+    R = O.restricted_tables(spec, pts)
+    g = np.zeros(spec.fov.N_points)
+    g[pts] = rng.standard_normal(len(pts))
+    u = rnd(op.N_observations, 102)
+    return spec, op, pts, g, u, O.forward(R, g[pts]), O.adjoint(R, u)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", TOL_FP32), ("split_bf16", TOL_SPLIT)])
+def test_config2_against_oracle(torch_cuda, c2, precision, tol):
+    """32 random waves (+-1 apodization, delays U[0, 5 us]) compounded into 16
+    mixes of all 128 elements with +-1 weights: sampled points against tables
+    the oracle restricts exactly to them."""
+    spec, op, pts, g, u, ref_f, ref_a = c2
+    op.set_precision(precision)
+    try:
+        e_f = O.rel_l2(op.forward(g), ref_f)
+        e_a = O.rel_l2(op.adjoint(u)[pts], ref_a)
+    finally:
+        op.set_precision("fp32")
+    log("config2 %s: forward rel-L2 %.2e adjoint rel-L2 %.2e" % (precision, e_f, e_a))
+    assert e_f < tol and e_a < tol
