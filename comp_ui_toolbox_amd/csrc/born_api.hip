@@ -90,6 +90,8 @@ Plan::~Plan() {
   if (stream) hipStreamSynchronize(stream);
   free_workspace(*this);
   dfree(ls.d_inst);
+  dfree(d_exty);
+  dfree(d_hrow);
   dfree(d_h);
   dfree(d_p);
   dfree(d_shift);
@@ -239,53 +241,117 @@ static void upload_index_structures(Plan& P, hipStream_t s) {
 // (setup_grid_symmetric.m:95-241): in 2-D, element e sees point (i, z) at
 // h_ref point (ext[i - e*Np + (N_e-1)*Np], z).  Derived from the map itself
 // and verified entry by entry; any mismatch selects the general path.
+// Structured path: the element's transfer function is a window of one
+// extended lateral row (2-D) or of one extended x-row chosen by the y offset
+// (3-D).  finish_structure takes the lateral maps extx (and exty), sizes the
+// tiles, builds the table row of every (y slice, row) and uploads the maps.
+static void finish_structure(Plan& P) {
+  P.structured = false;
+  const int Nex = P.Nex, Ney = (P.nd == 3) ? P.Ney : 1;
+  if (Nex > 128) return;
+  P.N_ext = P.Nx + (Nex - 1) * P.Np;
+  {
+    int mt = (Nex + 15) / 16;  // element tiles of a slice, rounded up to 1, 2, 4 or 8
+    mt = mt <= 1 ? 1 : (mt <= 2 ? 2 : (mt <= 4 ? 4 : 8));
+    P.Ne_pad = 16 * mt;
+  }
+  if (!fwd_structured_fits(P.Nx, P.Ne_pad, P.Np) || !adj_structured_fits(P.Nx, P.Ne_pad, P.Np)) return;
+  const int Ny = (P.nd == 3) ? P.Ny : 1;
+  P.n_rows = Ny * P.Nz;
+  const int off_y = (Ney - 1) * P.Npy;
+  std::vector<int> hrow((size_t)Ney * P.n_rows);
+  for (int ey = 0; ey < Ney; ++ey)
+    for (int iz = 0; iz < P.Nz; ++iz)
+      for (int iy = 0; iy < Ny; ++iy) {
+        const int ty = (P.nd == 3) ? P.exty[iy - ey * P.Npy + off_y] : 0;
+        hrow[(size_t)ey * P.n_rows + iy + (size_t)Ny * iz] = ty + P.HY * iz;
+      }
+  dfree(P.d_ext);
+  dfree(P.d_exty);
+  dfree(P.d_hrow);
+  P.d_ext = dupload(P.extx, P.stream);
+  if (P.nd == 3) P.d_exty = dupload(P.exty, P.stream);
+  P.d_hrow = dupload(hrow, P.stream);
+  P.structured = true;
+}
+
+// Lateral maps derived from a given shift map, verified entry by entry; any
+// mismatch leaves the plan on the general gather path.
 static void detect_structure(Plan& P) {
   P.structured = false;
-  if (P.nd != 2 || P.Nex != P.N_e || P.N_e > 128) return;
-  const int Nx = P.Nx, Nz = P.Nz;
+  if (P.nd == 3 && P.Nex * P.Ney != P.N_e) return;
+  if (P.nd == 2 && P.Nex != P.N_e) return;
+  const int Nx = P.Nx, Ny = (P.nd == 3) ? P.Ny : 1, Nz = P.Nz;
+  const int Nex = P.Nex, Ney = (P.nd == 3) ? P.Ney : 1;
   const int* sh = P.h_shift.data();
-  const int HR = P.HR;   // h_ref entries per depth row
-  int Np = 1;
-  if (P.N_e >= 2) {
-    // the last point of the first row: element 1 sees it Np points closer
+  const int HR = P.HR, HY = (P.nd == 3) ? P.HY : 1;
+  // points per pitch: element 1 (x) and element Nex (y) see the last point of
+  // the first x-row / the first point of the last y-row one pitch closer
+  int Np = 1, Npy = 1;
+  if (Nex >= 2) {
     const int p = Nx - 1;
     Np = sh[p] - sh[(size_t)1 * P.N_pts + p];
     if (Np < 1 || Np > HR - 1) return;
   }
-  const int N_ext = Nx + (P.N_e - 1) * Np;
-  std::vector<int> ext(N_ext, -1);
-  for (int e = 0; e < P.N_e; ++e)
-    for (int i = 0; i < Nx; ++i) {
-      const int t = i - e * Np + (P.N_e - 1) * Np;
-      const int v = sh[(size_t)e * P.N_pts + i];
-      if (v < 0 || v >= HR) return;
-      if (ext[t] < 0) ext[t] = v;
-      else if (ext[t] != v) return;
-    }
-  for (int t = 0; t < N_ext; ++t)
-    if (ext[t] < 0) return;
-  for (int e = 0; e < P.N_e; ++e)
-    for (int z = 0; z < Nz; ++z)
-      for (int i = 0; i < Nx; ++i) {
-        const int t = i - e * Np + (P.N_e - 1) * Np;
-        if (sh[(size_t)e * P.N_pts + (size_t)z * Nx + i] != z * HR + ext[t]) return;
-      }
-  P.Np = Np;
-  P.N_ext = N_ext;
-  {
-    int mt = (P.N_e + 15) / 16;  // element tiles, rounded up to 1, 2, 4 or 8
-    mt = mt <= 1 ? 1 : (mt <= 2 ? 2 : (mt <= 4 ? 4 : 8));
-    P.Ne_pad = 16 * mt;
+  if (Ney >= 2) {
+    const int p = Nx * (Ny - 1);
+    const int d = sh[p] - sh[(size_t)Nex * P.N_pts + p];
+    if (d % HR != 0) return;
+    Npy = d / HR;
+    if (Npy < 1 || Npy > HY - 1) return;
   }
-  if (!fwd_structured_fits(Nx, P.Ne_pad, Np) || !adj_structured_fits(Nx, P.Ne_pad, Np)) return;
-  P.d_ext = dupload(ext, P.stream);
-  P.structured = true;
+  const int N_ext = Nx + (Nex - 1) * Np, N_exty = Ny + (Ney - 1) * Npy;
+  std::vector<int> extx(N_ext, -1), exty(N_exty, -1);
+  for (int ex = 0; ex < Nex; ++ex)
+    for (int i = 0; i < Nx; ++i) {
+      const int t = i - ex * Np + (Nex - 1) * Np;
+      const int v = sh[(size_t)ex * P.N_pts + i];
+      if (v < 0 || v >= HR) return;
+      if (extx[t] < 0) extx[t] = v;
+      else if (extx[t] != v) return;
+    }
+  if (P.nd == 3) {
+    for (int ey = 0; ey < Ney; ++ey)
+      for (int iy = 0; iy < Ny; ++iy) {
+        const int t = iy - ey * Npy + (Ney - 1) * Npy;
+        const int v = sh[(size_t)(ey * Nex) * P.N_pts + (size_t)iy * Nx] - extx[(Nex - 1) * Np];
+        if (v % HR != 0) return;
+        const int r = v / HR;
+        if (r < 0 || r >= HY) return;
+        if (exty[t] < 0) exty[t] = r;
+        else if (exty[t] != r) return;
+      }
+    for (int t = 0; t < N_exty; ++t)
+      if (exty[t] < 0) return;
+  }
+  for (int t = 0; t < N_ext; ++t)
+    if (extx[t] < 0) return;
+  for (int ey = 0; ey < Ney; ++ey)
+    for (int ex = 0; ex < Nex; ++ex) {
+      const int* se = sh + (size_t)(ex + Nex * ey) * P.N_pts;
+      for (int z = 0; z < Nz; ++z)
+        for (int iy = 0; iy < Ny; ++iy) {
+          const int ty = (P.nd == 3) ? exty[iy - ey * Npy + (Ney - 1) * Npy] : 0;
+          const int base = HR * (ty + HY * z);
+          const int* row = se + (size_t)Nx * (iy + (size_t)Ny * z);
+          for (int i = 0; i < Nx; ++i)
+            if (row[i] != base + extx[i - ex * Np + (Nex - 1) * Np]) return;
+        }
+    }
+  P.Np = Np;
+  P.Npy = Npy;
+  P.extx = std::move(extx);
+  P.exty = std::move(exty);
+  finish_structure(P);
 }
 
 static bool use_structured(const Plan& P) { return P.structured && !P.force_general; }
 
+static void materialize_shift(Plan& P);
+
 static void ensure_shift_on_device(Plan& P) {
   if (P.d_shift) return;
+  if (P.h_shift.empty()) materialize_shift(P);
   P.d_shift = dupload(P.h_shift, P.stream);
 }
 
@@ -294,15 +360,17 @@ static void ensure_shift_on_device(Plan& P) {
 // the split kernel.
 static void set_blocking(Plan& P) {
   const int n_chunks = (int)P.chunks.size();
+  const int Ney = P.nd == 3 ? P.Ney : 1;
+  const int rows = P.n_rows > 0 ? P.n_rows : P.Nz;
   const int target = P.items_target;
-  int NZB = std::max(1, std::min(P.Nz, target / std::max(1, n_chunks)));
-  P.ZR = (P.Nz + NZB - 1) / NZB;
-  P.NZB = (P.Nz + P.ZR - 1) / P.ZR;
+  int NZB = std::max(1, std::min(rows, target / std::max(1, n_chunks * Ney)));
+  P.ZR = (rows + NZB - 1) / NZB;
+  P.NZB = (rows + P.ZR - 1) / P.ZR;
   P.nseg_s = (P.Nx + 255) / 256;
-  NZB = std::max(1, std::min(P.Nz, target / std::max(1, n_chunks * P.nseg_s)));
-  P.ZR_s = (P.Nz + NZB - 1) / NZB;
-  P.NZB_s = (P.Nz + P.ZR_s - 1) / P.ZR_s;
-  P.vblk = std::max(P.NZB, P.NZB_s * P.nseg_s);
+  NZB = std::max(1, std::min(rows, target / std::max(1, n_chunks * P.nseg_s)));
+  P.ZR_s = (rows + NZB - 1) / NZB;
+  P.NZB_s = (rows + P.ZR_s - 1) / P.ZR_s;
+  P.vblk = std::max(P.NZB * Ney, P.NZB_s * P.nseg_s);   // partial-sum blocks per chunk
 }
 
 // The record of a plan inside a structured launch (workspace must exist).
@@ -318,8 +386,11 @@ static Inst make_inst(const Plan& P) {
   I.Nx = P.Nx;
   I.Nz = P.Nz;
   I.N_pts = P.N_pts;
-  I.N_e = P.N_e;
+  I.N_e = P.Nex;   // elements of one slice
   I.Np = P.Np;
+  I.Ney = P.nd == 3 ? P.Ney : 1;
+  I.n_rows = P.n_rows;
+  I.hrow = (decltype(I.hrow))P.d_hrow;
   I.N_ext = P.N_ext;
   I.Hp = P.H_plane;
   I.HR = P.HR;
@@ -339,15 +410,15 @@ static void finalize_launchset(LaunchSet& L) {
   L.S_max = 1;
   for (Inst& I : L.inst) {
     I.item_begin = it;
-    it += I.n_chunks * I.NZB;
+    it += I.n_chunks * I.Ney * I.NZB;
     I.item_begin_s = its;
     its += I.n_chunks * I.NZB_s * I.nseg_s;
     I.adj_segs = (I.Nx + 64 * L.adj_mw - 1) / (64 * L.adj_mw);
     I.wg_begin = wg;
-    wg += I.adj_segs * I.Nz;
+    wg += I.adj_segs * I.n_rows;
     I.adj_segs_s = (I.Nx + 255) / 256;
     I.wg_begin_s = wgs;
-    wgs += I.adj_segs_s * I.Nz;
+    wgs += I.adj_segs_s * I.n_rows;
     L.S_max = std::max(L.S_max, I.S);
   }
   L.n_items = it;
@@ -397,7 +468,7 @@ static void ensure_workspace(Plan& P, int N_rhs) {
   P.d_V = dalloc<double2>((size_t)P.N_pairs * P.N_e * N_rhs);
   P.d_W = dalloc<float2>((size_t)P.N_pairs * P.N_e * N_rhs);
   if (P.structured) {
-    const size_t n = (size_t)n_chunks * P.Ne_pad * NC;
+    const size_t n = (size_t)n_chunks * (P.nd == 3 ? P.Ney : 1) * P.Ne_pad * NC;
     P.d_Wc = dalloc<float2>(n);
     HIP_CHECK(hipMemset(P.d_Wc, 0, n * sizeof(float2)));  // padding columns/elements stay zero
   }
@@ -447,14 +518,16 @@ static void common_dims(Plan& P, int nd, const int32_t* Npa, const int32_t* Nea,
   P.N_pts = N_points;
   P.N_e = N_elements;
   P.H_plane = N_points;
-  P.HR = N_points / P.Nz;
+  P.HR = P.Nx;
+  P.HY = P.Ny;
+  P.n_rows = P.Ny * P.Nz;
 }
 
-static void finish_plan(Plan& P) {
+static void finish_plan(Plan& P, bool structure_known = false) {
   int ncu = 0;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, P.device) == hipSuccess && ncu > 0)
     P.n_cu = ncu;
-  detect_structure(P);
+  if (!structure_known) detect_structure(P);
   if (!P.structured) ensure_shift_on_device(P);
   set_max_lds(160 * 1024);
   set_max_lds_split(160 * 1024);
@@ -529,52 +602,55 @@ static Plan* create_tables(const born_tables_desc* d) {
 // Extended layout (FOV narrower than the aperture, where the reflection is
 // undefined): h_ref covers the grid extended by (N_elements_axis - 1)
 // pitches before the FOV, and the lateral subscript is the offset itself.
-static void compute_shift_map(Plan& P, const int* Np_axis) {
+// Both are separable: per lateral axis, the table subscript of offset
+// t = i - e*Np + (Ne-1)*Np is ext[t].
+static void compute_layout(Plan& P, const int* Np_axis) {
   const int nlat = P.nd - 1;
-  const int Na[3] = {P.Nx, P.Ny, P.Nz};
+  const int Na[2] = {P.Nx, P.Ny};
   const int Ne[2] = {P.Nex, P.Ney};
-  int Nt[3] = {P.Nx, P.Ny, P.Nz};   // table grid
-  int off[2] = {0, 0};
   P.extended = false;
   for (int d = 0; d < nlat; ++d)
     if (Na[d] - (Ne[d] - 1) * Np_axis[d] < 1) P.extended = true;
-  if (P.extended)
-    for (int d = 0; d < nlat; ++d) {
-      off[d] = (Ne[d] - 1) * Np_axis[d];
-      Nt[d] = Na[d] + off[d];
+  std::vector<int> ext[2];
+  int Nt[2] = {P.Nx, P.Ny};
+  for (int d = 0; d < nlat; ++d) {
+    const int off = (Ne[d] - 1) * Np_axis[d];
+    ext[d].resize(Na[d] + off);
+    for (int t = 0; t < Na[d] + off; ++t) {
+      int v = t - off + 1;  // 1-based subscript s - e*Np
+      if (P.extended) v = t + 1;
+      else if (v <= 0) v = (Na[d] - (Ne[d] - 1) * Np_axis[d] + 1) - v;
+      ext[d][t] = v - 1;
     }
+    if (P.extended) Nt[d] = Na[d] + off;
+  }
   const int64_t plane = (int64_t)Nt[0] * (P.nd == 3 ? Nt[1] : 1) * P.Nz;
-  if (plane > INT32_MAX) throw Error(BORN_ERR_UNSUPPORTED, "extended table too large");
+  if (plane > INT32_MAX) throw Error(BORN_ERR_UNSUPPORTED, "table too large");
   P.H_plane = (int)plane;
-  P.HR = P.H_plane / P.Nz;
+  P.HR = Nt[0];
+  P.HY = (P.nd == 3) ? Nt[1] : 1;
+  P.Np = Np_axis[0];
+  P.Npy = (P.nd == 3) ? Np_axis[1] : 1;
+  P.extx = ext[0];
+  P.exty = (P.nd == 3) ? ext[1] : std::vector<int>{0};
+  P.h_shift.clear();
+}
+
+// The full shift map [N_e][N_pts] from the lateral maps (general path only).
+static void materialize_shift(Plan& P) {
+  const int Ny = (P.nd == 3) ? P.Ny : 1, Ney = (P.nd == 3) ? P.Ney : 1;
+  const int offx = (P.Nex - 1) * P.Np, offy = (Ney - 1) * P.Npy;
   P.h_shift.resize((size_t)P.N_e * P.N_pts);
   for (int e = 0; e < P.N_e; ++e) {
-    const int es[2] = {e % P.Nex, e / P.Nex};
-    for (int p = 0; p < P.N_pts; ++p) {
-      int s[3];
-      s[0] = p % P.Nx;
-      const int rest = p / P.Nx;
-      if (P.nd == 3) {
-        s[1] = rest % P.Ny;
-        s[2] = rest / P.Ny;
-      } else {
-        s[1] = rest;
+    const int ex = e % P.Nex, ey = e / P.Nex;
+    int* out = P.h_shift.data() + (size_t)e * P.N_pts;
+    for (int z = 0; z < P.Nz; ++z)
+      for (int iy = 0; iy < Ny; ++iy) {
+        const int ty = (P.nd == 3) ? P.exty[iy - ey * P.Npy + offy] : 0;
+        const int base = P.HR * (ty + P.HY * z);
+        int* row = out + (size_t)P.Nx * (iy + (size_t)Ny * z);
+        for (int i = 0; i < P.Nx; ++i) row[i] = base + P.extx[i - ex * P.Np + offx];
       }
-      int lin = 0, mul = 1;
-      for (int d = 0; d < P.nd; ++d) {
-        int v = s[d] + 1;  // 1-based subscript
-        if (d < nlat) {
-          v -= es[d] * Np_axis[d];
-          if (P.extended)
-            v += off[d];
-          else if (v <= 0)
-            v = (Na[d] - (Ne[d] - 1) * Np_axis[d] + 1) - v;
-        }
-        lin += (v - 1) * mul;
-        mul *= (P.nd == 3) ? Nt[d] : (d == 0 ? Nt[0] : P.Nz);
-      }
-      P.h_shift[(size_t)e * P.N_pts + p] = lin;
-    }
   }
 }
 
@@ -604,7 +680,7 @@ static Plan* create_geometry(const born_geometry_desc* d) {
     Np_axis[a] = n;
   }
   build_index_structures(*P, d->N_measurements, d->measurements);
-  compute_shift_map(*P, Np_axis);
+  compute_layout(*P, Np_axis);
   upload_index_structures(*P, P->stream);
   hipStream_t s = P->stream;
 
@@ -680,9 +756,14 @@ static Plan* create_geometry(const born_geometry_desc* d) {
     d_tx_el = dupload(tx_el, s);
     d_vd = dupload(vd, s);
     d_vd_off = dupload(vd_off, s);
-    ensure_shift_on_device(*P);
+    finish_structure(*P);
     P->d_p = alloc_p_table(*P);
-    launch_synth_pinc(*P, htx, d_meas_Nfm, d_tx_off, d_tx_el, d_vd, d_vd_off, s);
+    if (P->structured) {
+      launch_synth_pinc_structured(*P, htx, d_meas_Nfm, d_tx_off, d_tx_el, d_vd, d_vd_off, s);
+    } else {
+      ensure_shift_on_device(*P);
+      launch_synth_pinc(*P, htx, d_meas_Nfm, d_tx_off, d_tx_el, d_vd, d_vd_off, s);
+    }
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipStreamSynchronize(s));
   } catch (...) {
@@ -693,7 +774,7 @@ static Plan* create_geometry(const born_geometry_desc* d) {
   }
   dfree(dk); dfree(dfw); dfree(h64); dfree(h64tx);
   dfree(d_meas_Nfm); dfree(d_tx_off); dfree(d_tx_el); dfree(d_vd); dfree(d_vd_off);
-  finish_plan(*P);
+  finish_plan(*P, true);
   if (P->structured) dfree(P->d_shift);  // only the general path reads it
   return P.release();
 }
@@ -727,7 +808,7 @@ static void forward_device(Plan& P, const float2* gm, int N_rhs, void* u, bool o
       else
         launch_fwd_split(P.ls, P.n_cu, s);
     }
-    launch_fwd_reduce(P, N_rhs, P.precision == BORN_PRECISION_FP32 ? P.NZB : P.NZB_s * P.nseg_s, s);
+    launch_fwd_reduce(P, N_rhs, P.precision == BORN_PRECISION_FP32 ? P.NZB : P.NZB_s * P.nseg_s, s);  // per slice
   } else {
     ensure_shift_on_device(P);
     TimedLaunch t(P, s, 0);

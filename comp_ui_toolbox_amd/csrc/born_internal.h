@@ -39,6 +39,8 @@ struct Inst {
   int Nx, Nz, N_pts, N_e, Np, N_ext, Hp, HR;
   int n_chunks;
   int chunk_base;           // first chunk of the instance in the launch's chunk tables
+  int Ney, n_rows;          // y-element slices, rows (Ny * Nz)
+  BORN_GLOBAL const int* hrow;   // [Ney][n_rows] table row of (slice, row)
   int NZB, ZR, S;           // plain forward: z-blocks of ZR rows, S = ceil(Nx / 16)
   int NZB_s, ZR_s, nseg_s;  // split forward: z-blocks x 256-point segments
   int item_begin, item_begin_s;  // first forward work item of the instance (plain, split)
@@ -86,9 +88,17 @@ struct Plan {
   // the grid extended laterally by (N_elements_axis - 1) pitches, indexed by
   // lateral offset.  HR = H_plane / Nz entries per depth row.
   int H_plane = 0, HR = 0;
+  int HY = 1;       // x-rows of h_ref per depth plane (3-D: table grid Ny)
   bool extended = false;
-  int Np = 1;       // grid points per pitch (lateral, 2-D structured path)
+  // structured path: element e = ex + Nex * ey sees point (ix, iy, iz) at
+  // h_ref entry extx[ix - ex*Np + (Nex-1)*Np] + HR * (exty[iy - ey*Npy + (Ney-1)*Npy] + HY * iz).
+  // A "row" is (iy, iz); hrow[ey][row] = exty[...] + HY * iz is its table row.
+  int Np = 1;       // grid points per pitch along x
+  int Npy = 1;      // along y (3-D)
   int N_ext = 0;    // extended lateral length Nx + (Nex-1)*Np
+  int n_rows = 0;   // Ny * Nz
+  std::vector<int> extx, exty;
+  int* d_hrow = nullptr;          // [Ney][n_rows]
 
   // (measurement, unique-frequency-of-measurement) pairs: one p_inc row each
   int N_pairs = 0;
@@ -111,7 +121,8 @@ struct Plan {
   float2* d_h = nullptr;          // [N_fu][H_plane]
   float2* d_p = nullptr;          // [N_pairs + 1][N_pts], last row zero
   int* d_shift = nullptr;         // [N_e][N_pts], 0-based (general path)
-  int* d_ext = nullptr;           // [N_ext] lateral map (structured path)
+  int* d_ext = nullptr;           // [N_ext] lateral x map (structured path)
+  int* d_exty = nullptr;          // [Ny + (Ney-1)*Npy] lateral y map (3-D structured path)
   int* d_pair_fg = nullptr;
   int* d_pair_meas = nullptr;
   int* d_obs_mk = nullptr;        // [N_obs]
@@ -195,6 +206,8 @@ void launch_build_h(const GeomDev& g, int N_fu, const double2* k, const double2*
                     double2* h64, hipStream_t s);
 void launch_synth_pinc(const Plan& P, const double2* h64, const int* d_meas_Nfm, const int* d_tx_off,
                        const int* d_tx_el, const double2* d_vd, const int* d_vd_off, hipStream_t s);
+void launch_synth_pinc_structured(const Plan& P, const double2* h64, const int* d_meas_Nfm, const int* d_tx_off,
+                                  const int* d_tx_el, const double2* d_vd, const int* d_vd_off, hipStream_t s);
 void set_max_lds(size_t bytes);
 bool fwd_structured_fits(int Nx, int Ne_pad, int Np);
 bool fwd_split_supported(const Plan& P);

@@ -81,9 +81,11 @@ void launch_mask_gamma(const float2* g, int N_pts, int N_rhs, float2* gm, hipStr
 }
 
 // z-block partials -> V[pair][e][r] (FP64 sum)
+// V[pair][e][r] = sum of the depth-block partials of (chunk, y slice ey) at
+// element ex, with e = ex + Nex * ey.
 __global__ void k_fwd_reduce(const float2* __restrict__ vpart, const int* __restrict__ pr_chunk,
                              const int* __restrict__ pr_pos, int N_pairs, int N_e, int N_rhs, int NZB,
-                             int Ne_pad, double2* __restrict__ V) {
+                             int Ne_pad, int Nex, int Ney, double2* __restrict__ V) {
   const size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   const size_t total = (size_t)N_pairs * N_e * N_rhs;
   if (idx >= total) return;
@@ -92,9 +94,10 @@ __global__ void k_fwd_reduce(const float2* __restrict__ vpart, const int* __rest
   const int e = (int)(pe % N_e);
   const int pair = (int)(pe / N_e);
   const int ch = pr_chunk[pair * N_rhs + r], pos = pr_pos[pair * N_rhs + r];
+  const int ex = e % Nex, ey = e / Nex;
   double sx = 0.0, sy = 0.0;
   for (int zb = 0; zb < NZB; ++zb) {
-    const float2 v = vpart[(((size_t)ch * NZB + zb) * Ne_pad + e) * NC + pos];
+    const float2 v = vpart[((((size_t)ch * Ney + ey) * NZB + zb) * Ne_pad + ex) * NC + pos];
     sx += v.x;
     sy += v.y;
   }
@@ -104,7 +107,8 @@ __global__ void k_fwd_reduce(const float2* __restrict__ vpart, const int* __rest
 void launch_fwd_reduce(const Plan& P, int N_rhs, int nblk, hipStream_t s) {
   const size_t total = (size_t)P.N_pairs * P.N_e * N_rhs;
   hipLaunchKernelGGL(k_fwd_reduce, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, P.d_vpart,
-                     P.d_pr_chunk, P.d_pr_pos, P.N_pairs, P.N_e, N_rhs, nblk, P.Ne_pad, P.d_V);
+                     P.d_pr_chunk, P.d_pr_pos, P.N_pairs, P.N_e, N_rhs, nblk, P.Ne_pad, P.Nex,
+                     P.nd == 3 ? P.Ney : 1, P.d_V);
 }
 
 // ------------------------------------------------------ general forward
@@ -201,7 +205,7 @@ __global__ void k_adj_unmix(const float2* __restrict__ u, const int* __restrict_
                             const int* __restrict__ mk_meta, const int64_t* __restrict__ mk_off,
                             const double2* __restrict__ pref, int N_pairs, int N_e, int N_rhs, int64_t N_obs,
                             float2* __restrict__ W, float2* __restrict__ Wc, const int* __restrict__ pr_chunk,
-                            const int* __restrict__ pr_pos, int Ne_pad) {
+                            const int* __restrict__ pr_pos, int Ne_pad, int Nex, int Ney) {
   const size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (idx >= (size_t)N_pairs * N_e * N_rhs) return;
   const int r = (int)(idx % N_rhs);
@@ -222,7 +226,8 @@ __global__ void k_adj_unmix(const float2* __restrict__ u, const int* __restrict_
   const float2 res = make_float2((float)sx, (float)sy);
   if (Wc) {
     const int ch = pr_chunk[(size_t)pair * N_rhs + r], pos = pr_pos[(size_t)pair * N_rhs + r];
-    Wc[((size_t)ch * Ne_pad + e) * NC + pos] = res;
+    const int ex = e % Nex, ey = e / Nex;
+    Wc[(((size_t)ch * Ney + ey) * Ne_pad + ex) * NC + pos] = res;
   } else {
     W[idx] = res;
   }
@@ -232,7 +237,8 @@ void launch_adj_unmix(const Plan& P, const float2* u, int N_rhs, bool chunked, h
   const size_t total = (size_t)P.N_pairs * P.N_e * N_rhs;
   hipLaunchKernelGGL(k_adj_unmix, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, u, P.d_pair_list_off,
                      P.d_pair_list, P.d_slot, P.d_mk_meta, P.d_mk_off, P.d_pref, P.N_pairs, P.N_e, N_rhs,
-                     P.N_obs, P.d_W, chunked ? P.d_Wc : nullptr, P.d_pr_chunk, P.d_pr_pos, P.Ne_pad);
+                     P.N_obs, P.d_W, chunked ? P.d_Wc : nullptr, P.d_pr_chunk, P.d_pr_pos, P.Ne_pad, P.Nex,
+                     P.nd == 3 ? P.Ney : 1);
 }
 
 // ------------------------------------------------------ general adjoint
@@ -443,4 +449,50 @@ void launch_synth_pinc(const Plan& P, const double2* h64, const int* d_meas_Nfm,
                      P.N_pts, P.H_plane, P.d_p);
 }
 
+
+// p_in as above, with the shift map computed from the separable lateral maps
+// (structured geometries): element e = ex + Nex*ey sees point (ix, iy, iz) at
+// extx[ix - ex*Np + offx] + HR*(exty[iy - ey*Npy + offy] + HY*iz).
+__global__ __launch_bounds__(256) void k_synth_pinc_s(const double2* __restrict__ h64, const int* __restrict__ extx,
+                                                      const int* __restrict__ exty, int Nx, int Ny, int Nex,
+                                                      int Np, int Npy, int offx, int offy, int HR, int HY,
+                                                      const int* __restrict__ pair_fg,
+                                                      const int* __restrict__ pair_meas,
+                                                      const int* __restrict__ meas_pair_base,
+                                                      const int* __restrict__ meas_Nfm, const int* __restrict__ tx_off,
+                                                      const int* __restrict__ tx_el, const double2* __restrict__ vd,
+                                                      const int* __restrict__ vd_off, int N_pts, int H_plane,
+                                                      float2* __restrict__ P) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  const int pair = blockIdx.y;
+  if (p >= N_pts) return;
+  const int ix = p % Nx, rest = p / Nx;
+  const int iy = rest % Ny, iz = rest / Ny;
+  const int m = pair_meas[pair];
+  const int fm = pair - meas_pair_base[m];
+  const int Nfm = meas_Nfm[m];
+  const double2* hr = h64 + (size_t)pair_fg[pair] * H_plane;
+  double sx = 0.0, sy = 0.0;
+  for (int a = tx_off[m]; a < tx_off[m + 1]; ++a) {
+    const int e = tx_el[a];
+    const int ex = e % Nex, ey = e / Nex;
+    const int ty = exty ? exty[iy - ey * Npy + offy] : 0;
+    const double2 hv = hr[extx[ix - ex * Np + offx] + (size_t)HR * (ty + (size_t)HY * iz)];
+    const double2 v = vd[vd_off[m] + (size_t)(a - tx_off[m]) * Nfm + fm];
+    sx += hv.x * v.x - hv.y * v.y;
+    sy += hv.x * v.y + hv.y * v.x;
+  }
+  P[(size_t)pair * N_pts + p] = make_float2((float)sx, (float)sy);
+}
+
+void launch_synth_pinc_structured(const Plan& P, const double2* h64, const int* d_meas_Nfm, const int* d_tx_off,
+                                  const int* d_tx_el, const double2* d_vd, const int* d_vd_off, hipStream_t s) {
+  const int Ny = P.nd == 3 ? P.Ny : 1, Ney = P.nd == 3 ? P.Ney : 1;
+  hipLaunchKernelGGL(k_synth_pinc_s, dim3((P.N_pts + 255) / 256, P.N_pairs), dim3(256), 0, s, h64, P.d_ext,
+                     P.nd == 3 ? P.d_exty : nullptr, P.Nx, Ny, P.Nex, P.Np, P.Npy, (P.Nex - 1) * P.Np,
+                     (Ney - 1) * P.Npy, P.HR, P.HY, P.d_pair_fg, P.d_pair_meas, P.d_meas_pair_base, d_meas_Nfm,
+                     d_tx_off, d_tx_el, d_vd, d_vd_off, P.N_pts, P.H_plane, P.d_p);
+}
+
 }  // namespace born
+

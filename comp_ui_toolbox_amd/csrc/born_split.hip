@@ -634,7 +634,7 @@ static size_t adj_split_lds_bytes() {
 // launchers
 // ======================================================================
 bool fwd_split_supported(const Plan& P) {
-  return P.structured && P.Np == 4 && P.Ne_pad >= 32 && P.Ne_pad <= 128;
+  return P.structured && P.nd == 2 && P.Np == 4 && P.Ne_pad >= 32 && P.Ne_pad <= 128;
 }
 
 void launch_fwd_split(const LaunchSet& L, int n_cu, hipStream_t s) {

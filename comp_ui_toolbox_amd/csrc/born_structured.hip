@@ -70,6 +70,7 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(const Inst* __restrict__ ins
   int Np, XP, QL, HN;           // layout of the prefetched row
   int eidx[HPT];
   int fg, rr, pr[NC];           // chunk of the prefetched row
+  int ey;                       // y slice of the prefetched row (3-D; 0 in 2-D)
   int z, z_end, item = blockIdx.x;
 
 #define BORN_FWD_INST()                                                                   \
@@ -96,14 +97,16 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(const Inst* __restrict__ ins
   // columns read the zero row of the p_inc table
 #define BORN_FWD_DESC()                                                                   \
   {                                                                                       \
-    const int li_ = item - I->item_begin;                                                 \
-    const int ci_ = li_ / I->NZB;                                                         \
+    const int li_ = item - I->item_begin;      /* = (chunk * Ney + ey) * NZB + zb */       \
+    const int ci_ = li_ / (I->Ney * I->NZB);                                              \
+    const int rem_ = li_ - ci_ * (I->Ney * I->NZB);                                       \
+    ey = rem_ / I->NZB;                                                                   \
     const Chunk c_ = chunks[I->chunk_base + ci_];                                         \
     fg = c_.fg;                                                                           \
     rr = c_.r;                                                                            \
     _Pragma("unroll") for (int q = 0; q < NC; ++q) pr[q] = chunk_pairs[(I->chunk_base + ci_) * NC + q]; \
-    z = (li_ - ci_ * I->NZB) * I->ZR;                                                     \
-    z_end = min(I->Nz, z + I->ZR);                                                        \
+    z = (rem_ - ey * I->NZB) * I->ZR;                                                     \
+    z_end = min(I->n_rows, z + I->ZR);                                                    \
   }
 
   float2 hreg[HPT];
@@ -114,7 +117,7 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(const Inst* __restrict__ ins
   // address is valid (clamped), out-of-range entries are zeroed in the fill
 #define BORN_FWD_PREFETCH(zz)                                                           \
   {                                                                                     \
-    const float2* hrow_ = I->h + (size_t)fg * I->Hp + (size_t)(zz) * I->HR;             \
+    const float2* hrow_ = I->h + (size_t)fg * I->Hp + (size_t)I->hrow[ey * I->n_rows + (zz)] * I->HR; \
     _Pragma("unroll") for (int j = 0; j < HPT; ++j) hreg[j] = hrow_[max(eidx[j], 0)];  \
     const size_t rowo_ = (size_t)(zz) * I->Nx;                                          \
     const float2* grow_ = I->gm + (size_t)rr * I->N_pts + rowo_;                        \
@@ -361,20 +364,25 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d(const Inst* __restrict__ ins
   }
 
   float2 hreg[HPT], wreg[WPT], preg[PPT];
-  // Loads of chunk `cidx` (frequency FG, p_inc rows PV of this thread's
-  // entries), all from valid addresses: window entries outside the table are
-  // zeroed when written to LDS, unused columns meet zero weights, points past
-  // the row end are never written out.
-#define BORN_ADJ_PREFETCH(cidx, FG, PV)                                                          \
+  // A step is (chunk, y slice ey); 2-D has one slice.  Loads of a step: the
+  // window of the table row of (ey, this row) at frequency FG and the weights
+  // of the slice; the p_inc rows of the chunk's columns (PV) only with its
+  // first slice.  All addresses are valid: window entries outside the table
+  // are zeroed when written to LDS, unused columns meet zero weights, points
+  // past the row end are never written out.
+#define BORN_ADJ_PREFETCH_HW(cidx, EY, FG)                                                       \
   {                                                                                              \
-    const float2* hrow_ = I->h + (size_t)(FG) * I->Hp + (size_t)z * I->HR;                          \
+    const float2* hrow_ = I->h + (size_t)(FG) * I->Hp + (size_t)I->hrow[(EY) * I->n_rows + z] * I->HR; \
     _Pragma("unroll") for (int j = 0; j < HPT; ++j) hreg[j] = hrow_[max(eidx[j], 0)];           \
-    const float2* wblk_ = I->Wc + (size_t)(cidx) * NEP * NC;                                      \
+    const float2* wblk_ = I->Wc + ((size_t)(cidx) * I->Ney + (EY)) * NEP * NC;                   \
     _Pragma("unroll") for (int j = 0; j < WPT; ++j) wreg[j] = wblk_[tid + 256 * j];             \
+  }
+#define BORN_ADJ_PREFETCH_P(PV)                                                                  \
+  {                                                                                              \
     _Pragma("unroll") for (int j = 0; j < PPT; ++j) {                                           \
       const int idx_ = tid + 256 * j;                                                            \
-      const int gi_ = min(i_s + idx_ % SEG, I->Nx - 1);                                           \
-      preg[j] = I->P[(size_t)(PV)[j] * I->N_pts + (size_t)z * I->Nx + gi_];                         \
+      const int gi_ = min(i_s + idx_ % SEG, I->Nx - 1);                                          \
+      preg[j] = I->P[(size_t)(PV)[j] * I->N_pts + (size_t)z * I->Nx + gi_];                      \
     }                                                                                            \
   }
 #define BORN_ADJ_DESC(cidx, FG, R, PV)                                                           \
@@ -386,9 +394,12 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d(const Inst* __restrict__ ins
       (PV)[j] = chunk_pairs[(I->chunk_base + (cidx)) * NC + (tid + 256 * j) / SEG];              \
   }
 
+  const int Ney = I->Ney;
   int fg0, r0, pv0[PPT];
   BORN_ADJ_DESC(0, fg0, r0, pv0)
-  BORN_ADJ_PREFETCH(0, fg0, pv0)
+  BORN_ADJ_PREFETCH_HW(0, 0, fg0)
+  BORN_ADJ_PREFETCH_P(pv0)
+  int pf_fg = fg0;   // frequency of the chunk of the prefetched step
   int nfg = fg0, nr = r0, npv[PPT];
 #pragma unroll
   for (int j = 0; j < PPT; ++j) npv[j] = pv0[j];
@@ -402,8 +413,11 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d(const Inst* __restrict__ ins
     for (int v = 0; v < 4; ++v) gx[mw][v] = gy[mw][v] = 0.f;
 
   const int u_base = ib - i_s + col + (NEP - 1) * Np;
+  v4f P1[MW], P2[MW], P3[MW];
+  const int n_steps = I->n_chunks * Ney;
 
-  for (int ci = 0; ci < I->n_chunks; ++ci) {
+  for (int st = 0; st < n_steps; ++st) {
+    const int ci = st / Ney, ey = st - ci * Ney;
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < HPT; ++j) {
@@ -412,25 +426,37 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d(const Inst* __restrict__ ins
     }
 #pragma unroll
     for (int j = 0; j < WPT; ++j) Ws[((tid >> 4) + 16 * j) * NCS + (tid & 15)] = wreg[j];
+    if (ey == 0) {
 #pragma unroll
-    for (int j = 0; j < PPT; ++j) {
-      const int idx = tid + 256 * j;
-      const int pc = idx / SEG, pi = idx - pc * SEG;
-      Ps[pc * PS + pi] = preg[j];
+      for (int j = 0; j < PPT; ++j) {
+        const int idx = tid + 256 * j;
+        const int pc = idx / SEG, pi = idx - pc * SEG;
+        Ps[pc * PS + pi] = preg[j];
+      }
     }
     __syncthreads();
     const int r = cur_r;
-    const bool has_next = ci + 1 < I->n_chunks;
-    const int next_r = nr;
-    if (has_next) BORN_ADJ_PREFETCH(ci + 1, nfg, npv)
-    if (ci + 2 < I->n_chunks) BORN_ADJ_DESC(ci + 2, nfg, nr, npv)
+    const bool has_next = st + 1 < n_steps;
+    const bool last_slice = ey == Ney - 1;
+    const int next_r = last_slice ? nr : cur_r;
+    if (has_next) {
+      if (!last_slice) {
+        BORN_ADJ_PREFETCH_HW(ci, ey + 1, pf_fg)
+      } else {
+        BORN_ADJ_PREFETCH_HW(ci + 1, 0, nfg)
+        BORN_ADJ_PREFETCH_P(npv)
+        pf_fg = nfg;
+        if (ci + 2 < I->n_chunks) BORN_ADJ_DESC(ci + 2, nfg, nr, npv)
+      }
+    }
 
-    v4f P1[MW], P2[MW], P3[MW];
+    if (ey == 0) {
 #pragma unroll
-    for (int mw = 0; mw < MW; ++mw) {
-      P1[mw] = vzero();
-      P2[mw] = vzero();
-      P3[mw] = vzero();
+      for (int mw = 0; mw < MW; ++mw) {
+        P1[mw] = vzero();
+        P2[mw] = vzero();
+        P3[mw] = vzero();
+      }
     }
 #pragma unroll 4
     for (int k0 = 0; k0 < NEP; k0 += 4) {
@@ -447,6 +473,7 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d(const Inst* __restrict__ ins
         P3[mw] = mfma16(adiff, bsum, P3[mw]);
       }
     }
+    if (!last_slice) continue;
     // conj(A) W: Re = P1 + P2, Im = P3 - P1 + P2; then gamma_hat += conj(p) * Y
 #pragma unroll
     for (int mw = 0; mw < MW; ++mw) {
@@ -497,7 +524,8 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d(const Inst* __restrict__ ins
     }
     cur_r = next_r;
   }
-#undef BORN_ADJ_PREFETCH
+#undef BORN_ADJ_PREFETCH_HW
+#undef BORN_ADJ_PREFETCH_P
 #undef BORN_ADJ_DESC
 }
 

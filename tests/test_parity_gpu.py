@@ -74,7 +74,11 @@ def test_unstructured_shift_map_uses_general_path(torch_cuda):
     assert O.rel_l2(op.adjoint(u), O.adjoint(T, u)) < TOL_FP32
 
 
-def test_3d_tables_general_path(torch_cuda):
+@pytest.mark.parametrize("general", [False, True])
+def test_3d_tables(torch_cuda, general):
+    """3-D grid from the reference's tables: the separable lateral maps are
+    found in the shift map (structured path, one x-Toeplitz product per y
+    slice) or the shift map is gathered (general path)."""
     arr = C.ArraySpec((3, 2), (3e-4, 3e-4), (2.5e-4, 2.5e-4), (2, 2))
     fov = C.FovSpec((8, 6, 4), (1.5e-4, 1.5e-4, 1.5e-4), 2e-3)
     spec = C.SequenceSpec("t3", arr, C.MediumSpec(), fov, np.array([2.5e6, 4e6, 5e6]),
@@ -83,10 +87,37 @@ def test_3d_tables_general_path(torch_cuda):
                           [C.identity_mixes(6), C.identity_mixes(6)])
     T = O.build_tables(spec)
     op = OperatorBorn.from_tables(T)
+    assert op.path == "structured"
+    if general:
+        op.force_general(True)
     g = rnd((T.N_points, 2), 7)
     u = rnd((T.N_observations(), 2), 8)
-    assert O.rel_l2(op.forward(g), O.forward(T, g)) < TOL_FP32
-    assert O.rel_l2(op.adjoint(u), O.adjoint(T, u)) < TOL_FP32
+    e_f = O.rel_l2(op.forward(g), O.forward(T, g))
+    e_a = O.rel_l2(op.adjoint(u), O.adjoint(T, u))
+    log("3-D tables (%s): forward %.2e adjoint %.2e" % ("general" if general else "structured", e_f, e_a))
+    assert e_f < TOL_FP32 and e_a < TOL_FP32
+
+
+@pytest.mark.parametrize("shape", [(20, 12, 5), (10, 7, 3)])
+def test_3d_geometry_structured(torch_cuda, shape):
+    """3-D plans built from geometry on the structured path: an 8 x 4 matrix
+    array at pitch/2 over a FOV as wide as the aperture (mirrored tables) and
+    over a narrower one (tables on the extended grid), against every
+    element's own transfer function."""
+    arr = C.ArraySpec((8, 4), (3e-4, 3e-4), (2.7e-4, 2.7e-4), (2, 2))
+    fov = C.FovSpec(shape, (1.5e-4, 1.5e-4, 1.5e-4), 2e-3)
+    waves = [C.qsw_wave(arr, np.array([x, 0.0, -1e-2]), 1500.0) for x in (0.0, 6e-4)]
+    spec = C.SequenceSpec("t3g", arr, C.MediumSpec(), fov, np.array([2.5e6, 4e6, 5.5e6]), waves,
+                          [C.identity_mixes(32) for _ in waves])
+    op = OperatorBorn.from_sequence(spec)
+    assert op.path == "structured"
+    T = O.direct_tables(spec)
+    g = rnd((T.N_points, 2), 9)
+    u = rnd((T.N_observations(), 2), 10)
+    e_f = O.rel_l2(op.forward(g), O.forward(T, g))
+    e_a = O.rel_l2(op.adjoint(u), O.adjoint(T, u))
+    log("3-D geometry %s: forward %.2e adjoint %.2e" % (shape, e_f, e_a))
+    assert e_f < TOL_FP32 and e_a < TOL_FP32
 
 
 # -------------------------------------------------------------- config 0
