@@ -371,263 +371,298 @@ static size_t fwd_split_lds_bytes() {
 // fixed order with gamma_hat kept in registers (no atomics).
 // For Np = 4 the segment's points split into 4 phases ph = i mod 4 of 64
 // points s; for one phase the adjoint is the Toeplitz product
-//     Y[s][c] = sum_e conj(H(4 s + ph - 4 e + off)) W[e][c]
-// whose table index moves by one entry per element and per point.  Each
-// phase row is stored reversed (ascending in e) in four copies shifted by
-// 0..3 entries; an MFMA tile takes the 16 points s = s0 + 4 r (r = 0..15), so
-// all its lanes share one copy and every 8-byte read is aligned.
-// Wave w owns phase w (4 tiles, s0 = 0..3); K = elements in blocks of 32.
+//     Y[s][c] = sum_e conj(H(4 s + ph - 4 e + off)) W[e][c].
+// With the phase row stored reversed, Hr[m] with m = 64 - s + e, tile t
+// (points s = 16 t + r, r = 0..15) at K-block kb (elements 32 kb + 8 kk + j)
+// reads Hr[64 + 16 (2 kb - t) - r + 8 kk + j]: tile t + 2 at K-block kb + 1
+// reads the same data as tile t at kb.  A compute wave owns one phase and
+// walks the diagonals delta = 2 kb - t, loading one A set per diagonal
+// (double-buffered) with every K-block of W in registers.
+// The reversed row is stored in 4 copies, copy d holding Hr[m] at position
+// m - d + PADL, so lane r reads its 8 entries 8-byte aligned from copy
+// d = (-r) mod 4; the copies start at offsets chosen so that the 16 lanes of
+// a read cover all 32 banks once.
 //
-// LDS: Ar[comp][ph][copy][LR] (u16), Wt[comp][c][NEP + 8] (u16), Ps[c][SEG + 4] (float2)
+// The workgroup has 8 waves: waves 0-3 compute on one LDS buffer while
+// waves 4-7 load the next chunk and write it, split into BF16 hi/lo planes,
+// to the other buffer; one barrier per chunk swaps the buffers.
+//
+// LDS buffer: Ar[comp][ph][CPB] (u16, 4 copies per block), Wt[comp][c][WS] (u16), Ps[c][PSS] (float2)
 template <int MT>
-__global__ __launch_bounds__(256, 2) void k_adj_s2d_split(const Inst* __restrict__ inst, int n_inst, OutPtrs outs,
+struct AdjSplitLayout {
+  static constexpr int NP = 4;
+  static constexpr int NEP = 16 * MT;
+  static constexpr int SPH = SEG / NP;                          // points per phase
+  static constexpr int LQ = SPH + NEP;                          // reversed phase-row entries
+  static constexpr int PADL = 4;                                // lead pad of every copy
+  static constexpr int LR = ((LQ + PADL - 4 + 63) / 64) * 64 + 4;   // copy stride, 4 mod 64
+  static constexpr int CPB = ((4 * LR + 40 - 16 + 63) / 64) * 64 + 16;  // (comp, phase) block, 16 mod 64
+  static constexpr int WS = NEP + 8;                            // padded W row
+  static constexpr int PSS = SEG + 1;                           // padded p_inc row (float2)
+  static constexpr int AR_U16 = 4 * NP * CPB;
+  static constexpr int WT_U16 = 4 * NC * WS;
+  static constexpr int PS_OFF = ((AR_U16 + WT_U16 + 7) / 8) * 8;    // u16 offset of Ps
+  static constexpr int BUF_U16 = ((PS_OFF + NC * PSS * 4 + 7) / 8) * 8;  // one buffer, 16-byte multiple
+  static constexpr size_t bytes = (size_t)2 * BUF_U16 * 2;     // two buffers
+};
+__device__ __forceinline__ int copy_offset(int d) { return d == 1 ? 16 : (d == 2 ? 28 : (d == 3 ? 40 : 0)); }
+
+template <int MT>
+__global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict__ inst, int n_inst, OutPtrs outs,
                                                           const Chunk* __restrict__ chunks,
                                                           const int* __restrict__ chunk_pairs) {
   extern __shared__ unsigned short ldsu[];
-  constexpr int NP = 4;
-  constexpr int NEP = 16 * MT;
-  constexpr int SPH = SEG / NP;                 // points per phase
-  constexpr int LQ = SPH + NEP;                 // phase-row entries used
-  constexpr int LR = ((LQ + 3 + 3) / 4) * 4 + 4; // padded copy length
-  constexpr int WS = NEP + 8;                   // padded W row
-  constexpr int PSS = SEG + 4;                  // padded p_inc row (float2)
+  typedef AdjSplitLayout<MT> Lay;
+  constexpr int NP = Lay::NP, NEP = Lay::NEP, SPH = Lay::SPH, LQ = Lay::LQ, PADL = Lay::PADL;
+  constexpr int LR = Lay::LR, CPB = Lay::CPB, WS = Lay::WS, PSS = Lay::PSS;
+  constexpr int KB = MT / 2;                    // K-blocks of 32 elements
   constexpr int L = SEG + NP * (NEP - 1);       // extended-row window entries
   constexpr int HPT = (L + 255) / 256;
-  constexpr int WPT = NEP * NC / 256;           // W entries per thread
-  constexpr int PPT = NC * SEG / 256;           // p_inc entries per thread
-  unsigned short* Ar = ldsu;                                    // [4][NP][4][LR]
-  unsigned short* Wt = ldsu + 4 * NP * 4 * LR;                  // [4][NC][WS]
-  float2* Ps = reinterpret_cast<float2*>(ldsu + ((4 * NP * 4 * LR + 4 * NC * WS + 7) / 8) * 8);  // [NC][PSS]
+  constexpr int WPT = NEP * NC / 256;           // W entries per fill thread
+  constexpr int PPT = NC * SEG / 256;           // p_inc entries per fill thread
 
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, w = (tid >> 6) & 3;
+  const bool filler = tid >= 256;
+  const int ft = tid & 255;                     // thread index within its role
   const int kk = lane >> 4, col = lane & 15;
   int ki = 0;
   while (ki + 1 < n_inst && (int)blockIdx.x >= inst[ki + 1].wg_begin_s) ++ki;
   const Inst Iv = inst[ki];   // loop-invariant: held in scalar registers
   const Inst* I = &Iv;
-  float2* __restrict__ g_out = outs.p[ki];
   const int lw = (int)blockIdx.x - I->wg_begin_s;
   const int z = lw / I->adj_segs_s;
   const int i_s = (lw - z * I->adj_segs_s) * SEG;
-  const int t_shift = i_s - (NEP - I->N_e) * NP;  // window entry u holds Hext[u + t_shift]
+  const int n_chunks = I->n_chunks;
 
-  // gather indices of the window (chunk-invariant); hfull: all inside the table
-  const bool hfull = t_shift >= 0 && t_shift + L <= I->N_ext;
-  int eidx[HPT];
+  if (filler) {
+    // ---------------------------------------------------------------- fill waves
+    const int t_shift = i_s - (NEP - I->N_e) * NP;  // window entry u holds Hext[u + t_shift]
+    const bool hfull = t_shift >= 0 && t_shift + L <= I->N_ext;
+    int eidx[HPT];
 #pragma unroll
-  for (int j = 0; j < HPT; ++j) {
-    const int u = tid + 256 * j;
-    const int t = u + t_shift;
-    eidx[j] = (u < L && t >= 0 && t < I->N_ext) ? I->ext[t] : -1;
-  }
-  float2 hreg[HPT], wreg[WPT], preg[PPT];
-  // Loads of chunk `cidx` from its descriptor (frequency FG, column pairs PR),
-  // all from valid addresses: window entries outside the table are zeroed
-  // when written to LDS; p_inc rows of unused columns are read from row 0 and
-  // meet zero weights; points past the row end are never written out.
-#define BORN_ADJS_PREFETCH(cidx, FG, PR)                                                         \
-  {                                                                                              \
-    const float2* hrow_ = I->h + (size_t)(FG) * I->Hp + (size_t)z * I->HR;                          \
-    _Pragma("unroll") for (int j = 0; j < HPT; ++j) hreg[j] = hrow_[max(eidx[j], 0)];           \
-    const float2* wblk_ = I->Wc + (size_t)(cidx) * NEP * NC;                                      \
-    _Pragma("unroll") for (int j = 0; j < WPT; ++j) wreg[j] = wblk_[tid + 256 * j];             \
-    const int gi_ = min(i_s + tid, I->Nx - 1);                                                    \
-    _Pragma("unroll") for (int j = 0; j < PPT; ++j)                                             \
-      preg[j] = I->P[(size_t)(PR)[j] * I->N_pts + (size_t)z * I->Nx + gi_];                         \
-  }
-  // descriptor of a chunk: uniform, so it lives in scalar registers
-#define BORN_ADJS_DESC(cidx, FG, R, PR)                                                          \
-  {                                                                                              \
-    const Chunk c_ = chunks[I->chunk_base + (cidx)];                                             \
-    FG = c_.fg;                                                                                  \
-    R = c_.r;                                                                                    \
-    _Pragma("unroll") for (int q = 0; q < NC; ++q) (PR)[q] = chunk_pairs[(I->chunk_base + (cidx)) * NC + q]; \
+    for (int j = 0; j < HPT; ++j) {
+      const int u = ft + 256 * j;
+      const int t = u + t_shift;
+      eidx[j] = (u < L && t >= 0 && t < I->N_ext) ? I->ext[t] : -1;
+    }
+    // two register sets: the loads of chunk c + 2 are issued when chunk c has
+    // been written to LDS, so each load has two chunk periods to arrive
+    float2 hA[HPT], wA[WPT], pA[PPT], hB[HPT], wB[WPT], pB[PPT];
+    const int gi = min(i_s + ft, I->Nx - 1);
+    // Loads of chunk `cidx`, all from valid addresses: window entries outside
+    // the table are zeroed when written to LDS; p_inc rows of unused columns
+    // read the zero row; points past the row end are never written out.
+#define BORN_ADJS_PREFETCH(CIDX, H, WV, PV)                                                      \
+    {                                                                                            \
+      const int cidx_ = (CIDX);                                                                  \
+      const Chunk c_ = chunks[I->chunk_base + cidx_];                                            \
+      const float2* hrow_ = I->h + (size_t)c_.fg * I->Hp + (size_t)z * I->HR;                     \
+      _Pragma("unroll") for (int j = 0; j < HPT; ++j) H[j] = hrow_[max(eidx[j], 0)];             \
+      const float2* wblk_ = I->Wc + (size_t)cidx_ * NEP * NC;                                    \
+      _Pragma("unroll") for (int j = 0; j < WPT; ++j) WV[j] = wblk_[ft + 256 * j];               \
+      const int* prs_ = chunk_pairs + (size_t)(I->chunk_base + cidx_) * NC;                      \
+      _Pragma("unroll") for (int j = 0; j < PPT; ++j)                                            \
+        PV[j] = I->P[(size_t)prs_[j] * I->N_pts + (size_t)z * I->Nx + gi];                       \
+    }
+#define BORN_ADJS_FILL(BUF, H, WV, PV)                                                           \
+    {                                                                                            \
+      unsigned short* Ar = ldsu + (BUF) * Lay::BUF_U16;                                          \
+      unsigned short* Wt = Ar + Lay::AR_U16;                                                     \
+      float2* Ps = reinterpret_cast<float2*>(Ar + Lay::PS_OFF);                                  \
+      /* window entry u -> phase p = u mod 4, reversed index m = LQ - 1 - u / 4; */              \
+      /* copy d holds m at m - d + PADL (>= 1) */                                                \
+      _Pragma("unroll") for (int j = 0; j < HPT; ++j) {                                          \
+        const int u = ft + 256 * j;                                                              \
+        if (u < L) {                                                                             \
+          const int p = u & 3, m = (LQ - 1) - (u >> 2);                                          \
+          float2 hv = H[j];                                                                      \
+          if (!hfull && eidx[j] < 0) hv = make_float2(0.f, 0.f);                                 \
+          unsigned int hi, lo;                                                                   \
+          split_pair(hv.x, -hv.y, hi, lo); /* conj */                                            \
+          const unsigned short v[4] = {(unsigned short)hi, (unsigned short)lo,                   \
+                                       (unsigned short)(hi >> 16), (unsigned short)(lo >> 16)};  \
+          _Pragma("unroll") for (int cp = 0; cp < 4; ++cp) {                                     \
+            unsigned short* row = Ar + (cp * NP + p) * CPB + m + PADL;                           \
+            row[0] = v[cp];                                                                      \
+            row[LR + 16 - 1] = v[cp];                                                            \
+            row[2 * LR + 28 - 2] = v[cp];                                                        \
+            row[3 * LR + 40 - 3] = v[cp];                                                        \
+          }                                                                                      \
+        }                                                                                        \
+      }                                                                                          \
+      /* weights: entries j and j + 1 of a thread are elements k and k + 16 of one column */     \
+      _Pragma("unroll") for (int j = 0; j < WPT; j += 2) {                                       \
+        const int idx = ft + 256 * j;                                                            \
+        const int k = idx / NC, wc = idx - k * NC;                                               \
+        unsigned int rh, rl, ih, il;                                                             \
+        split_pair(WV[j].x, WV[j + 1].x, rh, rl);                                                \
+        split_pair(WV[j].y, WV[j + 1].y, ih, il);                                                \
+        unsigned short* wd = &Wt[wc * WS + k];                                                   \
+        wd[0 * NC * WS] = (unsigned short)rh;                                                    \
+        wd[0 * NC * WS + 16] = (unsigned short)(rh >> 16);                                       \
+        wd[1 * NC * WS] = (unsigned short)rl;                                                    \
+        wd[1 * NC * WS + 16] = (unsigned short)(rl >> 16);                                       \
+        wd[2 * NC * WS] = (unsigned short)ih;                                                    \
+        wd[2 * NC * WS + 16] = (unsigned short)(ih >> 16);                                       \
+        wd[3 * NC * WS] = (unsigned short)il;                                                    \
+        wd[3 * NC * WS + 16] = (unsigned short)(il >> 16);                                       \
+      }                                                                                          \
+      _Pragma("unroll") for (int j = 0; j < PPT; ++j) Ps[j * PSS + ft] = PV[j]; /* column j, point ft */ \
+    }
+    // every step issues its loads (clamped to the last chunk), so the loads in
+    // flight are the same on every path and each fill waits only for its own
+    const int last = n_chunks - 1;
+    BORN_ADJS_PREFETCH(0, hA, wA, pA)
+    BORN_ADJS_FILL(0, hA, wA, pA)
+    BORN_ADJS_PREFETCH(min(1, last), hB, wB, pB)
+    BORN_ADJS_PREFETCH(min(2, last), hA, wA, pA)
+    __syncthreads();
+    for (int ci = 0; ci < n_chunks; ci += 2) {
+      // step ci: chunk ci + 1 (set B) -> buffer (ci + 1) & 1; set B <- chunk ci + 3
+      if (ci + 1 < n_chunks) BORN_ADJS_FILL((ci + 1) & 1, hB, wB, pB)
+      BORN_ADJS_PREFETCH(min(ci + 3, last), hB, wB, pB)
+      __syncthreads();
+      if (ci + 1 >= n_chunks) break;
+      // step ci + 1: chunk ci + 2 (set A) -> buffer ci & 1; set A <- chunk ci + 4
+      if (ci + 2 < n_chunks) BORN_ADJS_FILL(ci & 1, hA, wA, pA)
+      BORN_ADJS_PREFETCH(min(ci + 4, last), hA, wA, pA)
+      __syncthreads();
+    }
+#undef BORN_ADJS_PREFETCH
+#undef BORN_ADJS_FILL
+    return;
   }
 
-  int fg0, r0, pr0[NC];
-  BORN_ADJS_DESC(0, fg0, r0, pr0)
-  BORN_ADJS_PREFETCH(0, fg0, pr0)
-  int nfg = fg0, nr = r0, npr[NC];
-#pragma unroll
-  for (int q = 0; q < NC; ++q) npr[q] = pr0[q];
-  if (I->n_chunks > 1) BORN_ADJS_DESC(1, nfg, nr, npr)
-  int cur_r = r0;
-
-  // gamma_hat partials: tile s0 (0..3), v (0..3) -> point s = s0 + 4*(4*kk + v), i = 4 s + ph
+  // ---------------------------------------------------------------- compute waves
+  float2* __restrict__ g_out = outs.p[ki];
+  // gamma_hat partials: tile t, v (0..3) -> point s = 16 t + 4 kk + v, i = 4 s + ph
   float gx[4][4], gy[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int v = 0; v < 4; ++v) gx[a][v] = gy[a][v] = 0.f;
-
   const int ph = w;
-  for (int ci = 0; ci < I->n_chunks; ++ci) {
-    __syncthreads();
-    // window entry u (point offset i = u - NP*(NEP-1) + NP*e ... ) -> phase rows:
-    // u = (i - i_s) + NP*(NEP - 1 - e); with i - i_s = NP*s + ph:
-    //   u = NP*(s + NEP - 1 - e) + ph  ->  phase ph, index q = s + NEP - 1 - e
-    // reversed: m = (LQ - 1) - q = (SPH - s) + e; copy d holds m + d at position m
-#pragma unroll
-    for (int j = 0; j < HPT; ++j) {
-      const int u = tid + 256 * j;
-      if (u < L) {
-        const int p = u & 3, q = u >> 2;
-        const int m = (LQ - 1) - q;
-        float2 hv = hreg[j];
-        if (!hfull && eidx[j] < 0) hv = make_float2(0.f, 0.f);
-        unsigned int hi, lo;
-        split_pair(hv.x, -hv.y, hi, lo);   // conj
-        const unsigned short v[4] = {(unsigned short)hi, (unsigned short)lo, (unsigned short)(hi >> 16),
-                                     (unsigned short)(lo >> 16)};
-#pragma unroll
-        for (int cp = 0; cp < 4; ++cp) {
-#pragma unroll
-          for (int d = 0; d < 4; ++d) {
-            const int mm = m - d;
-            if (mm >= 0) Ar[((cp * NP + p) * 4 + d) * LR + mm] = v[cp];
-          }
-        }
-      }
-    }
-    // weights: entries j and j + 1 of a thread are elements k and k + 16 of one column
-#pragma unroll
-    for (int j = 0; j < WPT; j += 2) {
-      const int idx = tid + 256 * j;
-      const int k = idx / NC, wc = idx - k * NC;
-      unsigned int rh, rl, ih, il;
-      split_pair(wreg[j].x, wreg[j + 1].x, rh, rl);
-      split_pair(wreg[j].y, wreg[j + 1].y, ih, il);
-      unsigned short* wd = &Wt[wc * WS + k];
-      wd[0 * NC * WS] = (unsigned short)rh;
-      wd[0 * NC * WS + 16] = (unsigned short)(rh >> 16);
-      wd[1 * NC * WS] = (unsigned short)rl;
-      wd[1 * NC * WS + 16] = (unsigned short)(rl >> 16);
-      wd[2 * NC * WS] = (unsigned short)ih;
-      wd[2 * NC * WS + 16] = (unsigned short)(ih >> 16);
-      wd[3 * NC * WS] = (unsigned short)il;
-      wd[3 * NC * WS + 16] = (unsigned short)(il >> 16);
-    }
-#pragma unroll
-    for (int j = 0; j < PPT; ++j) {
-      const int idx = tid + 256 * j;
-      const int pc = idx / SEG, pi = idx - pc * SEG;
-      Ps[pc * PSS + pi] = preg[j];
-    }
-    __syncthreads();
+  // lane's A read position (diagonal 0, comp 0): copy d = (-r) mod 4 of phase ph
+  const int dcp = (4 - (col & 3)) & 3;
+  const int abase = ph * CPB + dcp * LR + copy_offset(dcp) + (SPH - col + 8 * kk - dcp) + PADL;
+  int cur_r = chunks[I->chunk_base].r;
+  __syncthreads();
+  for (int ci = 0; ci < n_chunks; ++ci) {
+    const unsigned short* Ar = ldsu + (ci & 1) * Lay::BUF_U16;
+    const unsigned short* Wt = Ar + Lay::AR_U16;
+    const float2* Ps = reinterpret_cast<const float2*>(Ar + Lay::PS_OFF);
     const int r = cur_r;
-    const bool has_next = ci + 1 < I->n_chunks;
-    const int next_r = nr;
-    if (has_next) BORN_ADJS_PREFETCH(ci + 1, nfg, npr)
-    if (ci + 2 < I->n_chunks) BORN_ADJS_DESC(ci + 2, nfg, nr, npr)
+    const bool has_next = ci + 1 < n_chunks;
+    const int next_r = has_next ? chunks[I->chunk_base + ci + 1].r : r;
 
+    // every K-block of W (B operand)
+    uint4 W4[KB][4];
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+      for (int cp = 0; cp < 4; ++cp)
+        W4[kb][cp] = *reinterpret_cast<const uint4*>(&Wt[(cp * NC + col) * WS + 32 * kb + 8 * kk]);
     // Y for the 4 tiles of this phase: conj(H) W with conj folded into the
     // table (imag part stored negated): Re = Ar Wr - Ai' Wi, Im = Ar Wi + Ai' Wr
     v4f YR[4], YI[4];
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      YR[a] = vz();
-      YI[a] = vz();
+    for (int t = 0; t < 4; ++t) {
+      YR[t] = vz();
+      YI[t] = vz();
     }
-#pragma unroll 1
-    for (int kb = 0; kb < MT / 2; ++kb) {
-      const int eoff = 32 * kb + 8 * kk;
-      const uint4 wrh = *reinterpret_cast<const uint4*>(&Wt[(0 * NC + col) * WS + eoff]);
-      const uint4 wrl = *reinterpret_cast<const uint4*>(&Wt[(1 * NC + col) * WS + eoff]);
-      const uint4 wih = *reinterpret_cast<const uint4*>(&Wt[(2 * NC + col) * WS + eoff]);
-      const uint4 wil = *reinterpret_cast<const uint4*>(&Wt[(3 * NC + col) * WS + eoff]);
-      const uint4 nwih = neg8(wih), nwil = neg8(wil);
+    constexpr int DMIN = -3, DMAX = 2 * (KB - 1);
+    uint4 Ac[4], An[4];
+#define BORN_ADJS_LOAD_A(DST, DL)                                                                \
+  _Pragma("unroll") for (int cp = 0; cp < 4; ++cp) {                                            \
+    const unsigned short* a_ = Ar + cp * NP * CPB + abase + 16 * (DL);                           \
+    DST[cp] = join(*reinterpret_cast<const uint2*>(a_), *reinterpret_cast<const uint2*>(a_ + 4)); \
+  }
+    BORN_ADJS_LOAD_A(Ac, DMIN)
 #pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        // tile s0 = a, row r = col (lane's A row): s = a + 4*col; m0 = SPH - s + e,
-        // e = eoff + j; copy d = m0 & 3 is the same for all lanes of the tile
-        const int m0 = SPH - (a + 4 * col) + eoff;
-        const int d = m0 & 3;
-        const int base = ((0 * NP + ph) * 4 + d) * LR + (m0 - d);
-        const int cstride = NP * 4 * LR;
-        const uint4 arh = join(*reinterpret_cast<const uint2*>(&Ar[base]),
-                               *reinterpret_cast<const uint2*>(&Ar[base + 4]));
-        const uint4 arl = join(*reinterpret_cast<const uint2*>(&Ar[base + cstride]),
-                               *reinterpret_cast<const uint2*>(&Ar[base + cstride + 4]));
-        const uint4 aih = join(*reinterpret_cast<const uint2*>(&Ar[base + 2 * cstride]),
-                               *reinterpret_cast<const uint2*>(&Ar[base + 2 * cstride + 4]));
-        const uint4 ail = join(*reinterpret_cast<const uint2*>(&Ar[base + 3 * cstride]),
-                               *reinterpret_cast<const uint2*>(&Ar[base + 3 * cstride + 4]));
-        YR[a] = mfma32(arh, wrh, YR[a]);
-        YR[a] = mfma32(arh, wrl, YR[a]);
-        YR[a] = mfma32(arl, wrh, YR[a]);
-        YR[a] = mfma32(aih, nwih, YR[a]);
-        YR[a] = mfma32(aih, nwil, YR[a]);
-        YR[a] = mfma32(ail, nwih, YR[a]);
-        YI[a] = mfma32(arh, wih, YI[a]);
-        YI[a] = mfma32(arh, wil, YI[a]);
-        YI[a] = mfma32(arl, wih, YI[a]);
-        YI[a] = mfma32(aih, wrh, YI[a]);
-        YI[a] = mfma32(aih, wrl, YI[a]);
-        YI[a] = mfma32(ail, wrh, YI[a]);
+    for (int dl = DMIN; dl <= DMAX; ++dl) {
+      if (dl < DMAX) BORN_ADJS_LOAD_A(An, dl + 1)
+      const uint4 nih = neg8(Ac[2]), nil = neg8(Ac[3]);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int k2 = dl + t;
+        if (k2 >= 0 && (k2 & 1) == 0 && k2 / 2 < KB) {
+          const int kb = k2 / 2;
+          YR[t] = mfma32(Ac[0], W4[kb][0], YR[t]);
+          YR[t] = mfma32(Ac[0], W4[kb][1], YR[t]);
+          YR[t] = mfma32(Ac[1], W4[kb][0], YR[t]);
+          YR[t] = mfma32(nih, W4[kb][2], YR[t]);
+          YR[t] = mfma32(nih, W4[kb][3], YR[t]);
+          YR[t] = mfma32(nil, W4[kb][2], YR[t]);
+          YI[t] = mfma32(Ac[0], W4[kb][2], YI[t]);
+          YI[t] = mfma32(Ac[0], W4[kb][3], YI[t]);
+          YI[t] = mfma32(Ac[1], W4[kb][2], YI[t]);
+          YI[t] = mfma32(Ac[2], W4[kb][0], YI[t]);
+          YI[t] = mfma32(Ac[2], W4[kb][1], YI[t]);
+          YI[t] = mfma32(Ac[3], W4[kb][0], YI[t]);
+        }
+      }
+      if (dl < DMAX) {
+#pragma unroll
+        for (int cp = 0; cp < 4; ++cp) Ac[cp] = An[cp];
       }
     }
-    // gamma_hat += conj(p) Y; lane holds rows 4*kk + v (points s = a + 4*(4*kk + v)), column col
+#undef BORN_ADJS_LOAD_A
+    // gamma_hat += conj(p) Y; lane holds rows 4*kk + v of tile t (points s = 16 t + 4 kk + v), column col
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
+    for (int t = 0; t < 4; ++t) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int s = a + 4 * (4 * kk + v);
+        const int s = 16 * t + 4 * kk + v;
         const float2 p = Ps[col * PSS + NP * s + ph];
-        gx[a][v] += p.x * YR[a][v] + p.y * YI[a][v];
-        gy[a][v] += p.x * YI[a][v] - p.y * YR[a][v];
+        gx[t][v] += p.x * YR[t][v] + p.y * YI[t][v];
+        gy[t][v] += p.x * YI[t][v] - p.y * YR[t][v];
       }
     }
     if (!has_next || next_r != r) {
 #pragma unroll
-      for (int a = 0; a < 4; ++a) {
+      for (int t = 0; t < 4; ++t) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          float x = gx[a][v], y = gy[a][v];
+          float x = gx[t][v], y = gy[t][v];
 #pragma unroll
           for (int o = 1; o < 16; o <<= 1) {
             x += __shfl_xor(x, o, 64);
             y += __shfl_xor(y, o, 64);
           }
-          gx[a][v] = x;
-          gy[a][v] = y;
+          gx[t][v] = x;
+          gy[t][v] = y;
         }
       }
       // lane col < 4 writes v = col for every tile
       if (col < 4) {
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-          float x = gx[a][0], y = gy[a][0];
+        for (int t = 0; t < 4; ++t) {
+          float x = gx[t][0], y = gy[t][0];
 #pragma unroll
           for (int v = 1; v < 4; ++v) {
             if (col == v) {
-              x = gx[a][v];
-              y = gy[a][v];
+              x = gx[t][v];
+              y = gy[t][v];
             }
           }
-          const int s = a + 4 * (4 * kk + col);
+          const int s = 16 * t + 4 * kk + col;
           const int i = i_s + NP * s + ph;
           if (i < I->Nx) g_out[(size_t)r * I->N_pts + (size_t)z * I->Nx + i] = make_float2(x, y);
         }
       }
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+      for (int t = 0; t < 4; ++t)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) gx[a][v] = gy[a][v] = 0.f;
+        for (int v = 0; v < 4; ++v) gx[t][v] = gy[t][v] = 0.f;
     }
     cur_r = next_r;
+    __syncthreads();
   }
-#undef BORN_ADJS_PREFETCH
-#undef BORN_ADJS_DESC
 }
 
 template <int MT>
 static size_t adj_split_lds_bytes() {
-  constexpr int NEP = 16 * MT;
-  constexpr int LQ = SEG / 4 + NEP;
-  constexpr int LR = ((LQ + 3 + 3) / 4) * 4 + 4;
-  constexpr int WS = NEP + 8;
-  const size_t u16s = ((size_t)(4 * 4 * 4 * LR + 4 * NC * WS) + 7) / 8 * 8;
-  return u16s * sizeof(unsigned short) + (size_t)NC * (SEG + 4) * sizeof(float2);
+  return AdjSplitLayout<MT>::bytes;
 }
 
 // ======================================================================
@@ -662,9 +697,9 @@ void launch_adj_split(const LaunchSet& L, const OutPtrs& outs, hipStream_t s) {
   const dim3 grid(L.n_wg_s);
   const int n_inst = (int)L.inst.size();
   switch (L.MT) {
-    case 2: hipLaunchKernelGGL((k_adj_s2d_split<2>), grid, dim3(256), adj_split_lds_bytes<2>(), s, L.d_inst, n_inst, outs, L.d_chunks, L.d_chunk_pairs); break;
-    case 4: hipLaunchKernelGGL((k_adj_s2d_split<4>), grid, dim3(256), adj_split_lds_bytes<4>(), s, L.d_inst, n_inst, outs, L.d_chunks, L.d_chunk_pairs); break;
-    default: hipLaunchKernelGGL((k_adj_s2d_split<8>), grid, dim3(256), adj_split_lds_bytes<8>(), s, L.d_inst, n_inst, outs, L.d_chunks, L.d_chunk_pairs); break;
+    case 2: hipLaunchKernelGGL((k_adj_s2d_split<2>), grid, dim3(512), adj_split_lds_bytes<2>(), s, L.d_inst, n_inst, outs, L.d_chunks, L.d_chunk_pairs); break;
+    case 4: hipLaunchKernelGGL((k_adj_s2d_split<4>), grid, dim3(512), adj_split_lds_bytes<4>(), s, L.d_inst, n_inst, outs, L.d_chunks, L.d_chunk_pairs); break;
+    default: hipLaunchKernelGGL((k_adj_s2d_split<8>), grid, dim3(512), adj_split_lds_bytes<8>(), s, L.d_inst, n_inst, outs, L.d_chunks, L.d_chunk_pairs); break;
   }
 }
 
