@@ -303,6 +303,8 @@ void launch_fwd_structured(const LaunchSet& L, int n_cu, hipStream_t s) {
 #define BORN_FWD_NP(MTv)       \
   if (L.Np == 4) {             \
     BORN_FWD_PPT(MTv, 4)       \
+  } else if (L.Np == 2) {      \
+    BORN_FWD_PPT(MTv, 2)       \
   } else {                     \
     BORN_FWD_PPT(MTv, 0)       \
   }
@@ -557,6 +559,8 @@ void launch_adj_structured(const LaunchSet& L, const OutPtrs& outs, hipStream_t 
 #define BORN_ADJ_NP(MTv)        \
   if (L.Np == 4) {              \
     BORN_ADJ_MW(MTv, 4)         \
+  } else if (L.Np == 2) {       \
+    BORN_ADJ_MW(MTv, 2)         \
   } else {                      \
     BORN_ADJ_MW(MTv, 0)         \
   }
@@ -576,8 +580,10 @@ template <int MT>
 static void set_max_lds_mt(int b) {
 #define BORN_SET(K) hipFuncSetAttribute((const void*)(K), hipFuncAttributeMaxDynamicSharedMemorySize, b)
   BORN_SET((k_fwd_s2d<MT, 4, 1>)); BORN_SET((k_fwd_s2d<MT, 4, 2>)); BORN_SET((k_fwd_s2d<MT, 4, 4>));
+  BORN_SET((k_fwd_s2d<MT, 2, 1>)); BORN_SET((k_fwd_s2d<MT, 2, 2>)); BORN_SET((k_fwd_s2d<MT, 2, 4>));
   BORN_SET((k_fwd_s2d<MT, 0, 1>)); BORN_SET((k_fwd_s2d<MT, 0, 2>)); BORN_SET((k_fwd_s2d<MT, 0, 4>));
   BORN_SET((k_adj_s2d<MT, 2, 4>)); BORN_SET((k_adj_s2d<MT, 4, 4>));
+  BORN_SET((k_adj_s2d<MT, 2, 2>)); BORN_SET((k_adj_s2d<MT, 4, 2>));
   BORN_SET((k_adj_s2d<MT, 2, 0>)); BORN_SET((k_adj_s2d<MT, 4, 0>));
 #undef BORN_SET
 }

@@ -491,3 +491,85 @@ def direct_tables(spec, points: Optional[np.ndarray] = None) -> BornTables:
     return BornTables(h_rx, shift, meas, (n, 1), tuple(spec.array.N_elements_axis),
                       tuple(spec.N_points_per_pitch_axis))
 
+
+def _taper_all(spec, f: float, positions: np.ndarray, filt) -> np.ndarray:
+    """(n_pts, N_e) anti-aliasing taper of every element at one frequency:
+    filter_samples(filter_flags(...)) of each element, vectorized over the
+    elements (setup.m:902, :980-1017)."""
+    n = positions.shape[0]
+    N_e = spec.array.N_elements
+    if filt is None or filt.kind == "off":
+        return np.ones((n, N_e))
+    nd = spec.fov.N_dimensions
+    kr = float(np.real(wavenumbers(np.array([f]), spec.medium))[0])
+    rel = [d for d in range(nd - 1) if spec.array.N_elements_axis[d] > 1]
+    pitch = np.array([spec.array.pitch_axis[d] for d in rel])
+    ctr = spec.array.positions_ctr()
+    out = np.empty((n, N_e))
+    for e0 in range(0, N_e, 64):
+        c = ctr[e0:e0 + 64]
+        c3 = np.concatenate([c, np.zeros((c.shape[0], 1))], axis=1)
+        vec = positions[:, None, :] - c3[None, :, :]
+        dist = np.linalg.norm(vec, axis=2)
+        e = np.abs(vec[:, :, : nd - 1] / dist[..., None])[..., rel]
+        out[:, e0:e0 + c.shape[0]] = filter_samples(kr * pitch * e, filt)
+    return out
+
+
+def sampled_tables(spec, points: np.ndarray, f_sel: np.ndarray):
+    """The operator restricted to the grid points `points` and the global
+    frequencies `f_sel` (both 0-based), built like direct_tables -- every
+    element's own transfer function at each point, no shift or mirror map --
+    for a 3-D sequence, with the face sum of setup.m:1216-1281 evaluated by
+    the C routine ref_face_sum3 (pinned against transfer_function by the
+    tests).  Returns (T, rows): T's observations are the full operator's
+    observations at the sampled frequencies, in the full operator's order,
+    and rows holds their 0-based indices in the full observation vector."""
+    from . import ref_loop
+    if spec.fov.N_dimensions != 3:
+        raise ValueError("sampled_tables: 3-D sequences only")
+    N_e = spec.array.N_elements
+    points = np.asarray(points)
+    pos = spec.fov.positions()[points]
+    f_sel = np.asarray(sorted(set(int(i) for i in f_sel)))
+    f = spec.frequencies[f_sel]
+    k = wavenumbers(f, spec.medium)
+    offs, dA = spec.array.face_offsets()
+    ctr = spec.array.positions_ctr()
+    n = len(points)
+    same = (spec.filter_tx.kind, spec.filter_tx.param) == (spec.filter_rx.kind, spec.filter_rx.param)
+    h_rx = np.empty((len(f), n * N_e), dtype=np.complex128)
+    h_tx = h_rx if same else np.empty_like(h_rx)
+    for i in range(len(f)):
+        raw = ref_loop.face_sum3(pos, ctr, offs, dA, k[i])
+        h_rx[i] = (raw * _taper_all(spec, f[i], pos, spec.filter_rx)).reshape(-1)
+        if not same:
+            h_tx[i] = (raw * _taper_all(spec, f[i], pos, spec.filter_tx)).reshape(-1)
+    shift = (np.arange(n)[:, None] * N_e + np.arange(N_e)[None, :] + 1).astype(np.int64)
+    pos_of = {int(g): i for i, g in enumerate(f_sel)}      # global frequency -> sampled row
+    pref_u = prefactors_unique(spec)
+    meas, rows = [], []
+    obs = 0
+    for m in range(spec.N_measurements):
+        fidx = spec.measurement_frequencies(m)
+        keep = np.array([j for j, g in enumerate(fidx) if int(g) in pos_of], dtype=np.int64)  # measurement rows kept
+        sub_of = {int(j): i for i, j in enumerate(keep)}
+        vd = normal_velocities(spec, m)[keep]
+        act = np.asarray(spec.waves[m].indices_active)
+        hsel = h_tx[[pos_of[int(fidx[j])] for j in keep]].reshape(len(keep), n, N_e)
+        p = np.einsum("fpa,fa->fp", hsel[:, :, act], vd)
+        mixes = []
+        for mix in spec.mixes[m]:
+            fk = np.arange(len(fidx)) if mix.indices_f is None else np.asarray(mix.indices_f)
+            jj = [j for j in range(len(fk)) if int(fk[j]) in sub_of]
+            rows.extend(obs + j for j in jj)
+            fk_kept = np.array([fk[j] for j in jj], dtype=np.int64)
+            pref = pref_u[fidx[fk_kept]][:, None] * np.asarray(mix.weights)[None, :]
+            mixes.append(MixTables(np.array([sub_of[int(x)] for x in fk_kept], dtype=np.int64) + 1,
+                                   np.asarray(mix.indices_active) + 1, pref.astype(np.complex128)))
+            obs += len(fk)
+        meas.append(MeasTables(np.array([pos_of[int(fidx[j])] for j in keep], dtype=np.int64) + 1, p, mixes))
+    T = BornTables(h_rx, shift, meas, (n, 1), tuple(spec.array.N_elements_axis),
+                   tuple(spec.N_points_per_pitch_axis))
+    return T, np.asarray(rows, dtype=np.int64)
+

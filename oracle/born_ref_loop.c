@@ -151,3 +151,31 @@ int ref_apply(int mode, int N_pts, int N_fu, const double* h_, const int* shift,
   free(occ);
   return 0;
 }
+
+/* One frequency of the 3-D spatial transfer function of every element at
+ * arbitrary points, before the anti-aliasing taper (setup.m:1216-1281):
+ *     out[p][e] = -2 dA sum_q G(k, |r_p - f_eq|),  G(k, D) = -exp(-j k D) / (4 pi D)
+ * (greens_function.m:103), face points f_eq = (ctr_e + off_q, 0).
+ * pos: n_pts x 3, ctr: n_e x 2, off: n_q x 2 (row-major); out interleaved
+ * complex, n_pts x n_e.  k = k_re + j k_im (k_im <= 0: attenuation). */
+void ref_face_sum3(long n_pts, const double* pos, int n_e, const double* ctr, int n_q, const double* off,
+                   double dA, double k_re, double k_im, double* out) {
+  const double c0 = 2.0 * dA / (4.0 * 3.14159265358979323846);   /* -2 dA * (-1 / (4 pi)) */
+#pragma omp parallel for schedule(static)
+  for (long p = 0; p < n_pts; ++p) {
+    const double x = pos[3 * p], y = pos[3 * p + 1], z = pos[3 * p + 2];
+    for (int e = 0; e < n_e; ++e) {
+      double sr = 0.0, si = 0.0;
+      for (int q = 0; q < n_q; ++q) {
+        const double dx = x - (ctr[2 * e] + off[2 * q]), dy = y - (ctr[2 * e + 1] + off[2 * q + 1]);
+        const double D = sqrt(dx * dx + dy * dy + z * z);
+        /* exp(-j k D) = exp(k_im D) (cos(k_re D) - j sin(k_re D)) */
+        const double a = exp(k_im * D) / D;
+        sr += a * cos(k_re * D);
+        si -= a * sin(k_re * D);
+      }
+      out[2 * (p * (long)n_e + e)] = c0 * sr;
+      out[2 * (p * (long)n_e + e) + 1] = c0 * si;
+    }
+  }
+}

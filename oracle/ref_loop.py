@@ -25,6 +25,8 @@ def lib():
         L.ref_apply.argtypes = [C.c_int, C.c_int, C.c_int, dp, ip, C.c_int, C.c_int, ip, dp, C.c_int, ip, ip, ip, ip,
                                 dp, dp, dp, C.c_int, C.c_int]
         L.ref_apply.restype = C.c_int
+        L.ref_face_sum3.argtypes = [C.c_long, dp, C.c_int, dp, C.c_int, dp, C.c_double, C.c_double, C.c_double, dp]
+        L.ref_face_sum3.restype = None
         _LIB = L
     return _LIB
 
@@ -71,4 +73,15 @@ def apply_measurement(P: Prepared, mode: int, x: np.ndarray, fused: bool = False
                          _dp(out.view(np.float64)), int(fused), int(threads))
     if rc != 0:
         raise MemoryError("ref_apply failed to allocate")
+    return out
+
+
+def face_sum3(positions: np.ndarray, centres: np.ndarray, offsets: np.ndarray, dA: float, k: complex) -> np.ndarray:
+    """(n_pts, n_e) complex: the C face sum ref_face_sum3 for one wavenumber."""
+    pos = np.ascontiguousarray(positions, dtype=np.float64)
+    ctr = np.ascontiguousarray(centres, dtype=np.float64)
+    off = np.ascontiguousarray(offsets, dtype=np.float64)
+    out = np.empty((pos.shape[0], ctr.shape[0]), dtype=np.complex128)
+    lib().ref_face_sum3(pos.shape[0], _dp(pos), ctr.shape[0], _dp(ctr), off.shape[0], _dp(off), float(dA),
+                        float(np.real(k)), float(np.imag(k)), _dp(out.view(np.float64)))
     return out

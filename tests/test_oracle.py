@@ -287,3 +287,63 @@ def test_direct_tables_restrict_to_points():
     pts = np.array([0, 4, 11, 26])
     A_sub = O.assemble_matrix(O.direct_tables(spec, pts))
     assert O.rel_l2(A_sub, A[:, pts]) < 1e-14
+
+
+def tiny_spec_3d(filt="raised_cosine", mixes=None):
+    arr = C.ArraySpec((4, 3), (3e-4, 3e-4), (2.7e-4, 2.7e-4), (2, 2))
+    fov = C.FovSpec((9, 7, 3), (1.5e-4, 1.5e-4, 1.5e-4), 2e-3)
+    med = C.MediumSpec()
+    waves = [C.qsw_wave(arr, np.array([x, 0.0, -1e-2]), med.c_avg) for x in (0.0, 6e-4)]
+    if mixes is None:
+        mixes = [C.identity_mixes(12) for _ in waves]
+    spec = C.SequenceSpec("tiny3", arr, med, fov, np.array([2.5e6, 3.7e6, 4.4e6, 5.5e6]), waves, mixes)
+    spec.filter_tx = C.FilterSpec(filt, 0.5)
+    spec.filter_rx = C.FilterSpec(filt, 0.5)
+    return spec
+
+
+def test_face_sum3_matches_transfer_function():
+    """The C face sum times the taper equals transfer_function element by
+    element (3-D Green's function, every face point, complex wavenumber)."""
+    from oracle import ref_loop
+    spec = tiny_spec_3d()
+    pos = spec.fov.positions()[::5]
+    offs, dA = spec.array.face_offsets()
+    ctr = spec.array.positions_ctr()
+    k = O.wavenumbers(spec.frequencies, spec.medium)
+    for i, f in enumerate(spec.frequencies):
+        raw = ref_loop.face_sum3(pos, ctr, offs, dA, k[i])
+        blk = raw * O._taper_all(spec, f, pos, spec.filter_rx)
+        for e in range(spec.array.N_elements):
+            ref = O.transfer_function(spec, np.array([f]), e, spec.filter_rx, positions=pos)[0]
+            assert O.rel_l2(blk[:, e], ref) < 1e-13
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_sampled_tables_are_rows_and_columns_of_direct_tables(mixed):
+    """sampled_tables(points, f_sel) is the block of the full operator at the
+    sampled frequencies' observations and the chosen points: forward rows and
+    adjoint at the points equal direct_tables' operator on the same inputs,
+    with identity or random +-1 mixing."""
+    rng = np.random.Generator(np.random.PCG64(3))
+    mixes = None
+    if mixed:
+        mixes = []
+        for _ in range(2):
+            mm = []
+            for _k in range(3):
+                w = np.where(rng.random(12) < 0.5, -1.0, 1.0)
+                mm.append(C.MixSpec(np.arange(12), w))
+            mixes.append(mm)
+    spec = tiny_spec_3d(mixes=mixes)
+    pts = np.array([0, 7, 40, 64, 100, 150, 188])
+    f_sel = np.array([1, 3])
+    T_full = O.direct_tables(spec, pts)
+    T_s, rows = O.sampled_tables(spec, pts, f_sel)
+    assert T_s.N_observations() == len(rows)
+    g = rng.standard_normal((len(pts), 2)) + 1j * rng.standard_normal((len(pts), 2))
+    assert O.rel_l2(O.forward(T_s, g), O.forward(T_full, g)[rows]) < 1e-13
+    u = np.zeros((T_full.N_observations(), 2), dtype=np.complex128)
+    us = rng.standard_normal((len(rows), 2)) + 1j * rng.standard_normal((len(rows), 2))
+    u[rows] = us
+    assert O.rel_l2(O.adjoint(T_s, us), O.adjoint(T_full, u)) < 1e-13

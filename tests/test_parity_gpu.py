@@ -423,3 +423,61 @@ def test_config2_against_oracle(torch_cuda, c2, precision, tol):
         op.set_precision("fp32")
     log("config2 %s: forward rel-L2 %.2e adjoint rel-L2 %.2e" % (precision, e_f, e_a))
     assert e_f < tol and e_a < tol
+
+
+# ---------------------------------------------------------------- config 4 (3-D)
+C4_FREQS = np.array([0, 32, 64])
+
+
+@pytest.fixture(scope="module")
+def config4_case():
+    """The 3-D workload (32 x 32 matrix array, 128 x 128 x 192 FOV at pitch/2,
+    65 bins, 5 QSWs) with 4 gamma right-hand sides, and the FP64 oracle of
+    its block at a 32^3 sub-FOV (x, y in [48, 80), z in [80, 112)) and 3 of
+    the 65 frequencies (first, middle, last): the forward of gamma supported
+    on the sub-FOV at those frequencies' observations, and the adjoint at
+    the sub-FOV points of u supported on those observations."""
+    from oracle import ref_loop
+    spec = C.config4()
+    Nx, Ny, Nz = spec.fov.N_points_axis
+    ix, iy, iz = np.meshgrid(np.arange(48, 80), np.arange(48, 80), np.arange(80, 112), indexing="ij")
+    pts = np.sort((ix + Nx * (iy + Ny * iz)).reshape(-1))
+    log("config4: sampled oracle tables (%d points, %d frequencies)" % (len(pts), len(C4_FREQS)))
+    T, rows = O.sampled_tables(spec, pts, C4_FREQS)
+    nrhs = 4
+    rng = np.random.Generator(np.random.PCG64(5))
+    g_sub = rng.standard_normal((len(pts), nrhs)) + 1j * rng.standard_normal((len(pts), nrhs))
+    u_sub = rng.standard_normal((len(rows), nrhs)) + 1j * rng.standard_normal((len(rows), nrhs))
+    preps = [ref_loop.Prepared(T, m) for m in range(len(T.meas))]
+    u_ref = np.stack([np.concatenate([ref_loop.apply_measurement(P, 1, g_sub[:, r], fused=True) for P in preps])
+                      for r in range(nrhs)], axis=1)
+    g_ref = np.zeros((len(pts), nrhs), dtype=np.complex128)
+    for r in range(nrhs):
+        o = 0
+        for P in preps:
+            g_ref[:, r] += ref_loop.apply_measurement(P, 2, u_sub[o:o + P.n_obs, r], fused=True)
+            o += P.n_obs
+    log("config4: oracle done")
+    return dict(spec=spec, pts=pts, rows=rows, g_sub=g_sub, u_sub=u_sub, u_ref=u_ref, g_ref=g_ref)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", TOL_FP32)])
+def test_config4_subfov_oracle(torch_cuda, config4_case, precision, tol):
+    import torch
+    d = config4_case
+    op = OperatorBorn.from_sequence(d["spec"], precision=precision)
+    assert op.path == "structured"
+    nrhs = d["g_sub"].shape[1]
+    g = torch.zeros((nrhs, op.N_points), dtype=torch.complex64, device="cuda")
+    g[:, torch.from_numpy(d["pts"]).cuda()] = torch.from_numpy(d["g_sub"].T.astype(np.complex64)).cuda()
+    u = op.forward_dev(g)
+    u_gpu = u.cpu().numpy().T[d["rows"]]
+    e_f = O.rel_l2(u_gpu, d["u_ref"])
+    uu = torch.zeros((nrhs, op.N_observations), dtype=torch.complex64, device="cuda")
+    uu[:, torch.from_numpy(d["rows"]).cuda()] = torch.from_numpy(d["u_sub"].T.astype(np.complex64)).cuda()
+    gh = op.adjoint_dev(uu)
+    g_gpu = gh.cpu().numpy().T[d["pts"]]
+    e_a = O.rel_l2(g_gpu, d["g_ref"])
+    log("config4 %s (4 rhs, 32^3 sub-FOV, 3 frequencies): forward rel-L2 %.2e adjoint rel-L2 %.2e" % (precision, e_f, e_a))
+    assert e_f < tol and e_a < tol
+    op.close()
