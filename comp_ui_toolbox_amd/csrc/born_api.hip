@@ -367,10 +367,10 @@ static void set_blocking(Plan& P) {
   P.ZR = (rows + NZB - 1) / NZB;
   P.NZB = (rows + P.ZR - 1) / P.ZR;
   P.nseg_s = (P.Nx + 255) / 256;
-  NZB = std::max(1, std::min(rows, target / std::max(1, n_chunks * P.nseg_s)));
+  NZB = std::max(1, std::min(rows, target / std::max(1, n_chunks * Ney * P.nseg_s)));
   P.ZR_s = (rows + NZB - 1) / NZB;
   P.NZB_s = (rows + P.ZR_s - 1) / P.ZR_s;
-  P.vblk = std::max(P.NZB * Ney, P.NZB_s * P.nseg_s);   // partial-sum blocks per chunk
+  P.vblk = Ney * std::max(P.NZB, P.NZB_s * P.nseg_s);   // partial-sum blocks per chunk
 }
 
 // The record of a plan inside a structured launch (workspace must exist).
@@ -412,7 +412,7 @@ static void finalize_launchset(LaunchSet& L) {
     I.item_begin = it;
     it += I.n_chunks * I.Ney * I.NZB;
     I.item_begin_s = its;
-    its += I.n_chunks * I.NZB_s * I.nseg_s;
+    its += I.n_chunks * I.Ney * I.NZB_s * I.nseg_s;
     I.adj_segs = (I.Nx + 64 * L.adj_mw - 1) / (64 * L.adj_mw);
     I.wg_begin = wg;
     wg += I.adj_segs * I.n_rows;
@@ -840,7 +840,7 @@ static void check_precision(const Plan& P) {
     throw Error(BORN_ERR_UNSUPPORTED, "FP16 split precision is not available in this build");
   if (P.precision != BORN_PRECISION_FP32 && !(fwd_split_supported(P) && !P.force_general))
     throw Error(BORN_ERR_UNSUPPORTED,
-                "split precision needs the structured 2-D path with 4 grid points per pitch and 17 to 128 elements");
+                "split precision needs the structured path with 2 or 4 grid points per pitch and 17 to 128 elements per x row");
 }
 
 static void host_forward(Plan& P, const double* gamma, int N_rhs, double* u) {

@@ -67,12 +67,13 @@ __device__ __forceinline__ const void* byte_offset(const void* base, unsigned in
   return static_cast<const char*>(base) + off;
 }
 
-// Work item = (instance, chunk, z-block, 256-point segment xs), taken
-// persistently (the items of instance k are [item_begin_s_k,
-// item_begin_s_{k+1})); a step is one row z of the item.  The p_inc table of
-// every instance has a zero row after its N_pairs rows.  Waves: 2 element groups x 2 point
-// groups; wave (eg, pg) holds element tiles [eg*MTW, (eg+1)*MTW) and the
-// K-blocks pg*KBW .. pg*KBW + KBW-1 of the segment.  Points per pitch 4.
+// Work item = (instance, chunk, y slice ey, row block, 256-point segment xs),
+// taken persistently (the items of instance k are [item_begin_s_k,
+// item_begin_s_{k+1})); a step is one row (y, z) of the item (rows are depth
+// rows z in 2-D, Ney = 1).  The p_inc table of every instance has a zero row
+// after its N_pairs rows.  Waves: 2 element groups x 2 point groups; wave
+// (eg, pg) holds element tiles [eg*MTW, (eg+1)*MTW) and the K-blocks
+// pg*KBW .. pg*KBW + KBW-1 of the segment.  NP = 2 or 4 points per pitch.
 //
 // Loads of a step depend on nothing loaded in the step before: per item the
 // window gather offsets (registers) and the column rows (scalar registers,
@@ -81,15 +82,17 @@ __device__ __forceinline__ const void* byte_offset(const void* base, unsigned in
 //
 // LDS (u16 units):
 //   Aw[comp][copy][LAP]  window of the extended row (comp = re_hi, re_lo,
-//                        im_hi, im_lo); copy 1 is copy 0 shifted by 4 entries
-//                        so every lane's 8-entry read is 16-byte aligned
+//                        im_hi, im_lo); copy c is copy 0 shifted by NP*c
+//                        entries (8 / NP copies) so every lane's 8-entry read
+//                        is 16-byte aligned
 //   Bt[comp][c][SEGP]    p_inc*gamma of the segment, points contiguous
-template <int MT, bool EVEN>
+template <int MT, bool EVEN, int NP>
 __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict__ inst, int n_inst, int n_items,
                                                           int Ne_pad, const Chunk* __restrict__ chunks,
                                                           const int* __restrict__ chunk_pairs) {
   extern __shared__ unsigned short ldsu[];
-  constexpr int NP = 4;
+  constexpr int NCOPY = 8 / NP;         // window copies, shifted by NP entries each
+  constexpr int DSTEP = NP / 2;         // K-blocks of 32 points per element tile of 16
   constexpr int SEGP = SEG + 8;
   constexpr int EW = MT >= 2 ? 2 : 1;
   constexpr int PW = 4 / EW;
@@ -98,8 +101,8 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict
   constexpr int LA = SEG + NP * (16 * MT - 1);
   constexpr int LAP = ((LA + 4 + 7) / 8) * 8 + 8;
   constexpr int APT = (LA + 255) / 256;
-  unsigned short* Aw = ldsu;                 // [4][2][LAP]
-  unsigned short* Bt = ldsu + 4 * 2 * LAP;   // [4][NC][SEGP]
+  unsigned short* Aw = ldsu;                     // [4][NCOPY][LAP]
+  unsigned short* Bt = ldsu + 4 * NCOPY * LAP;   // [4][NC][SEGP]
   unsigned int* Bt32 = reinterpret_cast<unsigned int*>(Bt);
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -114,7 +117,7 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict
   const int bi = 2 * (tid & 127);
 
   int it = 0, item = blockIdx.x;
-  int ci, xs, z, z_end, fg, rr;
+  int ci, xs, z, z_end, fg, rr, ey;
   bool hfull, gfull;                 // window inside the table / segment inside the row
   unsigned int hoff[APT];            // window gather byte offsets within a row
   unsigned int hval = 0;             // bit j: window entry j inside the table
@@ -127,14 +130,16 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict
       Iv = inst[ki];                                                                      \
     }                                                                                     \
     const int tpad = (16 * MT - I->N_e) * NP;                                             \
-    const int NBLK = I->NZB_s * I->nseg_s;   /* partial-sum blocks per chunk */           \
-    const int li_ = (itm) - I->item_begin_s;                                              \
-    ci = li_ / NBLK;                                                                      \
-    const int rem_ = li_ - ci * NBLK;                                                     \
+    const int NBLK = I->NZB_s * I->nseg_s;   /* partial-sum blocks per chunk and slice */ \
+    const int li_ = (itm) - I->item_begin_s; /* ((ci * Ney + ey) * NZB_s + zb) * nseg_s + xs */ \
+    ci = li_ / (I->Ney * NBLK);                                                           \
+    const int rem1_ = li_ - ci * (I->Ney * NBLK);                                         \
+    ey = rem1_ / NBLK;                                                                    \
+    const int rem_ = rem1_ - ey * NBLK;                                                   \
     const int zb_ = rem_ / I->nseg_s;                                                     \
     xs = rem_ - zb_ * I->nseg_s;                                                          \
     z = zb_ * I->ZR_s;                                                                    \
-    z_end = min(I->Nz, z + I->ZR_s);                                                      \
+    z_end = min(I->n_rows, z + I->ZR_s);                                                  \
     const Chunk c_ = chunks[I->chunk_base + ci];                                          \
     fg = c_.fg;                                                                           \
     rr = c_.r;                                                                            \
@@ -165,7 +170,7 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict
 #define BORN_SPLIT_PREFETCH(zz)                                                                  \
   {                                                                                              \
     const size_t rowo_ = (size_t)(zz) * I->Nx;                                                    \
-    const float2* hrow_ = I->h + (size_t)fg * I->Hp + (size_t)(zz) * I->HR;                         \
+    const float2* hrow_ = I->h + (size_t)fg * I->Hp + (size_t)I->hrow[ey * I->n_rows + (zz)] * I->HR; \
     _Pragma("unroll") for (int j = 0; j < APT; ++j)                                             \
       hreg[j] = *static_cast<const float2*>(byte_offset(hrow_, hoff[j]));                        \
     const float2* grow_ = I->gm + (size_t)rr * I->N_pts + rowo_;                                   \
@@ -201,11 +206,11 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict
   }
 
   // lane A position: e = 16*(eg*MTW + m) + col, u = 32*k + 8*kk + NP*(16*MT - 1 - e);
-  // u mod 8 = 4 exactly when col is even
+  // u mod 8 is a multiple of NP, fixed per lane: copy (u mod 8) / NP is aligned
   const int e0 = 16 * (eg * MTW) + col;
   const int u00 = 8 * kk + NP * (16 * MT - 1 - e0);
-  const int acopy = (u00 & 7) ? 1 : 0;
-  const int abase = acopy * LAP + (u00 - 4 * acopy);
+  const int acopy = (u00 & 7) / NP;
+  const int abase = acopy * LAP + (u00 - NP * acopy);
   const int bbase = col * SEGP + 8 * kk;
 
   while (true) {
@@ -223,8 +228,9 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict
                                      (unsigned short)(lo >> 16)};
 #pragma unroll
         for (int cp = 0; cp < 4; ++cp) {
-          Aw[(cp * 2 + 0) * LAP + u] = v[cp];
-          if (u >= 4) Aw[(cp * 2 + 1) * LAP + u - 4] = v[cp];
+#pragma unroll
+          for (int c = 0; c < NCOPY; ++c)
+            if (u >= NP * c) Aw[(cp * NCOPY + c) * LAP + u - NP * c] = v[cp];
         }
       }
     }
@@ -268,10 +274,10 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict
     }
     if (has_next) BORN_SPLIT_PREFETCH(z)
 
-    // Tile m at K-block kb reads the window at 32*(pg*KBW + kb) - 64*m: the
-    // same data as tile 0 at K-block kb - 2m.  Walk the diagonals
-    // d = kb - 2m, holding the wave's B blocks in registers and streaming one
-    // A set per diagonal (double-buffered).
+    // Tile m at K-block kb reads the window at 32*(pg*KBW + kb) - 16*NP*m:
+    // the same data as tile 0 at K-block kb - DSTEP*m.  Walk the diagonals
+    // d = kb - DSTEP*m, holding the wave's B blocks in registers and
+    // streaming one A set per diagonal (double-buffered).
     uint4 B4[KBW][4], NB[KBW][2];
 #pragma unroll
     for (int kb = 0; kb < KBW; ++kb) {
@@ -282,23 +288,23 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict
       NB[kb][0] = neg8(B4[kb][2]);
       NB[kb][1] = neg8(B4[kb][3]);
     }
-    constexpr int D0 = -2 * (MTW - 1);
+    constexpr int D0 = -DSTEP * (MTW - 1);
     uint4 Acur[4], Anext[4];
     {
       const int ao = abase + 32 * (pg * KBW + D0);
 #pragma unroll
-      for (int cp = 0; cp < 4; ++cp) Acur[cp] = *reinterpret_cast<const uint4*>(&Aw[cp * 2 * LAP + ao]);
+      for (int cp = 0; cp < 4; ++cp) Acur[cp] = *reinterpret_cast<const uint4*>(&Aw[cp * NCOPY * LAP + ao]);
     }
 #pragma unroll
     for (int d = D0; d < KBW; ++d) {
       if (d + 1 < KBW) {
         const int ao = abase + 32 * (pg * KBW + d + 1);
 #pragma unroll
-        for (int cp = 0; cp < 4; ++cp) Anext[cp] = *reinterpret_cast<const uint4*>(&Aw[cp * 2 * LAP + ao]);
+        for (int cp = 0; cp < 4; ++cp) Anext[cp] = *reinterpret_cast<const uint4*>(&Aw[cp * NCOPY * LAP + ao]);
       }
 #pragma unroll
       for (int m = 0; m < MTW; ++m) {
-        const int kb = d + 2 * m;
+        const int kb = d + DSTEP * m;
         if (kb >= 0 && kb < KBW) {
           CR[m] = mfma32(Acur[0], B4[kb][0], CR[m]);
           CR[m] = mfma32(Acur[0], B4[kb][1], CR[m]);
@@ -353,12 +359,12 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict
 #undef BORN_SPLIT_ITEM
 }
 
-template <int MT>
+template <int MT, int NP>
 static size_t fwd_split_lds_bytes() {
   constexpr int SEGP = SEG + 8;
-  constexpr int LA = SEG + 4 * (16 * MT - 1);
+  constexpr int LA = SEG + NP * (16 * MT - 1);
   constexpr int LAP = ((LA + 4 + 7) / 8) * 8 + 8;
-  const size_t body = ((size_t)4 * 2 * LAP + (size_t)4 * NC * SEGP) * sizeof(unsigned short);
+  const size_t body = ((size_t)4 * (8 / NP) * LAP + (size_t)4 * NC * SEGP) * sizeof(unsigned short);
   const size_t red = (size_t)4 * (MT >= 2 ? MT / 2 : MT) * 8 * 64 * sizeof(float);
   return std::max(body, red);
 }
@@ -366,31 +372,34 @@ static size_t fwd_split_lds_bytes() {
 // ======================================================================
 // adjoint
 // ======================================================================
-// One workgroup = (instance, 256-point segment, depth row z); the workgroups
-// of instance k are [wg_begin_s_k, wg_begin_s_{k+1}).  Chunks are walked in a
-// fixed order with gamma_hat kept in registers (no atomics).
-// For Np = 4 the segment's points split into 4 phases ph = i mod 4 of 64
-// points s; for one phase the adjoint is the Toeplitz product
-//     Y[s][c] = sum_e conj(H(4 s + ph - 4 e + off)) W[e][c].
-// With the phase row stored reversed, Hr[m] with m = 64 - s + e, tile t
+// One workgroup = (instance, 256-point segment, row (y, z)); the workgroups
+// of instance k are [wg_begin_s_k, wg_begin_s_{k+1}).  Steps (chunk, y slice
+// ey) are walked in a fixed order with gamma_hat kept in registers (no
+// atomics); in 2-D a step is a chunk.
+// The segment's points split into NP phases ph = i mod NP of SPH = 256 / NP
+// points s; for one phase and slice the adjoint is the Toeplitz product
+//     Y[s][c] = sum_e conj(H(NP s + ph - NP e + off)) W[e][c],
+// summed over the slices of a chunk before gamma_hat += conj(p_inc) Y.
+// With the phase row stored reversed, Hr[m] with m = SPH - s + e, tile t
 // (points s = 16 t + r, r = 0..15) at K-block kb (elements 32 kb + 8 kk + j)
-// reads Hr[64 + 16 (2 kb - t) - r + 8 kk + j]: tile t + 2 at K-block kb + 1
-// reads the same data as tile t at kb.  A compute wave owns one phase and
-// walks the diagonals delta = 2 kb - t, loading one A set per diagonal
-// (double-buffered) with every K-block of W in registers.
+// reads Hr[SPH + 16 (2 kb - t) - r + 8 kk + j]: tile t + 2 at K-block kb + 1
+// reads the same data as tile t at kb.  A compute wave owns 4 tiles of one
+// phase and walks the diagonals delta = 2 kb - t, loading one A set per
+// diagonal (double-buffered) with every K-block of W in registers.
 // The reversed row is stored in 4 copies, copy d holding Hr[m] at position
 // m - d + PADL, so lane r reads its 8 entries 8-byte aligned from copy
 // d = (-r) mod 4; the copies start at offsets chosen so that the 16 lanes of
 // a read cover all 32 banks once.
 //
 // The workgroup has 8 waves: waves 0-3 compute on one LDS buffer while
-// waves 4-7 load the next chunk and write it, split into BF16 hi/lo planes,
-// to the other buffer; one barrier per chunk swaps the buffers.
+// waves 4-7 load the next step and write it, split into BF16 hi/lo planes,
+// to the other buffer; one barrier per step swaps the buffers.  p_inc of a
+// chunk has its own pair of buffers (by chunk parity), written with the
+// chunk's first step.
 //
-// LDS buffer: Ar[comp][ph][CPB] (u16, 4 copies per block), Wt[comp][c][WS] (u16), Ps[c][PSS] (float2)
-template <int MT>
+// LDS: AW[2] = {Ar[comp][ph][CPB] (u16, 4 copies per block), Wt[comp][c][WS] (u16)}, Ps[2][c][PSS] (float2)
+template <int MT, int NP>
 struct AdjSplitLayout {
-  static constexpr int NP = 4;
   static constexpr int NEP = 16 * MT;
   static constexpr int SPH = SEG / NP;                          // points per phase
   static constexpr int LQ = SPH + NEP;                          // reversed phase-row entries
@@ -400,26 +409,27 @@ struct AdjSplitLayout {
   static constexpr int WS = NEP + 8;                            // padded W row
   static constexpr int PSS = SEG + 1;                           // padded p_inc row (float2)
   static constexpr int AR_U16 = 4 * NP * CPB;
-  static constexpr int WT_U16 = 4 * NC * WS;
-  static constexpr int PS_OFF = ((AR_U16 + WT_U16 + 7) / 8) * 8;    // u16 offset of Ps
-  static constexpr int BUF_U16 = ((PS_OFF + NC * PSS * 4 + 7) / 8) * 8;  // one buffer, 16-byte multiple
-  static constexpr size_t bytes = (size_t)2 * BUF_U16 * 2;     // two buffers
+  static constexpr int AW_U16 = ((AR_U16 + 4 * NC * WS + 7) / 8) * 8;   // one A/W buffer
+  static constexpr int PS_U16 = ((NC * PSS * 4 + 7) / 8) * 8;           // one p_inc buffer
+  static constexpr int PS_OFF = 2 * AW_U16;
+  static constexpr size_t bytes = (size_t)(2 * AW_U16 + 2 * PS_U16) * 2;
 };
 __device__ __forceinline__ int copy_offset(int d) { return d == 1 ? 16 : (d == 2 ? 28 : (d == 3 ? 40 : 0)); }
 
-template <int MT>
+template <int MT, int NP, bool Y3>
 __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict__ inst, int n_inst, OutPtrs outs,
                                                           const Chunk* __restrict__ chunks,
                                                           const int* __restrict__ chunk_pairs) {
   extern __shared__ unsigned short ldsu[];
-  typedef AdjSplitLayout<MT> Lay;
-  constexpr int NP = Lay::NP, NEP = Lay::NEP, SPH = Lay::SPH, LQ = Lay::LQ, PADL = Lay::PADL;
+  typedef AdjSplitLayout<MT, NP> Lay;
+  constexpr int NEP = Lay::NEP, SPH = Lay::SPH, LQ = Lay::LQ, PADL = Lay::PADL;
   constexpr int LR = Lay::LR, CPB = Lay::CPB, WS = Lay::WS, PSS = Lay::PSS;
   constexpr int KB = MT / 2;                    // K-blocks of 32 elements
   constexpr int L = SEG + NP * (NEP - 1);       // extended-row window entries
   constexpr int HPT = (L + 255) / 256;
   constexpr int WPT = NEP * NC / 256;           // W entries per fill thread
   constexpr int PPT = NC * SEG / 256;           // p_inc entries per fill thread
+  constexpr int WPP = 4 / NP;                   // compute waves per phase
 
   const int tid = threadIdx.x, lane = tid & 63, w = (tid >> 6) & 3;
   const bool filler = tid >= 256;
@@ -430,9 +440,10 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict
   const Inst Iv = inst[ki];   // loop-invariant: held in scalar registers
   const Inst* I = &Iv;
   const int lw = (int)blockIdx.x - I->wg_begin_s;
-  const int z = lw / I->adj_segs_s;
+  const int z = lw / I->adj_segs_s;             // row (y, z)
   const int i_s = (lw - z * I->adj_segs_s) * SEG;
-  const int n_chunks = I->n_chunks;
+  const int Ney = Y3 ? I->Ney : 1;
+  const int n_steps = I->n_chunks * Ney;
 
   if (filler) {
     // ---------------------------------------------------------------- fill waves
@@ -445,36 +456,42 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict
       const int t = u + t_shift;
       eidx[j] = (u < L && t >= 0 && t < I->N_ext) ? I->ext[t] : -1;
     }
-    // two register sets: the loads of chunk c + 2 are issued when chunk c has
-    // been written to LDS, so each load has two chunk periods to arrive
+    // two register sets: the loads of step s + 2 are issued when step s has
+    // been written to LDS, so each load has two step periods to arrive
     float2 hA[HPT], wA[WPT], pA[PPT], hB[HPT], wB[WPT], pB[PPT];
     const int gi = min(i_s + ft, I->Nx - 1);
-    // Loads of chunk `cidx`, all from valid addresses: window entries outside
-    // the table are zeroed when written to LDS; p_inc rows of unused columns
-    // read the zero row; points past the row end are never written out.
-#define BORN_ADJS_PREFETCH(CIDX, H, WV, PV)                                                      \
+    // Loads of step `st` = (chunk, slice), all from valid addresses: window
+    // entries outside the table are zeroed when written to LDS; p_inc rows of
+    // unused columns read the zero row; points past the row end are never
+    // written out.  p_inc is loaded with a chunk's first slice.
+#define BORN_ADJS_PREFETCH(ST, H, WV, PV)                                                        \
     {                                                                                            \
-      const int cidx_ = (CIDX);                                                                  \
-      const Chunk c_ = chunks[I->chunk_base + cidx_];                                            \
-      const float2* hrow_ = I->h + (size_t)c_.fg * I->Hp + (size_t)z * I->HR;                     \
+      const int st_ = (ST);                                                                      \
+      const int c_ = Y3 ? st_ / Ney : st_, ey_ = Y3 ? st_ - c_ * Ney : 0;                        \
+      const Chunk ch_ = chunks[I->chunk_base + c_];                                              \
+      const int row_ = Y3 ? I->hrow[ey_ * I->n_rows + z] : z;                                    \
+      const float2* hrow_ = I->h + (size_t)ch_.fg * I->Hp + (size_t)row_ * I->HR;                \
       _Pragma("unroll") for (int j = 0; j < HPT; ++j) H[j] = hrow_[max(eidx[j], 0)];             \
-      const float2* wblk_ = I->Wc + (size_t)cidx_ * NEP * NC;                                    \
+      const float2* wblk_ = I->Wc + (size_t)st_ * NEP * NC;                                      \
       _Pragma("unroll") for (int j = 0; j < WPT; ++j) WV[j] = wblk_[ft + 256 * j];               \
-      const int* prs_ = chunk_pairs + (size_t)(I->chunk_base + cidx_) * NC;                      \
-      _Pragma("unroll") for (int j = 0; j < PPT; ++j)                                            \
-        PV[j] = I->P[(size_t)prs_[j] * I->N_pts + (size_t)z * I->Nx + gi];                       \
+      if (!Y3 || ey_ == 0) {                                                                     \
+        const int* prs_ = chunk_pairs + (size_t)(I->chunk_base + c_) * NC;                       \
+        _Pragma("unroll") for (int j = 0; j < PPT; ++j)                                          \
+          PV[j] = I->P[(size_t)prs_[j] * I->N_pts + (size_t)z * I->Nx + gi];                     \
+      }                                                                                          \
     }
-#define BORN_ADJS_FILL(BUF, H, WV, PV)                                                           \
+#define BORN_ADJS_FILL(ST, H, WV, PV)                                                            \
     {                                                                                            \
-      unsigned short* Ar = ldsu + (BUF) * Lay::BUF_U16;                                          \
+      const int st_ = (ST);                                                                      \
+      const int c_ = Y3 ? st_ / Ney : st_, ey_ = Y3 ? st_ - c_ * Ney : 0;                        \
+      unsigned short* Ar = ldsu + (st_ & 1) * Lay::AW_U16;                                       \
       unsigned short* Wt = Ar + Lay::AR_U16;                                                     \
-      float2* Ps = reinterpret_cast<float2*>(Ar + Lay::PS_OFF);                                  \
-      /* window entry u -> phase p = u mod 4, reversed index m = LQ - 1 - u / 4; */              \
+      /* window entry u -> phase p = u mod NP, reversed index m = LQ - 1 - u / NP; */            \
       /* copy d holds m at m - d + PADL (>= 1) */                                                \
       _Pragma("unroll") for (int j = 0; j < HPT; ++j) {                                          \
         const int u = ft + 256 * j;                                                              \
         if (u < L) {                                                                             \
-          const int p = u & 3, m = (LQ - 1) - (u >> 2);                                          \
+          const int p = u % NP, m = (LQ - 1) - u / NP;                                           \
           float2 hv = H[j];                                                                      \
           if (!hfull && eidx[j] < 0) hv = make_float2(0.f, 0.f);                                 \
           unsigned int hi, lo;                                                                   \
@@ -507,25 +524,29 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict
         wd[3 * NC * WS] = (unsigned short)il;                                                    \
         wd[3 * NC * WS + 16] = (unsigned short)(il >> 16);                                       \
       }                                                                                          \
-      _Pragma("unroll") for (int j = 0; j < PPT; ++j) Ps[j * PSS + ft] = PV[j]; /* column j, point ft */ \
+      if (!Y3 || ey_ == 0) {                                                                     \
+        float2* Ps = reinterpret_cast<float2*>(ldsu + Lay::PS_OFF + (c_ & 1) * Lay::PS_U16);     \
+        _Pragma("unroll") for (int j = 0; j < PPT; ++j) Ps[j * PSS + ft] = PV[j]; /* column j, point ft */ \
+      }                                                                                          \
     }
-    // every step issues its loads (clamped to the last chunk), so the loads in
-    // flight are the same on every path and each fill waits only for its own
-    const int last = n_chunks - 1;
+    // every step issues its loads (clamped to the last step), so in 2-D the
+    // loads in flight are the same on every path and each fill waits only
+    // for its own
+    const int last = n_steps - 1;
     BORN_ADJS_PREFETCH(0, hA, wA, pA)
     BORN_ADJS_FILL(0, hA, wA, pA)
     BORN_ADJS_PREFETCH(min(1, last), hB, wB, pB)
     BORN_ADJS_PREFETCH(min(2, last), hA, wA, pA)
     __syncthreads();
-    for (int ci = 0; ci < n_chunks; ci += 2) {
-      // step ci: chunk ci + 1 (set B) -> buffer (ci + 1) & 1; set B <- chunk ci + 3
-      if (ci + 1 < n_chunks) BORN_ADJS_FILL((ci + 1) & 1, hB, wB, pB)
-      BORN_ADJS_PREFETCH(min(ci + 3, last), hB, wB, pB)
+    for (int st = 0; st < n_steps; st += 2) {
+      // step st: step st + 1 (set B) -> its buffers; set B <- step st + 3
+      if (st + 1 < n_steps) BORN_ADJS_FILL(st + 1, hB, wB, pB)
+      BORN_ADJS_PREFETCH(min(st + 3, last), hB, wB, pB)
       __syncthreads();
-      if (ci + 1 >= n_chunks) break;
-      // step ci + 1: chunk ci + 2 (set A) -> buffer ci & 1; set A <- chunk ci + 4
-      if (ci + 2 < n_chunks) BORN_ADJS_FILL(ci & 1, hA, wA, pA)
-      BORN_ADJS_PREFETCH(min(ci + 4, last), hA, wA, pA)
+      if (st + 1 >= n_steps) break;
+      // step st + 1: step st + 2 (set A) -> its buffers; set A <- step st + 4
+      if (st + 2 < n_steps) BORN_ADJS_FILL(st + 2, hA, wA, pA)
+      BORN_ADJS_PREFETCH(min(st + 4, last), hA, wA, pA)
       __syncthreads();
     }
 #undef BORN_ADJS_PREFETCH
@@ -535,25 +556,27 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict
 
   // ---------------------------------------------------------------- compute waves
   float2* __restrict__ g_out = outs.p[ki];
-  // gamma_hat partials: tile t, v (0..3) -> point s = 16 t + 4 kk + v, i = 4 s + ph
+  const int ph = w / WPP, t0 = 4 * (w % WPP);   // phase and first tile of this wave
+  // gamma_hat partials: tile t0 + t, v (0..3) -> point s = 16 (t0 + t) + 4 kk + v, i = NP s + ph
   float gx[4][4], gy[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int v = 0; v < 4; ++v) gx[a][v] = gy[a][v] = 0.f;
-  const int ph = w;
-  // lane's A read position (diagonal 0, comp 0): copy d = (-r) mod 4 of phase ph
+  // lane's A read position (relative diagonal 0, comp 0): copy d = (-r) mod 4 of phase ph
   const int dcp = (4 - (col & 3)) & 3;
-  const int abase = ph * CPB + dcp * LR + copy_offset(dcp) + (SPH - col + 8 * kk - dcp) + PADL;
-  int cur_r = chunks[I->chunk_base].r;
+  const int abase = ph * CPB + dcp * LR + copy_offset(dcp) + (SPH - col + 8 * kk - dcp) + PADL - 16 * t0;
+  v4f YR[4], YI[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    YR[t] = vz();
+    YI[t] = vz();
+  }
   __syncthreads();
-  for (int ci = 0; ci < n_chunks; ++ci) {
-    const unsigned short* Ar = ldsu + (ci & 1) * Lay::BUF_U16;
+  for (int st = 0; st < n_steps; ++st) {
+    const int c = Y3 ? st / Ney : st, ey = Y3 ? st - c * Ney : 0;
+    const unsigned short* Ar = ldsu + (st & 1) * Lay::AW_U16;
     const unsigned short* Wt = Ar + Lay::AR_U16;
-    const float2* Ps = reinterpret_cast<const float2*>(Ar + Lay::PS_OFF);
-    const int r = cur_r;
-    const bool has_next = ci + 1 < n_chunks;
-    const int next_r = has_next ? chunks[I->chunk_base + ci + 1].r : r;
 
     // every K-block of W (B operand)
     uint4 W4[KB][4];
@@ -562,14 +585,9 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict
 #pragma unroll
       for (int cp = 0; cp < 4; ++cp)
         W4[kb][cp] = *reinterpret_cast<const uint4*>(&Wt[(cp * NC + col) * WS + 32 * kb + 8 * kk]);
-    // Y for the 4 tiles of this phase: conj(H) W with conj folded into the
-    // table (imag part stored negated): Re = Ar Wr - Ai' Wi, Im = Ar Wi + Ai' Wr
-    v4f YR[4], YI[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      YR[t] = vz();
-      YI[t] = vz();
-    }
+    // Y of the 4 tiles, summed over the chunk's slices: conj(H) W with conj
+    // folded into the table (imag part stored negated):
+    // Re = Ar Wr - Ai' Wi, Im = Ar Wi + Ai' Wr
     constexpr int DMIN = -3, DMAX = 2 * (KB - 1);
     uint4 Ac[4], An[4];
 #define BORN_ADJS_LOAD_A(DST, DL)                                                                \
@@ -607,69 +625,75 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict
       }
     }
 #undef BORN_ADJS_LOAD_A
-    // gamma_hat += conj(p) Y; lane holds rows 4*kk + v of tile t (points s = 16 t + 4 kk + v), column col
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int s = 16 * t + 4 * kk + v;
-        const float2 p = Ps[col * PSS + NP * s + ph];
-        gx[t][v] += p.x * YR[t][v] + p.y * YI[t][v];
-        gy[t][v] += p.x * YI[t][v] - p.y * YR[t][v];
-      }
-    }
-    if (!has_next || next_r != r) {
+    if (ey == Ney - 1) {
+      // gamma_hat += conj(p) Y; lane holds rows 4*kk + v of tile t0 + t, column col
+      const float2* Ps = reinterpret_cast<const float2*>(ldsu + Lay::PS_OFF + (c & 1) * Lay::PS_U16);
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          float x = gx[t][v], y = gy[t][v];
-#pragma unroll
-          for (int o = 1; o < 16; o <<= 1) {
-            x += __shfl_xor(x, o, 64);
-            y += __shfl_xor(y, o, 64);
-          }
-          gx[t][v] = x;
-          gy[t][v] = y;
+          const int s = 16 * (t0 + t) + 4 * kk + v;
+          const float2 p = Ps[col * PSS + NP * s + ph];
+          gx[t][v] += p.x * YR[t][v] + p.y * YI[t][v];
+          gy[t][v] += p.x * YI[t][v] - p.y * YR[t][v];
         }
+        YR[t] = vz();
+        YI[t] = vz();
       }
-      // lane col < 4 writes v = col for every tile
-      if (col < 4) {
+      const int r = chunks[I->chunk_base + c].r;
+      const bool has_next = c + 1 < I->n_chunks;
+      if (!has_next || chunks[I->chunk_base + c + 1].r != r) {
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-          float x = gx[t][0], y = gy[t][0];
 #pragma unroll
-          for (int v = 1; v < 4; ++v) {
-            if (col == v) {
-              x = gx[t][v];
-              y = gy[t][v];
+          for (int v = 0; v < 4; ++v) {
+            float x = gx[t][v], y = gy[t][v];
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) {
+              x += __shfl_xor(x, o, 64);
+              y += __shfl_xor(y, o, 64);
             }
+            gx[t][v] = x;
+            gy[t][v] = y;
           }
-          const int s = 16 * t + 4 * kk + col;
-          const int i = i_s + NP * s + ph;
-          if (i < I->Nx) g_out[(size_t)r * I->N_pts + (size_t)z * I->Nx + i] = make_float2(x, y);
         }
+        // lane col < 4 writes v = col for every tile
+        if (col < 4) {
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            float x = gx[t][0], y = gy[t][0];
+#pragma unroll
+            for (int v = 1; v < 4; ++v) {
+              if (col == v) {
+                x = gx[t][v];
+                y = gy[t][v];
+              }
+            }
+            const int s = 16 * (t0 + t) + 4 * kk + col;
+            const int i = i_s + NP * s + ph;
+            if (i < I->Nx) g_out[(size_t)r * I->N_pts + (size_t)z * I->Nx + i] = make_float2(x, y);
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) gx[t][v] = gy[t][v] = 0.f;
       }
-#pragma unroll
-      for (int t = 0; t < 4; ++t)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) gx[t][v] = gy[t][v] = 0.f;
     }
-    cur_r = next_r;
     __syncthreads();
   }
 }
 
-template <int MT>
+template <int MT, int NP>
 static size_t adj_split_lds_bytes() {
-  return AdjSplitLayout<MT>::bytes;
+  return AdjSplitLayout<MT, NP>::bytes;
 }
 
 // ======================================================================
 // launchers
 // ======================================================================
 bool fwd_split_supported(const Plan& P) {
-  return P.structured && P.nd == 2 && P.Np == 4 && P.Ne_pad >= 32 && P.Ne_pad <= 128;
+  return P.structured && (P.Np == 4 || P.Np == 2) && P.Ne_pad >= 32 && P.Ne_pad <= 128;
 }
 
 void launch_fwd_split(const LaunchSet& L, int n_cu, hipStream_t s) {
@@ -678,37 +702,68 @@ void launch_fwd_split(const LaunchSet& L, int n_cu, hipStream_t s) {
   for (const Inst& I : L.inst) even = even && (I.Nx % 2) == 0;
   const int n_inst = (int)L.inst.size();
   const int Ne_pad = 16 * L.MT;
-#define BORN_FWDS(MTv)                                                                               \
-  if (even)                                                                                          \
-    hipLaunchKernelGGL((k_fwd_s2d_split<MTv, true>), dim3(grid), dim3(256), fwd_split_lds_bytes<MTv>(), s, \
-                       L.d_inst, n_inst, L.n_items_s, Ne_pad, L.d_chunks, L.d_chunk_pairs);            \
-  else                                                                                               \
-    hipLaunchKernelGGL((k_fwd_s2d_split<MTv, false>), dim3(grid), dim3(256), fwd_split_lds_bytes<MTv>(), s, \
-                       L.d_inst, n_inst, L.n_items_s, Ne_pad, L.d_chunks, L.d_chunk_pairs);
+#define BORN_FWDS_L(MTv, EV, NPv)                                                                      \
+  hipLaunchKernelGGL((k_fwd_s2d_split<MTv, EV, NPv>), dim3(grid), dim3(256), (fwd_split_lds_bytes<MTv, NPv>()), s, \
+                     L.d_inst, n_inst, L.n_items_s, Ne_pad, L.d_chunks, L.d_chunk_pairs)
+#define BORN_FWDS_NP(MTv, EV)                 \
+  if (L.Np == 2) BORN_FWDS_L(MTv, EV, 2);     \
+  else BORN_FWDS_L(MTv, EV, 4);
+#define BORN_FWDS(MTv)          \
+  if (even) {                   \
+    BORN_FWDS_NP(MTv, true)     \
+  } else {                      \
+    BORN_FWDS_NP(MTv, false)    \
+  }
   switch (L.MT) {
     case 2: BORN_FWDS(2) break;
     case 4: BORN_FWDS(4) break;
     default: BORN_FWDS(8) break;
   }
 #undef BORN_FWDS
+#undef BORN_FWDS_NP
+#undef BORN_FWDS_L
 }
 
 void launch_adj_split(const LaunchSet& L, const OutPtrs& outs, hipStream_t s) {
   const dim3 grid(L.n_wg_s);
   const int n_inst = (int)L.inst.size();
-  switch (L.MT) {
-    case 2: hipLaunchKernelGGL((k_adj_s2d_split<2>), grid, dim3(512), adj_split_lds_bytes<2>(), s, L.d_inst, n_inst, outs, L.d_chunks, L.d_chunk_pairs); break;
-    case 4: hipLaunchKernelGGL((k_adj_s2d_split<4>), grid, dim3(512), adj_split_lds_bytes<4>(), s, L.d_inst, n_inst, outs, L.d_chunks, L.d_chunk_pairs); break;
-    default: hipLaunchKernelGGL((k_adj_s2d_split<8>), grid, dim3(512), adj_split_lds_bytes<8>(), s, L.d_inst, n_inst, outs, L.d_chunks, L.d_chunk_pairs); break;
+  bool y3 = false;
+  for (const Inst& I : L.inst) y3 = y3 || I.Ney > 1;
+#define BORN_ADJS_L(MTv, NPv, Y3v)                                                                    \
+  hipLaunchKernelGGL((k_adj_s2d_split<MTv, NPv, Y3v>), grid, dim3(512), (adj_split_lds_bytes<MTv, NPv>()), s, \
+                     L.d_inst, n_inst, outs, L.d_chunks, L.d_chunk_pairs)
+#define BORN_ADJS_NP(MTv)                                     \
+  if (L.Np == 2) {                                            \
+    if (y3) BORN_ADJS_L(MTv, 2, true); else BORN_ADJS_L(MTv, 2, false); \
+  } else {                                                    \
+    if (y3) BORN_ADJS_L(MTv, 4, true); else BORN_ADJS_L(MTv, 4, false); \
   }
+  switch (L.MT) {
+    case 2: BORN_ADJS_NP(2) break;
+    case 4: BORN_ADJS_NP(4) break;
+    default: BORN_ADJS_NP(8) break;
+  }
+#undef BORN_ADJS_NP
+#undef BORN_ADJS_L
+}
+
+template <int MT, int NP>
+static void set_max_lds_split_mt(int bytes) {
+#define BORN_SET(K) hipFuncSetAttribute((const void*)(K), hipFuncAttributeMaxDynamicSharedMemorySize, bytes)
+  BORN_SET((k_fwd_s2d_split<MT, true, NP>));
+  BORN_SET((k_fwd_s2d_split<MT, false, NP>));
+  BORN_SET((k_adj_s2d_split<MT, NP, false>));
+  BORN_SET((k_adj_s2d_split<MT, NP, true>));
+#undef BORN_SET
 }
 
 void set_max_lds_split(int bytes) {
-#define BORN_SET(K) hipFuncSetAttribute((const void*)(K), hipFuncAttributeMaxDynamicSharedMemorySize, bytes)
-  BORN_SET((k_fwd_s2d_split<2, true>)); BORN_SET((k_fwd_s2d_split<4, true>)); BORN_SET((k_fwd_s2d_split<8, true>));
-  BORN_SET((k_fwd_s2d_split<2, false>)); BORN_SET((k_fwd_s2d_split<4, false>)); BORN_SET((k_fwd_s2d_split<8, false>));
-  BORN_SET((k_adj_s2d_split<2>)); BORN_SET((k_adj_s2d_split<4>)); BORN_SET((k_adj_s2d_split<8>));
-#undef BORN_SET
+  set_max_lds_split_mt<2, 2>(bytes);
+  set_max_lds_split_mt<4, 2>(bytes);
+  set_max_lds_split_mt<8, 2>(bytes);
+  set_max_lds_split_mt<2, 4>(bytes);
+  set_max_lds_split_mt<4, 4>(bytes);
+  set_max_lds_split_mt<8, 4>(bytes);
 }
 
 }  // namespace born

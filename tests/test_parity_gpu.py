@@ -98,26 +98,28 @@ def test_3d_tables(torch_cuda, general):
     assert e_f < TOL_FP32 and e_a < TOL_FP32
 
 
+@pytest.mark.parametrize("precision,tol,nex", [("fp32", TOL_FP32, 8), ("split_bf16", 1e-4, 32)])
 @pytest.mark.parametrize("shape", [(20, 12, 5), (10, 7, 3)])
-def test_3d_geometry_structured(torch_cuda, shape):
-    """3-D plans built from geometry on the structured path: an 8 x 4 matrix
-    array at pitch/2 over a FOV as wide as the aperture (mirrored tables) and
-    over a narrower one (tables on the extended grid), against every
-    element's own transfer function."""
-    arr = C.ArraySpec((8, 4), (3e-4, 3e-4), (2.7e-4, 2.7e-4), (2, 2))
+def test_3d_geometry_structured(torch_cuda, shape, precision, tol, nex):
+    """3-D plans built from geometry on the structured path: an nex x 4
+    matrix array at pitch/2 over FOVs as wide as the aperture (mirrored
+    tables) or narrower (tables on the extended grid), against every
+    element's own transfer function.  The split form needs 17 or more
+    elements per x row."""
+    arr = C.ArraySpec((nex, 4), (3e-4, 3e-4), (2.7e-4, 2.7e-4), (2, 2))
     fov = C.FovSpec(shape, (1.5e-4, 1.5e-4, 1.5e-4), 2e-3)
     waves = [C.qsw_wave(arr, np.array([x, 0.0, -1e-2]), 1500.0) for x in (0.0, 6e-4)]
     spec = C.SequenceSpec("t3g", arr, C.MediumSpec(), fov, np.array([2.5e6, 4e6, 5.5e6]), waves,
-                          [C.identity_mixes(32) for _ in waves])
-    op = OperatorBorn.from_sequence(spec)
+                          [C.identity_mixes(4 * nex) for _ in waves])
+    op = OperatorBorn.from_sequence(spec, precision=precision)
     assert op.path == "structured"
     T = O.direct_tables(spec)
     g = rnd((T.N_points, 2), 9)
     u = rnd((T.N_observations(), 2), 10)
     e_f = O.rel_l2(op.forward(g), O.forward(T, g))
     e_a = O.rel_l2(op.adjoint(u), O.adjoint(T, u))
-    log("3-D geometry %s: forward %.2e adjoint %.2e" % (shape, e_f, e_a))
-    assert e_f < TOL_FP32 and e_a < TOL_FP32
+    log("3-D geometry %s %s: forward %.2e adjoint %.2e" % (shape, precision, e_f, e_a))
+    assert e_f < tol and e_a < tol
 
 
 # -------------------------------------------------------------- config 0
@@ -305,9 +307,10 @@ def test_combined_quick_gpu_contract(torch_cuda):
 TOL_SPLIT = 1e-4
 
 
-@pytest.mark.parametrize("N_e,Nx,Nz", [(20, 80, 4), (40, 300, 3), (128, 520, 2)])
-def test_split_bf16_small_tables(torch_cuda, N_e, Nx, Nz):
-    spec = tiny_spec(N_e=N_e, Nx=Nx, Nz=Nz, npp=4)
+@pytest.mark.parametrize("N_e,Nx,Nz,npp", [(20, 80, 4, 4), (40, 300, 3, 4), (128, 520, 2, 4),
+                                           (20, 40, 4, 2), (64, 300, 3, 2), (128, 260, 2, 2)])
+def test_split_bf16_small_tables(torch_cuda, N_e, Nx, Nz, npp):
+    spec = tiny_spec(N_e=N_e, Nx=Nx, Nz=Nz, npp=npp)
     T = O.build_tables(spec)
     op = OperatorBorn.from_tables(T, precision="split_bf16")
     g = rnd((T.N_points, 2), 71)
@@ -315,7 +318,7 @@ def test_split_bf16_small_tables(torch_cuda, N_e, Nx, Nz):
     e_f = O.rel_l2(op.forward(g), O.forward(T, g))
     e_a = O.rel_l2(op.adjoint(u), O.adjoint(T, u))
     assert op.path == "structured"
-    log("split bf16 N_e=%d Nx=%d: forward rel-L2 %.2e adjoint rel-L2 %.2e" % (N_e, Nx, e_f, e_a))
+    log("split bf16 N_e=%d Nx=%d npp=%d: forward rel-L2 %.2e adjoint rel-L2 %.2e" % (N_e, Nx, npp, e_f, e_a))
     assert e_f < TOL_SPLIT and e_a < TOL_SPLIT
 
 
@@ -461,7 +464,7 @@ def config4_case():
     return dict(spec=spec, pts=pts, rows=rows, g_sub=g_sub, u_sub=u_sub, u_ref=u_ref, g_ref=g_ref)
 
 
-@pytest.mark.parametrize("precision,tol", [("fp32", TOL_FP32)])
+@pytest.mark.parametrize("precision,tol", [("fp32", TOL_FP32), ("split_bf16", TOL_SPLIT)])
 def test_config4_subfov_oracle(torch_cuda, config4_case, precision, tol):
     import torch
     d = config4_case
