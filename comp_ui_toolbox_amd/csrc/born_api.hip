@@ -106,9 +106,8 @@ Plan::~Plan() {
   dfree(d_act);
   dfree(d_pref);
   dfree(d_meas_pair_base);
-  dfree(d_pair_list_off);
-  dfree(d_pair_list);
-  dfree(d_slot);
+  dfree(d_pe_off);
+  dfree(d_pe_ent);
   if (d_io_in) hipFree(d_io_in);
   if (d_io_out) hipFree(d_io_out);
   if (stream) hipStreamDestroy(stream);
@@ -215,26 +214,32 @@ static void upload_index_structures(Plan& P, hipStream_t s) {
   std::vector<double2> pref(P.pref.size() / 2);
   for (size_t i = 0; i < pref.size(); ++i) pref[i] = make_double2(P.pref[2 * i], P.pref[2 * i + 1]);
   P.d_pref = dupload(pref, s);
-  // un-mixing gather lists
-  std::vector<int> cnt(P.N_pairs + 1, 0);
+  // un-mixing gather lists: for every (pair, element) the (observation,
+  // prefactor) entries that read it, in (mix, frequency) order
+  std::vector<int64_t> peoff((size_t)P.N_pairs * P.N_e + 1, 0);
   for (int mk = 0; mk < P.N_mk; ++mk)
-    for (int f = 0; f < P.mk_Nf[mk]; ++f) cnt[P.meas_pair_base[P.mk_meas[mk]] + P.fmeas[P.mk_f_off[mk] + f] + 1]++;
-  for (int i = 0; i < P.N_pairs; ++i) cnt[i + 1] += cnt[i];
-  std::vector<int2> lst(cnt[P.N_pairs]);
+    for (int f = 0; f < P.mk_Nf[mk]; ++f) {
+      const int pr = P.meas_pair_base[P.mk_meas[mk]] + P.fmeas[P.mk_f_off[mk] + f];
+      for (int a = 0; a < P.mk_Nact[mk]; ++a) peoff[(size_t)pr * P.N_e + P.act[P.mk_a_off[mk] + a] + 1]++;
+    }
+  for (size_t i = 0; i + 1 < peoff.size(); ++i) peoff[i + 1] += peoff[i];
+  if (peoff.back() > INT32_MAX || P.N_obs > INT32_MAX || (int64_t)P.pref.size() / 2 > INT32_MAX)
+    throw Error(BORN_ERR_UNSUPPORTED, "too many observations for the un-mixing lists");
+  std::vector<int2> ent((size_t)peoff.back());
   {
-    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    std::vector<int64_t> fill(peoff.begin(), peoff.end() - 1);
     for (int mk = 0; mk < P.N_mk; ++mk)
       for (int f = 0; f < P.mk_Nf[mk]; ++f) {
         const int pr = P.meas_pair_base[P.mk_meas[mk]] + P.fmeas[P.mk_f_off[mk] + f];
-        lst[fill[pr]++] = make_int2(mk, f);
+        for (int a = 0; a < P.mk_Nact[mk]; ++a) {
+          const size_t pe = (size_t)pr * P.N_e + P.act[P.mk_a_off[mk] + a];
+          ent[fill[pe]++] = make_int2((int)(P.mk_obs_off[mk] + f),
+                                      (int)(P.mk_pref_off[mk] + (int64_t)a * P.mk_Nf[mk] + f));
+        }
       }
   }
-  P.d_pair_list_off = dupload(cnt, s);
-  P.d_pair_list = dupload(lst, s);
-  std::vector<int> slot((size_t)P.N_mk * P.N_e, -1);
-  for (int mk = 0; mk < P.N_mk; ++mk)
-    for (int a = 0; a < P.mk_Nact[mk]; ++a) slot[(size_t)mk * P.N_e + P.act[P.mk_a_off[mk] + a]] = a;
-  P.d_slot = dupload(slot, s);
+  P.d_pe_off = dupload(peoff, s);
+  P.d_pe_ent = dupload(ent, s);
 }
 
 // The lateral Toeplitz structure of setup_grid_symmetric.shift_lateral

@@ -134,9 +134,8 @@ struct Plan {
   double2* d_pref = nullptr;
   int* d_meas_pair_base = nullptr;
   // adjoint un-mixing gather lists: for each pair, the (mk, f) that read it
-  int* d_pair_list_off = nullptr; // [N_pairs+1]
-  int2* d_pair_list = nullptr;    // (mk, f)
-  int* d_slot = nullptr;          // [N_mk][N_e]: active slot of element or -1
+  int64_t* d_pe_off = nullptr;    // [N_pairs * N_e + 1]: un-mixing entries of (pair, element)
+  int2* d_pe_ent = nullptr;       // (observation, prefactor index)
   std::vector<int> h_shift;       // host copy of the 0-based shift map [N_e][N_pts]
 
   // per-N_rhs workspace

@@ -80,35 +80,41 @@ void launch_mask_gamma(const float2* g, int N_pts, int N_rhs, float2* gm, hipStr
   hipLaunchKernelGGL(k_mask_gamma, dim3((N_pts + 255) / 256), dim3(256), 0, s, g, N_pts, N_rhs, gm);
 }
 
-// z-block partials -> V[pair][e][r] (FP64 sum)
-// V[pair][e][r] = sum of the depth-block partials of (chunk, y slice ey) at
-// element ex, with e = ex + Nex * ey.
-__global__ void k_fwd_reduce(const float2* __restrict__ vpart, const int* __restrict__ pr_chunk,
-                             const int* __restrict__ pr_pos, int N_pairs, int N_e, int N_rhs, int NZB,
-                             int Ne_pad, int Nex, int Ney, double2* __restrict__ V) {
+// Row-block partials -> V[pair][e][r] (FP64 sum): V at element e = ex + Nex * ey
+// of the pair and rhs in column pos of chunk ch is the sum over the chunk's
+// blocks of slice ey.  Threads run over (chunk, slice, element, column), so
+// neighbouring threads read neighbouring partials.
+__global__ void k_fwd_reduce(const float2* __restrict__ vpart, const Chunk* __restrict__ chunks,
+                             const int* __restrict__ col_pair, int n_chunks, int NZB, int Ne_pad, int Nex, int Ney,
+                             int N_e, int N_rhs, double2* __restrict__ V) {
   const size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  const size_t total = (size_t)N_pairs * N_e * N_rhs;
-  if (idx >= total) return;
-  const int r = (int)(idx % N_rhs);
-  const size_t pe = idx / N_rhs;
-  const int e = (int)(pe % N_e);
-  const int pair = (int)(pe / N_e);
-  const int ch = pr_chunk[pair * N_rhs + r], pos = pr_pos[pair * N_rhs + r];
-  const int ex = e % Nex, ey = e / Nex;
+  if (idx >= (size_t)n_chunks * Ney * Ne_pad * NC) return;
+  const int pos = (int)(idx % NC);
+  size_t t = idx / NC;
+  const int ex = (int)(t % Ne_pad);
+  t /= Ne_pad;
+  const int ey = (int)(t % Ney);
+  const int ch = (int)(t / Ney);
+  const Chunk c = chunks[ch];
+  if (pos >= c.ncols || ex >= Nex) return;
   double sx = 0.0, sy = 0.0;
+  const float2* src = vpart + (((size_t)ch * Ney + ey) * NZB * Ne_pad + ex) * NC + pos;
   for (int zb = 0; zb < NZB; ++zb) {
-    const float2 v = vpart[((((size_t)ch * Ney + ey) * NZB + zb) * Ne_pad + ex) * NC + pos];
+    const float2 v = src[(size_t)zb * Ne_pad * NC];
     sx += v.x;
     sy += v.y;
   }
-  V[idx] = make_double2(sx, sy);
+  const int pair = col_pair[c.col_start + pos];
+  V[((size_t)pair * N_e + ey * Nex + ex) * N_rhs + c.r] = make_double2(sx, sy);
 }
 
 void launch_fwd_reduce(const Plan& P, int N_rhs, int nblk, hipStream_t s) {
-  const size_t total = (size_t)P.N_pairs * P.N_e * N_rhs;
-  hipLaunchKernelGGL(k_fwd_reduce, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, P.d_vpart,
-                     P.d_pr_chunk, P.d_pr_pos, P.N_pairs, P.N_e, N_rhs, nblk, P.Ne_pad, P.Nex,
-                     P.nd == 3 ? P.Ney : 1, P.d_V);
+  (void)N_rhs;   // the rhs of every column comes from its chunk
+  const int Ney = P.nd == 3 ? P.Ney : 1;
+  const int n_chunks = (int)P.chunks.size();
+  const size_t total = (size_t)n_chunks * Ney * P.Ne_pad * NC;
+  hipLaunchKernelGGL(k_fwd_reduce, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, P.d_vpart, P.d_chunks,
+                     P.d_col_pair, n_chunks, nblk, P.Ne_pad, P.Nex, Ney, P.N_e, P.ws_rhs, P.d_V);
 }
 
 // ------------------------------------------------------ general forward
@@ -196,16 +202,15 @@ void launch_fwd_mix(const Plan& P, int N_rhs, void* u, bool out_f64, hipStream_t
 
 // ------------------------------------------------------------- un-mixing
 // W[pair][e][r] = sum over the (mk, f) reading this pair and the active
-// slot a of element e: conj(pref[mk][f][a]) * u[r][obs(mk, f)].  A gather,
-// so the result is deterministic.  With Wc set, the result goes to the
-// chunk layout Wc[chunk(pair, r)][e][column(pair, r)] of the structured
-// kernels instead.
-__global__ void k_adj_unmix(const float2* __restrict__ u, const int* __restrict__ pair_list_off,
-                            const int2* __restrict__ pair_list, const int* __restrict__ slot,
-                            const int* __restrict__ mk_meta, const int64_t* __restrict__ mk_off,
-                            const double2* __restrict__ pref, int N_pairs, int N_e, int N_rhs, int64_t N_obs,
-                            float2* __restrict__ W, float2* __restrict__ Wc, const int* __restrict__ pr_chunk,
-                            const int* __restrict__ pr_pos, int Ne_pad, int Nex, int Ney) {
+// slot a of element e: conj(pref[mk][f][a]) * u[r][obs(mk, f)], walked from
+// the per-(pair, element) entry lists.  A gather, so the result is
+// deterministic.  With Wc set, the result goes to the chunk layout
+// Wc[chunk(pair, r)][e][column(pair, r)] of the structured kernels instead.
+__global__ void k_adj_unmix(const float2* __restrict__ u, const int64_t* __restrict__ pe_off,
+                            const int2* __restrict__ pe_ent, const double2* __restrict__ pref, int N_pairs,
+                            int N_e, int N_rhs, int64_t N_obs, float2* __restrict__ W, float2* __restrict__ Wc,
+                            const int* __restrict__ pr_chunk, const int* __restrict__ pr_pos, int Ne_pad, int Nex,
+                            int Ney) {
   const size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (idx >= (size_t)N_pairs * N_e * N_rhs) return;
   const int r = (int)(idx % N_rhs);
@@ -213,13 +218,11 @@ __global__ void k_adj_unmix(const float2* __restrict__ u, const int* __restrict_
   const int e = (int)(pe % N_e);
   const int pair = (int)(pe / N_e);
   double sx = 0.0, sy = 0.0;
-  for (int j = pair_list_off[pair]; j < pair_list_off[pair + 1]; ++j) {
-    const int2 mf = pair_list[j];
-    const int a = slot[(size_t)mf.x * N_e + e];
-    if (a < 0) continue;
-    const int Nf = mk_meta[mf.x * 4 + 1];
-    const double2 c = pref[mk_off[mf.x * 4 + 3] + (int64_t)a * Nf + mf.y];
-    const float2 v = u[(int64_t)r * N_obs + mk_off[mf.x * 4 + 0] + mf.y];
+  const float2* ur = u + (int64_t)r * N_obs;
+  for (int64_t j = pe_off[pe]; j < pe_off[pe + 1]; ++j) {
+    const int2 oe = pe_ent[j];
+    const double2 c = pref[oe.y];
+    const float2 v = ur[oe.x];
     sx += c.x * v.x + c.y * v.y;
     sy += c.x * v.y - c.y * v.x;
   }
@@ -235,10 +238,9 @@ __global__ void k_adj_unmix(const float2* __restrict__ u, const int* __restrict_
 
 void launch_adj_unmix(const Plan& P, const float2* u, int N_rhs, bool chunked, hipStream_t s) {
   const size_t total = (size_t)P.N_pairs * P.N_e * N_rhs;
-  hipLaunchKernelGGL(k_adj_unmix, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, u, P.d_pair_list_off,
-                     P.d_pair_list, P.d_slot, P.d_mk_meta, P.d_mk_off, P.d_pref, P.N_pairs, P.N_e, N_rhs,
-                     P.N_obs, P.d_W, chunked ? P.d_Wc : nullptr, P.d_pr_chunk, P.d_pr_pos, P.Ne_pad, P.Nex,
-                     P.nd == 3 ? P.Ney : 1);
+  hipLaunchKernelGGL(k_adj_unmix, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, u, P.d_pe_off,
+                     P.d_pe_ent, P.d_pref, P.N_pairs, P.N_e, N_rhs, P.N_obs, P.d_W, chunked ? P.d_Wc : nullptr,
+                     P.d_pr_chunk, P.d_pr_pos, P.Ne_pad, P.Nex, P.nd == 3 ? P.Ney : 1);
 }
 
 // ------------------------------------------------------ general adjoint

@@ -86,6 +86,14 @@ __device__ __forceinline__ const void* byte_offset(const void* base, unsigned in
 //                        entries (8 / NP copies) so every lane's 8-entry read
 //                        is 16-byte aligned
 //   Bt[comp][c][SEGP]    p_inc*gamma of the segment, points contiguous
+// Copy length of the forward's window planes: at least LA + 12 entries,
+// 16-byte multiple, and a stride between copies that puts the lanes of one
+// 8-lane read group on distinct banks (32 mod 64 entries for 2 copies,
+// 16 mod 64 for 4).
+__host__ __device__ constexpr int fwd_split_lap(int LA, int NP) {
+  return ((LA + 12 + 63 - (NP == 4 ? 32 : 16)) / 64) * 64 + (NP == 4 ? 32 : 16);
+}
+
 template <int MT, bool EVEN, int NP>
 __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict__ inst, int n_inst, int n_items,
                                                           int Ne_pad, const Chunk* __restrict__ chunks,
@@ -99,7 +107,7 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict
   constexpr int MTW = MT / EW;
   constexpr int KBW = (SEG / 32) / PW;
   constexpr int LA = SEG + NP * (16 * MT - 1);
-  constexpr int LAP = ((LA + 4 + 7) / 8) * 8 + 8;
+  constexpr int LAP = fwd_split_lap(LA, NP);
   constexpr int APT = (LA + 255) / 256;
   unsigned short* Aw = ldsu;                     // [4][NCOPY][LAP]
   unsigned short* Bt = ldsu + 4 * NCOPY * LAP;   // [4][NC][SEGP]
@@ -363,7 +371,7 @@ template <int MT, int NP>
 static size_t fwd_split_lds_bytes() {
   constexpr int SEGP = SEG + 8;
   constexpr int LA = SEG + NP * (16 * MT - 1);
-  constexpr int LAP = ((LA + 4 + 7) / 8) * 8 + 8;
+  constexpr int LAP = fwd_split_lap(LA, NP);
   const size_t body = ((size_t)4 * (8 / NP) * LAP + (size_t)4 * NC * SEGP) * sizeof(unsigned short);
   const size_t red = (size_t)4 * (MT >= 2 ? MT / 2 : MT) * 8 * 64 * sizeof(float);
   return std::max(body, red);
