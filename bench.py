@@ -110,20 +110,21 @@ def spec_gamma(spec):
 
 def time_form(op, precision, x, u, g, steps, warmup):
     """Warm up, then time `steps` forward+adjoint pairs of one numeric form.
-    Returns wall seconds and the per-launch kernel milliseconds."""
+    x (N_rhs, N_pts), u (N_rhs, N_obs), g (N_rhs, N_pts).  Returns wall
+    seconds and the per-launch kernel milliseconds."""
     import torch
     op.set_precision(precision)
     for _ in range(warmup):
-        op.forward_dev(x, out=u.view(1, -1))
-        op.adjoint_dev(u, out=g.view(1, -1))
+        op.forward_dev(x, out=u)
+        op.adjoint_dev(u, out=g)
     torch.cuda.synchronize()
     op.get_timing()
     op.set_timing(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        op.forward_dev(x, out=u.view(1, -1))
-        op.adjoint_dev(u, out=g.view(1, -1))
+        op.forward_dev(x, out=u)
+        op.adjoint_dev(u, out=g)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     op.set_timing(False)
@@ -263,6 +264,8 @@ def main():
     ap.add_argument("--no-compare", action="store_true",
                     help="do not time the plain FP32 form beside the selected one")
     ap.add_argument("--config", default="config1")
+    ap.add_argument("--rhs", type=int, default=0,
+                    help="gamma right-hand sides per application (default 4 for config4, else 1)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-mixes", type=int, default=128)
     ap.add_argument("--batched-split", action="store_true",
@@ -293,9 +296,14 @@ def main():
     t_tables = time.perf_counter() - t0
     print(f"[bench] tables built on device in {t_tables:.2f} s, path {op.path}", file=sys.stderr, flush=True)
 
-    x = torch.tensor(spec_gamma(spec).astype(np.complex64), device="cuda")
-    u = torch.empty(op.N_observations, dtype=torch.complex64, device="cuda")
-    g = torch.empty(op.N_points, dtype=torch.complex64, device="cuda")
+    n_rhs = args.rhs if args.rhs > 0 else (4 if args.config == "config4" else 1)
+    x0 = spec_gamma(spec)
+    rng = np.random.Generator(np.random.PCG64(spec.seed + 100))
+    # column 0: the phantom; further columns: independent N(0,1) speckle
+    xs = [x0] + [rng.standard_normal(x0.shape) for _ in range(n_rhs - 1)]
+    x = torch.tensor(np.stack(xs).astype(np.complex64), device="cuda")
+    u = torch.empty((n_rhs, op.N_observations), dtype=torch.complex64, device="cuda")
+    g = torch.empty((n_rhs, op.N_points), dtype=torch.complex64, device="cuda")
 
     forms = [args.precision]
     if not args.no_compare and args.precision != "fp32":
@@ -306,13 +314,13 @@ def main():
         # one pair from the same input, kept for the cross-form comparison
         uf = torch.empty_like(u)
         gf = torch.empty_like(g)
-        op.forward_dev(x, out=uf.view(1, -1))
-        op.adjoint_dev(uf, out=gf.view(1, -1))
+        op.forward_dev(x, out=uf)
+        op.adjoint_dev(uf, out=gf)
         outs[form] = (uf, gf)
     torch.cuda.synchronize()
 
-    F = algorithmic_flops(spec, 1)
-    B = algorithmic_bytes(spec, 1)
+    F = algorithmic_flops(spec, n_rhs)
+    B = algorithmic_bytes(spec, n_rhs)
     elapsed, kern = res[args.precision]
     pairs_per_s = args.steps / elapsed
     result = {
@@ -331,7 +339,7 @@ def main():
         "config": {
             "workload": f"{args.config}: {spec.array.N_elements} el, {spec.fov.N_points_axis[0]}x"
                         f"{spec.fov.N_points_axis[-1]} FOV, {len(spec.frequencies)} f, "
-                        f"{spec.N_measurements} waves, {spec.N_observations()} obs, 1 rhs",
+                        f"{spec.N_measurements} waves, {spec.N_observations()} obs, {n_rhs} rhs",
             "precision": args.precision,
             "path": op.path,
             "table_build_s": round(t_tables, 3),
