@@ -80,6 +80,15 @@ static void free_workspace(Plan& P) {
   dfree(P.d_W);
   dfree(P.d_Wc);
   dfree(P.d_gm);
+  dfree(P.d_fchunks);
+  dfree(P.d_fcol);
+  dfree(P.d_fchunk_pairs);
+  dfree(P.d_pexp);
+  dfree(P.d_ones);
+  dfree(P.d_fvpart);
+  P.fwd_expand = false;
+  P.fchunks.clear();
+  P.fcol.clear();
   P.ws_rhs = -1;
   P.chunks.clear();
   P.col_pair.clear();
@@ -90,6 +99,7 @@ Plan::~Plan() {
   if (stream) hipStreamSynchronize(stream);
   free_workspace(*this);
   dfree(ls.d_inst);
+  dfree(ls_fwd.d_inst);
   dfree(d_exty);
   dfree(d_hrow);
   dfree(d_h);
@@ -363,19 +373,32 @@ static void ensure_shift_on_device(Plan& P) {
 // Forward blocking: about items_target work items per launch, (chunk,
 // z-block) for the plain kernel and (chunk, z-block, 256-point segment) for
 // the split kernel.
-static void set_blocking(Plan& P) {
-  const int n_chunks = (int)P.chunks.size();
+struct Blocking {
+  int NZB, ZR, NZB_s, ZR_s, nseg_s, vblk;
+};
+static Blocking blocking_for(const Plan& P, int n_chunks) {
+  Blocking b;
   const int Ney = P.nd == 3 ? P.Ney : 1;
   const int rows = P.n_rows > 0 ? P.n_rows : P.Nz;
   const int target = P.items_target;
   int NZB = std::max(1, std::min(rows, target / std::max(1, n_chunks * Ney)));
-  P.ZR = (rows + NZB - 1) / NZB;
-  P.NZB = (rows + P.ZR - 1) / P.ZR;
-  P.nseg_s = (P.Nx + 255) / 256;
-  NZB = std::max(1, std::min(rows, target / std::max(1, n_chunks * Ney * P.nseg_s)));
-  P.ZR_s = (rows + NZB - 1) / NZB;
-  P.NZB_s = (rows + P.ZR_s - 1) / P.ZR_s;
-  P.vblk = Ney * std::max(P.NZB, P.NZB_s * P.nseg_s);   // partial-sum blocks per chunk
+  b.ZR = (rows + NZB - 1) / NZB;
+  b.NZB = (rows + b.ZR - 1) / b.ZR;
+  b.nseg_s = (P.Nx + 255) / 256;
+  NZB = std::max(1, std::min(rows, target / std::max(1, n_chunks * Ney * b.nseg_s)));
+  b.ZR_s = (rows + NZB - 1) / NZB;
+  b.NZB_s = (rows + b.ZR_s - 1) / b.ZR_s;
+  b.vblk = Ney * std::max(b.NZB, b.NZB_s * b.nseg_s);   // partial-sum blocks per chunk
+  return b;
+}
+static void set_blocking(Plan& P) {
+  const Blocking b = blocking_for(P, (int)P.chunks.size());
+  P.NZB = b.NZB;
+  P.ZR = b.ZR;
+  P.NZB_s = b.NZB_s;
+  P.ZR_s = b.ZR_s;
+  P.nseg_s = b.nseg_s;
+  P.vblk = b.vblk;
 }
 
 // The record of a plan inside a structured launch (workspace must exist).
@@ -434,6 +457,73 @@ static void finalize_launchset(LaunchSet& L) {
   HIP_CHECK(hipMemcpy(L.d_inst, L.inst.data(), L.inst.size() * sizeof(Inst), hipMemcpyHostToDevice));
 }
 
+// With several right-hand sides, chunks that pack the (pair, rhs) columns
+// of all right-hand sides of a frequency need fewer MFMA column tiles than
+// one chunk set per rhs.  The forward then runs on the products
+// p_inc * gamma per column (one extra pass over p_inc), when they fit in
+// the free device memory with 1 GiB to spare.
+static void setup_expanded_forward(Plan& P, int N_rhs) {
+  if (N_rhs <= 1) return;
+  std::vector<Chunk> fch;
+  std::vector<int> fcol;
+  for (int fg = 0; fg < P.N_fu; ++fg) {
+    const int b = P.fg_pair_off[fg], e = P.fg_pair_off[fg + 1];
+    const int ncol = (e - b) * N_rhs;
+    for (int j = 0; j < ncol; j += NC) {
+      Chunk c;
+      c.fg = fg;
+      c.r = 0;
+      c.col_start = (int)fcol.size();
+      c.ncols = std::min(NC, ncol - j);
+      for (int q = 0; q < c.ncols; ++q) {
+        const int k = j + q;   // column k of the frequency: pair b + k / N_rhs, rhs k % N_rhs
+        fcol.push_back(P.fg_pairs[b + k / N_rhs] * N_rhs + k % N_rhs);
+      }
+      fch.push_back(c);
+    }
+  }
+  if (fch.size() >= P.chunks.size()) return;   // no column tiles saved
+  const size_t rows = (size_t)P.N_pairs * N_rhs + 1;
+  const size_t bytes = rows * P.N_pts * sizeof(float2);
+  size_t free_b = 0, total_b = 0;
+  HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+  if (bytes + ((size_t)1 << 30) > free_b) return;
+  const int n_f = (int)fch.size();
+  std::vector<int> fpairs((size_t)n_f * NC, (int)(rows - 1));   // unused columns: the zero row
+  for (int c = 0; c < n_f; ++c)
+    for (int q = 0; q < fch[c].ncols; ++q) fpairs[(size_t)c * NC + q] = fcol[fch[c].col_start + q];
+  const Blocking bl = blocking_for(P, n_f);
+  P.fchunks = fch;
+  P.fcol = fcol;
+  P.d_fchunks = dupload(fch, P.stream);
+  P.d_fcol = dupload(fcol, P.stream);
+  P.d_fchunk_pairs = dupload(fpairs, P.stream);
+  P.d_pexp = dalloc<float2>(rows * P.N_pts);
+  HIP_CHECK(hipMemset(P.d_pexp + (rows - 1) * P.N_pts, 0, P.N_pts * sizeof(float2)));
+  std::vector<float2> ones(P.N_pts, make_float2(1.f, 0.f));
+  P.d_ones = dupload(ones, P.stream);
+  P.fvblk = bl.vblk;
+  P.d_fvpart = dalloc<float2>((size_t)n_f * bl.vblk * P.Ne_pad * NC);
+  Inst I = make_inst(P);
+  I.P = (decltype(I.P))P.d_pexp;
+  I.gm = (decltype(I.gm))P.d_ones;
+  I.vpart = (decltype(I.vpart))P.d_fvpart;
+  I.n_chunks = n_f;
+  I.NZB = bl.NZB;
+  I.ZR = bl.ZR;
+  I.NZB_s = bl.NZB_s;
+  I.ZR_s = bl.ZR_s;
+  I.nseg_s = bl.nseg_s;
+  P.ls_fwd.inst.assign(1, I);
+  P.ls_fwd.MT = P.Ne_pad / 16;
+  P.ls_fwd.Np = P.Np;
+  P.ls_fwd.adj_mw = P.ls.adj_mw;
+  P.ls_fwd.d_chunks = P.d_fchunks;
+  P.ls_fwd.d_chunk_pairs = P.d_fchunk_pairs;
+  finalize_launchset(P.ls_fwd);
+  P.fwd_expand = true;
+}
+
 // Chunks of <= NC (pair, rhs) columns per frequency, rhs-major.
 static void ensure_workspace(Plan& P, int N_rhs) {
   if (P.ws_rhs == N_rhs) return;
@@ -487,6 +577,7 @@ static void ensure_workspace(Plan& P, int N_rhs) {
     P.ls.d_chunks = P.d_chunks;
     P.ls.d_chunk_pairs = P.d_chunk_pairs;
     finalize_launchset(P.ls);
+    setup_expanded_forward(P, N_rhs);
   }
   HIP_CHECK(hipDeviceSynchronize());  // workspace ready before work on any stream
 }
@@ -804,7 +895,18 @@ struct TimedLaunch {
 };
 
 static void forward_device(Plan& P, const float2* gm, int N_rhs, void* u, bool out_f64, hipStream_t s) {
-  if (use_structured(P)) {
+  if (use_structured(P) && P.fwd_expand) {
+    {
+      TimedLaunch t(P, s, 0);
+      launch_expand_pg(P, s);
+      if (P.precision == BORN_PRECISION_FP32)
+        launch_fwd_structured(P.ls_fwd, P.n_cu, s);
+      else
+        launch_fwd_split(P.ls_fwd, P.n_cu, s);
+    }
+    const Inst& I = P.ls_fwd.inst[0];
+    launch_fwd_reduce_expanded(P, P.precision == BORN_PRECISION_FP32 ? I.NZB : I.NZB_s * I.nseg_s, s);
+  } else if (use_structured(P)) {
     (void)gm;   // the launch record points at the masked input P.d_gm
     {
       TimedLaunch t(P, s, 0);

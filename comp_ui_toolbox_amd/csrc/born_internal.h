@@ -158,6 +158,21 @@ struct Plan {
   float2* d_gm = nullptr;         // masked gamma [N_rhs][N_pts]
   int items_target = 4096;        // forward work items per launch (tunable)
   LaunchSet ls;                   // structured launch of this plan alone
+  // Forward with several right-hand sides and fewer than NC pairs per
+  // frequency: the products p_inc * gamma of every (pair, rhs) column are
+  // formed first (d_pexp, a zero row after them) and the forward runs on
+  // chunks that pack the columns of all right-hand sides of a frequency.
+  bool fwd_expand = false;
+  std::vector<Chunk> fchunks;     // chunks over expanded columns ep = pair * N_rhs + r
+  std::vector<int> fcol;          // expanded column of each chunk column
+  Chunk* d_fchunks = nullptr;
+  int* d_fcol = nullptr;
+  int* d_fchunk_pairs = nullptr;  // [n_fchunks][NC]: expanded column, N_pairs * N_rhs (zero row) past ncols
+  float2* d_pexp = nullptr;       // [N_pairs * N_rhs + 1][N_pts]
+  float2* d_ones = nullptr;       // [N_pts] of 1
+  float2* d_fvpart = nullptr;     // [n_fchunks][fvblk][Ne_pad][NC]
+  int fvblk = 1;
+  LaunchSet ls_fwd;               // forward launch over the expanded columns
   size_t io_bytes = 0;
   void* d_io_in = nullptr;        // host-path staging
   void* d_io_out = nullptr;
@@ -182,6 +197,8 @@ void launch_mask_gamma(const float2* g, int N_pts, int N_rhs, float2* gm, hipStr
 void launch_fwd_structured(const LaunchSet& L, int n_cu, hipStream_t s);
 void launch_fwd_general(const Plan& P, const float2* gm, int N_rhs, hipStream_t s);
 void launch_fwd_reduce(const Plan& P, int N_rhs, int nblk, hipStream_t s);
+void launch_fwd_reduce_expanded(const Plan& P, int nblk, hipStream_t s);
+void launch_expand_pg(const Plan& P, hipStream_t s);
 void launch_fwd_mix(const Plan& P, int N_rhs, void* u, bool out_f64, hipStream_t s);
 
 void launch_adj_unmix(const Plan& P, const float2* u, int N_rhs, bool chunked, hipStream_t s);

@@ -86,7 +86,7 @@ void launch_mask_gamma(const float2* g, int N_pts, int N_rhs, float2* gm, hipStr
 // neighbouring threads read neighbouring partials.
 __global__ void k_fwd_reduce(const float2* __restrict__ vpart, const Chunk* __restrict__ chunks,
                              const int* __restrict__ col_pair, int n_chunks, int NZB, int Ne_pad, int Nex, int Ney,
-                             int N_e, int N_rhs, double2* __restrict__ V) {
+                             int N_e, int N_rhs, double2* __restrict__ V, int expand_rhs) {
   const size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (idx >= (size_t)n_chunks * Ney * Ne_pad * NC) return;
   const int pos = (int)(idx % NC);
@@ -104,8 +104,39 @@ __global__ void k_fwd_reduce(const float2* __restrict__ vpart, const Chunk* __re
     sx += v.x;
     sy += v.y;
   }
-  const int pair = col_pair[c.col_start + pos];
-  V[((size_t)pair * N_e + ey * Nex + ex) * N_rhs + c.r] = make_double2(sx, sy);
+  // expanded columns (expand_rhs > 0) carry pair * expand_rhs + rhs
+  const int col = col_pair[c.col_start + pos];
+  const int pair = expand_rhs > 0 ? col / expand_rhs : col;
+  const int r = expand_rhs > 0 ? col - pair * expand_rhs : c.r;
+  V[((size_t)pair * N_e + ey * Nex + ex) * N_rhs + r] = make_double2(sx, sy);
+}
+
+void launch_fwd_reduce_expanded(const Plan& P, int nblk, hipStream_t s) {
+  const int Ney = P.nd == 3 ? P.Ney : 1;
+  const int n_chunks = (int)P.fchunks.size();
+  const size_t total = (size_t)n_chunks * Ney * P.Ne_pad * NC;
+  hipLaunchKernelGGL(k_fwd_reduce, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, P.d_fvpart, P.d_fchunks,
+                     P.d_fcol, n_chunks, nblk, P.Ne_pad, P.Nex, Ney, P.N_e, P.ws_rhs, P.d_V, P.ws_rhs);
+}
+
+// pexp[pair * N_rhs + r][p] = p_inc[pair][p] * gamma_masked[r][p]
+__global__ void k_expand_pg(const float2* __restrict__ Pt, const float2* __restrict__ gm, int N_pairs, int N_rhs,
+                            int N_pts, float2* __restrict__ pexp) {
+  const size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (idx >= (size_t)N_pairs * N_pts) return;
+  const int p = (int)(idx % N_pts);
+  const size_t pair = idx / N_pts;
+  const float2 a = Pt[idx];
+  for (int r = 0; r < N_rhs; ++r) {
+    const float2 g = gm[(size_t)r * N_pts + p];
+    pexp[(pair * N_rhs + r) * N_pts + p] = make_float2(a.x * g.x - a.y * g.y, a.x * g.y + a.y * g.x);
+  }
+}
+
+void launch_expand_pg(const Plan& P, hipStream_t s) {
+  const size_t total = (size_t)P.N_pairs * P.N_pts;
+  hipLaunchKernelGGL(k_expand_pg, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, P.d_p, P.d_gm, P.N_pairs,
+                     P.ws_rhs, P.N_pts, P.d_pexp);
 }
 
 void launch_fwd_reduce(const Plan& P, int N_rhs, int nblk, hipStream_t s) {
@@ -114,7 +145,7 @@ void launch_fwd_reduce(const Plan& P, int N_rhs, int nblk, hipStream_t s) {
   const int n_chunks = (int)P.chunks.size();
   const size_t total = (size_t)n_chunks * Ney * P.Ne_pad * NC;
   hipLaunchKernelGGL(k_fwd_reduce, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, P.d_vpart, P.d_chunks,
-                     P.d_col_pair, n_chunks, nblk, P.Ne_pad, P.Nex, Ney, P.N_e, P.ws_rhs, P.d_V);
+                     P.d_col_pair, n_chunks, nblk, P.Ne_pad, P.Nex, Ney, P.N_e, P.ws_rhs, P.d_V, 0);
 }
 
 // ------------------------------------------------------ general forward
