@@ -524,6 +524,71 @@ static void setup_expanded_forward(Plan& P, int N_rhs) {
   P.fwd_expand = true;
 }
 
+// With the forward on expanded columns, the adjoint can pack the right-hand
+// sides the same way: chunks of NC / k pairs of one frequency with k (the
+// rhs count rounded up to a power of 2) columns each, column = slot * k + r.
+// The adjoint kernels keep one rhs per column lane and write gamma_hat once.
+static void setup_column_rhs_adjoint(Plan& P, int N_rhs) {
+  int k = 1;
+  while (k < N_rhs) k <<= 1;
+  if (N_rhs <= 1 || k > NC) return;
+  const int per = NC / k;
+  size_t n2 = 0;
+  for (int fg = 0; fg < P.N_fu; ++fg) n2 += (P.fg_pair_off[fg + 1] - P.fg_pair_off[fg] + per - 1) / per;
+  if (n2 >= P.chunks.size()) return;
+  std::vector<Chunk> ch;
+  std::vector<int> colp;
+  std::vector<int> pr_chunk((size_t)P.N_pairs * N_rhs, -1), pr_pos((size_t)P.N_pairs * N_rhs, -1);
+  std::vector<int> cpairs(n2 * NC, P.N_pairs);   // unused columns: the zero row
+  for (int fg = 0; fg < P.N_fu; ++fg) {
+    const int b = P.fg_pair_off[fg], e = P.fg_pair_off[fg + 1];
+    for (int j = b; j < e; j += per) {
+      Chunk c;
+      c.fg = fg;
+      c.r = 0;
+      c.col_start = (int)colp.size();
+      const int ns = std::min(per, e - j);
+      c.ncols = ns * k;
+      for (int q = 0; q < ns; ++q) {
+        const int pr = P.fg_pairs[j + q];
+        for (int r = 0; r < k; ++r) {
+          colp.push_back(pr);
+          if (r < N_rhs) {
+            pr_chunk[(size_t)pr * N_rhs + r] = (int)ch.size();
+            pr_pos[(size_t)pr * N_rhs + r] = q * k + r;
+            cpairs[ch.size() * NC + q * k + r] = pr;
+          }
+        }
+      }
+      ch.push_back(c);
+    }
+  }
+  dfree(P.d_chunks);
+  dfree(P.d_col_pair);
+  dfree(P.d_chunk_pairs);
+  dfree(P.d_pr_chunk);
+  dfree(P.d_pr_pos);
+  dfree(P.d_Wc);
+  dfree(P.d_vpart);   // the forward runs on the expanded chunks
+  P.chunks = ch;
+  P.col_pair = colp;
+  P.d_chunks = dupload(ch, P.stream);
+  P.d_col_pair = dupload(colp, P.stream);
+  P.d_chunk_pairs = dupload(cpairs, P.stream);
+  P.d_pr_chunk = dupload(pr_chunk, P.stream);
+  P.d_pr_pos = dupload(pr_pos, P.stream);
+  const size_t n = n2 * (P.nd == 3 ? P.Ney : 1) * P.Ne_pad * NC;
+  P.d_Wc = dalloc<float2>(n);
+  HIP_CHECK(hipMemset(P.d_Wc, 0, n * sizeof(float2)));  // unused columns and padding stay zero
+  Inst I = make_inst(P);
+  I.col_rhs = k;
+  I.n_rhs = N_rhs;
+  P.ls.inst.assign(1, I);
+  P.ls.d_chunks = P.d_chunks;
+  P.ls.d_chunk_pairs = P.d_chunk_pairs;
+  finalize_launchset(P.ls);
+}
+
 // Chunks of <= NC (pair, rhs) columns per frequency, rhs-major.
 static void ensure_workspace(Plan& P, int N_rhs) {
   if (P.ws_rhs == N_rhs) return;
@@ -578,6 +643,7 @@ static void ensure_workspace(Plan& P, int N_rhs) {
     P.ls.d_chunk_pairs = P.d_chunk_pairs;
     finalize_launchset(P.ls);
     setup_expanded_forward(P, N_rhs);
+    if (P.fwd_expand) setup_column_rhs_adjoint(P, N_rhs);
   }
   HIP_CHECK(hipDeviceSynchronize());  // workspace ready before work on any stream
 }

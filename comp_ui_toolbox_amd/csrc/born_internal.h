@@ -46,6 +46,11 @@ struct Inst {
   int item_begin, item_begin_s;  // first forward work item of the instance (plain, split)
   int wg_begin, wg_begin_s;      // first adjoint workgroup of the instance (plain, split)
   int adj_segs, adj_segs_s;      // adjoint segments per row (plain, split)
+  // adjoint chunk columns: col_rhs = 0: every column of a chunk belongs to
+  // the chunk's rhs; col_rhs = k (power of 2): column c belongs to rhs
+  // c mod k (< n_rhs), and gamma_hat of all right-hand sides is written once
+  // after the last chunk
+  int col_rhs, n_rhs;
 };
 
 constexpr int MAX_BATCH = 64;

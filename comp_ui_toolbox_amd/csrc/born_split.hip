@@ -651,13 +651,15 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict
       const int r = chunks[I->chunk_base + c].r;
       const bool has_next = c + 1 < I->n_chunks;
       if (!has_next || chunks[I->chunk_base + c + 1].r != r) {
+        // sum over the column lanes of one rhs (all 16, or those with the
+        // same column mod col_rhs)
+        const int o0 = I->col_rhs > 0 ? I->col_rhs : 1;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
             float x = gx[t][v], y = gy[t][v];
-#pragma unroll
-            for (int o = 1; o < 16; o <<= 1) {
+            for (int o = o0; o < 16; o <<= 1) {
               x += __shfl_xor(x, o, 64);
               y += __shfl_xor(y, o, 64);
             }
@@ -665,8 +667,19 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict
             gy[t][v] = y;
           }
         }
-        // lane col < 4 writes v = col for every tile
-        if (col < 4) {
+        if (I->col_rhs > 0) {
+          // lane col < n_rhs holds rhs col for every point of its rows
+          if (col < I->n_rhs) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+              for (int v = 0; v < 4; ++v) {
+                const int i = i_s + NP * (16 * (t0 + t) + 4 * kk + v) + ph;
+                if (i < I->Nx) g_out[(size_t)col * I->N_pts + (size_t)z * I->Nx + i] = make_float2(gx[t][v], gy[t][v]);
+              }
+          }
+        } else if (col < 4) {
+          // lane col < 4 writes v = col for every tile
 #pragma unroll
           for (int t = 0; t < 4; ++t) {
             float x = gx[t][0], y = gy[t][0];

@@ -489,14 +489,15 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d(const Inst* __restrict__ ins
       }
     }
     if (!has_next || next_r != r) {
-      // sum over the 16 column lanes and write gamma_hat[:, r]
+      // sum over the column lanes of one rhs (all 16, or those with the same
+      // column mod col_rhs) and write gamma_hat
+      const int o0 = I->col_rhs > 0 ? I->col_rhs : 1;
 #pragma unroll
       for (int mw = 0; mw < MW; ++mw) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           float x = gx[mw][v], y = gy[mw][v];
-#pragma unroll
-          for (int o = 1; o < 16; o <<= 1) {
+          for (int o = o0; o < 16; o <<= 1) {
             x += __shfl_xor(x, o, 64);
             y += __shfl_xor(y, o, 64);
           }
@@ -504,7 +505,18 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d(const Inst* __restrict__ ins
           gy[mw][v] = y;
         }
       }
-      if (col < 4) {
+      if (I->col_rhs > 0) {
+        // lane col < n_rhs holds rhs col for every point of its rows
+        if (col < I->n_rhs) {
+#pragma unroll
+          for (int mw = 0; mw < MW; ++mw)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const int i = ib + 16 * mw + 4 * kk + v;
+              if (i < I->Nx) g_out[(size_t)col * I->N_pts + (size_t)z * I->Nx + i] = make_float2(gx[mw][v], gy[mw][v]);
+            }
+        }
+      } else if (col < 4) {
 #pragma unroll
         for (int mw = 0; mw < MW; ++mw) {
           const int i = ib + 16 * mw + 4 * kk + col;
