@@ -1449,6 +1449,43 @@ int born_adjoint_dev(born_plan_t plan, const void* u_c64, int32_t N_rhs, void* g
   });
 }
 
+static void energy_rx(Plan& P, int m, double* E_dev, hipStream_t s) {
+  if (m < 0 || m >= (int)P.meas_pair_base.size())
+    throw Error(BORN_ERR_INVALID_ARGUMENT, "operator_born.indices_measurement_sel must be a single index!");
+  ensure_shift_on_device(P);
+  launch_energy_rx(P, m, E_dev, s);
+  HIP_CHECK(hipGetLastError());
+}
+
+int born_energy_rx(born_plan_t plan, int32_t measurement, double* energy) {
+  return guarded([&] {
+    if (!plan) throw Error(BORN_ERR_NO_OPERATOR_BORN, "operator_born must be a single scattering.operator_born!");
+    if (!energy) throw Error(BORN_ERR_INVALID_ARGUMENT, "energy output is NULL");
+    Plan& P = *reinterpret_cast<Plan*>(plan);
+    HIP_CHECK(hipSetDevice(P.device));
+    double* d = dalloc<double>((size_t)P.N_pts);
+    try {
+      energy_rx(P, measurement, d, P.stream);
+      HIP_CHECK(hipMemcpyAsync(energy, d, (size_t)P.N_pts * sizeof(double), hipMemcpyDeviceToHost, P.stream));
+      HIP_CHECK(hipStreamSynchronize(P.stream));
+    } catch (...) {
+      dfree(d);
+      throw;
+    }
+    dfree(d);
+  });
+}
+
+int born_energy_rx_dev(born_plan_t plan, int32_t measurement, void* energy_f64, void* stream) {
+  return guarded([&] {
+    if (!plan) throw Error(BORN_ERR_NO_OPERATOR_BORN, "operator_born must be a single scattering.operator_born!");
+    if (!energy_f64) throw Error(BORN_ERR_INVALID_ARGUMENT, "energy output is NULL");
+    Plan& P = *reinterpret_cast<Plan*>(plan);
+    HIP_CHECK(hipSetDevice(P.device));
+    energy_rx(P, measurement, (double*)energy_f64, (hipStream_t)stream);
+  });
+}
+
 static void table_shape(const Plan& P, int which, int index, int64_t& rows, int64_t& cols) {
   if (which == 0) {
     rows = P.N_fu;

@@ -274,6 +274,50 @@ void launch_adj_unmix(const Plan& P, const float2* u, int N_rhs, bool chunked, h
                      P.d_pr_chunk, P.d_pr_pos, P.Ne_pad, P.Nex, P.nd == 3 ? P.Ney : 1);
 }
 
+// ------------------------------------------------------ received energy
+// E[p] = sum over the mixes k of measurement m and their frequencies f of
+// |p_inc[f, p]|^2 * |sum_a pref[k][f, a] * h_ref[f, shift(p, e_a)]|^2, the
+// squared norm of column p of the measurement's rows of A
+// (energy_rx_canonical, operator_born.m:718-859, identity TGC).  One thread
+// per point, FP64 sums.
+__global__ __launch_bounds__(256) void k_energy_rx(
+    const float2* __restrict__ h, const float2* __restrict__ Pt, const int* __restrict__ shift,
+    const int* __restrict__ pair_fg, const int* __restrict__ mk_meta, const int64_t* __restrict__ mk_off,
+    const int* __restrict__ fmeas, const int* __restrict__ act, const double2* __restrict__ pref, int mk0, int mk1,
+    int pair_base, int N_pts, int H_plane, double* __restrict__ E) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= N_pts) return;
+  double acc = 0.0;
+  for (int mk = mk0; mk < mk1; ++mk) {
+    const int Nf = mk_meta[mk * 4 + 1], Na = mk_meta[mk * 4 + 2];
+    const int64_t f_off = mk_off[mk * 4 + 1], a_off = mk_off[mk * 4 + 2], pref_off = mk_off[mk * 4 + 3];
+    for (int f = 0; f < Nf; ++f) {
+      const int pair = pair_base + fmeas[f_off + f];
+      const float2* hr = h + (size_t)pair_fg[pair] * H_plane;
+      double qx = 0.0, qy = 0.0;
+      for (int a = 0; a < Na; ++a) {
+        const float2 hv = hr[shift[(size_t)act[a_off + a] * N_pts + p]];
+        const double2 c = pref[pref_off + (int64_t)a * Nf + f];
+        qx += c.x * hv.x - c.y * hv.y;
+        qy += c.x * hv.y + c.y * hv.x;
+      }
+      const float2 pv = Pt[(size_t)pair * N_pts + p];
+      acc += ((double)pv.x * pv.x + (double)pv.y * pv.y) * (qx * qx + qy * qy);
+    }
+  }
+  E[p] = acc;
+}
+
+void launch_energy_rx(const Plan& P, int m, double* E, hipStream_t s) {
+  int mk0 = 0;
+  while (mk0 < P.N_mk && P.mk_meas[mk0] != m) ++mk0;
+  int mk1 = mk0;
+  while (mk1 < P.N_mk && P.mk_meas[mk1] == m) ++mk1;
+  hipLaunchKernelGGL(k_energy_rx, dim3((P.N_pts + 255) / 256), dim3(256), 0, s, P.d_h, P.d_p, P.d_shift, P.d_pair_fg,
+                     P.d_mk_meta, P.d_mk_off, P.d_fmeas, P.d_act, P.d_pref, mk0, mk1, P.meas_pair_base[m], P.N_pts,
+                     P.H_plane, E);
+}
+
 // ------------------------------------------------------ general adjoint
 __global__ __launch_bounds__(256) void k_adj_general(
     const float2* __restrict__ h, const float2* __restrict__ P, const float2* __restrict__ W,

@@ -165,6 +165,18 @@ int born_adjoint(born_plan_t plan, const double* u, int32_t N_rhs, double* gamma
 int born_forward_dev(born_plan_t plan, const void* gamma_c64, int32_t N_rhs, void* u_c64, void* stream);
 int born_adjoint_dev(born_plan_t plan, const void* u_c64, int32_t N_rhs, void* gamma_hat_c64, void* stream);
 
+/* Received energies of one pulse-echo measurement, canonical basis, no TGC:
+ * energy_rx_canonical( op ) (operator_born.m:718-859) -- for every grid
+ * point p, sum over the measurement's mixes and their frequencies of
+ * |sum_a pref(f, a) h_ref(f, shift(p, e_a)) p_incident(f, p)|^2, i.e. the
+ * squared norm of column p of the measurement's rows of A.  `measurement` is
+ * the 0-based index among the plan's measurements; the reference requires a
+ * single selected measurement (energy_rx_canonical:NoSingleMeasurementIndex).
+ * Host variant: energy is N_points doubles.  Device variant: N_points FP64
+ * on the device, enqueued on `stream`. */
+int born_energy_rx(born_plan_t plan, int32_t measurement, double* energy);
+int born_energy_rx_dev(born_plan_t plan, int32_t measurement, void* energy_f64, void* stream);
+
 /* Copies of the device tables, complex64, for checking the device table
  * build: which = 0 h_ref (N_f_unique x N_h), 1 p_incident of measurement
  * `index` (N_f_unique(m) x N_points).  Column-major like the reference

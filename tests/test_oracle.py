@@ -347,3 +347,22 @@ def test_sampled_tables_are_rows_and_columns_of_direct_tables(mixed):
     us = rng.standard_normal((len(rows), 2)) + 1j * rng.standard_normal((len(rows), 2))
     u[rows] = us
     assert O.rel_l2(O.adjoint(T_s, us), O.adjoint(T_full, u)) < 1e-13
+
+
+def test_energy_rx_is_column_norms_of_the_measurement_rows():
+    """energy_rx(T, m) equals the squared column norms of the rows of A that
+    belong to measurement m, on mixed tables with repeated frequencies and
+    several active elements per mix (and the sum over measurements is the
+    diagonal of A^H A)."""
+    _, T = mixed_tiny_tables()
+    A = O.assemble_matrix(T)
+    obs = 0
+    total = np.zeros(T.N_points)
+    for m, ms in enumerate(T.meas):
+        n = sum(len(mx.idx_f_to_meas) for mx in ms.mixes)
+        ref = np.sum(np.abs(A[obs:obs + n]) ** 2, axis=0)
+        E = O.energy_rx(T, m)
+        assert O.rel_l2(E, ref) < 1e-13
+        total += E
+        obs += n
+    assert O.rel_l2(total, np.real(np.diag(A.conj().T @ A))) < 1e-13

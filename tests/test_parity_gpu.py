@@ -484,3 +484,49 @@ def test_config4_subfov_oracle(torch_cuda, config4_case, precision, tol):
     log("config4 %s (4 rhs, 32^3 sub-FOV, 3 frequencies): forward rel-L2 %.2e adjoint rel-L2 %.2e" % (precision, e_f, e_a))
     assert e_f < tol and e_a < tol
     op.close()
+
+
+# ---------------------------------------------------------- received energy
+def test_energy_rx_mixed_tables(torch_cuda):
+    """Received energies per measurement (energy_rx_canonical) on mixed tables
+    with repeated frequencies and several active elements per mix, host and
+    device entry points, against the FP64 oracle."""
+    import torch
+    _, T = mixed_tiny_tables()
+    op = OperatorBorn.from_tables(T)
+    for m in range(len(T.meas)):
+        ref = O.energy_rx(T, m)
+        e_h = O.rel_l2(op.energy_rx(m), ref)
+        e_d = O.rel_l2(op.energy_rx_dev(m).cpu().numpy(), ref)
+        log("energy_rx measurement %d: host rel-L2 %.2e device rel-L2 %.2e" % (m, e_h, e_d))
+        assert e_h < TOL_FP32 and e_d < TOL_FP32
+    with pytest.raises(BornError):
+        op.energy_rx(len(T.meas))
+
+
+def test_energy_rx_3d(torch_cuda):
+    arr = C.ArraySpec((8, 4), (3e-4, 3e-4), (2.7e-4, 2.7e-4), (2, 2))
+    fov = C.FovSpec((10, 7, 3), (1.5e-4, 1.5e-4, 1.5e-4), 2e-3)
+    waves = [C.qsw_wave(arr, np.array([x, 0.0, -1e-2]), 1500.0) for x in (0.0, 6e-4)]
+    spec = C.SequenceSpec("t3e", arr, C.MediumSpec(), fov, np.array([2.5e6, 4e6, 5.5e6]), waves,
+                          [C.identity_mixes(32) for _ in waves])
+    op = OperatorBorn.from_sequence(spec)
+    T = O.direct_tables(spec)
+    for m in range(2):
+        e = O.rel_l2(op.energy_rx(m), O.energy_rx(T, m))
+        log("energy_rx 3-D measurement %d: rel-L2 %.2e" % (m, e))
+        assert e < TOL_FP32
+
+
+def test_energy_rx_config1(torch_cuda, c1):
+    """Config 1, first and last measurement, at 24 sampled points against
+    tables restricted exactly to those points."""
+    spec, op = c1
+    rng = np.random.Generator(np.random.PCG64(83))
+    pts = np.sort(rng.choice(spec.fov.N_points, 24, replace=False))
+    R = O.restricted_tables(spec, pts)
+    for m in (0, spec.N_measurements - 1):
+        E = op.energy_rx(m)
+        e = O.rel_l2(E[pts], O.energy_rx(R, m))
+        log("energy_rx config1 measurement %d at 24 points: rel-L2 %.2e" % (m, e))
+        assert e < TOL_FP32
