@@ -26,7 +26,8 @@ FILTER_CODES = {"off": 0, "raised_cosine": 1, "logistic": 2, "boxcar": 3}
 EXPORTED = (
     "born_plan_create_tables", "born_plan_create_geometry", "born_plan_destroy", "born_plan_size",
     "born_plan_set_precision", "born_forward", "born_adjoint", "born_forward_dev", "born_adjoint_dev",
-    "born_plan_get_table", "born_plan_table_shape", "born_energy_rx", "born_energy_rx_dev", "born_plan_path", "born_plan_force_general", "born_plan_set_timing",
+    "born_plan_get_table", "born_plan_table_shape", "born_energy_rx", "born_energy_rx_dev",
+    "born_time_signals", "born_plan_path", "born_plan_force_general", "born_plan_set_timing",
     "born_plan_get_timing", "born_last_error", "born_status_id",
     "born_batch_create", "born_batch_destroy", "born_batch_forward_dev", "born_batch_adjoint_dev",
     "born_batch_set_graph", "born_batch_autotune", "born_plan_autotune", "born_batch_set_timing",
@@ -114,6 +115,8 @@ def lib() -> C.CDLL:
     L.born_plan_table_shape.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.born_energy_rx.argtypes = [vp, C.c_int32, pd]
     L.born_energy_rx_dev.argtypes = [vp, C.c_int32, vp, vp]
+    L.born_time_signals.argtypes = [vp, pd, C.c_double, C.c_double, vp, C.c_int32, vp, C.c_int64,
+                                    C.POINTER(C.c_int32), vp]
     L.born_plan_path.argtypes = [vp]
     L.born_plan_force_general.argtypes = [vp, C.c_int32]
     L.born_plan_set_timing.argtypes = [vp, C.c_int32]

@@ -1486,6 +1486,74 @@ int born_energy_rx_dev(born_plan_t plan, int32_t measurement, void* energy_f64, 
   });
 }
 
+int born_time_signals(born_plan_t plan, const double* frequencies, double delta_f, double delta_t,
+                      const void* u_c64, int32_t N_rhs, void* rf_f32, int64_t capacity, int32_t* N_t_out,
+                      void* stream) {
+  return guarded([&] {
+    if (!plan) throw Error(BORN_ERR_NO_OPERATOR_BORN, "operator_born must be a single scattering.operator_born!");
+    if (!frequencies || !u_c64 || !rf_f32 || N_rhs <= 0)
+      throw Error(BORN_ERR_INVALID_ARGUMENT, "frequencies, voltages and output are required");
+    Plan& P = *reinterpret_cast<Plan*>(plan);
+    HIP_CHECK(hipSetDevice(P.device));
+    // regular quantized frequency axis: f = q * delta_f with integer q
+    if (delta_f <= 0.0) {
+      std::vector<double> f(frequencies, frequencies + P.N_fu);
+      std::sort(f.begin(), f.end());
+      delta_f = 0.0;
+      for (size_t i = 0; i + 1 < f.size(); ++i) {
+        const double d = f[i + 1] - f[i];
+        if (d > 0.0 && (delta_f == 0.0 || d < delta_f)) delta_f = d;
+      }
+      if (delta_f == 0.0) delta_f = f[0];
+    }
+    std::vector<int> q_fg(P.N_fu);
+    int q_max = 0;
+    for (int i = 0; i < P.N_fu; ++i) {
+      const double q = frequencies[i] / delta_f;
+      if (q < -0.5 || std::fabs(q - std::round(q)) > 1e-6 * std::max(1.0, std::fabs(q)))
+        throw Error(BORN_ERR_UNSUPPORTED, "signal: the frequencies are not integer multiples of delta_f");
+      q_fg[i] = (int)std::lround(q);
+      q_max = std::max(q_max, q_fg[i]);
+    }
+    int N_t;
+    if (delta_t > 0.0) {
+      const double n = 1.0 / (delta_f * delta_t);
+      N_t = (int)std::lround(n);
+      if (std::fabs(n - N_t) > 1e-6 * std::max(1.0, n))
+        throw Error(BORN_ERR_INVALID_ARGUMENT, "Ts_rec must be integer multiples of delta!");
+    } else {
+      N_t = 2 * q_max;   // delta = 1 / (2 q_ub delta_f) (signal_matrix.m:333)
+    }
+    if (N_t <= 0 || q_max > N_t / 2)
+      throw Error(BORN_ERR_INVALID_ARGUMENT, "Number of spectral samples exceeds the maximum!");
+    const size_t need = (size_t)N_rhs * P.N_mk * N_t;
+    if (capacity < (int64_t)need) throw Error(BORN_ERR_INVALID_ARGUMENT, "output capacity too small");
+    std::vector<int> q_obs(P.N_obs);
+    for (int mk = 0; mk < P.N_mk; ++mk) {
+      std::vector<int> qs(P.mk_Nf[mk]);
+      for (int f = 0; f < P.mk_Nf[mk]; ++f) {
+        const int pr = P.meas_pair_base[P.mk_meas[mk]] + P.fmeas[P.mk_f_off[mk] + f];
+        qs[f] = q_obs[P.mk_obs_off[mk] + f] = q_fg[P.pair_fg[pr]];
+      }
+      std::sort(qs.begin(), qs.end());
+      if (std::adjacent_find(qs.begin(), qs.end()) != qs.end())
+        throw Error(BORN_ERR_UNSUPPORTED, "signal: a mix repeats a frequency, so it has no frequency axis");
+    }
+    const hipStream_t s = (hipStream_t)stream;
+    int* d_q = dupload(q_obs, s);
+    try {
+      launch_time_signals(P, (const float2*)u_c64, d_q, N_rhs, N_t, (float*)rf_f32, s);
+      HIP_CHECK(hipGetLastError());
+      HIP_CHECK(hipStreamSynchronize(s));
+    } catch (...) {
+      dfree(d_q);
+      throw;
+    }
+    dfree(d_q);
+    if (N_t_out) *N_t_out = N_t;
+  });
+}
+
 static void table_shape(const Plan& P, int which, int index, int64_t& rows, int64_t& cols) {
   if (which == 0) {
     rows = P.N_fu;

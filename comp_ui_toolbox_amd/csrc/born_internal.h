@@ -205,6 +205,8 @@ void launch_fwd_reduce(const Plan& P, int N_rhs, int nblk, hipStream_t s);
 void launch_fwd_reduce_expanded(const Plan& P, int nblk, hipStream_t s);
 void launch_expand_pg(const Plan& P, hipStream_t s);
 void launch_energy_rx(const Plan& P, int m, double* E, hipStream_t s);
+void launch_time_signals(const Plan& P, const float2* u, const int* q_obs, int N_rhs, int N_t, float* rf,
+                         hipStream_t s);
 void launch_fwd_mix(const Plan& P, int N_rhs, void* u, bool out_f64, hipStream_t s);
 
 void launch_adj_unmix(const Plan& P, const float2* u, int N_rhs, bool chunked, hipStream_t s);

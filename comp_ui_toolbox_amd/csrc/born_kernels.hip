@@ -318,6 +318,47 @@ void launch_energy_rx(const Plan& P, int m, double* E, hipStream_t s) {
                      P.H_plane, E);
 }
 
+// ------------------------------------------------------ time-domain signals
+// rf[r][mk][n] = sum over the frequencies f of mix mk of
+//   w_q * Re(u[r][obs(mk, f)] * exp(+j 2 pi q n / N_t)),
+// q = f / delta_f the integer frequency index, w_q = 1 for q = 0 and for the
+// Nyquist index N_t / 2, else 2: N_t * ifft(X, N_t, 'symmetric') of the
+// zero-padded spectrum (signal_matrix.m:379-391, lbs_q = 0).  The phase is
+// reduced exactly in integers, (q n) mod N_t, before the FP32 sincos.
+__global__ __launch_bounds__(256) void k_time_signals(const float2* __restrict__ u, const int* __restrict__ q_obs,
+                                                      const int64_t* __restrict__ mk_off,
+                                                      const int* __restrict__ mk_meta, int N_mk, int64_t N_obs,
+                                                      int N_rhs, int N_t, float* __restrict__ rf) {
+  const size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (idx >= (size_t)N_rhs * N_mk * N_t) return;
+  const int n = (int)(idx % N_t);
+  const size_t rs = idx / N_t;
+  const int mk = (int)(rs % N_mk);
+  const int r = (int)(rs / N_mk);
+  const int64_t o0 = mk_off[mk * 4 + 0];
+  const int Nf = mk_meta[mk * 4 + 1];
+  const float2* ur = u + (size_t)r * N_obs;
+  const float w0 = 6.283185307179586f / (float)N_t;
+  float acc = 0.f;
+  for (int f = 0; f < Nf; ++f) {
+    const int q = q_obs[o0 + f];
+    const float2 x = ur[o0 + f];
+    const int ph = (int)(((int64_t)q * n) % N_t);
+    float sn, cs;
+    __sincosf(w0 * (float)ph, &sn, &cs);
+    const float wq = (q == 0 || 2 * q == N_t) ? 1.f : 2.f;
+    acc += wq * (x.x * cs - x.y * sn);
+  }
+  rf[idx] = acc;
+}
+
+void launch_time_signals(const Plan& P, const float2* u, const int* q_obs, int N_rhs, int N_t, float* rf,
+                         hipStream_t s) {
+  const size_t total = (size_t)N_rhs * P.N_mk * N_t;
+  hipLaunchKernelGGL(k_time_signals, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, u, q_obs, P.d_mk_off,
+                     P.d_mk_meta, P.N_mk, P.N_obs, N_rhs, N_t, rf);
+}
+
 // ------------------------------------------------------ general adjoint
 __global__ __launch_bounds__(256) void k_adj_general(
     const float2* __restrict__ h, const float2* __restrict__ P, const float2* __restrict__ W,

@@ -78,6 +78,12 @@ class _Meas:
         self.mixes = mixes
 
 
+def _min_gap(f: np.ndarray) -> float:
+    g = np.diff(np.sort(f))
+    g = g[g > 0]
+    return float(g.min()) if len(g) else float(f.min())
+
+
 class OperatorBorn:
     """Born scattering operator A (N_obs x N_pts) on one MI355X."""
 
@@ -108,7 +114,8 @@ class OperatorBorn:
         plan = C.c_void_p()
         check(L.born_plan_create_tables(C.byref(d), C.byref(plan)))
         return cls._wrap(plan, device, {"rows_h": np.asarray(T.h_ref).shape[0],
-                                        "rows_p": [len(ms.idx_f_to_global) for ms in T.meas]})
+                                        "rows_p": [len(ms.idx_f_to_global) for ms in T.meas],
+                                        "n_mix_signals": sum(len(ms.mixes) for ms in T.meas)})
 
     @classmethod
     def from_sequence(cls, spec, device: int = 0, precision: str = "fp32") -> "OperatorBorn":
@@ -154,7 +161,8 @@ class OperatorBorn:
         check(L.born_plan_create_geometry(C.byref(d), C.byref(plan)))
         return cls._wrap(plan, device, {"spec": spec.name, "rows_h": len(spec.frequencies),
                                         "rows_p": [len(spec.measurement_frequencies(m))
-                                                   for m in range(spec.N_measurements)]})
+                                                   for m in range(spec.N_measurements)],
+                                        "n_mix_signals": sum(len(mm) for mm in spec.mixes)})
 
     @classmethod
     def _wrap(cls, plan, device, meta=None):
@@ -235,6 +243,30 @@ class OperatorBorn:
         s = torch.cuda.current_stream(out.device).cuda_stream if stream is None else stream
         check(_lib.lib().born_energy_rx_dev(self._plan, int(measurement), C.c_void_p(out.data_ptr()), C.c_void_p(s)))
         return out
+
+    def time_signals(self, u, frequencies, delta_f: float = 0.0, delta_t: float = 0.0, stream=None):
+        """Time-domain mixed voltage signals (format_voltages + signal,
+        operator.m:900-1007, signal_matrix.m:293-397): u (N_rhs, N_obs)
+        complex64 device tensor -> float32 tensor (N_rhs, N_mix_signals, N_t)
+        and N_t.  `frequencies`: the plan's global unique frequencies (Hz)."""
+        import torch
+        uu = u.reshape(1, -1) if u.dim() == 1 else u
+        self._check_dev(uu, self.N_observations)
+        f = np.ascontiguousarray(np.asarray(frequencies, dtype=np.float64))
+        n_t = C.c_int32()
+        # size query: N_t is at most 2 * max(q) for the default, or 1 / (delta_f delta_t)
+        df = delta_f if delta_f > 0 else _min_gap(f)
+        nt_max = int(round(1.0 / (df * delta_t))) if delta_t > 0 else 2 * int(round(f.max() / df))
+        n_sig = self._n_mix_signals()
+        out = torch.empty((uu.shape[0], n_sig, max(nt_max, 1)), dtype=torch.float32, device=uu.device)
+        s = torch.cuda.current_stream(uu.device).cuda_stream if stream is None else stream
+        check(_lib.lib().born_time_signals(self._plan, dptr(f), float(delta_f), float(delta_t),
+                                           C.c_void_p(uu.data_ptr()), uu.shape[0], C.c_void_p(out.data_ptr()),
+                                           out.numel(), C.byref(n_t), C.c_void_p(s)))
+        return out[:, :, : n_t.value], n_t.value
+
+    def _n_mix_signals(self) -> int:
+        return int(self.meta["n_mix_signals"])
 
     # ---------------------------------------------------------- host FP64 path
     def forward(self, gamma) -> np.ndarray:

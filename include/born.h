@@ -177,6 +177,23 @@ int born_adjoint_dev(born_plan_t plan, const void* u_c64, int32_t N_rhs, void* g
 int born_energy_rx(born_plan_t plan, int32_t measurement, double* energy);
 int born_energy_rx_dev(born_plan_t plan, int32_t measurement, void* energy_f64, void* stream);
 
+/* Time-domain mixed voltage signals: for every right-hand side and mix
+ * signal (measurement-major, the order of the observations) the real signal
+ * N_t * ifft(X, N_t, 'symmetric') of its zero-padded Fourier coefficients
+ * X[q] = u(obs(mix, f)), q = f / delta_f -- format_voltages then
+ * signal_matrix.signal (operator.m:900-1007; signal_matrix.m:293-397) with
+ * time-axis lower bound 0.  `frequencies` are the plan's N_f_unique global
+ * unique frequencies in Hz; delta_f <= 0 takes the smallest gap between
+ * them; delta_t <= 0 takes 1 / (2 q_max delta_f) (signal_matrix.m:333), so
+ * N_t = round(1 / (delta_f delta_t)).  u: complex64 device N_rhs x N_obs;
+ * rf: float32 device N_rhs x N_mix_signals x N_t.  Synchronous on `stream`.
+ * A mix must not repeat a frequency (its axis is increasing in the
+ * reference), else BORN_ERR_UNSUPPORTED.  No counterpart in the MEX (a
+ * MATLAB-side format). */
+int born_time_signals(born_plan_t plan, const double* frequencies, double delta_f, double delta_t,
+                      const void* u_c64, int32_t N_rhs, void* rf_f32, int64_t capacity, int32_t* N_t,
+                      void* stream);
+
 /* Copies of the device tables, complex64, for checking the device table
  * build: which = 0 h_ref (N_f_unique x N_h), 1 p_incident of measurement
  * `index` (N_f_unique(m) x N_points).  Column-major like the reference

@@ -385,6 +385,32 @@ def energy_rx(T: BornTables, m: int = 0) -> np.ndarray:
     return E
 
 
+def time_signals(T: BornTables, frequencies: np.ndarray, u: np.ndarray, delta_f: float,
+                 N_t: int) -> np.ndarray:
+    """Time-domain mixed voltage signals (format_voltages, operator.m:900-1007,
+    then signal_matrix.signal, signal_matrix.m:293-397, lbs_q = 0): for each
+    mix of each measurement, its coefficients u(obs) placed at the integer
+    frequency indices q = f / delta_f of a length floor(N_t/2)+1 spectrum,
+    then N_t * irfft(., N_t) (numpy's real inverse FFT reads the spectrum as
+    conjugate symmetric, as MATLAB's ifft(., 'symmetric') does).
+    u: (N_obs,) or (N_obs, N_rhs).  Returns (N_rhs, N_mix_signals, N_t)."""
+    uu = np.asarray(u)
+    if uu.ndim == 1:
+        uu = uu[:, None]
+    q_fg = np.rint(np.asarray(frequencies) / delta_f).astype(np.int64)
+    sig = []
+    obs = 0
+    for ms in T.meas:
+        fg_meas = ms.idx_f_to_global - 1
+        for mx in ms.mixes:
+            fm = mx.idx_f_to_meas - 1
+            X = np.zeros((N_t // 2 + 1, uu.shape[1]), dtype=np.complex128)
+            X[q_fg[fg_meas[fm]]] = uu[obs:obs + len(fm)]
+            sig.append(N_t * np.fft.irfft(X, N_t, axis=0))
+            obs += len(fm)
+    return np.stack(sig, axis=0).transpose(2, 0, 1)
+
+
 def adjoint(T: BornTables, u: np.ndarray, points: Optional[np.ndarray] = None) -> np.ndarray:
     """gamma_hat = A^H u_M over all grid points (or the 0-based subset
     `points`), the loop nest of adjoint_quick_cpu (operator_born.m:511-593):

@@ -174,16 +174,18 @@ def test_incident_field_equals_direct_sum():
 
 
 # ------------------------------------------------------------- operator pair
-def mixed_tiny_tables(seed=3):
+def mixed_tiny_tables(seed=3, freqs=None, distinct=False):
+    """Two measurements with frequency subsets and mixes of one to three
+    active elements; one mix repeats a frequency unless `distinct`."""
     rng = np.random.Generator(np.random.PCG64(seed))
-    freqs = np.array([2.0e6, 2.6e6, 3.1e6, 4.5e6, 5.0e6])
+    freqs = np.array([2.0e6, 2.6e6, 3.1e6, 4.5e6, 5.0e6]) if freqs is None else freqs
     spec = tiny_spec(freqs=freqs)
     spec.meas_f_indices = [np.array([0, 1, 3, 4]), np.array([4, 2, 1])]
     spec.mixes = [
         [C.MixSpec(np.array([0, 2]), np.array([1.0, -1.0]), np.array([3, 0, 2])),
          C.MixSpec(np.array([3, 1, 0]), rng.standard_normal(3) + 1j * rng.standard_normal(3))],
         [C.MixSpec(np.array([1]), np.array([2.0])),
-         C.MixSpec(np.array([2, 3]), np.array([1.0, 0.5]), np.array([2, 1, 1]))],
+         C.MixSpec(np.array([2, 3]), np.array([1.0, 0.5]), np.array([2, 1, 0]) if distinct else np.array([2, 1, 1]))],
     ]
     return spec, O.build_tables(spec)
 
@@ -366,3 +368,38 @@ def test_energy_rx_is_column_norms_of_the_measurement_rows():
         total += E
         obs += n
     assert O.rel_l2(total, np.real(np.diag(A.conj().T @ A))) < 1e-13
+
+
+def test_time_signals_single_coefficients():
+    """A single Fourier coefficient X at index q gives 2|X| cos(2 pi q n / N_t
+    + arg X) (0 < q < N_t/2), Re(X) at q = 0 and Re(X) (-1)^n at the Nyquist
+    index; signals are linear in u and ordered measurement -> mix."""
+    freqs = np.array([0.0, 0.5e6, 2.0e6, 2.5e6, 5.0e6])
+    spec, T = mixed_tiny_tables(freqs=freqs, distinct=True)
+    N_t = 20
+    n = np.arange(N_t)
+    rng = np.random.Generator(np.random.PCG64(7))
+    for obs in range(T.N_observations()):
+        u = np.zeros(T.N_observations(), dtype=np.complex128)
+        X = rng.standard_normal() + 1j * rng.standard_normal()
+        u[obs] = X
+        s = O.time_signals(T, freqs, u, 0.5e6, N_t)[0]
+        # which signal and frequency does obs belong to
+        k = 0
+        o = 0
+        for ms in T.meas:
+            for mx in ms.mixes:
+                if o <= obs < o + len(mx.idx_f_to_meas):
+                    q = int(round(freqs[ms.idx_f_to_global[mx.idx_f_to_meas[obs - o] - 1] - 1] / 0.5e6))
+                    sig = k
+                o += len(mx.idx_f_to_meas)
+                k += 1
+        if q == 0:
+            ref = np.full(N_t, X.real)
+        elif 2 * q == N_t:
+            ref = X.real * (-1.0) ** n
+        else:
+            ref = 2 * abs(X) * np.cos(2 * np.pi * q * n / N_t + np.angle(X))
+        expect = np.zeros_like(s)
+        expect[sig] = ref
+        assert np.max(np.abs(s - expect)) < 1e-12 * max(1.0, abs(X))

@@ -530,3 +530,36 @@ def test_energy_rx_config1(torch_cuda, c1):
         e = O.rel_l2(E[pts], O.energy_rx(R, m))
         log("energy_rx config1 measurement %d at 24 points: rel-L2 %.2e" % (m, e))
         assert e < TOL_FP32
+
+
+# ---------------------------------------------------------- time-domain signals
+@pytest.mark.parametrize("delta_t", [0.0, 1.0 / (0.5e6 * 32)])
+def test_time_signals(torch_cuda, delta_t):
+    """Mixed voltage signals in the time domain (format_voltages + signal)
+    for two right-hand sides, against the oracle's real inverse FFT."""
+    import torch
+    freqs = np.array([2.0e6, 2.5e6, 3.0e6, 4.5e6, 5.0e6])
+    _, T = mixed_tiny_tables(freqs=freqs, distinct=True)
+    op = OperatorBorn.from_tables(T)
+    u = rnd((T.N_observations(), 2), 95)
+    ut = torch.from_numpy(u.T.astype(np.complex64)).cuda().contiguous()
+    rf, n_t = op.time_signals(ut, freqs, delta_t=delta_t)
+    assert n_t == (20 if delta_t == 0.0 else 32)
+    ref = O.time_signals(T, freqs, u, 0.5e6, n_t)
+    e = O.rel_l2(rf.cpu().numpy(), ref)
+    log("time signals (N_t=%d): rel-L2 %.2e" % (n_t, e))
+    assert e < TOL_FP32
+
+
+def test_time_signals_rejects_unquantized_axis_and_repeated_frequencies(torch_cuda):
+    import torch
+    _, T = mixed_tiny_tables(distinct=True)   # 2.0, 2.6, 3.1, 4.5, 5.0 MHz: no common delta_f at the smallest gap
+    op = OperatorBorn.from_tables(T)
+    ut = torch.zeros((1, T.N_observations()), dtype=torch.complex64, device="cuda")
+    with pytest.raises(BornError):
+        op.time_signals(ut, np.array([2.0e6, 2.6e6, 3.1e6, 4.5e6, 5.0e6]))
+    freqs = np.array([2.0e6, 2.5e6, 3.0e6, 4.5e6, 5.0e6])
+    _, T = mixed_tiny_tables(freqs=freqs)   # one mix repeats a frequency: no frequency axis
+    op = OperatorBorn.from_tables(T)
+    with pytest.raises(BornError):
+        op.time_signals(ut, freqs)
