@@ -264,6 +264,8 @@ def main():
     ap.add_argument("--no-compare", action="store_true",
                     help="do not time the plain FP32 form beside the selected one")
     ap.add_argument("--config", default="config1")
+    ap.add_argument("--autotune", action="store_true",
+                    help="run the library's autotuner for each form before timing it")
     ap.add_argument("--rhs", type=int, default=0,
                     help="gamma right-hand sides per application (default 4 for config4, else 1)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -309,7 +311,13 @@ def main():
     if not args.no_compare and args.precision != "fp32":
         forms.append("fp32")
     res, outs = {}, {}
+    tuned = {}
     for form in forms:
+        if args.autotune:
+            op.set_precision(form)
+            tuned[form] = op.autotune()
+            print(f"[bench] {form}: autotuned contraction fwd {tuned[form][0]:.3f} ms adj {tuned[form][1]:.3f} ms",
+                  file=sys.stderr, flush=True)
         res[form] = time_form(op, form, x, u, g, args.steps, args.warmup)
         # one pair from the same input, kept for the cross-form comparison
         uf = torch.empty_like(u)

@@ -563,3 +563,27 @@ def test_time_signals_rejects_unquantized_axis_and_repeated_frequencies(torch_cu
     op = OperatorBorn.from_tables(T)
     with pytest.raises(BornError):
         op.time_signals(ut, freqs)
+
+
+# ---------------------------------------------------- several right-hand sides
+@pytest.mark.parametrize("precision,tol", [("fp32", TOL_FP32), ("split_bf16", TOL_SPLIT)])
+@pytest.mark.parametrize("n_rhs", [1, 2, 3, 5, 16, 17])
+def test_many_rhs(torch_cuda, precision, tol, n_rhs):
+    """Right-hand-side counts that pack differently into the 16 MFMA columns
+    (2 waves per frequency: 1 chunk per rhs, or rhs packed per frequency with
+    k = 2, 4, 8, 16 lanes per pair, or 17 > 16 one chunk per rhs again),
+    through the host and the device entry points, against the oracle."""
+    import torch
+    spec = tiny_spec(N_e=40, Nx=160, Nz=3, npp=4)
+    T = O.build_tables(spec)
+    op = OperatorBorn.from_tables(T, precision=precision)
+    g = rnd((T.N_points, n_rhs), 101)
+    u = rnd((T.N_observations(), n_rhs), 102)
+    e_f = O.rel_l2(op.forward(g), O.forward(T, g))
+    e_a = O.rel_l2(op.adjoint(u), O.adjoint(T, u))
+    gd = torch.from_numpy(g.T.astype(np.complex64)).cuda().contiguous()
+    ud = torch.from_numpy(u.T.astype(np.complex64)).cuda().contiguous()
+    e_fd = O.rel_l2(op.forward_dev(gd).cpu().numpy().T, O.forward(T, g.astype(np.complex64)))
+    e_ad = O.rel_l2(op.adjoint_dev(ud).cpu().numpy().T, O.adjoint(T, u.astype(np.complex64)))
+    log("%d rhs %s: forward %.2e adjoint %.2e (device %.2e %.2e)" % (n_rhs, precision, e_f, e_a, e_fd, e_ad))
+    assert max(e_f, e_a, e_fd, e_ad) < tol
