@@ -100,6 +100,9 @@ Plan::~Plan() {
   free_workspace(*this);
   dfree(ls.d_inst);
   dfree(ls_fwd.d_inst);
+  dfree(d_fscale);
+  dfree(d_f16max);
+  dfree(d_f16part);
   dfree(d_exty);
   dfree(d_hrow);
   dfree(d_h);
@@ -419,6 +422,7 @@ static Inst make_inst(const Plan& P) {
   I.Ney = P.nd == 3 ? P.Ney : 1;
   I.n_rows = P.n_rows;
   I.hrow = (decltype(I.hrow))P.d_hrow;
+  I.fscale = (decltype(I.fscale))P.d_fscale;
   I.N_ext = P.N_ext;
   I.Hp = P.H_plane;
   I.HR = P.HR;
@@ -691,6 +695,9 @@ static void finish_plan(Plan& P, bool structure_known = false) {
     P.n_cu = ncu;
   if (!structure_known) detect_structure(P);
   if (!P.structured) ensure_shift_on_device(P);
+  P.d_fscale = dupload(std::vector<float>(5, 1.f), P.stream);
+  P.d_f16max = dalloc<float>(2);
+  P.d_f16part = dalloc<float>(256);
   set_max_lds(160 * 1024);
   set_max_lds_split(160 * 1024);
   HIP_CHECK(hipStreamSynchronize(P.stream));
@@ -960,7 +967,28 @@ struct TimedLaunch {
   }
 };
 
+// FP16 split form: maxima of the tables once per plan, then the operand
+// scales of one apply (forward: from the masked gamma; adjoint: from W)
+static void fp16_scales(Plan& P, bool forward, int N_rhs, hipStream_t s) {
+  if (P.precision != BORN_PRECISION_SPLIT_FP16) return;
+  if (!P.f16_maxima) {
+    launch_fp16_scales(P, (const float*)P.d_h, (size_t)2 * P.N_fu * P.H_plane, -1, s);
+    launch_fp16_scales(P, (const float*)P.d_p, (size_t)2 * (P.N_pairs + 1) * P.N_pts, -2, s);
+    P.f16_maxima = true;
+  }
+  if (forward) {
+    launch_fp16_scales(P, (const float*)P.d_gm, (size_t)2 * N_rhs * P.N_pts, 1, s);
+  } else {
+    const size_t n = P.chunks.size() * (size_t)(P.nd == 3 ? P.Ney : 1) * P.Ne_pad * NC;
+    launch_fp16_scales(P, (const float*)P.d_Wc, 2 * n, 2, s);
+  }
+}
+
 static void forward_device(Plan& P, const float2* gm, int N_rhs, void* u, bool out_f64, hipStream_t s) {
+  if (use_structured(P)) {
+    P.ls.f16 = P.ls_fwd.f16 = P.precision == BORN_PRECISION_SPLIT_FP16;
+    fp16_scales(P, true, N_rhs, s);
+  }
   if (use_structured(P) && P.fwd_expand) {
     {
       TimedLaunch t(P, s, 0);
@@ -993,6 +1021,10 @@ static void forward_device(Plan& P, const float2* gm, int N_rhs, void* u, bool o
 
 static void adjoint_device(Plan& P, const float2* u, int N_rhs, float2* g, hipStream_t s) {
   launch_adj_unmix(P, u, N_rhs, use_structured(P), s);
+  if (use_structured(P)) {
+    P.ls.f16 = P.precision == BORN_PRECISION_SPLIT_FP16;
+    fp16_scales(P, false, N_rhs, s);
+  }
   TimedLaunch t(P, s, 1);
   if (use_structured(P)) {
     OutPtrs outs;
@@ -1009,8 +1041,6 @@ static void adjoint_device(Plan& P, const float2* u, int N_rhs, float2* g, hipSt
 }
 
 static void check_precision(const Plan& P) {
-  if (P.precision == BORN_PRECISION_SPLIT_FP16)
-    throw Error(BORN_ERR_UNSUPPORTED, "FP16 split precision is not available in this build");
   if (P.precision != BORN_PRECISION_FP32 && !(fwd_split_supported(P) && !P.force_general))
     throw Error(BORN_ERR_UNSUPPORTED,
                 "split precision needs the structured path with 2 or 4 grid points per pitch and 17 to 128 elements per x row");
@@ -1163,7 +1193,9 @@ static void batch_forward_enqueue(Batch& B, const void* const* in, void* const* 
   for (size_t i = 0; i < B.plans.size(); ++i) {
     Plan& P = *B.plans[i];
     launch_mask_gamma((const float2*)in[i], P.N_pts, 1, P.d_gm, s);
+    fp16_scales(P, true, 1, s);
   }
+  B.ls.f16 = B.precision == BORN_PRECISION_SPLIT_FP16;
   hipEvent_t a = nullptr, b = nullptr;
   if (B.timing) {
     HIP_CHECK(hipEventCreate(&a));
@@ -1191,8 +1223,10 @@ static void batch_adjoint_enqueue(Batch& B, const void* const* in, void* const* 
   for (size_t i = 0; i < B.plans.size(); ++i) {
     Plan& P = *B.plans[i];
     launch_adj_unmix(P, (const float2*)in[i], 1, true, s);
+    fp16_scales(P, false, 1, s);
     outs.p[i] = (float2*)out[i];
   }
+  B.ls.f16 = B.precision == BORN_PRECISION_SPLIT_FP16;
   hipEvent_t a = nullptr, b = nullptr;
   if (B.timing) {
     HIP_CHECK(hipEventCreate(&a));
@@ -1301,6 +1335,7 @@ static TuneResult autotune(std::vector<Plan*>& plans, LaunchSet& L, int precisio
   const int items_c[] = {1024, 2048, 4096, 8192};
   const int grid_c[] = {1, 2};
   TuneResult best{plans[0]->items_target, L.grid_mult, L.adj_mw, 1e30f, 1e30f};
+  L.f16 = precision == BORN_PRECISION_SPLIT_FP16;
   OutPtrs outs;
   std::vector<float2*> scratch;
   try {

@@ -51,6 +51,10 @@ struct Inst {
   // c mod k (< n_rhs), and gamma_hat of all right-hand sides is written once
   // after the last chunk
   int col_rhs, n_rhs;
+  // FP16 split operand scales (powers of 2): [0] h_ref, [1] p_inc * gamma,
+  // [2] forward unscale 1 / ([0] [1]), [3] adjoint weights, [4] adjoint
+  // unscale 1 / ([0] [3])
+  BORN_GLOBAL const float* fscale;
 };
 
 constexpr int MAX_BATCH = 64;
@@ -73,6 +77,7 @@ struct LaunchSet {
   int n_items = 0, n_items_s = 0;
   int n_wg = 0, n_wg_s = 0;
   int grid_mult = 2;              // persistent forward workgroups per CU
+  bool f16 = false;               // split kernels: FP16 (true) or BF16 hi/lo operands
 };
 
 // Device tables and index maps.  All device arrays are owned by the plan.
@@ -163,6 +168,12 @@ struct Plan {
   float2* d_gm = nullptr;         // masked gamma [N_rhs][N_pts]
   int items_target = 4096;        // forward work items per launch (tunable)
   LaunchSet ls;                   // structured launch of this plan alone
+  // FP16 split form: operand scales [5], table maxima [2] (h_ref, p_inc),
+  // block partial maxima [256]
+  float* d_fscale = nullptr;
+  float* d_f16max = nullptr;
+  float* d_f16part = nullptr;
+  bool f16_maxima = false;        // table maxima computed
   // Forward with several right-hand sides and fewer than NC pairs per
   // frequency: the products p_inc * gamma of every (pair, rhs) column are
   // formed first (d_pexp, a zero row after them) and the forward runs on
@@ -205,6 +216,7 @@ void launch_fwd_reduce(const Plan& P, int N_rhs, int nblk, hipStream_t s);
 void launch_fwd_reduce_expanded(const Plan& P, int nblk, hipStream_t s);
 void launch_expand_pg(const Plan& P, hipStream_t s);
 void launch_energy_rx(const Plan& P, int m, double* E, hipStream_t s);
+void launch_fp16_scales(const Plan& P, const float* x, size_t n_floats, int which, hipStream_t s);
 void launch_time_signals(const Plan& P, const float2* u, const int* q_obs, int N_rhs, int N_t, float* rf,
                          hipStream_t s);
 void launch_fwd_mix(const Plan& P, int N_rhs, void* u, bool out_f64, hipStream_t s);

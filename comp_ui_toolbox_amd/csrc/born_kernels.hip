@@ -359,6 +359,70 @@ void launch_time_signals(const Plan& P, const float2* u, const int* q_obs, int N
                      P.d_mk_meta, P.N_mk, P.N_obs, N_rhs, N_t, rf);
 }
 
+// ------------------------------------------------------ FP16 split scales
+// FP16 has a 5-bit exponent, so the split form scales every operand by a
+// power of 2 (exact) that brings its largest entry just below 2^14:
+// s = 2^(14 - e) with max < 2^e.  The contraction results are scaled back by
+// the inverse product.  Largest |re| or |im| of n floats, per block:
+__global__ __launch_bounds__(256) void k_absmax_part(const float* __restrict__ x, size_t n, float* __restrict__ part) {
+  __shared__ float red[256];
+  float m = 0.f;
+  for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) m = fmaxf(m, fabsf(x[i]));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__device__ __forceinline__ int max_exponent(float m) {   // m < 2^e (e = 0 for m = 0)
+  int e = 0;
+  if (m > 0.f) frexpf(m, &e);
+  return e;
+}
+
+// which = 0: maxima of h_ref and p_inc -> maxima[0..1]; which = 1: forward
+// scales from the maxima and max |gamma|; which = 2: adjoint scales from
+// max |h_ref| and max |W|.  One block.
+__global__ __launch_bounds__(256) void k_fp16_scales(const float* __restrict__ part, int n_part, int which,
+                                                     float* __restrict__ maxima, float* __restrict__ fscale) {
+  __shared__ float red[256];
+  float m = 0.f;
+  for (int i = threadIdx.x; i < n_part; i += 256) m = fmaxf(m, part[i]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  const float x = red[0];
+  if (which < 0) {
+    maxima[-which - 1] = x;
+    return;
+  }
+  const int eh = max_exponent(maxima[0]), ex = max_exponent(x);
+  fscale[0] = ldexpf(1.f, 14 - eh);
+  if (which == 1) {
+    const int ep = max_exponent(maxima[1]);
+    fscale[1] = ldexpf(1.f, 14 - ep - ex);
+    fscale[2] = ldexpf(1.f, eh + ep + ex - 28);
+  } else {
+    fscale[3] = ldexpf(1.f, 14 - ex);
+    fscale[4] = ldexpf(1.f, eh + ex - 28);
+  }
+}
+
+// which: -1 / -2 store max |x| as the h_ref / p_inc maximum; 1 forward
+// scales (x = gamma); 2 adjoint scales (x = W)
+void launch_fp16_scales(const Plan& P, const float* x, size_t n_floats, int which, hipStream_t s) {
+  const int nb = (int)std::min<size_t>(256, std::max<size_t>(1, (n_floats + 255) / 256));
+  hipLaunchKernelGGL(k_absmax_part, dim3(nb), dim3(256), 0, s, x, n_floats, P.d_f16part);
+  hipLaunchKernelGGL(k_fp16_scales, dim3(1), dim3(256), 0, s, P.d_f16part, nb, which, P.d_f16max, P.d_fscale);
+}
+
 // ------------------------------------------------------ general adjoint
 __global__ __launch_bounds__(256) void k_adj_general(
     const float2* __restrict__ h, const float2* __restrict__ P, const float2* __restrict__ W,

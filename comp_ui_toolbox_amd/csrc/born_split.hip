@@ -16,24 +16,19 @@ namespace born {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
-typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 
 namespace {
 
 constexpr int SEG = 256;          // points per step (forward) / per workgroup (adjoint)
 
-__device__ __forceinline__ unsigned short bf_bits(float x) {
-  return __builtin_bit_cast(unsigned short, (__bf16)x);  // round to nearest even
-}
-__device__ __forceinline__ float bf_val(unsigned short b) { return __uint_as_float(((unsigned int)b) << 16); }
-
-__device__ __forceinline__ void split2(float x, unsigned short& hi, unsigned short& lo) {
-  hi = bf_bits(x);
-  lo = bf_bits(x - bf_val(hi));
-}
-
+// 16x16x32 MFMA on BF16 or FP16 hi/lo operands, FP32 accumulation
+template <bool F16>
 __device__ __forceinline__ v4f mfma32(uint4 a, uint4 b, v4f c) {
-  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a), __builtin_bit_cast(v8bf, b), c, 0, 0, 0);
+  if constexpr (F16)
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(v8h, a), __builtin_bit_cast(v8h, b), c, 0, 0, 0);
+  else
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a), __builtin_bit_cast(v8bf, b), c, 0, 0, 0);
 }
 __device__ __forceinline__ uint4 neg8(uint4 v) {
   return make_uint4(v.x ^ 0x80008000u, v.y ^ 0x80008000u, v.z ^ 0x80008000u, v.w ^ 0x80008000u);
@@ -57,11 +52,28 @@ __device__ __forceinline__ unsigned int bf_pack(float x0, float x1, float& r0, f
   r1 = __uint_as_float(p & 0xffff0000u);
   return p;
 }
-// hi/lo split of a pair: hi = bf16(x), lo = bf16(x - hi), each packed (x0 low)
+// Round two floats to FP16 in one conversion; returns the packed pair and
+// the rounded values as floats.
+__device__ __forceinline__ unsigned int h_pack(float x0, float x1, float& r0, float& r1) {
+  typedef float v2f __attribute__((ext_vector_type(2)));
+  typedef _Float16 v2h __attribute__((ext_vector_type(2)));
+  const v2h h = __builtin_convertvector((v2f{x0, x1}), v2h);
+  r0 = (float)h[0];
+  r1 = (float)h[1];
+  return __builtin_bit_cast(unsigned int, h);
+}
+// hi/lo split of a pair: hi = t(x), lo = t(x - hi) for t = BF16 or FP16,
+// each packed (x0 low)
+template <bool F16>
 __device__ __forceinline__ void split_pair(float x0, float x1, unsigned int& hi, unsigned int& lo) {
   float h0, h1, d0, d1;
-  hi = bf_pack(x0, x1, h0, h1);
-  lo = bf_pack(x0 - h0, x1 - h1, d0, d1);
+  if constexpr (F16) {
+    hi = h_pack(x0, x1, h0, h1);
+    lo = h_pack(x0 - h0, x1 - h1, d0, d1);
+  } else {
+    hi = bf_pack(x0, x1, h0, h1);
+    lo = bf_pack(x0 - h0, x1 - h1, d0, d1);
+  }
 }
 __device__ __forceinline__ const void* byte_offset(const void* base, unsigned int off) {
   return static_cast<const char*>(base) + off;
@@ -94,7 +106,7 @@ __host__ __device__ constexpr int fwd_split_lap(int LA, int NP) {
   return ((LA + 12 + 63 - (NP == 4 ? 32 : 16)) / 64) * 64 + (NP == 4 ? 32 : 16);
 }
 
-template <int MT, bool EVEN, int NP>
+template <int MT, bool EVEN, int NP, bool F16>
 __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict__ inst, int n_inst, int n_items,
                                                           int Ne_pad, const Chunk* __restrict__ chunks,
                                                           const int* __restrict__ chunk_pairs) {
@@ -223,15 +235,18 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict
 
   while (true) {
     __syncthreads();
-    // window -> hi/lo planes, two copies
+    // FP16 operands are scaled by powers of 2 into range (see k_fp16_scales)
+    const float s_h = F16 ? I->fscale[0] : 1.f, s_b = F16 ? I->fscale[1] : 1.f;
+    // window -> hi/lo planes, NCOPY copies
 #pragma unroll
     for (int j = 0; j < APT; ++j) {
       const int u = tid + 256 * j;
       if (u < LA) {
         float2 hv = hreg[j];
         if (!hfull && !((hval >> j) & 1u)) hv = make_float2(0.f, 0.f);
+        if (F16) hv = make_float2(hv.x * s_h, hv.y * s_h);
         unsigned int hi, lo;   // (re, im) packed
-        split_pair(hv.x, hv.y, hi, lo);
+        split_pair<F16>(hv.x, hv.y, hi, lo);
         const unsigned short v[4] = {(unsigned short)hi, (unsigned short)lo, (unsigned short)(hi >> 16),
                                      (unsigned short)(lo >> 16)};
 #pragma unroll
@@ -252,11 +267,17 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         const float2 p0 = preg[q][0], p1 = preg[q][1];
-        const float re0 = p0.x * g0.x - p0.y * g0.y, im0 = p0.x * g0.y + p0.y * g0.x;
-        const float re1 = p1.x * g1.x - p1.y * g1.y, im1 = p1.x * g1.y + p1.y * g1.x;
+        float re0 = p0.x * g0.x - p0.y * g0.y, im0 = p0.x * g0.y + p0.y * g0.x;
+        float re1 = p1.x * g1.x - p1.y * g1.y, im1 = p1.x * g1.y + p1.y * g1.x;
+        if (F16) {
+          re0 *= s_b;
+          im0 *= s_b;
+          re1 *= s_b;
+          im1 *= s_b;
+        }
         unsigned int rh, rl, ih, il;
-        split_pair(re0, re1, rh, rl);
-        split_pair(im0, im1, ih, il);
+        split_pair<F16>(re0, re1, rh, rl);
+        split_pair<F16>(im0, im1, ih, il);
         const int o = ((bc0 + q) * SEGP + bi) >> 1;
         Bt32[0 * (NC * SEGP / 2) + o] = rh;
         Bt32[1 * (NC * SEGP / 2) + o] = rl;
@@ -270,6 +291,7 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict
     const int item_done_id = item;
     float2* const vpart_done = Iv.vpart;
     const int ib_done = Iv.item_begin_s;
+    const float unscale_done = F16 ? Iv.fscale[2] : 1.f;
     const bool item_done = z + 1 >= z_end;
     bool has_next = true;
     if (!item_done) {
@@ -314,18 +336,18 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict
       for (int m = 0; m < MTW; ++m) {
         const int kb = d + DSTEP * m;
         if (kb >= 0 && kb < KBW) {
-          CR[m] = mfma32(Acur[0], B4[kb][0], CR[m]);
-          CR[m] = mfma32(Acur[0], B4[kb][1], CR[m]);
-          CR[m] = mfma32(Acur[1], B4[kb][0], CR[m]);
-          CR[m] = mfma32(Acur[2], NB[kb][0], CR[m]);
-          CR[m] = mfma32(Acur[2], NB[kb][1], CR[m]);
-          CR[m] = mfma32(Acur[3], NB[kb][0], CR[m]);
-          CI[m] = mfma32(Acur[0], B4[kb][2], CI[m]);
-          CI[m] = mfma32(Acur[0], B4[kb][3], CI[m]);
-          CI[m] = mfma32(Acur[1], B4[kb][2], CI[m]);
-          CI[m] = mfma32(Acur[2], B4[kb][0], CI[m]);
-          CI[m] = mfma32(Acur[2], B4[kb][1], CI[m]);
-          CI[m] = mfma32(Acur[3], B4[kb][0], CI[m]);
+          CR[m] = mfma32<F16>(Acur[0], B4[kb][0], CR[m]);
+          CR[m] = mfma32<F16>(Acur[0], B4[kb][1], CR[m]);
+          CR[m] = mfma32<F16>(Acur[1], B4[kb][0], CR[m]);
+          CR[m] = mfma32<F16>(Acur[2], NB[kb][0], CR[m]);
+          CR[m] = mfma32<F16>(Acur[2], NB[kb][1], CR[m]);
+          CR[m] = mfma32<F16>(Acur[3], NB[kb][0], CR[m]);
+          CI[m] = mfma32<F16>(Acur[0], B4[kb][2], CI[m]);
+          CI[m] = mfma32<F16>(Acur[0], B4[kb][3], CI[m]);
+          CI[m] = mfma32<F16>(Acur[1], B4[kb][2], CI[m]);
+          CI[m] = mfma32<F16>(Acur[2], B4[kb][0], CI[m]);
+          CI[m] = mfma32<F16>(Acur[2], B4[kb][1], CI[m]);
+          CI[m] = mfma32<F16>(Acur[3], B4[kb][0], CI[m]);
         }
       }
       if (d + 1 < KBW) {
@@ -357,7 +379,7 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict
         for (int p_g = 0; p_g < PW; ++p_g) sum += red[((p_g * EW + e_g) * MTW + mtw) * 512 + rest];
         const int ln = idx & 63, j = (idx >> 6) & 7;
         const int e = 16 * mt + 4 * (ln >> 4) + (j & 3);
-        out[((size_t)e * NC + (ln & 15)) * 2 + (j >> 2)] = sum;
+        out[((size_t)e * NC + (ln & 15)) * 2 + (j >> 2)] = F16 ? sum * unscale_done : sum;
       }
       ++it;
     }
@@ -424,7 +446,7 @@ struct AdjSplitLayout {
 };
 __device__ __forceinline__ int copy_offset(int d) { return d == 1 ? 16 : (d == 2 ? 28 : (d == 3 ? 40 : 0)); }
 
-template <int MT, int NP, bool Y3>
+template <int MT, int NP, bool Y3, bool F16>
 __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict__ inst, int n_inst, OutPtrs outs,
                                                           const Chunk* __restrict__ chunks,
                                                           const int* __restrict__ chunk_pairs) {
@@ -457,6 +479,8 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict
     // ---------------------------------------------------------------- fill waves
     const int t_shift = i_s - (NEP - I->N_e) * NP;  // window entry u holds Hext[u + t_shift]
     const bool hfull = t_shift >= 0 && t_shift + L <= I->N_ext;
+    // FP16 operands are scaled by powers of 2 into range (see k_fp16_scales)
+    const float s_h = F16 ? I->fscale[0] : 1.f, s_w = F16 ? I->fscale[3] : 1.f;
     int eidx[HPT];
 #pragma unroll
     for (int j = 0; j < HPT; ++j) {
@@ -502,8 +526,9 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict
           const int p = u % NP, m = (LQ - 1) - u / NP;                                           \
           float2 hv = H[j];                                                                      \
           if (!hfull && eidx[j] < 0) hv = make_float2(0.f, 0.f);                                 \
+          if (F16) hv = make_float2(hv.x * s_h, hv.y * s_h);                                     \
           unsigned int hi, lo;                                                                   \
-          split_pair(hv.x, -hv.y, hi, lo); /* conj */                                            \
+          split_pair<F16>(hv.x, -hv.y, hi, lo); /* conj */                                       \
           const unsigned short v[4] = {(unsigned short)hi, (unsigned short)lo,                   \
                                        (unsigned short)(hi >> 16), (unsigned short)(lo >> 16)};  \
           _Pragma("unroll") for (int cp = 0; cp < 4; ++cp) {                                     \
@@ -520,8 +545,9 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict
         const int idx = ft + 256 * j;                                                            \
         const int k = idx / NC, wc = idx - k * NC;                                               \
         unsigned int rh, rl, ih, il;                                                             \
-        split_pair(WV[j].x, WV[j + 1].x, rh, rl);                                                \
-        split_pair(WV[j].y, WV[j + 1].y, ih, il);                                                \
+        const float wsc_ = F16 ? s_w : 1.f;                                                      \
+        split_pair<F16>(WV[j].x * wsc_, WV[j + 1].x * wsc_, rh, rl);                             \
+        split_pair<F16>(WV[j].y * wsc_, WV[j + 1].y * wsc_, ih, il);                             \
         unsigned short* wd = &Wt[wc * WS + k];                                                   \
         wd[0 * NC * WS] = (unsigned short)rh;                                                    \
         wd[0 * NC * WS + 16] = (unsigned short)(rh >> 16);                                       \
@@ -613,18 +639,18 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict
         const int k2 = dl + t;
         if (k2 >= 0 && (k2 & 1) == 0 && k2 / 2 < KB) {
           const int kb = k2 / 2;
-          YR[t] = mfma32(Ac[0], W4[kb][0], YR[t]);
-          YR[t] = mfma32(Ac[0], W4[kb][1], YR[t]);
-          YR[t] = mfma32(Ac[1], W4[kb][0], YR[t]);
-          YR[t] = mfma32(nih, W4[kb][2], YR[t]);
-          YR[t] = mfma32(nih, W4[kb][3], YR[t]);
-          YR[t] = mfma32(nil, W4[kb][2], YR[t]);
-          YI[t] = mfma32(Ac[0], W4[kb][2], YI[t]);
-          YI[t] = mfma32(Ac[0], W4[kb][3], YI[t]);
-          YI[t] = mfma32(Ac[1], W4[kb][2], YI[t]);
-          YI[t] = mfma32(Ac[2], W4[kb][0], YI[t]);
-          YI[t] = mfma32(Ac[2], W4[kb][1], YI[t]);
-          YI[t] = mfma32(Ac[3], W4[kb][0], YI[t]);
+          YR[t] = mfma32<F16>(Ac[0], W4[kb][0], YR[t]);
+          YR[t] = mfma32<F16>(Ac[0], W4[kb][1], YR[t]);
+          YR[t] = mfma32<F16>(Ac[1], W4[kb][0], YR[t]);
+          YR[t] = mfma32<F16>(nih, W4[kb][2], YR[t]);
+          YR[t] = mfma32<F16>(nih, W4[kb][3], YR[t]);
+          YR[t] = mfma32<F16>(nil, W4[kb][2], YR[t]);
+          YI[t] = mfma32<F16>(Ac[0], W4[kb][2], YI[t]);
+          YI[t] = mfma32<F16>(Ac[0], W4[kb][3], YI[t]);
+          YI[t] = mfma32<F16>(Ac[1], W4[kb][2], YI[t]);
+          YI[t] = mfma32<F16>(Ac[2], W4[kb][0], YI[t]);
+          YI[t] = mfma32<F16>(Ac[2], W4[kb][1], YI[t]);
+          YI[t] = mfma32<F16>(Ac[3], W4[kb][0], YI[t]);
         }
       }
       if (dl < DMAX) {
@@ -634,6 +660,14 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict
     }
 #undef BORN_ADJS_LOAD_A
     if (ey == Ney - 1) {
+      if (F16) {
+        const float us = I->fscale[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          YR[t] *= us;
+          YI[t] *= us;
+        }
+      }
       // gamma_hat += conj(p) Y; lane holds rows 4*kk + v of tile t0 + t, column col
       const float2* Ps = reinterpret_cast<const float2*>(ldsu + Lay::PS_OFF + (c & 1) * Lay::PS_U16);
 #pragma unroll
@@ -724,8 +758,12 @@ void launch_fwd_split(const LaunchSet& L, int n_cu, hipStream_t s) {
   const int n_inst = (int)L.inst.size();
   const int Ne_pad = 16 * L.MT;
 #define BORN_FWDS_L(MTv, EV, NPv)                                                                      \
-  hipLaunchKernelGGL((k_fwd_s2d_split<MTv, EV, NPv>), dim3(grid), dim3(256), (fwd_split_lds_bytes<MTv, NPv>()), s, \
-                     L.d_inst, n_inst, L.n_items_s, Ne_pad, L.d_chunks, L.d_chunk_pairs)
+  if (L.f16)                                                                                         \
+    hipLaunchKernelGGL((k_fwd_s2d_split<MTv, EV, NPv, true>), dim3(grid), dim3(256), (fwd_split_lds_bytes<MTv, NPv>()), s, \
+                       L.d_inst, n_inst, L.n_items_s, Ne_pad, L.d_chunks, L.d_chunk_pairs);            \
+  else                                                                                               \
+    hipLaunchKernelGGL((k_fwd_s2d_split<MTv, EV, NPv, false>), dim3(grid), dim3(256), (fwd_split_lds_bytes<MTv, NPv>()), s, \
+                       L.d_inst, n_inst, L.n_items_s, Ne_pad, L.d_chunks, L.d_chunk_pairs)
 #define BORN_FWDS_NP(MTv, EV)                 \
   if (L.Np == 2) BORN_FWDS_L(MTv, EV, 2);     \
   else BORN_FWDS_L(MTv, EV, 4);
@@ -751,8 +789,12 @@ void launch_adj_split(const LaunchSet& L, const OutPtrs& outs, hipStream_t s) {
   bool y3 = false;
   for (const Inst& I : L.inst) y3 = y3 || I.Ney > 1;
 #define BORN_ADJS_L(MTv, NPv, Y3v)                                                                    \
-  hipLaunchKernelGGL((k_adj_s2d_split<MTv, NPv, Y3v>), grid, dim3(512), (adj_split_lds_bytes<MTv, NPv>()), s, \
-                     L.d_inst, n_inst, outs, L.d_chunks, L.d_chunk_pairs)
+  if (L.f16)                                                                                         \
+    hipLaunchKernelGGL((k_adj_s2d_split<MTv, NPv, Y3v, true>), grid, dim3(512), (adj_split_lds_bytes<MTv, NPv>()), s, \
+                       L.d_inst, n_inst, outs, L.d_chunks, L.d_chunk_pairs);                           \
+  else                                                                                               \
+    hipLaunchKernelGGL((k_adj_s2d_split<MTv, NPv, Y3v, false>), grid, dim3(512), (adj_split_lds_bytes<MTv, NPv>()), s, \
+                       L.d_inst, n_inst, outs, L.d_chunks, L.d_chunk_pairs)
 #define BORN_ADJS_NP(MTv)                                     \
   if (L.Np == 2) {                                            \
     if (y3) BORN_ADJS_L(MTv, 2, true); else BORN_ADJS_L(MTv, 2, false); \
@@ -771,10 +813,14 @@ void launch_adj_split(const LaunchSet& L, const OutPtrs& outs, hipStream_t s) {
 template <int MT, int NP>
 static void set_max_lds_split_mt(int bytes) {
 #define BORN_SET(K) hipFuncSetAttribute((const void*)(K), hipFuncAttributeMaxDynamicSharedMemorySize, bytes)
-  BORN_SET((k_fwd_s2d_split<MT, true, NP>));
-  BORN_SET((k_fwd_s2d_split<MT, false, NP>));
-  BORN_SET((k_adj_s2d_split<MT, NP, false>));
-  BORN_SET((k_adj_s2d_split<MT, NP, true>));
+  BORN_SET((k_fwd_s2d_split<MT, true, NP, false>));
+  BORN_SET((k_fwd_s2d_split<MT, false, NP, false>));
+  BORN_SET((k_adj_s2d_split<MT, NP, false, false>));
+  BORN_SET((k_adj_s2d_split<MT, NP, true, false>));
+  BORN_SET((k_fwd_s2d_split<MT, true, NP, true>));
+  BORN_SET((k_fwd_s2d_split<MT, false, NP, true>));
+  BORN_SET((k_adj_s2d_split<MT, NP, false, true>));
+  BORN_SET((k_adj_s2d_split<MT, NP, true, true>));
 #undef BORN_SET
 }
 

@@ -98,7 +98,8 @@ def test_3d_tables(torch_cuda, general):
     assert e_f < TOL_FP32 and e_a < TOL_FP32
 
 
-@pytest.mark.parametrize("precision,tol,nex", [("fp32", TOL_FP32, 8), ("split_bf16", 1e-4, 32)])
+@pytest.mark.parametrize("precision,tol,nex", [("fp32", TOL_FP32, 8), ("split_bf16", 1e-4, 32),
+                                               ("split_fp16", 1e-4, 32)])
 @pytest.mark.parametrize("shape", [(20, 12, 5), (10, 7, 3)])
 def test_3d_geometry_structured(torch_cuda, shape, precision, tol, nex):
     """3-D plans built from geometry on the structured path: an nex x 4
@@ -307,22 +308,24 @@ def test_combined_quick_gpu_contract(torch_cuda):
 TOL_SPLIT = 1e-4
 
 
+@pytest.mark.parametrize("form", ["split_bf16", "split_fp16"])
 @pytest.mark.parametrize("N_e,Nx,Nz,npp", [(20, 80, 4, 4), (40, 300, 3, 4), (128, 520, 2, 4),
                                            (20, 40, 4, 2), (64, 300, 3, 2), (128, 260, 2, 2)])
-def test_split_bf16_small_tables(torch_cuda, N_e, Nx, Nz, npp):
+def test_split_small_tables(torch_cuda, N_e, Nx, Nz, npp, form):
     spec = tiny_spec(N_e=N_e, Nx=Nx, Nz=Nz, npp=npp)
     T = O.build_tables(spec)
-    op = OperatorBorn.from_tables(T, precision="split_bf16")
+    op = OperatorBorn.from_tables(T, precision=form)
     g = rnd((T.N_points, 2), 71)
     u = rnd((T.N_observations(), 2), 72)
     e_f = O.rel_l2(op.forward(g), O.forward(T, g))
     e_a = O.rel_l2(op.adjoint(u), O.adjoint(T, u))
     assert op.path == "structured"
-    log("split bf16 N_e=%d Nx=%d npp=%d: forward rel-L2 %.2e adjoint rel-L2 %.2e" % (N_e, Nx, npp, e_f, e_a))
+    log("%s N_e=%d Nx=%d npp=%d: forward rel-L2 %.2e adjoint rel-L2 %.2e" % (form, N_e, Nx, npp, e_f, e_a))
     assert e_f < TOL_SPLIT and e_a < TOL_SPLIT
 
 
-def test_split_bf16_config1(torch_cuda, c1):
+@pytest.mark.parametrize("form", ["split_bf16", "split_fp16"])
+def test_split_config1(torch_cuda, c1, form):
     spec, op = c1
     rng = np.random.Generator(np.random.PCG64(81))
     pts = np.sort(rng.choice(spec.fov.N_points, 24, replace=False))
@@ -330,7 +333,7 @@ def test_split_bf16_config1(torch_cuda, c1):
     g = np.zeros(spec.fov.N_points)
     g[pts] = rng.standard_normal(len(pts))
     u = rnd(op.N_observations, 82)
-    op.set_precision("split_bf16")
+    op.set_precision(form)
     try:
         e_f = O.rel_l2(op.forward(g), O.forward(R, g[pts]))
         e_a = O.rel_l2(op.adjoint(u)[pts], O.adjoint(R, u))
@@ -342,7 +345,7 @@ def test_split_bf16_config1(torch_cuda, c1):
         e_d = ((us - u32).norm() / u32.norm()).item()
     finally:
         op.set_precision("fp32")
-    log("split bf16 config1: forward rel-L2 %.2e adjoint rel-L2 %.2e dense vs fp32 %.2e" % (e_f, e_a, e_d))
+    log("%s config1: forward rel-L2 %.2e adjoint rel-L2 %.2e dense vs fp32 %.2e" % (form, e_f, e_a, e_d))
     assert e_f < TOL_SPLIT and e_a < TOL_SPLIT and e_d < TOL_SPLIT
 
 
@@ -464,7 +467,7 @@ def config4_case():
     return dict(spec=spec, pts=pts, rows=rows, g_sub=g_sub, u_sub=u_sub, u_ref=u_ref, g_ref=g_ref)
 
 
-@pytest.mark.parametrize("precision,tol", [("fp32", TOL_FP32), ("split_bf16", TOL_SPLIT)])
+@pytest.mark.parametrize("precision,tol", [("fp32", TOL_FP32), ("split_bf16", TOL_SPLIT), ("split_fp16", TOL_SPLIT)])
 def test_config4_subfov_oracle(torch_cuda, config4_case, precision, tol):
     import torch
     d = config4_case
@@ -566,7 +569,7 @@ def test_time_signals_rejects_unquantized_axis_and_repeated_frequencies(torch_cu
 
 
 # ---------------------------------------------------- several right-hand sides
-@pytest.mark.parametrize("precision,tol", [("fp32", TOL_FP32), ("split_bf16", TOL_SPLIT)])
+@pytest.mark.parametrize("precision,tol", [("fp32", TOL_FP32), ("split_bf16", TOL_SPLIT), ("split_fp16", TOL_SPLIT)])
 @pytest.mark.parametrize("n_rhs", [1, 2, 3, 5, 16, 17])
 def test_many_rhs(torch_cuda, precision, tol, n_rhs):
     """Right-hand-side counts that pack differently into the 16 MFMA columns
