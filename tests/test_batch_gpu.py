@@ -138,10 +138,11 @@ def test_plan_autotune_keeps_results(torch_cuda, fp32_set):
     assert rel(op.forward_dev(g[0]), f0) < 1e-6 and rel(op.adjoint_dev(u[0]), a0) < 1e-6
 
 
-def test_batch_split_bf16(torch_cuda):
+@pytest.mark.parametrize("form", ["split_bf16", "split_fp16"])
+def test_batch_split(torch_cuda, form):
     torch = torch_cuda
     specs = uneven_specs(20)
-    ops = [OperatorBorn.from_sequence(s, precision="split_bf16") for s in specs]
+    ops = [OperatorBorn.from_sequence(s, precision=form) for s in specs]
     B = OperatorBatch(ops)
     g, u = inputs(torch, ops, 16)
     fb = B.forward_dev(g)
@@ -150,8 +151,22 @@ def test_batch_split_bf16(torch_cuda):
         T = O.direct_tables(spec)
         ef = O.rel_l2(fb[i].cpu().numpy(), O.forward(T, g[i].cpu().numpy().astype(np.complex128)))
         ea = O.rel_l2(ab[i].cpu().numpy(), O.adjoint(T, u[i].cpu().numpy().astype(np.complex128)))
-        log("split batch instance %d vs FP64 oracle: forward %.2e adjoint %.2e" % (i, ef, ea))
+        log("%s batch instance %d vs FP64 oracle: forward %.2e adjoint %.2e" % (form, i, ef, ea))
         assert ef < TOL_SPLIT and ea < TOL_SPLIT
+    # graph replays (the FP16 operand scales are computed inside the graph)
+    ref_f = [x.clone() for x in fb]
+    ref_a = [x.clone() for x in ab]
+    B.set_graph(True)
+    f2 = [torch.empty_like(x) for x in ref_f]
+    a2 = [torch.empty_like(x) for x in ref_a]
+    for rep in range(3):
+        B.forward_dev(g, outs=f2)
+        B.adjoint_dev(u, outs=a2)
+        torch.cuda.synchronize()
+        for i in range(len(ops)):
+            assert torch.equal(f2[i], ref_f[i]) and torch.equal(a2[i], ref_a[i]), (rep, i)
+        ops[rep % len(ops)].forward_dev(g[rep % len(ops)] * 3.0)   # eager work with other scales between replays
+        torch.cuda.synchronize()
 
 
 def test_batch_rejects_mixed_precision(torch_cuda, fp32_set):
