@@ -121,6 +121,11 @@ Plan::~Plan() {
   dfree(d_meas_pair_base);
   dfree(d_pe_off);
   dfree(d_pe_ent);
+  dfree(d_tgc_tiles);
+  dfree(d_tgc_half);
+  dfree(d_tgc_off);
+  dfree(d_tgc_k32);
+  dfree(d_tgc_k64);
   if (d_io_in) hipFree(d_io_in);
   if (d_io_out) hipFree(d_io_out);
   if (stream) hipStreamDestroy(stream);
@@ -1586,6 +1591,108 @@ int born_time_signals(born_plan_t plan, const double* frequencies, double delta_
     }
     dfree(d_q);
     if (N_t_out) *N_t_out = N_t;
+  });
+}
+
+static void tgc_clear(Plan& P) {
+  dfree(P.d_tgc_tiles);
+  dfree(P.d_tgc_half);
+  dfree(P.d_tgc_off);
+  dfree(P.d_tgc_k32);
+  dfree(P.d_tgc_k64);
+  P.tgc = false;
+  P.n_tgc_tiles = 0;
+}
+
+int born_tgc_set(born_plan_t plan, const int32_t* half_lengths, const double* kernels_c128) {
+  return guarded([&] {
+    if (!plan) throw Error(BORN_ERR_NO_OPERATOR_BORN, "operator_born must be a single scattering.operator_born!");
+    Plan& P = *reinterpret_cast<Plan*>(plan);
+    HIP_CHECK(hipSetDevice(P.device));
+    if (P.stream) HIP_CHECK(hipStreamSynchronize(P.stream));
+    HIP_CHECK(hipDeviceSynchronize());   // no launch may still read the old taps
+    tgc_clear(P);
+    if (!half_lengths) return;           // identity (regularization.tgc.off)
+    if (!kernels_c128) throw Error(BORN_ERR_INVALID_ARGUMENT, "TGC kernels are NULL");
+    std::vector<int> half(P.N_mk);
+    std::vector<int64_t> off(P.N_mk);
+    std::vector<double2> k64;
+    std::vector<int2> tiles;
+    int64_t src = 0;
+    for (int mk = 0; mk < P.N_mk; ++mk) {
+      const int M = half_lengths[mk];
+      const int N = P.mk_Nf[mk];
+      if (M < 0) throw Error(BORN_ERR_INVALID_ARGUMENT, "TGC kernels must have odd lengths 2 M + 1");
+      // a kernel reaching past the mix is cut to 2 N - 1 taps (convolution.m:76-90)
+      const int Mc = std::min(M, N - 1);
+      const int64_t cut = M - Mc;
+      half[mk] = Mc;
+      off[mk] = (int64_t)k64.size();
+      for (int64_t t = cut; t <= 2 * (int64_t)M - cut; ++t) {
+        const double* k = kernels_c128 + 2 * (src + t);
+        if (!std::isfinite(k[0]) || !std::isfinite(k[1]))
+          throw Error(BORN_ERR_INVALID_ARGUMENT, "TGC kernels must be finite");
+        k64.push_back(make_double2(k[0], k[1]));
+      }
+      src += 2 * (int64_t)M + 1;
+      for (int i0 = 0; i0 < N; i0 += 256) tiles.push_back(make_int2(mk, i0));
+    }
+    std::vector<float2> k32(k64.size());
+    for (size_t i = 0; i < k64.size(); ++i) k32[i] = make_float2((float)k64[i].x, (float)k64[i].y);
+    try {
+      P.d_tgc_half = dupload(half, P.stream);
+      P.d_tgc_off = dupload(off, P.stream);
+      P.d_tgc_tiles = dupload(tiles, P.stream);
+      P.d_tgc_k64 = dupload(k64, P.stream);
+      P.d_tgc_k32 = dupload(k32, P.stream);
+    } catch (...) {
+      tgc_clear(P);
+      throw;
+    }
+    P.n_tgc_tiles = (int)tiles.size();
+    P.tgc = true;
+  });
+}
+
+static void tgc_check(const Plan& P, int32_t N_rhs) {
+  if (!P.tgc) throw Error(BORN_ERR_INVALID_ARGUMENT, "no TGC kernels are set on this operator");
+  if (N_rhs <= 0 || N_rhs > 65535) throw Error(BORN_ERR_INVALID_ARGUMENT, "N_rhs must lie in [1, 65535]");
+}
+
+int born_tgc_apply(born_plan_t plan, int32_t mode, const double* in, int32_t N_rhs, double* out) {
+  return guarded([&] {
+    if (!plan) throw Error(BORN_ERR_NO_OPERATOR_BORN, "operator_born must be a single scattering.operator_born!");
+    if (!in || !out) throw Error(BORN_ERR_NO_VALID_DOUBLE_MATRIX, "input and output are required");
+    Plan& P = *reinterpret_cast<Plan*>(plan);
+    HIP_CHECK(hipSetDevice(P.device));
+    tgc_check(P, N_rhs);
+    const size_t n = (size_t)N_rhs * P.N_obs;
+    double2* d = dalloc<double2>(2 * n);
+    try {
+      HIP_CHECK(hipMemcpyAsync(d, in, n * sizeof(double2), hipMemcpyHostToDevice, P.stream));
+      launch_tgc(P, mode != 1, true, d, d + n, N_rhs, P.stream);
+      HIP_CHECK(hipGetLastError());
+      HIP_CHECK(hipMemcpyAsync(out, d + n, n * sizeof(double2), hipMemcpyDeviceToHost, P.stream));
+      HIP_CHECK(hipStreamSynchronize(P.stream));
+    } catch (...) {
+      dfree(d);
+      throw;
+    }
+    dfree(d);
+  });
+}
+
+int born_tgc_apply_dev(born_plan_t plan, int32_t mode, const void* in_c64, void* out_c64, int32_t N_rhs,
+                       void* stream) {
+  return guarded([&] {
+    if (!plan) throw Error(BORN_ERR_NO_OPERATOR_BORN, "operator_born must be a single scattering.operator_born!");
+    if (!in_c64 || !out_c64) throw Error(BORN_ERR_NO_VALID_DOUBLE_MATRIX, "input and output are required");
+    if (in_c64 == out_c64) throw Error(BORN_ERR_INVALID_ARGUMENT, "TGC input and output must not be the same buffer");
+    Plan& P = *reinterpret_cast<Plan*>(plan);
+    HIP_CHECK(hipSetDevice(P.device));
+    tgc_check(P, N_rhs);
+    launch_tgc(P, mode != 1, false, in_c64, out_c64, N_rhs, (hipStream_t)stream);
+    HIP_CHECK(hipGetLastError());
   });
 }
 

@@ -189,6 +189,16 @@ struct Plan {
   float2* d_fvpart = nullptr;     // [n_fchunks][fvblk][Ne_pad][NC]
   int fvblk = 1;
   LaunchSet ls_fwd;               // forward launch over the expanded columns
+  // time gain compensation: per mix the half length M (after the cut-off
+  // truncation) and the offset of its 2M + 1 taps; tiles (mix, first row) of
+  // 256 outputs
+  bool tgc = false;
+  int n_tgc_tiles = 0;
+  int2* d_tgc_tiles = nullptr;
+  int* d_tgc_half = nullptr;
+  int64_t* d_tgc_off = nullptr;
+  float2* d_tgc_k32 = nullptr;
+  double2* d_tgc_k64 = nullptr;
   size_t io_bytes = 0;
   void* d_io_in = nullptr;        // host-path staging
   void* d_io_out = nullptr;
@@ -220,6 +230,7 @@ void launch_fp16_scales(const Plan& P, const float* x, size_t n_floats, int whic
 void launch_time_signals(const Plan& P, const float2* u, const int* q_obs, int N_rhs, int N_t, float* rf,
                          hipStream_t s);
 void launch_fwd_mix(const Plan& P, int N_rhs, void* u, bool out_f64, hipStream_t s);
+void launch_tgc(const Plan& P, bool adjoint, bool f64, const void* in, void* out, int N_rhs, hipStream_t s);
 
 void launch_adj_unmix(const Plan& P, const float2* u, int N_rhs, bool chunked, hipStream_t s);
 void launch_adj_structured(const LaunchSet& L, const OutPtrs& outs, hipStream_t s);

@@ -359,6 +359,85 @@ void launch_time_signals(const Plan& P, const float2* u, const int* q_obs, int N
                      P.d_mk_meta, P.N_mk, P.N_obs, N_rhs, N_t, rf);
 }
 
+// ------------------------------------------------------ time gain compensation
+// One discrete convolution along the frequency coefficients of every mix,
+// cut off to the mix's own length N (linear_transforms.convolutions.fft,
+// fft.m:85-126 with cut_off; convolution.m:62-98).  Taps k[d], |d| <= M:
+//   forward  y[i] = sum_d k[d] x[i - d]         (0 <= i - d < N)
+//   adjoint  y[j] = sum_d conj(k[d]) x[j + d]   (0 <= j + d < N)
+// A block computes 256 outputs of one (mix, right-hand side); the taps pass
+// through LDS in chunks of 256 together with the 511 inputs a chunk reaches.
+__device__ __forceinline__ void cfma(float2& a, float2 k, float2 x) {
+  a.x = fmaf(k.x, x.x, fmaf(-k.y, x.y, a.x));
+  a.y = fmaf(k.x, x.y, fmaf(k.y, x.x, a.y));
+}
+__device__ __forceinline__ void cfma(double2& a, double2 k, double2 x) {
+  a.x = fma(k.x, x.x, fma(-k.y, x.y, a.x));
+  a.y = fma(k.x, x.y, fma(k.y, x.x, a.y));
+}
+
+template <typename T, bool ADJ>
+__global__ __launch_bounds__(256) void k_tgc(const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ taps,
+                                             const int2* __restrict__ tiles, const int* __restrict__ half,
+                                             const int64_t* __restrict__ tap_off, const int64_t* __restrict__ mk_off,
+                                             const int* __restrict__ mk_meta, int64_t N_obs) {
+  __shared__ T xs[511];
+  __shared__ T ks[256];
+  const int2 tl = tiles[blockIdx.x];
+  const int mk = tl.x, i0 = tl.y;
+  const int M = half[mk];
+  const int N = mk_meta[mk * 4 + 1];
+  const int64_t o0 = mk_off[mk * 4 + 0];
+  const T* kt = taps + tap_off[mk] + M;   // kt[d], -M <= d <= M
+  const T* xr = x + (size_t)blockIdx.y * N_obs + o0;
+  const int li = threadIdx.x;
+  T acc;
+  acc.x = 0;
+  acc.y = 0;
+  for (int d0 = -M; d0 <= M; d0 += 256) {
+    T k;
+    k.x = 0;
+    k.y = 0;
+    if (d0 + li <= M) {
+      k = kt[d0 + li];
+      if (ADJ) k.y = -k.y;
+    }
+    ks[li] = k;
+    for (int t = li; t < 511; t += 256) {
+      const int j = ADJ ? i0 + d0 + t : i0 - d0 - 255 + t;
+      T v;
+      v.x = 0;
+      v.y = 0;
+      if (j >= 0 && j < N) v = xr[j];
+      xs[t] = v;
+    }
+    __syncthreads();
+    const int nd = min(256, M - d0 + 1);
+    for (int dd = 0; dd < nd; ++dd) cfma(acc, ks[dd], ADJ ? xs[li + dd] : xs[li - dd + 255]);
+    __syncthreads();
+  }
+  if (i0 + li < N) y[(size_t)blockIdx.y * N_obs + o0 + i0 + li] = acc;
+}
+
+void launch_tgc(const Plan& P, bool adjoint, bool f64, const void* in, void* out, int N_rhs, hipStream_t s) {
+  const dim3 grid((unsigned)P.n_tgc_tiles, (unsigned)N_rhs);
+  if (f64) {
+    if (adjoint)
+      hipLaunchKernelGGL((k_tgc<double2, true>), grid, dim3(256), 0, s, (const double2*)in, (double2*)out,
+                         P.d_tgc_k64, P.d_tgc_tiles, P.d_tgc_half, P.d_tgc_off, P.d_mk_off, P.d_mk_meta, P.N_obs);
+    else
+      hipLaunchKernelGGL((k_tgc<double2, false>), grid, dim3(256), 0, s, (const double2*)in, (double2*)out,
+                         P.d_tgc_k64, P.d_tgc_tiles, P.d_tgc_half, P.d_tgc_off, P.d_mk_off, P.d_mk_meta, P.N_obs);
+  } else {
+    if (adjoint)
+      hipLaunchKernelGGL((k_tgc<float2, true>), grid, dim3(256), 0, s, (const float2*)in, (float2*)out,
+                         P.d_tgc_k32, P.d_tgc_tiles, P.d_tgc_half, P.d_tgc_off, P.d_mk_off, P.d_mk_meta, P.N_obs);
+    else
+      hipLaunchKernelGGL((k_tgc<float2, false>), grid, dim3(256), 0, s, (const float2*)in, (float2*)out,
+                         P.d_tgc_k32, P.d_tgc_tiles, P.d_tgc_half, P.d_tgc_off, P.d_mk_off, P.d_mk_meta, P.N_obs);
+  }
+}
+
 // ------------------------------------------------------ FP16 split scales
 // FP16 has a 5-bit exponent, so the split form scales every operand by a
 // power of 2 (exact) that brings its largest entry just below 2^14:

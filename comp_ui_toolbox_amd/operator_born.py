@@ -115,7 +115,8 @@ class OperatorBorn:
         check(L.born_plan_create_tables(C.byref(d), C.byref(plan)))
         return cls._wrap(plan, device, {"rows_h": np.asarray(T.h_ref).shape[0],
                                         "rows_p": [len(ms.idx_f_to_global) for ms in T.meas],
-                                        "n_mix_signals": sum(len(ms.mixes) for ms in T.meas)})
+                                        "n_mix_signals": sum(len(ms.mixes) for ms in T.meas),
+                                        "mix_n_obs": [len(mx.idx_f_to_meas) for ms in T.meas for mx in ms.mixes]})
 
     @classmethod
     def from_sequence(cls, spec, device: int = 0, precision: str = "fp32") -> "OperatorBorn":
@@ -162,7 +163,8 @@ class OperatorBorn:
         return cls._wrap(plan, device, {"spec": spec.name, "rows_h": len(spec.frequencies),
                                         "rows_p": [len(spec.measurement_frequencies(m))
                                                    for m in range(spec.N_measurements)],
-                                        "n_mix_signals": sum(len(mm) for mm in spec.mixes)})
+                                        "n_mix_signals": sum(len(mm) for mm in spec.mixes),
+                                        "mix_n_obs": [len(mx.idx_f_to_meas) for ms in meas for mx in ms.mixes]})
 
     @classmethod
     def _wrap(cls, plan, device, meta=None):
@@ -268,6 +270,63 @@ class OperatorBorn:
     def _n_mix_signals(self) -> int:
         return int(self.meta["n_mix_signals"])
 
+    # ---------------------------------------------------------- TGC
+    def set_tgc(self, kernels=None):
+        """Time gain compensation applied by op_A(..., tgc=True): one odd-length
+        complex kernel per mix signal (measurement-major), e.g. from
+        tgc.exponential_kernels; None clears it (regularization.tgc.off)."""
+        L = _lib.lib()
+        if kernels is None:
+            check(L.born_tgc_set(self._plan, None, None))
+            return
+        if len(kernels) != self._n_mix_signals():
+            raise BornError(10, "born:InvalidArgument",
+                            f"expected {self._n_mix_signals()} TGC kernels, one per mix signal")
+        half = (C.c_int32 * len(kernels))()
+        for i, k in enumerate(kernels):
+            n = np.asarray(k).size
+            if n % 2 != 1:
+                raise BornError(10, "born:InvalidArgument", "TGC kernels must have odd lengths 2 M + 1")
+            half[i] = (n - 1) // 2
+        taps = np.ascontiguousarray(np.concatenate([np.asarray(k, dtype=np.complex128).reshape(-1)
+                                                    for k in kernels]))
+        check(L.born_tgc_set(self._plan, half, dptr(taps.view(np.float64))))
+
+    def _tgc_host(self, u, mode: int) -> np.ndarray:
+        uu = np.asarray(u)
+        squeeze = uu.ndim == 1
+        u2 = uu[:, None] if squeeze else uu
+        if u2.shape[0] != self.N_observations:
+            raise BornError(7, "combined_quick_gpu:NoValidDoubleMatrix",
+                            f"input must be a double matrix with {self.N_observations} rows!")
+        uin = _c128_colmajor(u2)
+        out = np.empty((u2.shape[1], self.N_observations), dtype=np.complex128)
+        check(_lib.lib().born_tgc_apply(self._plan, int(mode), dptr(uin), u2.shape[1], dptr(out.view(np.float64))))
+        res = out.T
+        return res[:, 0].copy() if squeeze else np.ascontiguousarray(res)
+
+    def tgc_forward(self, u) -> np.ndarray:
+        """TGC of u (N_obs,) or (N_obs, N_rhs), FP64 (forward_transform)."""
+        return self._tgc_host(u, 1)
+
+    def tgc_adjoint(self, u) -> np.ndarray:
+        """Adjoint TGC (adjoint_transform), FP64."""
+        return self._tgc_host(u, 2)
+
+    def tgc_dev(self, u, mode: int = 1, out=None, stream=None):
+        """TGC on the device: u (N_obs,) or (N_rhs, N_obs) complex64; mode 1
+        forward, 2 adjoint.  Returns a new tensor unless `out` is given."""
+        import torch
+        squeeze = u.dim() == 1
+        uu = u.reshape(1, -1) if squeeze else u
+        self._check_dev(uu, self.N_observations)
+        if out is None:
+            out = torch.empty_like(uu)
+        s = torch.cuda.current_stream(uu.device).cuda_stream if stream is None else stream
+        check(_lib.lib().born_tgc_apply_dev(self._plan, int(mode), C.c_void_p(uu.data_ptr()),
+                                            C.c_void_p(out.data_ptr()), uu.shape[0], C.c_void_p(s)))
+        return out.reshape(-1) if squeeze else out
+
     # ---------------------------------------------------------- host FP64 path
     def forward(self, gamma) -> np.ndarray:
         """u_M = A gamma with MEX semantics (occupancy rule, zero output when
@@ -353,15 +412,18 @@ class OperatorBorn:
                             f"expected a contiguous complex64 device tensor with last dimension {rows}")
 
     # ---------------------------------------------------------- solver contract
-    def op_A(self, x, mode: int):
+    def op_A(self, x, mode: int, tgc: bool = False):
         """operator.combined_scalar: mode 0 size, 1 forward, 2 adjoint
-        (operator.m:1019-1072)."""
+        (operator.m:1019-1072).  tgc=True applies the kernels of set_tgc
+        after the forward and their adjoint before the adjoint
+        (forward_scalar / adjoint_scalar, operator_born.m:222-236, :263-277)."""
         if mode == 0:
             return list(self.size)
         if mode == 1:
-            return self.forward(x)
+            u = self.forward(x)
+            return self.tgc_forward(u) if tgc else u
         if mode == 2:
-            return self.adjoint(x)
+            return self.adjoint(self.tgc_adjoint(x) if tgc else x)
         raise BornError(6, "combined_scalar:InvalidMode", "mode must be 0, 1 or 2")
 
 

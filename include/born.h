@@ -194,6 +194,30 @@ int born_time_signals(born_plan_t plan, const double* frequencies, double delta_
                       const void* u_c64, int32_t N_rhs, void* rf_f32, int64_t capacity, int32_t* N_t,
                       void* stream);
 
+/* Time gain compensation (TGC), the linear transform forward_scalar applies
+ * to u after the operator and adjoint_scalar applies before it
+ * (operator_born.m:222-236, :263-277).  regularization.tgc.exponential
+ * builds it as one discrete convolution per mix along the mix's Fourier
+ * coefficients, cut off to the mix's length (tgc/exponential.m:118-213;
+ * linear_transforms.convolutions.fft, fft.m:85-126; convolution.m:29-102):
+ *   forward  y[i] = sum_{|d| <= M} k[d] x[i - d],
+ *   adjoint  y[j] = sum_{|d| <= M} conj(k[d]) x[j + d],
+ * with x = 0 outside the mix's N_f rows.
+ * born_tgc_set: half_lengths holds M of every mix signal (measurement-major,
+ * the order of the observations); kernels_c128 the concatenated 2 M + 1 taps
+ * k[-M..M] of each, interleaved complex FP64.  A kernel with M + 1 > N_f is
+ * cut symmetrically to 2 N_f - 1 taps, as convolution.m:76-90 does.
+ * half_lengths = NULL clears it (regularization.tgc.off: identity).
+ * born_tgc_apply: host FP64, in and out column-major N_obs x N_rhs
+ * interleaved complex (in == out allowed), computed in FP64, synchronous.
+ * born_tgc_apply_dev: complex64 device N_rhs x N_obs, in and out must not
+ * overlap, enqueued on `stream`.  mode 1 = forward, any other value adjoint.
+ * No counterpart in the MEX (a MATLAB-side linear transform). */
+int born_tgc_set(born_plan_t plan, const int32_t* half_lengths, const double* kernels_c128);
+int born_tgc_apply(born_plan_t plan, int32_t mode, const double* in, int32_t N_rhs, double* out);
+int born_tgc_apply_dev(born_plan_t plan, int32_t mode, const void* in_c64, void* out_c64, int32_t N_rhs,
+                       void* stream);
+
 /* Copies of the device tables, complex64, for checking the device table
  * build: which = 0 h_ref (N_f_unique x N_h), 1 p_incident of measurement
  * `index` (N_f_unique(m) x N_points).  Column-major like the reference

@@ -411,6 +411,40 @@ def time_signals(T: BornTables, frequencies: np.ndarray, u: np.ndarray, delta_f:
     return np.stack(sig, axis=0).transpose(2, 0, 1)
 
 
+def tgc_apply(kernels: Sequence[np.ndarray], mix_n_obs: Sequence[int], u: np.ndarray, mode: int) -> np.ndarray:
+    """Time gain compensation as the reference computes it: per mix, the
+    DFT-based discrete convolution linear_transforms.convolutions.fft with
+    cut_off (fft.m:85-126; kernel truncation convolution.m:62-98).  Forward:
+    y = ifft(fft(k, N_dft) .* fft(x, N_dft)) cut to rows M+1..end-M; adjoint:
+    x padded by M zeros on both sides, y = ifft(conj(fft(k)) .* fft(x)) cut
+    to its first N rows.  N_dft = len(k) + N - 1.  mode 1 forward, else
+    adjoint.  u: (N_obs,) or (N_obs, N_rhs), mixes in observation order."""
+    uu = np.asarray(u, dtype=np.complex128)
+    squeeze = uu.ndim == 1
+    if squeeze:
+        uu = uu[:, None]
+    out = np.empty_like(uu)
+    obs = 0
+    for k, N in zip(kernels, mix_n_obs):
+        k = np.asarray(k, dtype=np.complex128).reshape(-1)
+        M = (len(k) - 1) // 2
+        if M + 1 > N:
+            cut = M - N + 1
+            k = k[cut:len(k) - cut]
+            M = N - 1
+        x = uu[obs:obs + N]
+        n_dft = len(k) + N - 1
+        kd = np.fft.fft(k, n_dft)
+        if mode == 1:
+            y = np.fft.ifft(kd[:, None] * np.fft.fft(x, n_dft, axis=0), axis=0)[M:n_dft - M]
+        else:
+            xp = np.concatenate([np.zeros((M, x.shape[1])), x, np.zeros((M, x.shape[1]))])
+            y = np.fft.ifft(np.conj(kd)[:, None] * np.fft.fft(xp, n_dft, axis=0), axis=0)[:N]
+        out[obs:obs + N] = y
+        obs += N
+    return out[:, 0] if squeeze else out
+
+
 def adjoint(T: BornTables, u: np.ndarray, points: Optional[np.ndarray] = None) -> np.ndarray:
     """gamma_hat = A^H u_M over all grid points (or the 0-based subset
     `points`), the loop nest of adjoint_quick_cpu (operator_born.m:511-593):

@@ -403,3 +403,39 @@ def test_time_signals_single_coefficients():
         expect = np.zeros_like(s)
         expect[sig] = ref
         assert np.max(np.abs(s - expect)) < 1e-12 * max(1.0, abs(X))
+
+
+# ------------------------------------------------------ time gain compensation
+@pytest.mark.parametrize("mode", [1, 2])
+def test_tgc_apply_is_cut_off_linear_convolution(mode):
+    """The DFT-based convolution (fft.m) equals the banded Toeplitz matrix
+    C[i, j] = k[i - j + M] (|i - j| <= M) or its conjugate transpose, built
+    entry by entry from the untruncated kernel, for kernels shorter than,
+    as long as and longer than the mix (truncated), and non-Hermitian taps
+    so that a missing conjugation shows."""
+    rng = np.random.Generator(np.random.PCG64(41))
+    sizes = [5, 1, 9, 3, 12]
+    halves = [2, 0, 12, 1, 11]
+    kernels = [rng.standard_normal(2 * M + 1) + 1j * rng.standard_normal(2 * M + 1) for M in halves]
+    u = rng.standard_normal((sum(sizes), 2)) + 1j * rng.standard_normal((sum(sizes), 2))
+    got = O.tgc_apply(kernels, sizes, u, mode)
+    obs = 0
+    for k, M, N in zip(kernels, halves, sizes):
+        Cm = np.zeros((N, N), dtype=np.complex128)
+        for i in range(N):
+            for j in range(N):
+                if abs(i - j) <= M:
+                    Cm[i, j] = k[i - j + M]
+        ref = (Cm if mode == 1 else Cm.conj().T) @ u[obs:obs + N]
+        assert np.max(np.abs(got[obs:obs + N] - ref)) < 1e-12 * max(1.0, np.max(np.abs(ref)))
+        obs += N
+    y = rng.standard_normal(sum(sizes)) + 1j * rng.standard_normal(sum(sizes))
+    lhs = np.vdot(y, O.tgc_apply(kernels, sizes, u[:, 0], 1))
+    rhs = np.vdot(O.tgc_apply(kernels, sizes, y, 2), u[:, 0])
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+
+
+def test_tgc_apply_identity_kernel():
+    u = np.arange(7) + 1j * np.arange(7)[::-1]
+    for mode in (1, 2):
+        assert np.allclose(O.tgc_apply([np.array([1.0 + 0j])], [7], u, mode), u, rtol=0, atol=1e-13)
