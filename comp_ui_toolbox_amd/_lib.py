@@ -32,6 +32,7 @@ EXPORTED = (
     "born_batch_create", "born_batch_destroy", "born_batch_forward_dev", "born_batch_adjoint_dev",
     "born_batch_set_graph", "born_batch_autotune", "born_plan_autotune", "born_batch_set_timing",
     "born_batch_get_timing", "born_tgc_set", "born_tgc_apply", "born_tgc_apply_dev",
+    "born_energy_rx_tgc", "born_energy_rx_tgc_dev",
 )
 
 
@@ -120,6 +121,8 @@ def lib() -> C.CDLL:
     L.born_tgc_set.argtypes = [vp, C.POINTER(C.c_int32), pd]
     L.born_tgc_apply.argtypes = [vp, C.c_int32, pd, C.c_int32, pd]
     L.born_tgc_apply_dev.argtypes = [vp, C.c_int32, vp, vp, C.c_int32, vp]
+    L.born_energy_rx_tgc.argtypes = [vp, C.c_int32, pd]
+    L.born_energy_rx_tgc_dev.argtypes = [vp, C.c_int32, vp, vp]
     L.born_plan_path.argtypes = [vp]
     L.born_plan_force_general.argtypes = [vp, C.c_int32]
     L.born_plan_set_timing.argtypes = [vp, C.c_int32]

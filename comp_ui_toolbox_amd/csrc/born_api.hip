@@ -1516,6 +1516,63 @@ int born_energy_rx(born_plan_t plan, int32_t measurement, double* energy) {
   });
 }
 
+// Energy under the plan's TGC; synchronous on s (a spill buffer for long
+// mixes lives for the launch only).
+static void energy_rx_tgc(Plan& P, int m, double* E_dev, hipStream_t s) {
+  if (m < 0 || m >= (int)P.meas_pair_base.size())
+    throw Error(BORN_ERR_INVALID_ARGUMENT, "operator_born.indices_measurement_sel must be a single index!");
+  if (!P.tgc) throw Error(BORN_ERR_INVALID_ARGUMENT, "no TGC kernels are set on this operator");
+  ensure_shift_on_device(P);
+  int rows = 1;
+  for (int mk = 0; mk < P.N_mk; ++mk)
+    if (P.mk_meas[mk] == m) rows = std::max(rows, P.mk_Nf[mk]);
+  if (rows <= ETGC_LDS_ROWS) {
+    launch_energy_rx_tgc(P, m, rows, nullptr, 0, E_dev, s);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(s));
+    return;
+  }
+  const int blocks = std::min((P.N_pts + 63) / 64, 1024);
+  float2* spill = dalloc<float2>((size_t)blocks * rows * 64);
+  try {
+    launch_energy_rx_tgc(P, m, rows, spill, blocks, E_dev, s);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(s));
+  } catch (...) {
+    dfree(spill);
+    throw;
+  }
+  dfree(spill);
+}
+
+int born_energy_rx_tgc(born_plan_t plan, int32_t measurement, double* energy) {
+  return guarded([&] {
+    if (!plan) throw Error(BORN_ERR_NO_OPERATOR_BORN, "operator_born must be a single scattering.operator_born!");
+    if (!energy) throw Error(BORN_ERR_INVALID_ARGUMENT, "energy output is NULL");
+    Plan& P = *reinterpret_cast<Plan*>(plan);
+    HIP_CHECK(hipSetDevice(P.device));
+    double* d = dalloc<double>((size_t)P.N_pts);
+    try {
+      energy_rx_tgc(P, measurement, d, P.stream);
+      HIP_CHECK(hipMemcpy(energy, d, (size_t)P.N_pts * sizeof(double), hipMemcpyDeviceToHost));
+    } catch (...) {
+      dfree(d);
+      throw;
+    }
+    dfree(d);
+  });
+}
+
+int born_energy_rx_tgc_dev(born_plan_t plan, int32_t measurement, void* energy_f64, void* stream) {
+  return guarded([&] {
+    if (!plan) throw Error(BORN_ERR_NO_OPERATOR_BORN, "operator_born must be a single scattering.operator_born!");
+    if (!energy_f64) throw Error(BORN_ERR_INVALID_ARGUMENT, "energy output is NULL");
+    Plan& P = *reinterpret_cast<Plan*>(plan);
+    HIP_CHECK(hipSetDevice(P.device));
+    energy_rx_tgc(P, measurement, (double*)energy_f64, (hipStream_t)stream);
+  });
+}
+
 int born_energy_rx_dev(born_plan_t plan, int32_t measurement, void* energy_f64, void* stream) {
   return guarded([&] {
     if (!plan) throw Error(BORN_ERR_NO_OPERATOR_BORN, "operator_born must be a single scattering.operator_born!");

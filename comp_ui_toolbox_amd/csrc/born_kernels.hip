@@ -438,6 +438,93 @@ void launch_tgc(const Plan& P, bool adjoint, bool f64, const void* in, void* out
   }
 }
 
+// ------------------------------------------------------ received energy, TGC
+// E[p] = sum over the mixes k of measurement m of sum_i |sum_j C_k[i, j] Phi_k[j, p]|^2
+// with Phi_k[j, p] = p_inc[f_j, p] sum_a pref[k][j, a] h_ref[f_j, shift(p, e_a)]
+// and C_k the mix's TGC convolution (C[i, j] = k[i - j], |i - j| <= M):
+// energy_rx_canonical with a diagonal LT_tgc (operator_born.m:830-848).
+// A block holds 64 points (the lanes) and loops over the mixes; its 4 waves
+// first fill Phi[j][lane] for all rows j of the mix (FP32, in LDS, or in
+// the block's slice of `spill` when the mix is too long for LDS), then each
+// wave takes every 4th row i of the band product.  FP64 sums of |y|^2.
+template <bool SPILL>
+__global__ __launch_bounds__(256) void k_energy_rx_tgc(
+    const float2* __restrict__ h, const float2* __restrict__ Pt, const int* __restrict__ shift,
+    const int* __restrict__ pair_fg, const int* __restrict__ mk_meta, const int64_t* __restrict__ mk_off,
+    const int* __restrict__ fmeas, const int* __restrict__ act, const double2* __restrict__ pref,
+    const float2* __restrict__ taps, const int* __restrict__ half, const int64_t* __restrict__ tap_off, int mk0,
+    int mk1, int pair_base, int N_pts, int H_plane, int n_tiles, float2* __restrict__ spill, int spill_rows,
+    double* __restrict__ E) {
+  extern __shared__ float2 phi_lds[];
+  __shared__ double red[4][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  float2* phi = SPILL ? spill + (size_t)blockIdx.x * spill_rows * 64 : phi_lds;
+  for (int tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+    const int p = tile * 64 + lane;
+    const bool valid = p < N_pts;
+    double acc = 0.0;
+    for (int mk = mk0; mk < mk1; ++mk) {
+      const int Nf = mk_meta[mk * 4 + 1], Na = mk_meta[mk * 4 + 2];
+      const int64_t f_off = mk_off[mk * 4 + 1], a_off = mk_off[mk * 4 + 2], pref_off = mk_off[mk * 4 + 3];
+      for (int j = w; j < Nf; j += 4) {
+        float2 v = make_float2(0.f, 0.f);
+        if (valid) {
+          const int pair = pair_base + fmeas[f_off + j];
+          const float2* hr = h + (size_t)pair_fg[pair] * H_plane;
+          double qx = 0.0, qy = 0.0;
+          for (int a = 0; a < Na; ++a) {
+            const float2 hv = hr[shift[(size_t)act[a_off + a] * N_pts + p]];
+            const double2 c = pref[pref_off + (int64_t)a * Nf + j];
+            qx += c.x * hv.x - c.y * hv.y;
+            qy += c.x * hv.y + c.y * hv.x;
+          }
+          const float2 pv = Pt[(size_t)pair * N_pts + p];
+          v = make_float2((float)(qx * pv.x - qy * pv.y), (float)(qx * pv.y + qy * pv.x));
+        }
+        phi[j * 64 + lane] = v;
+      }
+      __syncthreads();
+      const int M = half[mk];
+      const float2* kt = taps + tap_off[mk] + M;   // kt[d], -M <= d <= M
+      for (int i = w; i < Nf; i += 4) {
+        const int j0 = max(0, i - M), j1 = min(Nf - 1, i + M);
+        float2 y = make_float2(0.f, 0.f);
+        for (int j = j0; j <= j1; ++j) cfma(y, kt[i - j], phi[j * 64 + lane]);
+        acc += (double)y.x * y.x + (double)y.y * y.y;
+      }
+      __syncthreads();
+    }
+    red[w][lane] = acc;
+    __syncthreads();
+    if (w == 0 && valid) E[p] = red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
+    __syncthreads();
+  }
+}
+
+// rows: the longest mix of the measurement
+void launch_energy_rx_tgc(const Plan& P, int m, int rows, float2* spill, int spill_blocks, double* E,
+                          hipStream_t s) {
+  int mk0 = 0;
+  while (mk0 < P.N_mk && P.mk_meas[mk0] != m) ++mk0;
+  int mk1 = mk0;
+  while (mk1 < P.N_mk && P.mk_meas[mk1] == m) ++mk1;
+  const int n_tiles = (P.N_pts + 63) / 64;
+  if (!spill) {
+    const size_t lds = (size_t)std::max(rows, 1) * 64 * sizeof(float2);
+    hipFuncSetAttribute((const void*)k_energy_rx_tgc<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        ETGC_LDS_ROWS * 64 * (int)sizeof(float2));
+    hipLaunchKernelGGL(k_energy_rx_tgc<false>, dim3(n_tiles), dim3(256), lds, s, P.d_h, P.d_p, P.d_shift,
+                       P.d_pair_fg, P.d_mk_meta, P.d_mk_off, P.d_fmeas, P.d_act, P.d_pref, P.d_tgc_k32,
+                       P.d_tgc_half, P.d_tgc_off, mk0, mk1, P.meas_pair_base[m], P.N_pts, P.H_plane, n_tiles,
+                       (float2*)nullptr, 0, E);
+  } else {
+    hipLaunchKernelGGL(k_energy_rx_tgc<true>, dim3(std::min(n_tiles, spill_blocks)), dim3(256), 0, s, P.d_h,
+                       P.d_p, P.d_shift, P.d_pair_fg, P.d_mk_meta, P.d_mk_off, P.d_fmeas, P.d_act, P.d_pref,
+                       P.d_tgc_k32, P.d_tgc_half, P.d_tgc_off, mk0, mk1, P.meas_pair_base[m], P.N_pts, P.H_plane,
+                       n_tiles, spill, rows, E);
+  }
+}
+
 // ------------------------------------------------------ FP16 split scales
 // FP16 has a 5-bit exponent, so the split form scales every operand by a
 // power of 2 (exact) that brings its largest entry just below 2^14:

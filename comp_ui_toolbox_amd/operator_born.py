@@ -225,17 +225,19 @@ class OperatorBorn:
         check(_lib.lib().born_plan_get_table(self._plan, w, index, buf.ctypes.data_as(C.c_void_p), cap))
         return buf.reshape(cols.value, rows.value).T.copy()
 
-    def energy_rx(self, measurement: int = 0) -> np.ndarray:
-        """Received energies of one measurement, canonical basis, no TGC
+    def energy_rx(self, measurement: int = 0, tgc: bool = False) -> np.ndarray:
+        """Received energies of one measurement, canonical basis
         (energy_rx_canonical, operator_born.m:718-859): per grid point the
-        squared norm of its column of the measurement's rows of A.
+        squared norm of its column of the measurement's rows of A, or of
+        TGC applied to them when tgc=True (kernels of set_tgc, :836-843).
         `measurement` is 0-based among the plan's measurements.  (N_pts,)
         float64."""
         out = np.empty(self.N_points, dtype=np.float64)
-        check(_lib.lib().born_energy_rx(self._plan, int(measurement), dptr(out)))
+        fn = _lib.lib().born_energy_rx_tgc if tgc else _lib.lib().born_energy_rx
+        check(fn(self._plan, int(measurement), dptr(out)))
         return out
 
-    def energy_rx_dev(self, measurement: int = 0, out=None, stream=None):
+    def energy_rx_dev(self, measurement: int = 0, out=None, stream=None, tgc: bool = False):
         """Device variant: float64 tensor of N_pts energies on the plan's device."""
         import torch
         if out is None:
@@ -243,7 +245,8 @@ class OperatorBorn:
         if out.dtype != torch.float64 or not out.is_cuda or not out.is_contiguous() or out.numel() != self.N_points:
             raise BornError(7, "combined_quick_gpu:NoValidDoubleMatrix", "expected a contiguous float64 device tensor")
         s = torch.cuda.current_stream(out.device).cuda_stream if stream is None else stream
-        check(_lib.lib().born_energy_rx_dev(self._plan, int(measurement), C.c_void_p(out.data_ptr()), C.c_void_p(s)))
+        fn = _lib.lib().born_energy_rx_tgc_dev if tgc else _lib.lib().born_energy_rx_dev
+        check(fn(self._plan, int(measurement), C.c_void_p(out.data_ptr()), C.c_void_p(s)))
         return out
 
     def time_signals(self, u, frequencies, delta_f: float = 0.0, delta_t: float = 0.0, stream=None):

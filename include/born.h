@@ -218,6 +218,16 @@ int born_tgc_apply(born_plan_t plan, int32_t mode, const double* in, int32_t N_r
 int born_tgc_apply_dev(born_plan_t plan, int32_t mode, const void* in_c64, void* out_c64, int32_t N_rhs,
                        void* stream);
 
+/* Received energies of one measurement under the plan's TGC (born_tgc_set):
+ * energy_rx_canonical( op, LT_tgc ) with a diagonal LT_tgc
+ * (operator_born.m:718-859, TGC at :836-843) -- for every grid point p, the
+ * sum over the measurement's mixes of ||C_k Phi_k(:, p)||^2, Phi_k the mix's
+ * rows of A at p and C_k its TGC convolution.  Host variant: N_points
+ * doubles; device variant: N_points FP64 on the device, enqueued on `stream`
+ * and complete on return.  BORN_ERR_INVALID_ARGUMENT when no TGC is set. */
+int born_energy_rx_tgc(born_plan_t plan, int32_t measurement, double* energy);
+int born_energy_rx_tgc_dev(born_plan_t plan, int32_t measurement, void* energy_f64, void* stream);
+
 /* Copies of the device tables, complex64, for checking the device table
  * build: which = 0 h_ref (N_f_unique x N_h), 1 p_incident of measurement
  * `index` (N_f_unique(m) x N_points).  Column-major like the reference

@@ -367,20 +367,24 @@ def forward(T: BornTables, gamma: np.ndarray, rule: str = "mex",
     return out[:, 0] if squeeze else out
 
 
-def energy_rx(T: BornTables, m: int = 0) -> np.ndarray:
-    """Received energies of measurement m, canonical basis, identity TGC
+def energy_rx(T: BornTables, m: int = 0, kernels: Optional[Sequence[np.ndarray]] = None) -> np.ndarray:
+    """Received energies of measurement m, canonical basis
     (energy_rx_canonical, operator_born.m:718-859): per mix, Phi_M = sum_a
     h_ref(fmap, shift(:, e_a)) .* p_inc(fm, :) .* pref(:, a) (:786, :830-831),
+    with a diagonal TGC Phi_M = forward_transform(LT_tgc.transforms{k}, Phi_M)
+    (:836-843; `kernels`: one per mix of measurement m, applied by tgc_apply),
     then E += vecnorm(Phi_M, 2, 1).^2 (:848).  Returns (N_pts,)."""
     ms = T.meas[m]
     fg_meas = ms.idx_f_to_global - 1
     E = np.zeros(T.N_points)
-    for mx in ms.mixes:
+    for k, mx in enumerate(ms.mixes):
         fm = mx.idx_f_to_meas - 1
         Phi = np.zeros((len(fm), T.N_points), dtype=np.complex128)
         for a, e1 in enumerate(mx.idx_active):
             sh = T.shift[:, e1 - 1] - 1
             Phi += T.h_ref[np.ix_(fg_meas[fm], sh)] * ms.p_incident[fm] * mx.prefactors[:, a][:, None]
+        if kernels is not None:
+            Phi = tgc_apply([kernels[k]], [len(fm)], Phi, 1)
         E += np.sum(np.abs(Phi) ** 2, axis=0)
     return E
 

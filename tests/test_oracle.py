@@ -439,3 +439,23 @@ def test_tgc_apply_identity_kernel():
     u = np.arange(7) + 1j * np.arange(7)[::-1]
     for mode in (1, 2):
         assert np.allclose(O.tgc_apply([np.array([1.0 + 0j])], [7], u, mode), u, rtol=0, atol=1e-13)
+
+
+def test_energy_rx_with_tgc_is_column_norms_of_compensated_rows():
+    """energy_rx with TGC kernels equals the squared column norms of the
+    measurement's rows of the assembled A after the TGC of each mix; with
+    one-tap unit kernels it reduces to the plain energies."""
+    _, T = mixed_tiny_tables()
+    A = O.assemble_matrix(T)
+    rng = np.random.Generator(np.random.PCG64(61))
+    obs = 0
+    for m, ms in enumerate(T.meas):
+        sizes = [len(mx.idx_f_to_meas) for mx in ms.mixes]
+        rows = A[obs:obs + sum(sizes)]
+        obs += sum(sizes)
+        kernels = [rng.standard_normal(2 * M + 1) + 1j * rng.standard_normal(2 * M + 1) for M in (1, 3)[: len(sizes)]]
+        ref = np.sum(np.abs(O.tgc_apply(kernels, sizes, rows, 1)) ** 2, axis=0)
+        got = O.energy_rx(T, m, kernels)
+        assert np.max(np.abs(got - ref)) < 1e-12 * np.max(ref)
+        ones = [np.ones(1, dtype=np.complex128)] * len(sizes)
+        assert np.allclose(O.energy_rx(T, m, ones), O.energy_rx(T, m), rtol=1e-14, atol=0)

@@ -136,3 +136,63 @@ def test_tgc_errors(torch_cuda):
     op.set_tgc(None)
     with pytest.raises(BornError):
         op.tgc_forward(u)
+
+
+def _kernels_of_measurement(op, T, kernels, m):
+    first = sum(len(ms.mixes) for ms in T.meas[:m])
+    return kernels[first:first + len(T.meas[m].mixes)]
+
+
+def test_energy_rx_tgc_mixed_tables(torch_cuda):
+    """Received energies under TGC (energy_rx_canonical with a diagonal
+    LT_tgc) on mixed tables, host and device entry points, both
+    measurements, against the oracle."""
+    _, T = mixed_tiny_tables()
+    op = OperatorBorn.from_tables(T)
+    sizes = op.meta["mix_n_obs"]
+    kernels = random_kernels(sizes, [1, 0, 5, 2], 71)
+    op.set_tgc(kernels)
+    for m in range(len(T.meas)):
+        ref = O.energy_rx(T, m, _kernels_of_measurement(op, T, kernels, m))
+        e_h = O.rel_l2(op.energy_rx(m, tgc=True), ref)
+        e_d = O.rel_l2(op.energy_rx_dev(m, tgc=True).cpu().numpy(), ref)
+        log("energy_rx with TGC measurement %d: host %.2e device %.2e" % (m, e_h, e_d))
+        assert e_h < TOL_OP and e_d < TOL_OP
+    op.set_tgc(None)
+    with pytest.raises(BornError):
+        op.energy_rx(0, tgc=True)
+
+
+def test_energy_rx_tgc_long_mixes(torch_cuda):
+    """Mixes of 600 frequencies, longer than the LDS holds: the rows go to
+    the spill buffer."""
+    freqs = 2.0e6 + 5e3 * np.arange(600)
+    spec = tiny_spec(N_e=2, Nx=9, Nz=3, npp=4, freqs=freqs)
+    T = O.build_tables(spec)
+    op = OperatorBorn.from_tables(T)
+    kernels = random_kernels(op.meta["mix_n_obs"], [40, 0, 350, 7], 72)
+    op.set_tgc(kernels)
+    for m in range(2):
+        ref = O.energy_rx(T, m, _kernels_of_measurement(op, T, kernels, m))
+        e = O.rel_l2(op.energy_rx(m, tgc=True), ref)
+        log("energy_rx with TGC, 600-frequency mixes, measurement %d: %.2e" % (m, e))
+        assert e < TOL_OP
+
+
+def test_energy_rx_tgc_config1(torch_cuda):
+    """Config 1, first measurement (128 mixes of 257 frequencies), exponential
+    TGC, at 24 sampled points against tables restricted exactly to them."""
+    spec = C.config1()
+    op = OperatorBorn.from_sequence(spec)
+    sizes = op.meta["mix_n_obs"]
+    df = float(spec.frequencies[1] - spec.frequencies[0])
+    a = np.log(100.0) / 28e-6
+    kernels = tgc.exponential_kernels(a, np.tile([[2.0e-6, 30.0e-6]], (len(sizes), 1)), df, -60.0)
+    op.set_tgc(kernels)
+    rng = np.random.Generator(np.random.PCG64(73))
+    pts = np.sort(rng.choice(spec.fov.N_points, 24, replace=False))
+    R = O.restricted_tables(spec, pts)
+    E = op.energy_rx(0, tgc=True)
+    e = O.rel_l2(E[pts], O.energy_rx(R, 0, kernels[: len(R.meas[0].mixes)]))
+    log("energy_rx with TGC, config 1 measurement 0 at 24 points: %.2e" % e)
+    assert e < TOL_OP
