@@ -12,6 +12,10 @@ are resident in HBM.  Prints one JSON line (see the keys below).
 `value` is pairs/s of the selected form (split BF16 by default); the plain
 FP32 form is timed beside it on the same plan and input, and the ratio and
 the relative L2 difference between the two forms' outputs are reported.
+`cpu_baseline` times the oracle's C FP64 loop nest (Phi materialized, as
+the reference's CPU path) on a sample of the config-1 pair, with the fused
+loop beside it; `config0_full` times config 0 in full on both the CPU loops
+and the device.
 """
 import argparse
 import json
@@ -53,12 +57,10 @@ def algorithmic_bytes(spec, n_rhs: int) -> float:
         8.0 * spec.N_observations() * n_rhs
 
 
-def cpu_baseline(op, spec, n_mixes: int, threads: int):
-    """The oracle's C restatement of forward_quick_cpu / adjoint_quick_cpu
-    (Phi materialized per active element, then a GEMV) on measurement 1 and
-    the first `n_mixes` receive channels, scaled to the full pair."""
+def _cpu_tables(op, spec, n_mixes: int):
+    """FP64 copies of the device tables of measurement 1 and its first
+    `n_mixes` mixes, in the oracle's table layout."""
     from oracle import born_oracle as O
-    from oracle import ref_loop as R
 
     class _T:
         pass
@@ -82,29 +84,74 @@ def cpu_baseline(op, spec, n_mixes: int, threads: int):
         mt.prefactors = pref_u[fidx][:, None] * np.asarray(mix.weights)[None, :]
         ms.mixes.append(mt)
     T.meas = [ms]
+    return T
+
+
+def _cpu_pair_seconds(T, g, fused: bool, threads: int) -> float:
+    from oracle import ref_loop as R
     P = R.Prepared(T, 0)
-    g = np.ascontiguousarray(spec_gamma(spec), dtype=np.complex128)
+    g = np.ascontiguousarray(g, dtype=np.complex128)
     t0 = time.perf_counter()
-    u = R.apply_measurement(P, 1, g, fused=False, threads=threads)
-    R.apply_measurement(P, 2, u, fused=False, threads=threads)
-    dt = time.perf_counter() - t0
+    u = R.apply_measurement(P, 1, g, fused=fused, threads=threads)
+    R.apply_measurement(P, 2, u, fused=fused, threads=threads)
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(op, spec, n_mixes: int, threads: int):
+    """The oracle's C restatement of forward_quick_cpu / adjoint_quick_cpu
+    (Phi materialized per active element, then a GEMV) on measurement 1 and
+    the first `n_mixes` receive channels, scaled to the full pair; beside it
+    the oracle's fused loop (no Phi) on the same sample."""
+    T = _cpu_tables(op, spec, n_mixes)
+    g = spec_gamma(spec)
+    dt = _cpu_pair_seconds(T, g, False, threads)
+    dt_f = _cpu_pair_seconds(T, g, True, threads)
     n_units = sum(len(mm) for mm in spec.mixes)          # (measurement, mix) pairs in a full pair
-    t_pair = dt / n_mixes * n_units
+    scale = n_units / n_mixes
     return {
-        "value": 1.0 / t_pair, "unit": "pairs/s", "cores": threads, "kind": "port",
+        "value": 1.0 / (dt * scale), "unit": "pairs/s", "cores": threads, "kind": "port",
         "sample": f"C FP64 loop nest (operator_born.m:332-596, Phi materialized) on measurement 1, "
                   f"{n_mixes} of {len(spec.mixes[0])} receive channels, forward+adjoint in {dt:.1f} s, "
-                  f"scaled x{n_units / n_mixes:.0f} to the full pair",
+                  f"scaled x{scale:.0f} to the full pair",
+        "fused": {"value": 1.0 / (dt_f * scale), "unit": "pairs/s",
+                  "sample": f"the oracle's fused FP64 loop (no Phi) on the same sample in {dt_f:.1f} s"},
     }
+
+
+def config0_full(threads: int, steps: int):
+    """Config 0 in full (64 elements, 256x256 FOV, 65 bins, one wave, 30
+    occupied points): the C FP64 loop nest, Phi materialized and fused, and
+    the device pair in both forms, all on the same input."""
+    import torch
+    from comp_ui_toolbox_amd import OperatorBorn
+    from comp_ui_toolbox_amd import configs as C
+    spec = C.config0()
+    op = OperatorBorn.from_sequence(spec, precision="fp32")
+    g0 = C.phantom_points(spec)
+    T = _cpu_tables(op, spec, len(spec.mixes[0]))
+    dt = _cpu_pair_seconds(T, g0, False, threads)
+    dt_f = _cpu_pair_seconds(T, g0, True, threads)
+    x = torch.tensor(g0.astype(np.complex64)[None, :], device="cuda")
+    u = torch.empty((1, op.N_observations), dtype=torch.complex64, device="cuda")
+    g = torch.empty((1, op.N_points), dtype=torch.complex64, device="cuda")
+    gpu = {}
+    for form in ("split_bf16", "fp32"):
+        el, _ = time_form(op, form, x, u, g, steps, 3)
+        gpu[form] = steps / el
+    op.close()
+    return {"workload": "config0: 64 el, 256x256 FOV, 65 f, 1 wave, 4160 obs, 30 occupied points, 1 rhs",
+            "cpu_pairs_per_s": 1.0 / dt, "cpu_fused_pairs_per_s": 1.0 / dt_f, "cores": threads,
+            "gpu_pairs_per_s": gpu}
 
 
 _GAMMA = {}
 
 
 def spec_gamma(spec):
+    """Config 0: 30 occupied points; the others: the speckle phantom."""
     from comp_ui_toolbox_amd import configs as C
     if spec.name not in _GAMMA:
-        _GAMMA[spec.name] = C.phantom_speckle(spec)
+        _GAMMA[spec.name] = C.phantom_points(spec) if spec.name == "config0" else C.phantom_speckle(spec)
     return _GAMMA[spec.name]
 
 
@@ -373,6 +420,12 @@ def main():
             result["cpu_baseline"] = cpu_baseline(op, spec, args.cpu_mixes, threads)
         except Exception as e:  # report, do not hide
             result["cpu_baseline"] = {"value": None, "error": repr(e)}
+        if args.config == "config1":
+            op.close()
+            try:
+                result["config0_full"] = config0_full(threads, args.steps)
+            except Exception as e:
+                result["config0_full"] = {"error": repr(e)}
     else:
         result["cpu_baseline"] = None
     print(json.dumps(result), flush=True)
