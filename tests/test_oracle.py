@@ -459,3 +459,19 @@ def test_energy_rx_with_tgc_is_column_norms_of_compensated_rows():
         assert np.max(np.abs(got - ref)) < 1e-12 * np.max(ref)
         ones = [np.ones(1, dtype=np.complex128)] * len(sizes)
         assert np.allclose(O.energy_rx(T, m, ones), O.energy_rx(T, m), rtol=1e-14, atol=0)
+
+
+def test_prefactors_lossless_and_lossy():
+    """pref = -dV k^2 (setup.m:759): in a lossless, dispersion-free medium
+    -dx dz (2 pi f / c)^2 with dV the area of a 2-D grid cell; with losses
+    k = k' - j k'' (k'' > 0), so k^2 (not |k|^2) has Im(pref) = 2 dV k' k'' > 0."""
+    spec = tiny_spec()
+    spec.medium = C.MediumSpec(absorption=0.0, dispersion=False)
+    f = spec.frequencies
+    dx, dz = spec.fov.delta_axis
+    assert np.allclose(O.prefactors_unique(spec), -dx * dz * (2 * np.pi * f / 1500.0) ** 2, rtol=1e-14, atol=0)
+    spec.medium = C.MediumSpec(absorption=2.0)
+    p = O.prefactors_unique(spec)
+    k = O.wavenumbers(f, spec.medium)
+    assert np.all(p.imag > 0)
+    assert np.allclose(p.imag, 2 * dx * dz * k.real * (-k.imag), rtol=1e-13)
