@@ -326,7 +326,7 @@ def main():
         copy_probe(args.copy_probe)
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               f"(the operator pair runs on a single MI355X).", file=sys.stderr, flush=True)
         sys.exit(2)
 
