@@ -97,6 +97,20 @@ def _cpu_pair_seconds(T, g, fused: bool, threads: int) -> float:
     return time.perf_counter() - t0
 
 
+def host_cpu() -> dict:
+    """CPU model and logical CPU count of the host that runs the CPU legs."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count()}
+
+
 def cpu_baseline(op, spec, n_mixes: int, threads: int):
     """The oracle's C restatement of forward_quick_cpu / adjoint_quick_cpu
     (Phi materialized per active element, then a GEMV) on measurement 1 and
@@ -115,6 +129,7 @@ def cpu_baseline(op, spec, n_mixes: int, threads: int):
                   f"scaled x{scale:.0f} to the full pair",
         "fused": {"value": 1.0 / (dt_f * scale), "unit": "pairs/s",
                   "sample": f"the oracle's fused FP64 loop (no Phi) on the same sample in {dt_f:.1f} s"},
+        "host": host_cpu(),
     }
 
 
