@@ -506,7 +506,8 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict
       const float2* hrow_ = I->h + (size_t)ch_.fg * I->Hp + (size_t)row_ * I->HR;                \
       _Pragma("unroll") for (int j = 0; j < HPT; ++j) H[j] = hrow_[max(eidx[j], 0)];             \
       const float2* wblk_ = I->Wc + (size_t)st_ * NEP * NC;                                      \
-      _Pragma("unroll") for (int j = 0; j < WPT; ++j) WV[j] = wblk_[ft + 256 * j];               \
+      _Pragma("unroll") for (int j = 0; j < WPT; ++j)                                           \
+        WV[j] = wblk_[(2 * ((ft >> 4) + 16 * (j >> 1)) + (j & 1)) * NC + (ft & 15)];             \
       if (!Y3 || ey_ == 0) {                                                                     \
         const int* prs_ = chunk_pairs + (size_t)(I->chunk_base + c_) * NC;                       \
         _Pragma("unroll") for (int j = 0; j < PPT; ++j)                                          \
@@ -541,23 +542,19 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict
           }                                                                                      \
         }                                                                                        \
       }                                                                                          \
-      /* weights: entries j and j + 1 of a thread are elements k and k + 16 of one column */     \
+      /* weights: entries j and j + 1 of a thread are elements k and k + 1 of column */          \
+      /* ft mod 16, packed into one 32-bit word per plane */                                     \
       _Pragma("unroll") for (int j = 0; j < WPT; j += 2) {                                       \
-        const int idx = ft + 256 * j;                                                            \
-        const int k = idx / NC, wc = idx - k * NC;                                               \
+        const int k = 2 * ((ft >> 4) + 16 * (j >> 1)), wc = ft & 15;                             \
         unsigned int rh, rl, ih, il;                                                             \
         const float wsc_ = F16 ? s_w : 1.f;                                                      \
         split_pair<F16>(WV[j].x * wsc_, WV[j + 1].x * wsc_, rh, rl);                             \
         split_pair<F16>(WV[j].y * wsc_, WV[j + 1].y * wsc_, ih, il);                             \
-        unsigned short* wd = &Wt[wc * WS + k];                                                   \
-        wd[0 * NC * WS] = (unsigned short)rh;                                                    \
-        wd[0 * NC * WS + 16] = (unsigned short)(rh >> 16);                                       \
-        wd[1 * NC * WS] = (unsigned short)rl;                                                    \
-        wd[1 * NC * WS + 16] = (unsigned short)(rl >> 16);                                       \
-        wd[2 * NC * WS] = (unsigned short)ih;                                                    \
-        wd[2 * NC * WS + 16] = (unsigned short)(ih >> 16);                                       \
-        wd[3 * NC * WS] = (unsigned short)il;                                                    \
-        wd[3 * NC * WS + 16] = (unsigned short)(il >> 16);                                       \
+        unsigned int* wd = reinterpret_cast<unsigned int*>(&Wt[wc * WS + k]);                    \
+        wd[0 * NC * WS / 2] = rh;                                                                \
+        wd[1 * NC * WS / 2] = rl;                                                                \
+        wd[2 * NC * WS / 2] = ih;                                                                \
+        wd[3 * NC * WS / 2] = il;                                                                \
       }                                                                                          \
       if (!Y3 || ey_ == 0) {                                                                     \
         float2* Ps = reinterpret_cast<float2*>(ldsu + Lay::PS_OFF + (c_ & 1) * Lay::PS_U16);     \
