@@ -121,7 +121,7 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict
   constexpr int KBW = (SEG / 32) / PW;
   constexpr int LA = SEG + NP * (16 * MT - 1);
   constexpr int LAP = fwd_split_lap(LA, NP);
-  constexpr int APT = (LA + 255) / 256;
+  constexpr int APT = 2 * ((LA + 511) / 512);   // window entries per thread: pairs (u, u + 1), u = 2 tid + 512 j
   unsigned short* Aw = ldsu;                     // [4][NCOPY][LAP]
   unsigned short* Bt = ldsu + 4 * NCOPY * LAP;   // [4][NC][SEGP]
   unsigned int* Bt32 = reinterpret_cast<unsigned int*>(Bt);
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict
     gfull = xs * SEG + SEG <= I->Nx;                                                       \
     hval = 0;                                                                             \
     _Pragma("unroll") for (int j = 0; j < APT; ++j) {                                    \
-      const int u_ = tid + 256 * j;                                                       \
+      const int u_ = 2 * tid + 512 * (j >> 1) + (j & 1);                                  \
       const int t_ = u_ + t0_;                                                            \
       const bool ok_ = u_ < LA && t_ >= 0 && t_ < I->N_ext;                                \
       hoff[j] = ok_ ? 8u * (unsigned int)I->ext[t_] : 0u;                                  \
@@ -238,23 +238,32 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict
     __syncthreads();
     // FP16 operands are scaled by powers of 2 into range (see k_fp16_scales)
     const float s_h = F16 ? I->fscale[0] : 1.f, s_b = F16 ? I->fscale[1] : 1.f;
-    // window -> hi/lo planes, NCOPY copies
+    // window -> hi/lo planes, NCOPY copies; a thread writes the entry pair
+    // (u, u + 1) as one 32-bit word per plane (u and the copy shifts are even)
+    {
+      unsigned int* Aw32 = reinterpret_cast<unsigned int*>(Aw);
 #pragma unroll
-    for (int j = 0; j < APT; ++j) {
-      const int u = tid + 256 * j;
-      if (u < LA) {
-        float2 hv = hreg[j];
-        if (!hfull && !((hval >> j) & 1u)) hv = make_float2(0.f, 0.f);
-        if (F16) hv = make_float2(hv.x * s_h, hv.y * s_h);
-        unsigned int hi, lo;   // (re, im) packed
-        split_pair<F16>(hv.x, hv.y, hi, lo);
-        const unsigned short v[4] = {(unsigned short)hi, (unsigned short)lo, (unsigned short)(hi >> 16),
-                                     (unsigned short)(lo >> 16)};
+      for (int j = 0; j < APT; j += 2) {
+        const int u = 2 * tid + 512 * (j >> 1);
+        if (u < LA) {
+          float2 h0 = hreg[j], h1 = hreg[j + 1];
+          if (!hfull) {
+            if (!((hval >> j) & 1u)) h0 = make_float2(0.f, 0.f);
+            if (!((hval >> (j + 1)) & 1u)) h1 = make_float2(0.f, 0.f);
+          }
+          if (F16) {
+            h0 = make_float2(h0.x * s_h, h0.y * s_h);
+            h1 = make_float2(h1.x * s_h, h1.y * s_h);
+          }
+          unsigned int v[4];   // re_hi, re_lo, im_hi, im_lo; entry u in the low half
+          split_pair<F16>(h0.x, h1.x, v[0], v[1]);
+          split_pair<F16>(h0.y, h1.y, v[2], v[3]);
 #pragma unroll
-        for (int cp = 0; cp < 4; ++cp) {
+          for (int cp = 0; cp < 4; ++cp) {
 #pragma unroll
-          for (int c = 0; c < NCOPY; ++c)
-            if (u >= NP * c) Aw[(cp * NCOPY + c) * LAP + u - NP * c] = v[cp];
+            for (int c = 0; c < NCOPY; ++c)
+              if (u >= NP * c) Aw32[((cp * NCOPY + c) * LAP + u - NP * c) >> 1] = v[cp];
+          }
         }
       }
     }
