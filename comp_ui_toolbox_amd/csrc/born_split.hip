@@ -466,7 +466,13 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict
   constexpr int LR = Lay::LR, CPB = Lay::CPB, WS = Lay::WS, PSS = Lay::PSS;
   constexpr int KB = MT / 2;                    // K-blocks of 32 elements
   constexpr int L = SEG + NP * (NEP - 1);       // extended-row window entries
-  constexpr int HPT = (L + 255) / 256;
+  // window fill: thread (p, r) = (ft mod NP, ft / NP) holds the entries
+  // u = NP q + p, q = 4 r + i (i = 0..3): four consecutive reversed positions
+  // of one phase row
+  constexpr int HPT = 4;
+  constexpr int WG4 = ((L + NP - 1) / NP + 3) / 4;   // 4-entry groups per phase
+  static_assert(NP * WG4 <= 256, "window groups exceed the fill threads");
+  static_assert(LQ % 4 == 0 && LR % 4 == 0 && CPB % 4 == 0, "copy alignment classes assume multiples of 4");
   constexpr int WPT = NEP * NC / 256;           // W entries per fill thread
   constexpr int PPT = NC * SEG / 256;           // p_inc entries per fill thread
   constexpr int WPP = 4 / NP;                   // compute waves per phase
@@ -488,15 +494,14 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict
   if (filler) {
     // ---------------------------------------------------------------- fill waves
     const int t_shift = i_s - (NEP - I->N_e) * NP;  // window entry u holds Hext[u + t_shift]
-    const bool hfull = t_shift >= 0 && t_shift + L <= I->N_ext;
     // FP16 operands are scaled by powers of 2 into range (see k_fp16_scales)
     const float s_h = F16 ? I->fscale[0] : 1.f, s_w = F16 ? I->fscale[3] : 1.f;
     int eidx[HPT];
 #pragma unroll
     for (int j = 0; j < HPT; ++j) {
-      const int u = ft + 256 * j;
+      const int u = NP * (4 * (ft / NP) + j) + ft % NP;
       const int t = u + t_shift;
-      eidx[j] = (u < L && t >= 0 && t < I->N_ext) ? I->ext[t] : -1;
+      eidx[j] = (ft < NP * WG4 && u < L && t >= 0 && t < I->N_ext) ? I->ext[t] : -1;
     }
     // two register sets: the loads of step s + 2 are issued when step s has
     // been written to LDS, so each load has two step periods to arrive
@@ -529,26 +534,42 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict
       const int c_ = Y3 ? st_ / Ney : st_, ey_ = Y3 ? st_ - c_ * Ney : 0;                        \
       unsigned short* Ar = ldsu + (st_ & 1) * Lay::AW_U16;                                       \
       unsigned short* Wt = Ar + Lay::AR_U16;                                                     \
-      /* window entry u -> phase p = u mod NP, reversed index m = LQ - 1 - u / NP; */            \
-      /* copy d holds m at m - d + PADL (>= 1) */                                                \
-      _Pragma("unroll") for (int j = 0; j < HPT; ++j) {                                          \
-        const int u = ft + 256 * j;                                                              \
-        if (u < L) {                                                                             \
-          const int p = u % NP, m = (LQ - 1) - u / NP;                                           \
-          float2 hv = H[j];                                                                      \
-          if (!hfull && eidx[j] < 0) hv = make_float2(0.f, 0.f);                                 \
+      /* window entry u -> phase p = u mod NP, reversed index m = LQ - 1 - u / NP;  */            \
+      /* copy d holds m at m - d + PADL (>= 1).  The thread's entries q = 4 r + 3 .. 4 r */       \
+      /* sit at positions P .. P + 3 of every copy; P mod 4 = -d mod 4, so copy 0 takes */       \
+      /* one 8-byte store, copy 2 two 4-byte stores, copies 1 and 3 2 + 4 + 2 bytes.    */       \
+      /* Entries outside the window (eidx < 0) are written as zero.                     */       \
+      if (ft < NP * WG4) {                                                                       \
+        const int p = ft % NP, r = ft / NP;                                                      \
+        float xr[4], xi[4];   /* position order: index i holds q = 4 r + 3 - i */                \
+        _Pragma("unroll") for (int i = 0; i < 4; ++i) {                                          \
+          float2 hv = H[3 - i];                                                                  \
+          if (eidx[3 - i] < 0) hv = make_float2(0.f, 0.f);                                       \
           if (F16) hv = make_float2(hv.x * s_h, hv.y * s_h);                                     \
-          unsigned int hi, lo;                                                                   \
-          split_pair<F16>(hv.x, -hv.y, hi, lo); /* conj */                                       \
-          const unsigned short v[4] = {(unsigned short)hi, (unsigned short)lo,                   \
-                                       (unsigned short)(hi >> 16), (unsigned short)(lo >> 16)};  \
-          _Pragma("unroll") for (int cp = 0; cp < 4; ++cp) {                                     \
-            unsigned short* row = Ar + (cp * NP + p) * CPB + m + PADL;                           \
-            row[0] = v[cp];                                                                      \
-            row[LR + 16 - 1] = v[cp];                                                            \
-            row[2 * LR + 28 - 2] = v[cp];                                                        \
-            row[3 * LR + 40 - 3] = v[cp];                                                        \
-          }                                                                                      \
+          xr[i] = hv.x;                                                                          \
+          xi[i] = -hv.y; /* conj */                                                              \
+        }                                                                                        \
+        unsigned int pr[4][2]; /* [comp][half]: half 0 = positions P, P+1; 1 = P+2, P+3 */       \
+        split_pair<F16>(xr[0], xr[1], pr[0][0], pr[1][0]);                                       \
+        split_pair<F16>(xr[2], xr[3], pr[0][1], pr[1][1]);                                       \
+        split_pair<F16>(xi[0], xi[1], pr[2][0], pr[3][0]);                                       \
+        split_pair<F16>(xi[2], xi[3], pr[2][1], pr[3][1]);                                       \
+        _Pragma("unroll") for (int cp = 0; cp < 4; ++cp) {                                       \
+          unsigned short* P0 = Ar + (cp * NP + p) * CPB + PADL + (LQ - 4 - 4 * r);               \
+          const unsigned int A_ = pr[cp][0], B_ = pr[cp][1];                                     \
+          const unsigned int mid_ = (A_ >> 16) | (B_ << 16);                                     \
+          *reinterpret_cast<uint2*>(P0) = make_uint2(A_, B_);                                    \
+          unsigned short* P1 = P0 + LR + 16 - 1;                                                 \
+          P1[0] = (unsigned short)A_;                                                            \
+          *reinterpret_cast<unsigned int*>(P1 + 1) = mid_;                                       \
+          P1[3] = (unsigned short)(B_ >> 16);                                                    \
+          unsigned short* P2 = P0 + 2 * LR + 28 - 2;                                             \
+          *reinterpret_cast<unsigned int*>(P2) = A_;                                             \
+          *reinterpret_cast<unsigned int*>(P2 + 2) = B_;                                         \
+          unsigned short* P3 = P0 + 3 * LR + 40 - 3;                                             \
+          P3[0] = (unsigned short)A_;                                                            \
+          *reinterpret_cast<unsigned int*>(P3 + 1) = mid_;                                       \
+          P3[3] = (unsigned short)(B_ >> 16);                                                    \
         }                                                                                        \
       }                                                                                          \
       /* weights: entries j and j + 1 of a thread are elements k and k + 1 of column */          \
