@@ -138,7 +138,7 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict
   const int bi = 2 * (tid & 127);
 
   int it = 0, item = blockIdx.x;
-  int ci, xs, z, z_end, fg, rr, ey;
+  int ci, xs, z, z_end, fg, rr, ey, lid;
   bool hfull, gfull;                 // window inside the table / segment inside the row
   unsigned int hoff[APT];            // window gather byte offsets within a row
   unsigned int hval = 0;             // bit j: window entry j inside the table
@@ -152,7 +152,10 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict
     }                                                                                     \
     const int tpad = (16 * MT - I->N_e) * NP;                                             \
     const int NBLK = I->NZB_s * I->nseg_s;   /* partial-sum blocks per chunk and slice */ \
-    const int li_ = (itm) - I->item_begin_s; /* ((ci * Ney + ey) * NZB_s + zb) * nseg_s + xs */ \
+    /* items run in reverse: the forward reads p_inc from the last frequency down, */      \
+    /* where the adjoint before it stopped, so the tail it left in the cache is reused */  \
+    lid = I->n_chunks * I->Ney * NBLK - 1 - ((itm) - I->item_begin_s);                    \
+    const int li_ = lid; /* ((ci * Ney + ey) * NZB_s + zb) * nseg_s + xs */              \
     ci = li_ / (I->Ney * NBLK);                                                           \
     const int rem1_ = li_ - ci * (I->Ney * NBLK);                                         \
     ey = rem1_ / NBLK;                                                                    \
@@ -298,9 +301,8 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict
     __syncthreads();
 
     // next step: the next row of this item, else the first row of the next item
-    const int item_done_id = item;
+    const int lid_done = lid;
     float2* const vpart_done = Iv.vpart;
-    const int ib_done = Iv.item_begin_s;
     const float unscale_done = F16 ? Iv.fscale[2] : 1.f;
     const bool item_done = z + 1 >= z_end;
     bool has_next = true;
@@ -381,7 +383,7 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict
       }
       __syncthreads();
       // vpart block of the item: [chunk][zb * nseg + xs] = local item index
-      float* out = reinterpret_cast<float*>(vpart_done + (size_t)(item_done_id - ib_done) * Ne_pad * NC);
+      float* out = reinterpret_cast<float*>(vpart_done + (size_t)lid_done * Ne_pad * NC);
       for (int idx = tid; idx < MT * 512; idx += 256) {
         const int mt = idx >> 9, e_g = mt / MTW, mtw = mt - e_g * MTW, rest = idx & 511;
         float sum = 0.f;
