@@ -32,7 +32,8 @@ EXPORTED = (
     "born_batch_create", "born_batch_destroy", "born_batch_forward_dev", "born_batch_adjoint_dev",
     "born_batch_set_graph", "born_batch_autotune", "born_plan_autotune", "born_batch_set_timing",
     "born_batch_get_timing", "born_tgc_set", "born_tgc_apply", "born_tgc_apply_dev",
-    "born_energy_rx_tgc", "born_energy_rx_tgc_dev",
+    "born_energy_rx_tgc", "born_energy_rx_tgc_dev", "born_debug_fail_allocation",
+    "born_debug_live_allocations",
 )
 
 
@@ -123,6 +124,8 @@ def lib() -> C.CDLL:
     L.born_tgc_apply_dev.argtypes = [vp, C.c_int32, vp, vp, C.c_int32, vp]
     L.born_energy_rx_tgc.argtypes = [vp, C.c_int32, pd]
     L.born_energy_rx_tgc_dev.argtypes = [vp, C.c_int32, vp, vp]
+    L.born_debug_fail_allocation.argtypes = [C.c_int64]
+    L.born_debug_live_allocations.argtypes = []
     L.born_plan_path.argtypes = [vp]
     L.born_plan_force_general.argtypes = [vp, C.c_int32]
     L.born_plan_set_timing.argtypes = [vp, C.c_int32]
@@ -140,9 +143,10 @@ def lib() -> C.CDLL:
     L.born_status_id.restype = C.c_char_p
     L.born_status_id.argtypes = [C.c_int]
     for name in EXPORTED:
-        if name not in ("born_last_error", "born_status_id", "born_plan_path"):
+        if name not in ("born_last_error", "born_status_id", "born_plan_path", "born_debug_live_allocations"):
             getattr(L, name).restype = C.c_int
     L.born_plan_path.restype = C.c_int
+    L.born_debug_live_allocations.restype = C.c_int64
     _lib = L
     return L
 

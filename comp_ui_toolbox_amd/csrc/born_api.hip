@@ -8,6 +8,8 @@
 #include "born.h"
 #include "born_internal.h"
 
+#include <atomic>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -35,21 +37,36 @@ struct Error : std::runtime_error {
                                        #expr + " at " + __FILE__ + ":" + std::to_string(__LINE__)); \
   } while (0)
 
+// live device allocations of the library (error-path tests check it returns
+// to its value before a failed call)
+static std::atomic<int64_t> g_live_alloc{0};
+
 template <typename T>
 static void dfree(T*& p) {
-  if (p) hipFree((void*)p);
+  if (p) {
+    hipFree((void*)p);
+    g_live_alloc.fetch_sub(1);
+  }
   p = nullptr;
 }
+
+// Error-path testing: the g_fail_alloc-th device allocation from now fails
+// (0: the next one); negative: off.
+static std::atomic<int64_t> g_fail_alloc{-1};
 
 template <typename T>
 static T* dalloc(size_t n) {
   if (n == 0) n = 1;
+  if (g_fail_alloc.load() >= 0 && g_fail_alloc.fetch_sub(1) == 0)
+    throw Error(BORN_ERR_OUT_OF_MEMORY, "device allocation of " + std::to_string(n * sizeof(T)) +
+                                            " bytes failed (injected)");
   void* p = nullptr;
   hipError_t e = hipMalloc(&p, n * sizeof(T));
   if (e != hipSuccess) {
     hipGetLastError();
     throw Error(BORN_ERR_OUT_OF_MEMORY, "device allocation of " + std::to_string(n * sizeof(T)) + " bytes failed");
   }
+  g_live_alloc.fetch_add(1);
   return (T*)p;
 }
 
@@ -57,7 +74,12 @@ static T* dalloc(size_t n) {
 template <typename T>
 static T* dupload(const std::vector<T>& v, hipStream_t) {
   T* p = dalloc<T>(v.size());
-  if (!v.empty()) HIP_CHECK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  try {
+    if (!v.empty()) HIP_CHECK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  } catch (...) {
+    dfree(p);
+    throw;
+  }
   return p;
 }
 
@@ -65,7 +87,12 @@ static T* dupload(const std::vector<T>& v, hipStream_t) {
 // columns of the structured kernels read.
 static float2* alloc_p_table(const Plan& P) {
   float2* d = dalloc<float2>((size_t)(P.N_pairs + 1) * P.N_pts);
-  HIP_CHECK(hipMemset(d + (size_t)P.N_pairs * P.N_pts, 0, (size_t)P.N_pts * sizeof(float2)));
+  try {
+    HIP_CHECK(hipMemset(d + (size_t)P.N_pairs * P.N_pts, 0, (size_t)P.N_pts * sizeof(float2)));
+  } catch (...) {
+    dfree(d);
+    throw;
+  }
   return d;
 }
 
@@ -126,8 +153,8 @@ Plan::~Plan() {
   dfree(d_tgc_off);
   dfree(d_tgc_k32);
   dfree(d_tgc_k64);
-  if (d_io_in) hipFree(d_io_in);
-  if (d_io_out) hipFree(d_io_out);
+  dfree(d_io_in);
+  dfree(d_io_out);
   if (stream) hipStreamDestroy(stream);
 }
 
@@ -599,9 +626,22 @@ static void setup_column_rhs_adjoint(Plan& P, int N_rhs) {
 }
 
 // Chunks of <= NC (pair, rhs) columns per frequency, rhs-major.
+static void build_workspace(Plan& P, int N_rhs);
+
+// A failure while building frees the partial workspace, so a plan that
+// could not get its workspace holds none.
 static void ensure_workspace(Plan& P, int N_rhs) {
   if (P.ws_rhs == N_rhs) return;
   free_workspace(P);
+  try {
+    build_workspace(P, N_rhs);
+  } catch (...) {
+    free_workspace(P);
+    throw;
+  }
+}
+
+static void build_workspace(Plan& P, int N_rhs) {
   P.chunks.clear();
   P.col_pair.clear();
   std::vector<int> pr_chunk((size_t)P.N_pairs * N_rhs, -1), pr_pos((size_t)P.N_pairs * N_rhs, -1);
@@ -642,7 +682,6 @@ static void ensure_workspace(Plan& P, int N_rhs) {
     HIP_CHECK(hipMemset(P.d_Wc, 0, n * sizeof(float2)));  // padding columns/elements stay zero
   }
   P.d_gm = dalloc<float2>((size_t)N_rhs * P.N_pts);
-  P.ws_rhs = N_rhs;
   if (P.structured) {
     P.ls.inst.assign(1, make_inst(P));
     P.ls.MT = P.Ne_pad / 16;
@@ -655,13 +694,16 @@ static void ensure_workspace(Plan& P, int N_rhs) {
     if (P.fwd_expand) setup_column_rhs_adjoint(P, N_rhs);
   }
   HIP_CHECK(hipDeviceSynchronize());  // workspace ready before work on any stream
+  // valid only once complete: a failure above leaves ws_rhs = -1, so the next
+  // call frees the partial workspace and builds it again
+  P.ws_rhs = N_rhs;
 }
 
 static void ensure_io(Plan& P, size_t in_bytes, size_t out_bytes) {
   const size_t need = std::max(in_bytes, out_bytes);
   if (P.io_bytes >= need) return;
-  if (P.d_io_in) hipFree(P.d_io_in);
-  if (P.d_io_out) hipFree(P.d_io_out);
+  dfree(P.d_io_in);
+  dfree(P.d_io_out);
   P.d_io_in = P.d_io_out = nullptr;
   P.io_bytes = 0;
   P.d_io_in = dalloc<char>(need);
@@ -759,10 +801,11 @@ static Plan* create_tables(const born_tables_desc* d) {
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipStreamSynchronize(P->stream));
   } catch (...) {
-    hipFree(stage);
+    hipStreamSynchronize(P->stream);   // conversions already enqueued read the staging buffer
+    dfree(stage);
     throw;
   }
-  hipFree(stage);
+  dfree(stage);
   finish_plan(*P);
   return P.release();
 }
@@ -880,14 +923,14 @@ static Plan* create_geometry(const born_geometry_desc* d) {
 
   std::vector<double2> kv(P->N_fu);
   for (int f = 0; f < P->N_fu; ++f) kv[f] = make_double2(d->wavenumbers[2 * f], d->wavenumbers[2 * f + 1]);
-  double2* dk = dupload(kv, s);
-  double2* dfw = nullptr;
+  std::vector<double2> fw;
   if (d->face_weights) {
     const size_t nfw = (size_t)g.Nfx * g.Nfy * P->N_fu;
-    std::vector<double2> fw(nfw);
+    fw.resize(nfw);
     for (size_t i = 0; i < nfw; ++i) fw[i] = make_double2(d->face_weights[2 * i], d->face_weights[2 * i + 1]);
-    dfw = dupload(fw, s);
   }
+  double2* dk = nullptr;
+  double2* dfw = nullptr;
   double2* h64 = nullptr;
   double2* h64tx = nullptr;
   int* d_meas_Nfm = nullptr;
@@ -896,6 +939,8 @@ static Plan* create_geometry(const born_geometry_desc* d) {
   double2* d_vd = nullptr;
   int* d_vd_off = nullptr;
   try {
+    dk = dupload(kv, s);
+    if (d->face_weights) dfw = dupload(fw, s);
     const size_t nh = (size_t)P->N_fu * P->H_plane;
     h64 = dalloc<double2>(nh);
     g.filt_kind = d->filter_rx_kind;
@@ -1966,6 +2011,13 @@ int born_batch_get_timing(born_batch_t batch, double* ms, int64_t* launches) {
 }
 
 const char* born_last_error(void) { return g_last_error.c_str(); }
+
+int born_debug_fail_allocation(int64_t n) {
+  g_fail_alloc.store(n < 0 ? -1 : n);
+  return BORN_OK;
+}
+
+int64_t born_debug_live_allocations(void) { return g_live_alloc.load(); }
 
 const char* born_status_id(int status) {
   switch (status) {

@@ -288,6 +288,17 @@ const char* born_last_error(void);
 /* Reference error identifier for a status code. */
 const char* born_status_id(int status);
 
+/* Error-path testing: the n-th device allocation from now fails with
+ * BORN_ERR_OUT_OF_MEMORY (n = 0: the next one); n < 0 switches it off.
+ * Process-wide.  No counterpart in the reference; it lets tests force the
+ * allocation failures that the MEX reports as adjoint_quick_gpu:ErrorCUDA
+ * (combined_quick_gpu.h:31-44) and check that every entry point cleans up. */
+int born_debug_fail_allocation(int64_t n);
+/* Number of device allocations the library holds now (all plans and
+ * batches); with born_debug_fail_allocation it shows that a failed call
+ * released what it had allocated.  No counterpart in the reference. */
+int64_t born_debug_live_allocations(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -152,6 +152,19 @@ def test_argument_checks_and_error_identifiers(mex):
     assert mex.call([real_h, one, x]) == (None, "combined_quick_gpu:NoComplexDoubles")
 
 
+def test_missing_property_is_reported(mex):
+    """An operator object without a property the MEX reads (here the incident
+    waves) is rejected before any table is touched."""
+    _, T = mixed_tiny_tables()
+    op = operator_object(mex, T)
+    bare = mex.obj("scattering.operator_born", [{
+        "sequence": mex.L.mock_object(b"scattering.sequences.sequence", 1),
+        "indices_measurement_sel": mex.double(1.0)}])
+    x = mex.double(np.zeros((T.N_points, 1)))
+    assert mex.call([bare, mex.double(1.0), x]) == (None, "combined_quick_gpu:NoOperatorBorn")
+    assert mex.call([op, mex.double(1.0), x])[1] != "combined_quick_gpu:NoOperatorBorn"
+
+
 @pytest.mark.gpu
 def test_forward_adjoint_through_the_shim(mex, torch_cuda):
     """mode 1 = forward, any other value = adjoint (cu:404-409); real input is
