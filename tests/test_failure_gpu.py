@@ -117,3 +117,26 @@ def test_tgc_and_energy_paths(torch_cuda):
     assert O.rel_l2(E, O.energy_rx(T, 0, kernels[:first])) < TOL
     op.close()
     assert live() == base
+
+
+def test_batch_creation(torch_cuda):
+    """Batch creation over two uneven plans, then an eager forward through it."""
+    import torch
+    from comp_ui_toolbox_amd import OperatorBatch
+    s1 = tiny_spec(N_e=5, Nx=17, Nz=7, npp=3)
+    s2 = tiny_spec(N_e=5, Nx=21, Nz=4, npp=3)
+    T1, T2 = O.build_tables(s1), O.build_tables(s2)
+    base = live()
+    ops = [OperatorBorn.from_tables(T1), OperatorBorn.from_tables(T2)]
+    B, n = sweep(lambda: OperatorBatch(ops), strict=False)   # plans keep the workspaces built for the batch
+    assert n >= 1
+    g1, g2 = rnd(T1.N_points, 6), rnd(T2.N_points, 7)
+    xs = [torch.tensor(g.astype(np.complex64), device="cuda") for g in (g1, g2)]
+    us = B.forward_dev(xs)
+    torch.cuda.synchronize()
+    assert O.rel_l2(us[0].cpu().numpy(), O.forward(T1, g1.astype(np.complex64))) < TOL
+    assert O.rel_l2(us[1].cpu().numpy(), O.forward(T2, g2.astype(np.complex64))) < TOL
+    B.close()
+    for op in ops:
+        op.close()
+    assert live() == base
