@@ -33,7 +33,7 @@ EXPORTED = (
     "born_batch_set_graph", "born_batch_autotune", "born_plan_autotune", "born_batch_set_timing",
     "born_batch_get_timing", "born_tgc_set", "born_tgc_apply", "born_tgc_apply_dev",
     "born_energy_rx_tgc", "born_energy_rx_tgc_dev", "born_debug_fail_allocation",
-    "born_debug_live_allocations",
+    "born_debug_live_allocations", "born_columns", "born_columns_dev",
 )
 
 
@@ -125,6 +125,8 @@ def lib() -> C.CDLL:
     L.born_energy_rx_tgc.argtypes = [vp, C.c_int32, pd]
     L.born_energy_rx_tgc_dev.argtypes = [vp, C.c_int32, vp, vp]
     L.born_debug_fail_allocation.argtypes = [C.c_int64]
+    L.born_columns.argtypes = [vp, pd, C.c_int32, pd]
+    L.born_columns_dev.argtypes = [vp, pd, C.c_int32, vp, vp]
     L.born_debug_live_allocations.argtypes = []
     L.born_plan_path.argtypes = [vp]
     L.born_plan_force_general.argtypes = [vp, C.c_int32]

@@ -1696,6 +1696,51 @@ int born_time_signals(born_plan_t plan, const double* frequencies, double delta_
   });
 }
 
+// Columns of A at 1-based points; out complex FP64 column-major N_obs x n
+// (host) or complex64 n x N_obs (device); complete on return.
+static void columns(Plan& P, const double* points, int32_t n, void* out, bool host, hipStream_t s) {
+  if (!points || !out) throw Error(BORN_ERR_INVALID_ARGUMENT, "points and output are required");
+  if (n <= 0) throw Error(BORN_ERR_INVALID_ARGUMENT, "at least one point is required");
+  std::vector<int> pts(n);
+  for (int j = 0; j < n; ++j) pts[j] = idx1(points[j], P.N_pts, "points");
+  ensure_shift_on_device(P);
+  int* d_pts = dupload(pts, s);
+  void* d_out = nullptr;
+  try {
+    const size_t bytes = (size_t)n * P.N_obs * (host ? sizeof(double2) : sizeof(float2));
+    d_out = host ? (void*)dalloc<char>(bytes) : out;
+    launch_columns(P, d_pts, n, d_out, host, s);
+    HIP_CHECK(hipGetLastError());
+    if (host) HIP_CHECK(hipMemcpyAsync(out, d_out, bytes, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+  } catch (...) {
+    hipStreamSynchronize(s);
+    dfree(d_pts);
+    if (host) dfree(d_out);
+    throw;
+  }
+  dfree(d_pts);
+  if (host) dfree(d_out);
+}
+
+int born_columns(born_plan_t plan, const double* points, int32_t n, double* out) {
+  return guarded([&] {
+    if (!plan) throw Error(BORN_ERR_NO_OPERATOR_BORN, "operator_born must be a single scattering.operator_born!");
+    Plan& P = *reinterpret_cast<Plan*>(plan);
+    HIP_CHECK(hipSetDevice(P.device));
+    columns(P, points, n, out, true, P.stream);
+  });
+}
+
+int born_columns_dev(born_plan_t plan, const double* points, int32_t n, void* out_c64, void* stream) {
+  return guarded([&] {
+    if (!plan) throw Error(BORN_ERR_NO_OPERATOR_BORN, "operator_born must be a single scattering.operator_born!");
+    Plan& P = *reinterpret_cast<Plan*>(plan);
+    HIP_CHECK(hipSetDevice(P.device));
+    columns(P, points, n, out_c64, false, (hipStream_t)stream);
+  });
+}
+
 static void tgc_clear(Plan& P) {
   dfree(P.d_tgc_tiles);
   dfree(P.d_tgc_half);

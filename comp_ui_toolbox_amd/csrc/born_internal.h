@@ -231,6 +231,7 @@ void launch_time_signals(const Plan& P, const float2* u, const int* q_obs, int N
                          hipStream_t s);
 void launch_fwd_mix(const Plan& P, int N_rhs, void* u, bool out_f64, hipStream_t s);
 void launch_tgc(const Plan& P, bool adjoint, bool f64, const void* in, void* out, int N_rhs, hipStream_t s);
+void launch_columns(const Plan& P, const int* pts, int n, void* u, bool out_f64, hipStream_t s);
 // received energy under the plan's TGC; mixes of up to ETGC_LDS_ROWS rows keep
 // their rows in LDS, longer ones in `spill` (spill_blocks x rows x 64 float2)
 constexpr int ETGC_LDS_ROWS = 304;   // 304 rows x 64 points x 8 B = 152 KiB

@@ -231,6 +231,56 @@ void launch_fwd_mix(const Plan& P, int N_rhs, void* u, bool out_f64, hipStream_t
                        (float2*)u);
 }
 
+// ------------------------------------------------------------- columns
+// Columns of A: u[j][obs] = p_inc[pair][p_j] * sum_a pref[mk][f][a] *
+// h_ref[fg(pair)][shift(p_j, e_a)], the forward of the unit vector at p_j
+// (forward_quick_cpu with one occupied point, operator_born.m:428-452).
+// One thread per (column, observation); FP64 sums.
+template <typename OutT>
+__global__ void k_columns(const float2* __restrict__ h, const float2* __restrict__ Pt, const int* __restrict__ shift,
+                          const int* __restrict__ pair_fg, const int* __restrict__ obs_mk,
+                          const int* __restrict__ obs_f, const int* __restrict__ mk_meta,
+                          const int64_t* __restrict__ mk_off, const int* __restrict__ fmeas,
+                          const int* __restrict__ act, const double2* __restrict__ pref,
+                          const int* __restrict__ meas_pair_base, int64_t N_obs, int N_pts, int H_plane,
+                          const int* __restrict__ pts, int n, OutT* __restrict__ u) {
+  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (idx >= N_obs * n) return;
+  const int j = (int)(idx / N_obs);
+  const int64_t obs = idx - (int64_t)j * N_obs;
+  const int p = pts[j];
+  const int mk = obs_mk[obs], f = obs_f[obs];
+  const int m = mk_meta[mk * 4 + 0], Nf = mk_meta[mk * 4 + 1], Nact = mk_meta[mk * 4 + 2];
+  const int64_t f_off = mk_off[mk * 4 + 1], a_off = mk_off[mk * 4 + 2], p_off = mk_off[mk * 4 + 3];
+  const int pair = meas_pair_base[m] + fmeas[f_off + f];
+  const float2* hr = h + (size_t)pair_fg[pair] * H_plane;
+  double qx = 0.0, qy = 0.0;
+  for (int a = 0; a < Nact; ++a) {
+    const double2 c = pref[p_off + (int64_t)a * Nf + f];
+    const float2 hv = hr[shift[(size_t)act[a_off + a] * N_pts + p]];
+    qx += c.x * hv.x - c.y * hv.y;
+    qy += c.x * hv.y + c.y * hv.x;
+  }
+  const float2 pv = Pt[(size_t)pair * N_pts + p];
+  OutT o;
+  o.x = qx * pv.x - qy * pv.y;
+  o.y = qx * pv.y + qy * pv.x;
+  u[idx] = o;
+}
+
+void launch_columns(const Plan& P, const int* pts, int n, void* u, bool out_f64, hipStream_t s) {
+  const int64_t total = P.N_obs * n;
+  const dim3 grid((unsigned)((total + 255) / 256)), block(256);
+  if (out_f64)
+    hipLaunchKernelGGL(k_columns<double2>, grid, block, 0, s, P.d_h, P.d_p, P.d_shift, P.d_pair_fg, P.d_obs_mk,
+                       P.d_obs_f, P.d_mk_meta, P.d_mk_off, P.d_fmeas, P.d_act, P.d_pref, P.d_meas_pair_base, P.N_obs,
+                       P.N_pts, P.H_plane, pts, n, (double2*)u);
+  else
+    hipLaunchKernelGGL(k_columns<float2>, grid, block, 0, s, P.d_h, P.d_p, P.d_shift, P.d_pair_fg, P.d_obs_mk,
+                       P.d_obs_f, P.d_mk_meta, P.d_mk_off, P.d_fmeas, P.d_act, P.d_pref, P.d_meas_pair_base, P.N_obs,
+                       P.N_pts, P.H_plane, pts, n, (float2*)u);
+}
+
 // ------------------------------------------------------------- un-mixing
 // W[pair][e][r] = sum over the (mk, f) reading this pair and the active
 // slot a of element e: conj(pref[mk][f][a]) * u[r][obs(mk, f)], walked from

@@ -273,6 +273,29 @@ class OperatorBorn:
     def _n_mix_signals(self) -> int:
         return int(self.meta["n_mix_signals"])
 
+    # ---------------------------------------------------------- columns
+    def columns(self, points) -> np.ndarray:
+        """A[:, points] (0-based grid points), complex128 (N_obs, n): the
+        forward of the unit vectors at those points, gathered directly
+        (born_columns)."""
+        pts = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1) + 1.0)
+        out = np.empty((len(pts), self.N_observations), dtype=np.complex128)
+        check(_lib.lib().born_columns(self._plan, dptr(pts), len(pts), dptr(out.view(np.float64))))
+        return np.ascontiguousarray(out.T)
+
+    def columns_dev(self, points, out=None, stream=None):
+        """Device variant: complex64 tensor (n, N_obs), row j = A[:, points[j]]."""
+        import torch
+        pts = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1) + 1.0)
+        if out is None:
+            out = torch.empty((len(pts), self.N_observations), dtype=torch.complex64, device=f"cuda:{self.device}")
+        self._check_dev(out, self.N_observations)
+        if out.numel() != len(pts) * self.N_observations:
+            raise BornError(7, "combined_quick_gpu:NoValidDoubleMatrix", "output must hold n x N_obs entries")
+        s = torch.cuda.current_stream(out.device).cuda_stream if stream is None else stream
+        check(_lib.lib().born_columns_dev(self._plan, dptr(pts), len(pts), C.c_void_p(out.data_ptr()), C.c_void_p(s)))
+        return out
+
     # ---------------------------------------------------------- TGC
     def set_tgc(self, kernels=None):
         """Time gain compensation applied by op_A(..., tgc=True): one odd-length

@@ -194,6 +194,17 @@ int born_time_signals(born_plan_t plan, const double* frequencies, double delta_
                       const void* u_c64, int32_t N_rhs, void* rf_f32, int64_t capacity, int32_t* N_t,
                       void* stream);
 
+/* Columns of A at the given grid points: the forward of unit vectors, which
+ * the solvers use to extract columns (omp.m:177-184; cosamp.m:205-223
+ * applies the operator to batches of up to 1024 unit vectors).  Gathered
+ * from the tables directly, O(N_observations) per column, instead of one
+ * full forward per unit vector.  points: n 1-based doubles.  Host variant:
+ * out complex FP64 N_obs x n, column-major.  Device variant: complex64
+ * n x N_obs on the device (the layout of born_forward_dev), complete on
+ * return.  No counterpart in the MEX. */
+int born_columns(born_plan_t plan, const double* points, int32_t n, double* out);
+int born_columns_dev(born_plan_t plan, const double* points, int32_t n, void* out_c64, void* stream);
+
 /* Time gain compensation (TGC), the linear transform forward_scalar applies
  * to u after the operator and adjoint_scalar applies before it
  * (operator_born.m:222-236, :263-277).  regularization.tgc.exponential

@@ -590,3 +590,49 @@ def test_many_rhs(torch_cuda, precision, tol, n_rhs):
     e_ad = O.rel_l2(op.adjoint_dev(ud).cpu().numpy().T, O.adjoint(T, u.astype(np.complex64)))
     log("%d rhs %s: forward %.2e adjoint %.2e (device %.2e %.2e)" % (n_rhs, precision, e_f, e_a, e_fd, e_ad))
     assert max(e_f, e_a, e_fd, e_ad) < tol
+
+
+# ---------------------------------------------------------------- columns
+def test_columns_tiny(torch_cuda):
+    """Columns of A (born_columns): against the assembled matrix, between
+    the host and device entry points, and against the dense forward of the
+    unit vectors; a repeated point is allowed, out-of-range points are not."""
+    _, T = mixed_tiny_tables()
+    op = OperatorBorn.from_tables(T)
+    A = O.assemble_matrix(T)
+    pts = np.array([0, 5, T.N_points - 1, 5])
+    e_h = O.rel_l2(op.columns(pts), A[:, pts])
+    e_d = O.rel_l2(op.columns_dev(pts).cpu().numpy().T, A[:, pts])
+    E = np.zeros((T.N_points, len(pts)))
+    E[pts, np.arange(len(pts))] = 1.0
+    e_f = O.rel_l2(op.columns(pts), op.forward(E))
+    log("columns (tiny): host %.2e device %.2e vs dense forward %.2e" % (e_h, e_d, e_f))
+    assert max(e_h, e_d, e_f) < TOL_FP32
+    with pytest.raises(BornError):
+        op.columns([T.N_points])
+    with pytest.raises(BornError):
+        op.columns([])
+
+
+def test_columns_config1(torch_cuda, c1):
+    """Config 1: 16 columns against tables restricted exactly to those
+    points; 1024 columns (a CoSaMP batch) timed on the device."""
+    import torch
+    spec, op = c1
+    rng = np.random.Generator(np.random.PCG64(91))
+    pts = np.sort(rng.choice(spec.fov.N_points, 16, replace=False))
+    R = O.restricted_tables(spec, pts)
+    e = O.rel_l2(op.columns(pts), O.forward(R, np.eye(len(pts))))
+    big = rng.choice(spec.fov.N_points, 1024, replace=False)
+    out = torch.empty((1024, op.N_observations), dtype=torch.complex64, device="cuda")
+    op.columns_dev(big, out=out)
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record()
+    op.columns_dev(big, out=out)
+    t1.record()
+    torch.cuda.synchronize()
+    log("columns (config 1): 16 columns rel-L2 %.2e; 1024 columns in %.2f ms" % (e, t0.elapsed_time(t1)))
+    assert e < TOL_FP32
+    assert O.rel_l2(out[3].cpu().numpy(), op.columns(big[3:4])[:, 0]) < 1e-6
