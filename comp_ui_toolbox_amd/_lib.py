@@ -34,6 +34,7 @@ EXPORTED = (
     "born_batch_get_timing", "born_tgc_set", "born_tgc_apply", "born_tgc_apply_dev",
     "born_energy_rx_tgc", "born_energy_rx_tgc_dev", "born_debug_fail_allocation",
     "born_debug_live_allocations", "born_columns", "born_columns_dev",
+    "born_plan_set_sparse_forward",
 )
 
 
@@ -126,6 +127,7 @@ def lib() -> C.CDLL:
     L.born_energy_rx_tgc_dev.argtypes = [vp, C.c_int32, vp, vp]
     L.born_debug_fail_allocation.argtypes = [C.c_int64]
     L.born_columns.argtypes = [vp, pd, C.c_int32, pd]
+    L.born_plan_set_sparse_forward.argtypes = [vp, C.c_int32]
     L.born_columns_dev.argtypes = [vp, pd, C.c_int32, vp, vp]
     L.born_debug_live_allocations.argtypes = []
     L.born_plan_path.argtypes = [vp]

@@ -1096,6 +1096,59 @@ static void check_precision(const Plan& P) {
                 "split precision needs the structured path with 2 or 4 grid points per pitch and 17 to 128 elements per x row");
 }
 
+// Gather path or contraction for a host forward with n_occ occupied points:
+// the gather costs about n_occ * sum over mixes of (N_f * N_active) table
+// reads, the contraction about N_pairs * N_e * N_pts multiply-adds; the
+// gather is taken when clearly cheaper (factor 512), or by the plan's limit.
+static bool use_sparse_forward(const Plan& P, size_t n_occ) {
+  if (P.sparse_max == 0) return false;
+  if (P.sparse_max > 0) return n_occ <= (size_t)P.sparse_max;
+  double gather = 0.0;
+  for (int mk = 0; mk < P.N_mk; ++mk) gather += (double)P.mk_Nf[mk] * P.mk_Nact[mk];
+  return (double)n_occ * gather * 512.0 <= (double)P.N_pairs * P.N_e * P.N_pts;
+}
+
+// Host forward through k_fwd_sparse: the occupied points (the MEX rule on
+// the doubles, cu:466-488) with their FP64 values, FP64 sums over FP32 tables.
+static void host_forward_sparse(Plan& P, const double* gamma, int N_rhs, size_t n_occ, double* u) {
+  std::vector<int> pts;
+  pts.reserve(n_occ);
+  for (int p = 0; p < P.N_pts; ++p) {
+    bool occ = false;
+    for (int r = 0; r < N_rhs && !occ; ++r) {
+      const double* d = gamma + 2 * ((size_t)r * P.N_pts + p);
+      occ = std::fabs(d[0]) > 2.220446049250313080847e-16 || std::fabs(d[1]) > 2.220446049250313080847e-16;
+    }
+    if (occ) pts.push_back(p);
+  }
+  const int n = (int)pts.size();
+  std::vector<double2> g((size_t)n * N_rhs);
+  for (int r = 0; r < N_rhs; ++r)
+    for (int j = 0; j < n; ++j) {
+      const double* d = gamma + 2 * ((size_t)r * P.N_pts + pts[j]);
+      g[(size_t)r * n + j] = make_double2(d[0], d[1]);
+    }
+  const size_t nout = (size_t)P.N_obs * N_rhs;
+  ensure_shift_on_device(P);
+  ensure_io(P, 0, nout * sizeof(double2));
+  int* d_pts = dupload(pts, P.stream);
+  double2* d_g = nullptr;
+  try {
+    d_g = dupload(g, P.stream);
+    launch_fwd_sparse(P, d_pts, n, d_g, N_rhs, (double2*)P.d_io_out, P.stream);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(u, P.d_io_out, nout * sizeof(double2), hipMemcpyDeviceToHost, P.stream));
+    HIP_CHECK(hipStreamSynchronize(P.stream));
+  } catch (...) {
+    hipStreamSynchronize(P.stream);
+    dfree(d_pts);
+    dfree(d_g);
+    throw;
+  }
+  dfree(d_pts);
+  dfree(d_g);
+}
+
 static void host_forward(Plan& P, const double* gamma, int N_rhs, double* u) {
   if (!gamma || !u) throw Error(BORN_ERR_NO_VALID_DOUBLE_MATRIX, "input must be a double matrix with N_points rows!");
   if (N_rhs <= 0) throw Error(BORN_ERR_NO_VALID_DOUBLE_MATRIX, "input must be a nonempty double matrix!");
@@ -1120,6 +1173,10 @@ static void host_forward(Plan& P, const double* gamma, int N_rhs, double* u) {
   const size_t nout = (size_t)P.N_obs * N_rhs;
   if (n_occ == 0) {  // quick exit (cu:496): zeros
     std::memset(u, 0, sizeof(double) * 2 * nout);
+    return;
+  }
+  if (use_sparse_forward(P, n_occ)) {
+    host_forward_sparse(P, gamma, N_rhs, n_occ, u);
     return;
   }
   ensure_workspace(P, N_rhs);
@@ -1884,6 +1941,13 @@ int born_plan_get_table(born_plan_t plan, int32_t which, int32_t index, void* ou
 int born_plan_path(born_plan_t plan) {
   if (!plan) return -1;
   return use_structured(*reinterpret_cast<Plan*>(plan)) ? 0 : 1;
+}
+
+int born_plan_set_sparse_forward(born_plan_t plan, int32_t max_points) {
+  return guarded([&] {
+    if (!plan) throw Error(BORN_ERR_NO_OPERATOR_BORN, "operator_born must be a single scattering.operator_born!");
+    reinterpret_cast<Plan*>(plan)->sparse_max = max_points < 0 ? -1 : max_points;
+  });
 }
 
 int born_plan_force_general(born_plan_t plan, int32_t on) {

@@ -189,6 +189,9 @@ struct Plan {
   float2* d_fvpart = nullptr;     // [n_fchunks][fvblk][Ne_pad][NC]
   int fvblk = 1;
   LaunchSet ls_fwd;               // forward launch over the expanded columns
+  // host forward: the gather path for at most sparse_max occupied points
+  // (-1: the cost rule, 0: never)
+  int sparse_max = -1;
   // time gain compensation: per mix the half length M (after the cut-off
   // truncation) and the offset of its 2M + 1 taps; tiles (mix, first row) of
   // 256 outputs
@@ -232,6 +235,7 @@ void launch_time_signals(const Plan& P, const float2* u, const int* q_obs, int N
 void launch_fwd_mix(const Plan& P, int N_rhs, void* u, bool out_f64, hipStream_t s);
 void launch_tgc(const Plan& P, bool adjoint, bool f64, const void* in, void* out, int N_rhs, hipStream_t s);
 void launch_columns(const Plan& P, const int* pts, int n, void* u, bool out_f64, hipStream_t s);
+void launch_fwd_sparse(const Plan& P, const int* pts, int n, const double2* g, int N_rhs, double2* u, hipStream_t s);
 // received energy under the plan's TGC; mixes of up to ETGC_LDS_ROWS rows keep
 // their rows in LDS, longer ones in `spill` (spill_blocks x rows x 64 float2)
 constexpr int ETGC_LDS_ROWS = 304;   // 304 rows x 64 points x 8 B = 152 KiB

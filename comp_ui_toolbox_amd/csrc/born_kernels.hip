@@ -281,6 +281,55 @@ void launch_columns(const Plan& P, const int* pts, int n, void* u, bool out_f64,
                        P.N_pts, P.H_plane, pts, n, (float2*)u);
 }
 
+// ------------------------------------------------------------- sparse forward
+// u[r][obs] = sum_j A[obs, p_j] g[r][j] over the n occupied points p_j: the
+// forward with the MEX's occupancy compaction (cu:454-499) as a gather, for
+// inputs with few occupied points.  One thread per (rhs, observation).
+__global__ void k_fwd_sparse(const float2* __restrict__ h, const float2* __restrict__ Pt,
+                             const int* __restrict__ shift, const int* __restrict__ pair_fg,
+                             const int* __restrict__ obs_mk, const int* __restrict__ obs_f,
+                             const int* __restrict__ mk_meta, const int64_t* __restrict__ mk_off,
+                             const int* __restrict__ fmeas, const int* __restrict__ act,
+                             const double2* __restrict__ pref, const int* __restrict__ meas_pair_base,
+                             int64_t N_obs, int N_pts, int H_plane, const int* __restrict__ pts, int n,
+                             const double2* __restrict__ g, int N_rhs, double2* __restrict__ u) {
+  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (idx >= N_obs * N_rhs) return;
+  const int r = (int)(idx / N_obs);
+  const int64_t obs = idx - (int64_t)r * N_obs;
+  const int mk = obs_mk[obs], f = obs_f[obs];
+  const int m = mk_meta[mk * 4 + 0], Nf = mk_meta[mk * 4 + 1], Nact = mk_meta[mk * 4 + 2];
+  const int64_t f_off = mk_off[mk * 4 + 1], a_off = mk_off[mk * 4 + 2], p_off = mk_off[mk * 4 + 3];
+  const int pair = meas_pair_base[m] + fmeas[f_off + f];
+  const float2* hr = h + (size_t)pair_fg[pair] * H_plane;
+  const float2* pr = Pt + (size_t)pair * N_pts;
+  const double2* gr = g + (size_t)r * n;
+  double sx = 0.0, sy = 0.0;
+  for (int j = 0; j < n; ++j) {
+    const int p = pts[j];
+    double qx = 0.0, qy = 0.0;
+    for (int a = 0; a < Nact; ++a) {
+      const double2 c = pref[p_off + (int64_t)a * Nf + f];
+      const float2 hv = hr[shift[(size_t)act[a_off + a] * N_pts + p]];
+      qx += c.x * hv.x - c.y * hv.y;
+      qy += c.x * hv.y + c.y * hv.x;
+    }
+    const float2 pv = pr[p];
+    const double ax = qx * pv.x - qy * pv.y, ay = qx * pv.y + qy * pv.x;
+    const double2 gv = gr[j];
+    sx += ax * gv.x - ay * gv.y;
+    sy += ax * gv.y + ay * gv.x;
+  }
+  u[idx] = make_double2(sx, sy);
+}
+
+void launch_fwd_sparse(const Plan& P, const int* pts, int n, const double2* g, int N_rhs, double2* u, hipStream_t s) {
+  const int64_t total = P.N_obs * N_rhs;
+  hipLaunchKernelGGL(k_fwd_sparse, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, P.d_h, P.d_p, P.d_shift,
+                     P.d_pair_fg, P.d_obs_mk, P.d_obs_f, P.d_mk_meta, P.d_mk_off, P.d_fmeas, P.d_act, P.d_pref,
+                     P.d_meas_pair_base, P.N_obs, P.N_pts, P.H_plane, pts, n, g, N_rhs, u);
+}
+
 // ------------------------------------------------------------- un-mixing
 // W[pair][e][r] = sum over the (mk, f) reading this pair and the active
 // slot a of element e: conj(pref[mk][f][a]) * u[r][obs(mk, f)], walked from

@@ -194,6 +194,12 @@ class OperatorBorn:
     def path(self) -> str:
         return {0: "structured", 1: "general"}.get(_lib.lib().born_plan_path(self._plan), "invalid")
 
+    def set_sparse_forward(self, max_points: int = -1):
+        """Host forward with few occupied points as gathers (the MEX's
+        occupancy compaction): -1 by estimated cost, 0 never, k > 0 whenever
+        at most k points are occupied."""
+        check(_lib.lib().born_plan_set_sparse_forward(self._plan, int(max_points)))
+
     def force_general(self, on: bool = True):
         check(_lib.lib().born_plan_force_general(self._plan, int(on)))
 

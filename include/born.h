@@ -194,6 +194,15 @@ int born_time_signals(born_plan_t plan, const double* frequencies, double delta_
                       const void* u_c64, int32_t N_rhs, void* rf_f32, int64_t capacity, int32_t* N_t,
                       void* stream);
 
+/* Forward with few occupied points (host entry point born_forward): the MEX
+ * compacts the input to its occupied points (cu:454-499); with few of them
+ * the plan computes u = sum_j A[:, p_j] gamma_j by gathers from the tables
+ * (FP64 sums) instead of the full contraction.  max_points < 0: the plan
+ * decides by estimated cost (default); 0: never; k > 0: whenever at most k
+ * points are occupied.  Results agree with the contraction within its
+ * tolerance.  No counterpart in the MEX. */
+int born_plan_set_sparse_forward(born_plan_t plan, int32_t max_points);
+
 /* Columns of A at the given grid points: the forward of unit vectors, which
  * the solvers use to extract columns (omp.m:177-184; cosamp.m:205-223
  * applies the operator to batches of up to 1024 unit vectors).  Gathered

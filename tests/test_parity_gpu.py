@@ -636,3 +636,52 @@ def test_columns_config1(torch_cuda, c1):
     log("columns (config 1): 16 columns rel-L2 %.2e; 1024 columns in %.2f ms" % (e, t0.elapsed_time(t1)))
     assert e < TOL_FP32
     assert O.rel_l2(out[3].cpu().numpy(), op.columns(big[3:4])[:, 0]) < 1e-6
+
+
+# ---------------------------------------------------------------- sparse forward
+def test_sparse_forward_paths_agree(torch_cuda):
+    """Host forward with few occupied points: the gather path (forced) and
+    the contraction (forced off) against the oracle, two right-hand sides
+    with different occupied points (the union is occupied, cu:466-488), an
+    entry below DBL_EPSILON that does not occupy its point, and zero input."""
+    _, T = mixed_tiny_tables()
+    op = OperatorBorn.from_tables(T)
+    g = np.zeros((T.N_points, 2), dtype=np.complex128)
+    g[[3, 17], 0] = [1.5 - 0.5j, -2.0]
+    g[[17, 40], 1] = [0.25j, 1.0]
+    g[50, 1] = 1e-17                      # not occupied
+    ref = O.forward(T, g)
+    op.set_sparse_forward(1000)
+    u_s = op.forward(g)
+    op.set_sparse_forward(0)
+    u_d = op.forward(g)
+    op.set_sparse_forward(-1)
+    e_s, e_d = O.rel_l2(u_s, ref), O.rel_l2(u_d, ref)
+    log("sparse forward: gather %.2e contraction %.2e" % (e_s, e_d))
+    assert e_s < 1e-6 and e_d < TOL_FP32
+    op.set_sparse_forward(1000)
+    assert np.all(op.forward(np.zeros((T.N_points, 1))) == 0)
+
+
+def test_sparse_forward_config0(torch_cuda, c0_tables):
+    """Config 0's input (30 occupied points) takes the gather path by the
+    cost rule; it agrees with the contraction and the oracle, and is faster."""
+    import time
+    spec = C.config0()
+    T = c0_tables
+    op = OperatorBorn.from_tables(T)
+    g = C.phantom_points(spec)
+    ref = O.forward(T, g)
+    times = {}
+    outs = {}
+    for mode, lim in (("gather", -1), ("contraction", 0)):
+        op.set_sparse_forward(lim)
+        op.forward(g)
+        t0 = time.perf_counter()
+        for _ in range(5):
+            outs[mode] = op.forward(g)
+        times[mode] = (time.perf_counter() - t0) / 5 * 1e3
+    e_g, e_c = O.rel_l2(outs["gather"], ref), O.rel_l2(outs["contraction"], ref)
+    log("config0 forward, 30 points: gather %.2e in %.2f ms, contraction %.2e in %.2f ms (host entry)"
+        % (e_g, times["gather"], e_c, times["contraction"]))
+    assert e_g < 1e-6 and e_c < TOL_FP32
