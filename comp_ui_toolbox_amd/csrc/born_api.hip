@@ -1758,6 +1758,7 @@ int born_time_signals(born_plan_t plan, const double* frequencies, double delta_
 static void columns(Plan& P, const double* points, int32_t n, void* out, bool host, hipStream_t s) {
   if (!points || !out) throw Error(BORN_ERR_INVALID_ARGUMENT, "points and output are required");
   if (n <= 0) throw Error(BORN_ERR_INVALID_ARGUMENT, "at least one point is required");
+  if ((double)n * (double)P.N_obs > 2.0e11) throw Error(BORN_ERR_INVALID_ARGUMENT, "too many columns for one call");
   std::vector<int> pts(n);
   for (int j = 0; j < n; ++j) pts[j] = idx1(points[j], P.N_pts, "points");
   ensure_shift_on_device(P);
