@@ -685,3 +685,24 @@ def test_sparse_forward_config0(torch_cuda, c0_tables):
     log("config0 forward, 30 points: gather %.2e in %.2f ms, contraction %.2e in %.2f ms (host entry)"
         % (e_g, times["gather"], e_c, times["contraction"]))
     assert e_g < 1e-6 and e_c < TOL_FP32
+
+
+def test_columns_and_sparse_forward_3d(torch_cuda):
+    """3-D plan: columns and the gather forward against the direct oracle tables."""
+    arr = C.ArraySpec((8, 4), (3e-4, 3e-4), (2.7e-4, 2.7e-4), (2, 2))
+    fov = C.FovSpec((10, 7, 3), (1.5e-4, 1.5e-4, 1.5e-4), 2e-3)
+    waves = [C.qsw_wave(arr, np.array([x, 0.0, -1e-2]), 1500.0) for x in (0.0, 6e-4)]
+    spec = C.SequenceSpec("t3c", arr, C.MediumSpec(), fov, np.array([2.5e6, 4e6, 5.5e6]), waves,
+                          [C.identity_mixes(32) for _ in waves])
+    op = OperatorBorn.from_sequence(spec)
+    T = O.direct_tables(spec)
+    pts = np.array([0, 37, 123, spec.fov.N_points - 1])
+    E = np.zeros((T.N_points, len(pts)))
+    E[pts, np.arange(len(pts))] = 1.0
+    e_c = O.rel_l2(op.columns(pts), O.forward(T, E))
+    g = np.zeros(T.N_points, dtype=np.complex128)
+    g[pts] = [1.0, -0.5j, 2.0, 0.25]
+    op.set_sparse_forward(100)
+    e_s = O.rel_l2(op.forward(g), O.forward(T, g))
+    log("3-D columns %.2e, gather forward %.2e" % (e_c, e_s))
+    assert e_c < TOL_FP32 and e_s < TOL_FP32
