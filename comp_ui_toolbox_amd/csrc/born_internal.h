@@ -166,7 +166,7 @@ struct Plan {
   float2* d_W = nullptr;          // [N_pairs][N_e][N_rhs] (general path)
   float2* d_Wc = nullptr;         // [n_chunks][Ne_pad][NC] (structured path; unused slots stay zero)
   float2* d_gm = nullptr;         // masked gamma [N_rhs][N_pts]
-  int items_target = 4096;        // forward work items per launch (tunable)
+  int items_target = 8192;        // forward work items per launch (tunable)
   LaunchSet ls;                   // structured launch of this plan alone
   // FP16 split form: operand scales [5], table maxima [2] (h_ref, p_inc),
   // block partial maxima [256]
