@@ -246,12 +246,53 @@ def qsw_wave(array: ArraySpec, position_src: np.ndarray, c_avg: float,
 
 
 def random_wave(array: ArraySpec, rng: np.random.Generator, t_max: float) -> WaveSpec:
-    """Random apodization (+-1, Bernoulli 0.5) and random delays U[0, t_max]
-    (rad.m:72-117)."""
+    """The config-2 waves: random apodization +-1 (Bernoulli 0.5, as ra.m and
+    rad.m draw it) and delays drawn U[0, t_max].  The toolbox's own rad
+    synthesis permutes a linear delay ramp instead (rad_wave)."""
     n = array.N_elements
     apod = np.where(rng.random(n) < 0.5, -1.0, 1.0)
     tau = rng.uniform(0.0, t_max, n)
-    return WaveSpec(np.arange(n), apod, tau, label="rad")
+    return WaveSpec(np.arange(n), apod, tau, label="random")
+
+
+def _random_apodization(n: int, rng: np.random.Generator) -> np.ndarray:
+    # rand >= 0.5 -> +1, else -1 (ra.m, rad.m compute_delays_scalar)
+    return np.where(rng.random(n) >= 0.5, 1.0, -1.0)
+
+
+def _permuted_ramp(array: ArraySpec, rng: np.random.Generator, e_theta_lateral, c_avg: float) -> np.ndarray:
+    # t_shift_max = sum over lateral axes with more than one element of
+    # (N_axis - 1) * pitch_axis * |e_theta component| / c_avg; the delays are a
+    # random permutation of k * t_shift_max / (N_e - 1), k = 0..N_e-1
+    # (rd.m, rad.m compute_delays_scalar)
+    e = np.abs(np.atleast_1d(np.asarray(e_theta_lateral, dtype=np.float64)))
+    t_max = sum((na - 1) * pa * e[d] for d, (na, pa) in enumerate(zip(array.N_elements_axis, array.pitch_axis))
+                if na > 1) / c_avg
+    n = array.N_elements
+    t_inc = t_max / (n - 1) if n > 1 else 0.0
+    return rng.permutation(n).astype(np.float64) * t_inc
+
+
+def ra_wave(array: ArraySpec, rng: np.random.Generator) -> WaveSpec:
+    """Random apodization, no delays (syntheses/random/ra.m)."""
+    n = array.N_elements
+    return WaveSpec(np.arange(n), _random_apodization(n, rng), np.zeros(n), label="ra")
+
+
+def rd_wave(array: ArraySpec, rng: np.random.Generator, e_theta_lateral, c_avg: float) -> WaveSpec:
+    """Unit apodization, randomly permuted delay ramp (syntheses/random/rd.m);
+    e_theta_lateral: the lateral components of the steering direction."""
+    n = array.N_elements
+    return WaveSpec(np.arange(n), np.ones(n), _permuted_ramp(array, rng, e_theta_lateral, c_avg), label="rd")
+
+
+def rad_wave(array: ArraySpec, rng_apo: np.random.Generator, rng_del: np.random.Generator, e_theta_lateral,
+             c_avg: float) -> WaveSpec:
+    """Random apodization and randomly permuted delay ramp, from two
+    generators (syntheses/random/rad.m)."""
+    n = array.N_elements
+    return WaveSpec(np.arange(n), _random_apodization(n, rng_apo),
+                    _permuted_ramp(array, rng_del, e_theta_lateral, c_avg), label="rad")
 
 
 def identity_mixes(N_elements: int) -> List[MixSpec]:

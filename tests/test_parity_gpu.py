@@ -706,3 +706,21 @@ def test_columns_and_sparse_forward_3d(torch_cuda):
     e_s = O.rel_l2(op.forward(g), O.forward(T, g))
     log("3-D columns %.2e, gather forward %.2e" % (e_c, e_s))
     assert e_c < TOL_FP32 and e_s < TOL_FP32
+
+
+def test_random_syntheses_plan(torch_cuda):
+    """Plans built on the device from ra / rd / rad waves (syntheses/random)
+    against the oracle's tables."""
+    spec = tiny_spec(N_e=6, Nx=24, Nz=5, npp=4)
+    arr, c = spec.array, spec.medium.c_avg
+    spec.waves = [C.ra_wave(arr, np.random.default_rng(11)),
+                  C.rd_wave(arr, np.random.default_rng(12), [math.sin(0.25)], c),
+                  C.rad_wave(arr, np.random.default_rng(13), np.random.default_rng(14), [math.sin(-0.1)], c)]
+    spec.mixes = [C.identity_mixes(arr.N_elements) for _ in spec.waves]
+    T = O.build_tables(spec)
+    op = OperatorBorn.from_sequence(spec)
+    g = rnd((T.N_points, 2), 121)
+    u = rnd((T.N_observations(), 2), 122)
+    e_f, e_a = O.rel_l2(op.forward(g), O.forward(T, g)), O.rel_l2(op.adjoint(u), O.adjoint(T, u))
+    log("ra/rd/rad plan: forward %.2e adjoint %.2e" % (e_f, e_a))
+    assert e_f < TOL_FP32 and e_a < TOL_FP32
