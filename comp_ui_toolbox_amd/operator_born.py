@@ -444,18 +444,24 @@ class OperatorBorn:
                             f"expected a contiguous complex64 device tensor with last dimension {rows}")
 
     # ---------------------------------------------------------- solver contract
-    def op_A(self, x, mode: int, tgc: bool = False):
+    def op_A(self, x, mode: int, tgc: bool = False, dictionary=None):
         """operator.combined_scalar: mode 0 size, 1 forward, 2 adjoint
         (operator.m:1019-1072).  tgc=True applies the kernels of set_tgc
-        after the forward and their adjoint before the adjoint
-        (forward_scalar / adjoint_scalar, operator_born.m:222-236, :263-277)."""
+        after the forward and their adjoint before the adjoint; a dictionary
+        (forward_transform / adjoint_transform, e.g. wavelet.D4Dictionary)
+        synthesizes gamma = Psi^T x before the forward and analyses the
+        adjoint's output (forward_scalar / adjoint_scalar,
+        operator_born.m:190-236, :263-306).  Mode 0 returns the operator's
+        own size, as the reference does (operator.m:1044-1045)."""
         if mode == 0:
             return list(self.size)
         if mode == 1:
-            u = self.forward(x)
+            g = dictionary.adjoint_transform(x) if dictionary is not None else x
+            u = self.forward(g)
             return self.tgc_forward(u) if tgc else u
         if mode == 2:
-            return self.adjoint(self.tgc_adjoint(x) if tgc else x)
+            g = self.adjoint(self.tgc_adjoint(x) if tgc else x)
+            return dictionary.forward_transform(g) if dictionary is not None else g
         raise BornError(6, "combined_scalar:InvalidMode", "mode must be 0, 1 or 2")
 
 

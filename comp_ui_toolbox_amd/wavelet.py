@@ -92,3 +92,48 @@ def idwt2(coef: torch.Tensor, levels: int) -> torch.Tensor:
         blk = _synthesis_last(blk)
         out = torch.cat([torch.cat([blk, out[..., :n0, n1:]], dim=-1), out[..., n0:, :]], dim=-2)
     return out
+
+
+class D4Dictionary:
+    """The orthogonal D4 wavelet as the dictionary of op_A, with the
+    linear_transforms interface forward_scalar / adjoint_scalar use
+    (operator_born.m:190-200, :300-306): forward_transform = analysis Psi,
+    adjoint_transform = synthesis Psi^T.  It acts on FOV vectors of a 2-D grid
+    (N_x, N_z), points x fastest, as (N_pts,) or (N_pts, N_rhs) numpy arrays
+    (real or complex; real and imaginary parts are transformed separately)
+    or torch tensors, on `device`."""
+
+    def __init__(self, grid_shape, levels=None, device="cuda"):
+        if len(grid_shape) != 2:
+            raise ValueError("the D4 dictionary acts on 2-D grids")
+        self.nx, self.nz = int(grid_shape[0]), int(grid_shape[1])
+        self.levels = max_levels((self.nz, self.nx)) if levels is None else int(levels)
+        if self.levels > max_levels((self.nz, self.nx)):
+            raise ValueError("too many levels for this grid")
+        self.device = device
+
+    def _apply(self, x, fn):
+        import numpy as np
+        is_np = not isinstance(x, torch.Tensor)
+        t = torch.as_tensor(np.asarray(x) if is_np else x)
+        squeeze = t.dim() == 1
+        cols = t.reshape(-1, 1) if squeeze else t
+        if cols.shape[0] != self.nx * self.nz:
+            raise ValueError(f"expected {self.nx * self.nz} rows")
+        dev = cols.device if not is_np else torch.device(self.device)
+        imgs = cols.T.reshape(-1, self.nz, self.nx).to(dev)
+        if torch.is_complex(imgs):
+            out = torch.complex(fn(imgs.real.double(), self.levels), fn(imgs.imag.double(), self.levels))
+        else:
+            out = fn(imgs.double(), self.levels)
+        out = out.reshape(-1, self.nx * self.nz).T
+        out = out.reshape(-1) if squeeze else out
+        return out.cpu().numpy() if is_np else out
+
+    def forward_transform(self, x):
+        """Psi x (analysis)."""
+        return self._apply(x, dwt2)
+
+    def adjoint_transform(self, x):
+        """Psi^T x (synthesis, = Psi^-1 x)."""
+        return self._apply(x, idwt2)

@@ -724,3 +724,25 @@ def test_random_syntheses_plan(torch_cuda):
     e_f, e_a = O.rel_l2(op.forward(g), O.forward(T, g)), O.rel_l2(op.adjoint(u), O.adjoint(T, u))
     log("ra/rd/rad plan: forward %.2e adjoint %.2e" % (e_f, e_a))
     assert e_f < TOL_FP32 and e_a < TOL_FP32
+
+
+def test_op_A_with_dictionary_and_tgc(torch_cuda):
+    """op_A(theta, 1, dictionary=D4) = A Psi^T theta and op_A(y, 2) =
+    Psi A^H y (forward_scalar / adjoint_scalar), with and without TGC; the
+    pair stays adjoint."""
+    from comp_ui_toolbox_amd.wavelet import D4Dictionary
+    spec = tiny_spec(N_e=4, Nx=16, Nz=8, npp=4)
+    T = O.build_tables(spec)
+    op = OperatorBorn.from_tables(T)
+    D = D4Dictionary((16, 8))
+    theta = rnd(T.N_points, 131)
+    y = rnd(T.N_observations(), 132)
+    e_f = O.rel_l2(op.op_A(theta, 1, dictionary=D), O.forward(T, D.adjoint_transform(theta)))
+    e_a = O.rel_l2(op.op_A(y, 2, dictionary=D), D.forward_transform(O.adjoint(T, y)))
+    op.set_tgc([np.array([0.1, 1.0, 0.1])] * len(op.meta["mix_n_obs"]))
+    for tgc in (False, True):
+        lhs = np.vdot(y, op.op_A(theta, 1, tgc=tgc, dictionary=D))
+        rhs = np.vdot(op.op_A(y, 2, tgc=tgc, dictionary=D), theta)
+        assert abs(lhs - rhs) < TOL_FP32 * abs(lhs)
+    log("op_A with D4 dictionary: forward %.2e adjoint %.2e" % (e_f, e_a))
+    assert e_f < TOL_FP32 and e_a < TOL_FP32

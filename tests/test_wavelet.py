@@ -43,3 +43,20 @@ def test_max_levels_non_dyadic():
     assert W.max_levels((384, 384)) == 7
     assert W.max_levels((128, 128)) == 6
     assert W.max_levels((6, 6)) == 1
+
+
+def test_d4_dictionary_on_fov_vectors():
+    """The dictionary acts on FOV vectors (x fastest), complex columns part
+    by part, and is orthogonal: Psi^T Psi = I and <Psi x, y> = <x, Psi^T y>."""
+    import numpy as np
+    from comp_ui_toolbox_amd.wavelet import D4Dictionary
+    D = D4Dictionary((16, 8), device="cpu")
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((128, 2)) + 1j * rng.standard_normal((128, 2))
+    y = rng.standard_normal((128, 2)) + 1j * rng.standard_normal((128, 2))
+    assert np.allclose(D.adjoint_transform(D.forward_transform(x)), x, atol=1e-12)
+    assert abs(np.vdot(D.forward_transform(x), y) - np.vdot(x, D.adjoint_transform(y))) < 1e-10
+    img = x[:, 0].reshape(8, 16)                      # rows z, columns x
+    from comp_ui_toolbox_amd.wavelet import dwt2
+    ref = dwt2(torch.tensor(img.real), D.levels).numpy().reshape(-1)
+    assert np.allclose(D.forward_transform(x[:, 0].real), ref, atol=1e-14)
