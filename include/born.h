@@ -152,7 +152,9 @@ int born_plan_set_precision(born_plan_t plan, int32_t precision);
 
 /* combined_quick_gpu( op, 1, gamma ): host FP64 in/out, MEX semantics --
  * forward occupancy rule (cu:466-488), zero output when nothing is occupied
- * (cu:496).  gamma: complex N_points x N_rhs; u: complex N_obs x N_rhs. */
+ * (cu:496); with few occupied points the gather path of
+ * born_plan_set_sparse_forward.  gamma: complex N_points x N_rhs; u:
+ * complex N_obs x N_rhs. */
 int born_forward(born_plan_t plan, const double* gamma, int32_t N_rhs, double* u);
 
 /* combined_quick_gpu( op, 2, u ): host FP64 in/out; all N_points rows
