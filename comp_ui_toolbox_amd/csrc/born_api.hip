@@ -114,6 +114,7 @@ static void free_workspace(Plan& P) {
   dfree(P.d_ones);
   dfree(P.d_fvpart);
   P.fwd_expand = false;
+  P.fwd_table_expanded = false;
   P.fchunks.clear();
   P.fcol.clear();
   P.ws_rhs = -1;
@@ -498,6 +499,46 @@ static void finalize_launchset(LaunchSet& L) {
 // one chunk set per rhs.  The forward then runs on the products
 // p_inc * gamma per column (one extra pass over p_inc), when they fit in
 // the free device memory with 1 GiB to spare.
+// Whether packing the right-hand sides per column lane (setup_column_rhs_adjoint)
+// needs fewer adjoint chunks than one chunk set per rhs.
+static bool column_rhs_saves(const Plan& P, int N_rhs) {
+  int k = 1;
+  while (k < N_rhs) k <<= 1;
+  if (N_rhs <= 1 || k > NC) return false;
+  const int per = NC / k;
+  size_t n2 = 0;
+  for (int fg = 0; fg < P.N_fu; ++fg) n2 += (P.fg_pair_off[fg + 1] - P.fg_pair_off[fg] + per - 1) / per;
+  return n2 < P.chunks.size();
+}
+
+// The forward's own launch set over the rhs-major chunks of the workspace
+// (the p_inc table, gamma of each chunk's rhs, the plan's partial sums), so
+// that the adjoint can replace the plan's chunks by its packed ones.
+static void setup_rhs_major_forward(Plan& P) {
+  const int n_f = (int)P.chunks.size();
+  std::vector<int> fpairs((size_t)n_f * NC, P.N_pairs);   // unused columns: the zero row
+  for (int c = 0; c < n_f; ++c)
+    for (int q = 0; q < P.chunks[c].ncols; ++q) fpairs[(size_t)c * NC + q] = P.col_pair[P.chunks[c].col_start + q];
+  P.fchunks = P.chunks;
+  P.fcol = P.col_pair;
+  P.d_fchunks = dupload(P.fchunks, P.stream);
+  P.d_fcol = dupload(P.fcol, P.stream);
+  P.d_fchunk_pairs = dupload(fpairs, P.stream);
+  P.fvblk = P.vblk;
+  Inst I = make_inst(P);
+  P.d_fvpart = P.d_vpart;   // the plan's partials move to the forward's launch set
+  P.d_vpart = nullptr;
+  P.ls_fwd.inst.assign(1, I);
+  P.ls_fwd.MT = P.Ne_pad / 16;
+  P.ls_fwd.Np = P.Np;
+  P.ls_fwd.adj_mw = P.ls.adj_mw;
+  P.ls_fwd.d_chunks = P.d_fchunks;
+  P.ls_fwd.d_chunk_pairs = P.d_fchunk_pairs;
+  finalize_launchset(P.ls_fwd);
+  P.fwd_expand = true;
+  P.fwd_table_expanded = false;
+}
+
 static void setup_expanded_forward(Plan& P, int N_rhs) {
   if (N_rhs <= 1) return;
   std::vector<Chunk> fch;
@@ -518,12 +559,23 @@ static void setup_expanded_forward(Plan& P, int N_rhs) {
       fch.push_back(c);
     }
   }
-  if (fch.size() >= P.chunks.size()) return;   // no column tiles saved
+  // The expanded table costs a write and a read of N_rhs tables per forward
+  // on top of the reads it replaces.  The FP32 contraction is bound by its
+  // MFMA issue, so any saved column tile pays; the split contraction is bound
+  // by its operand traffic (LDS and power), so the table pays only where it
+  // at least halves the column tiles (few pairs per frequency, as with 5
+  // waves and 4 rhs).  Otherwise, when the adjoint packs the right-hand
+  // sides per column lane, the forward keeps the rhs-major chunks in a
+  // launch set of its own.
   const size_t rows = (size_t)P.N_pairs * N_rhs + 1;
   const size_t bytes = rows * P.N_pts * sizeof(float2);
   size_t free_b = 0, total_b = 0;
   HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
-  if (bytes + ((size_t)1 << 30) > free_b) return;
+  const bool pays = P.ws_split ? 2 * fch.size() <= P.chunks.size() : fch.size() < P.chunks.size();
+  if (!pays || bytes + ((size_t)1 << 30) > free_b) {
+    if (column_rhs_saves(P, N_rhs)) setup_rhs_major_forward(P);
+    return;
+  }
   const int n_f = (int)fch.size();
   std::vector<int> fpairs((size_t)n_f * NC, (int)(rows - 1));   // unused columns: the zero row
   for (int c = 0; c < n_f; ++c)
@@ -558,6 +610,7 @@ static void setup_expanded_forward(Plan& P, int N_rhs) {
   P.ls_fwd.d_chunk_pairs = P.d_fchunk_pairs;
   finalize_launchset(P.ls_fwd);
   P.fwd_expand = true;
+  P.fwd_table_expanded = true;
 }
 
 // With the forward on expanded columns, the adjoint can pack the right-hand
@@ -631,8 +684,10 @@ static void build_workspace(Plan& P, int N_rhs);
 // A failure while building frees the partial workspace, so a plan that
 // could not get its workspace holds none.
 static void ensure_workspace(Plan& P, int N_rhs) {
-  if (P.ws_rhs == N_rhs) return;
+  const bool split = P.precision != BORN_PRECISION_FP32;
+  if (P.ws_rhs == N_rhs && (N_rhs == 1 || P.ws_split == split)) return;
   free_workspace(P);
+  P.ws_split = split;
   try {
     build_workspace(P, N_rhs);
   } catch (...) {
@@ -697,6 +752,7 @@ static void build_workspace(Plan& P, int N_rhs) {
   // valid only once complete: a failure above leaves ws_rhs = -1, so the next
   // call frees the partial workspace and builds it again
   P.ws_rhs = N_rhs;
+  ++P.ws_gen;
 }
 
 static void ensure_io(Plan& P, size_t in_bytes, size_t out_bytes) {
@@ -1042,7 +1098,7 @@ static void forward_device(Plan& P, const float2* gm, int N_rhs, void* u, bool o
   if (use_structured(P) && P.fwd_expand) {
     {
       TimedLaunch t(P, s, 0);
-      launch_expand_pg(P, s);
+      if (P.fwd_table_expanded) launch_expand_pg(P, s);
       if (P.precision == BORN_PRECISION_FP32)
         launch_fwd_structured(P.ls_fwd, P.n_cu, s);
       else
@@ -1220,6 +1276,7 @@ struct Batch {
   int n_cu = 256;
   int precision = 0;
   std::vector<Plan*> plans;
+  std::vector<uint64_t> gen;     // workspace generation of each plan when the records were made
   LaunchSet ls;
   Chunk* d_chunks = nullptr;
   int* d_chunk_pairs = nullptr;
@@ -1268,6 +1325,7 @@ Batch::~Batch() {
 // Instance records, concatenated chunk tables and ranges of the batch.
 static void build_batch_launch(Batch& B) {
   B.ls.inst.clear();
+  B.gen.clear();
   size_t n_chunks = 0;
   for (Plan* P : B.plans) n_chunks += P->chunks.size();
   dfree(B.d_chunks);
@@ -1284,6 +1342,7 @@ static void build_batch_launch(Batch& B) {
     Inst I = make_inst(*P);
     I.chunk_base = (int)base;
     B.ls.inst.push_back(I);
+    B.gen.push_back(P->ws_gen);
     base += nc;
     small = small && P->Nx <= 128;
   }
@@ -1353,7 +1412,24 @@ static void batch_adjoint_enqueue(Batch& B, const void* const* in, void* const* 
 
 // Run one direction, replaying a captured graph when the stream and the
 // buffers are those it was captured with.
+// A plan of the batch applied on its own with another rhs count gets a new
+// workspace, which frees the one the batch's records point at: rebuild the
+// plan's one-rhs workspace and the records (and drop the captured graphs).
+static void batch_refresh(Batch& B) {
+  bool stale = false;
+  for (size_t i = 0; i < B.plans.size(); ++i) {
+    Plan& P = *B.plans[i];
+    if (P.precision != B.precision) throw Error(BORN_ERR_UNSUPPORTED, "a plan of the batch changed its precision");
+    if (P.ws_rhs != 1 || P.ws_gen != B.gen[i]) {
+      ensure_workspace(P, 1);
+      stale = true;
+    }
+  }
+  if (stale) build_batch_launch(B);
+}
+
 static void batch_apply(Batch& B, int dir, const void* const* in, void* const* out, hipStream_t s) {
+  batch_refresh(B);
   const size_t n = B.plans.size();
   if (!B.graph || B.timing) {
     if (dir == 0) batch_forward_enqueue(B, in, out, s);
@@ -2066,6 +2142,7 @@ int born_batch_autotune(born_batch_t batch, void* stream, double* fwd_ms, double
     Batch& B = *reinterpret_cast<Batch*>(batch);
     HIP_CHECK(hipSetDevice(B.device));
     hipStream_t s = (hipStream_t)stream;
+    batch_refresh(B);
     HIP_CHECK(hipDeviceSynchronize());
     const TuneResult r = autotune(B.plans, B.ls, B.precision, B.n_cu, [&] { build_batch_launch(B); }, s);
     if (fwd_ms) *fwd_ms = r.fwd_ms;

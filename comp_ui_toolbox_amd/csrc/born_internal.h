@@ -150,6 +150,8 @@ struct Plan {
 
   // per-N_rhs workspace
   int ws_rhs = -1;
+  bool ws_split = false;          // built for a split precision (the forward's expansion rule)
+  uint64_t ws_gen = 0;            // counts workspace builds: batches holding its pointers compare it
   std::vector<Chunk> chunks;
   std::vector<int> col_pair;
   Chunk* d_chunks = nullptr;
@@ -178,7 +180,8 @@ struct Plan {
   // frequency: the products p_inc * gamma of every (pair, rhs) column are
   // formed first (d_pexp, a zero row after them) and the forward runs on
   // chunks that pack the columns of all right-hand sides of a frequency.
-  bool fwd_expand = false;
+  bool fwd_expand = false;         // the forward runs on ls_fwd (below)
+  bool fwd_table_expanded = false; // ls_fwd reads d_pexp (else d_p with the rhs of each chunk)
   std::vector<Chunk> fchunks;     // chunks over expanded columns ep = pair * N_rhs + r
   std::vector<int> fcol;          // expanded column of each chunk column
   Chunk* d_fchunks = nullptr;

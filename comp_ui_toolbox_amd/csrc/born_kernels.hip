@@ -116,7 +116,8 @@ void launch_fwd_reduce_expanded(const Plan& P, int nblk, hipStream_t s) {
   const int n_chunks = (int)P.fchunks.size();
   const size_t total = (size_t)n_chunks * Ney * P.Ne_pad * NC;
   hipLaunchKernelGGL(k_fwd_reduce, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, P.d_fvpart, P.d_fchunks,
-                     P.d_fcol, n_chunks, nblk, P.Ne_pad, P.Nex, Ney, P.N_e, P.ws_rhs, P.d_V, P.ws_rhs);
+                     P.d_fcol, n_chunks, nblk, P.Ne_pad, P.Nex, Ney, P.N_e, P.ws_rhs, P.d_V,
+                     P.fwd_table_expanded ? P.ws_rhs : 0);
 }
 
 // pexp[pair * N_rhs + r][p] = p_inc[pair][p] * gamma_masked[r][p]

@@ -179,6 +179,39 @@ def test_batch_rejects_mixed_precision(torch_cuda, fp32_set):
         OperatorBatch([ops[0], s20])
 
 
+def test_batch_follows_plan_workspace_changes(torch_cuda):
+    """A plan of the batch applied on its own with two right-hand sides gets
+    a new workspace; the batch then rebuilds its records (and its captured
+    graph) instead of reading the freed one.  A plan whose precision changed
+    is refused."""
+    torch = torch_cuda
+    from comp_ui_toolbox_amd import BornError
+    specs = uneven_specs(16)[:3]
+    ops = [OperatorBorn.from_sequence(s) for s in specs]
+    B = OperatorBatch(ops, graph=True)
+    g, u = inputs(torch, ops, 19)
+    fb0 = [t.clone() for t in B.forward_dev(g)]
+    rng = np.random.Generator(np.random.PCG64(20))
+    two = rng.standard_normal((ops[1].N_points, 2)) + 1j * rng.standard_normal((ops[1].N_points, 2))
+    cols = np.stack([ops[1].forward(two[:, r]) for r in range(2)], axis=1)
+    assert O.rel_l2(ops[1].forward(two), cols) < 1e-6   # plan 1 now holds a 2-rhs workspace
+    fb = B.forward_dev(g)
+    ab = B.adjoint_dev(u)
+    torch.cuda.synchronize()
+    for i, op in enumerate(ops):
+        assert rel(fb[i], fb0[i]) < 1e-6
+        assert rel(fb[i], op.forward_dev(g[i])) < 1e-6
+        assert rel(ab[i], op.adjoint_dev(u[i])) < 1e-6
+    ops[2].set_precision("split_bf16")
+    with pytest.raises(BornError):
+        B.forward_dev(g)
+    ops[2].set_precision("fp32")
+    fb = B.forward_dev(g)
+    torch.cuda.synchronize()
+    assert all(rel(fb[i], fb0[i]) < 1e-6 for i in range(len(ops)))
+    B.close()
+
+
 def test_config3_batch(torch_cuda):
     """The 24 uneven config-3 instances (most with a FOV narrower than the
     aperture) as one batch: every instance equals its plan applied alone, and
