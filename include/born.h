@@ -284,7 +284,10 @@ int born_plan_get_timing(born_plan_t plan, double* ms, int64_t* launches);
  * contraction launch per direction covers every instance.  No counterpart in
  * the reference (each MEX call serves one operator).  The plans must be on
  * the structured path, share the device, the precision, the element padding
- * and the points per pitch, and outlive the batch; at most 64 plans.
+ * and the points per pitch, and outlive the batch; at most 64 plans.  A plan
+ * may still be applied on its own (with any rhs count): the next batch call
+ * rebuilds the plan's one-rhs workspace and the batch's records.  A plan
+ * whose precision was changed makes batch calls fail with born:Unsupported.
  * gamma_c64[i] / u_c64[i] are the device buffers of plan i (complex64,
  * N_points(i) and N_observations(i) entries). */
 typedef struct born_batch_s* born_batch_t;
