@@ -592,6 +592,32 @@ def test_many_rhs(torch_cuda, precision, tol, n_rhs):
     assert max(e_f, e_a, e_fd, e_ad) < tol
 
 
+@pytest.mark.parametrize("precision,tol", [("fp32", TOL_FP32), ("split_bf16", TOL_SPLIT), ("split_fp16", TOL_SPLIT)])
+@pytest.mark.parametrize("n_rhs", [3, 4])
+def test_many_rhs_many_waves(torch_cuda, precision, tol, n_rhs):
+    """11 waves per frequency, as config 1.  With 3 right-hand sides packing
+    saves no chunk (9 either way); with 4 it saves 3 of 12, too few for the
+    split forms, whose forward then keeps one chunk set per rhs beside the
+    packed adjoint, while FP32 packs both directions."""
+    import torch
+    med = C.MediumSpec()
+    spec = tiny_spec(N_e=40, Nx=160, Nz=3, npp=4)
+    spec.waves = [C.qpw_wave(spec.array, a, med.c_avg) for a in np.linspace(-0.3, 0.3, 11)]
+    spec.mixes = [C.identity_mixes(40) for _ in spec.waves]
+    T = O.build_tables(spec)
+    op = OperatorBorn.from_tables(T, precision=precision)
+    g = rnd((T.N_points, n_rhs), 103)
+    u = rnd((T.N_observations(), n_rhs), 104)
+    e_f = O.rel_l2(op.forward(g), O.forward(T, g))
+    e_a = O.rel_l2(op.adjoint(u), O.adjoint(T, u))
+    gd = torch.from_numpy(g.T.astype(np.complex64)).cuda().contiguous()
+    ud = torch.from_numpy(u.T.astype(np.complex64)).cuda().contiguous()
+    e_fd = O.rel_l2(op.forward_dev(gd).cpu().numpy().T, O.forward(T, g.astype(np.complex64)))
+    e_ad = O.rel_l2(op.adjoint_dev(ud).cpu().numpy().T, O.adjoint(T, u.astype(np.complex64)))
+    log("11 waves, %d rhs %s: forward %.2e adjoint %.2e (device %.2e %.2e)" % (n_rhs, precision, e_f, e_a, e_fd, e_ad))
+    assert max(e_f, e_a, e_fd, e_ad) < tol
+
+
 # ---------------------------------------------------------------- columns
 def test_columns_tiny(torch_cuda):
     """Columns of A (born_columns): against the assembled matrix, between
