@@ -327,7 +327,8 @@ def main():
                     help="do not time the plain FP32 form beside the selected one")
     ap.add_argument("--config", default="config1")
     ap.add_argument("--autotune", action="store_true",
-                    help="run the library's autotuner for each form before timing it")
+                    help="run the library's autotuner for each form before timing it (its candidate "
+                         "launches then share kernel names with the timed ones in a profile)")
     ap.add_argument("--rhs", type=int, default=0,
                     help="gamma right-hand sides per application (default 4 for config4, else 1)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -412,6 +413,7 @@ def main():
                         f"{spec.N_measurements} waves, {spec.N_observations()} obs, {n_rhs} rhs",
             "precision": args.precision,
             "path": op.path,
+            "autotuned": bool(args.autotune),
             "table_build_s": round(t_tables, 3),
             "kernel_ms": {k: round(v, 4) for k, v in kern.items()},
             "flops_per_pass": F,
