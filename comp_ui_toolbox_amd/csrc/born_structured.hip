@@ -47,7 +47,12 @@ __device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
 // The LDS layout follows the instance (row length), so the layout of the
 // row held in LDS (used by the MFMA phase) and that of the prefetched row
 // (used by the next fill) are tracked separately.
-template <int MT, int NP, int PPT>
+//
+// DUAL (rows of at most 128 points): a step holds two consecutive rows, their
+// B tiles side by side (points [0, XP) and [XP, 2 XP)) and their two
+// extended h rows one after the other, so the point groups of the waves
+// split at the row boundary and each barrier pair serves twice the MFMAs.
+template <int MT, int NP, int PPT, bool DUAL>
 __global__ __launch_bounds__(256, 2) void k_fwd_s2d(const Inst* __restrict__ inst, int n_inst, int n_items,
                                                     int Ne_pad, const Chunk* __restrict__ chunks,
                                                     const int* __restrict__ chunk_pairs) {
@@ -72,6 +77,7 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(const Inst* __restrict__ ins
   int fg, rr, pr[NC];           // chunk of the prefetched row
   int ey;                       // y slice of the prefetched row (3-D; 0 in 2-D)
   int z, z_end, item = blockIdx.x;
+  constexpr int ZS = DUAL ? 2 : 1;   // rows per step
 
 #define BORN_FWD_INST()                                                                   \
   {                                                                                       \
@@ -110,8 +116,10 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(const Inst* __restrict__ ins
   }
 
   float2 hreg[HPT];
+  float2 hreg2[DUAL ? HPT : 1];   // the step's second h row (DUAL)
   float2 praw[PPT][NC];
   float2 greg[PPT];
+  int gvalid = 0;                 // bit j: B point tid + 256 j inside its row (DUAL)
 
   // issue the loads of row zz of the current chunk into registers; every
   // address is valid (clamped), out-of-range entries are zeroed in the fill
@@ -119,13 +127,32 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(const Inst* __restrict__ ins
   {                                                                                     \
     const float2* hrow_ = I->h + (size_t)fg * I->Hp + (size_t)I->hrow[ey * I->n_rows + (zz)] * I->HR; \
     _Pragma("unroll") for (int j = 0; j < HPT; ++j) hreg[j] = hrow_[max(eidx[j], 0)];  \
-    const size_t rowo_ = (size_t)(zz) * I->Nx;                                          \
-    const float2* grow_ = I->gm + (size_t)rr * I->N_pts + rowo_;                        \
-    _Pragma("unroll") for (int j = 0; j < PPT; ++j) {                                   \
-      const int i_ = min(tid + 256 * j, I->Nx - 1);                                     \
-      greg[j] = grow_[i_];                                                              \
-      _Pragma("unroll") for (int c_ = 0; c_ < NC; ++c_)                                 \
-        praw[j][c_] = I->P[(size_t)pr[c_] * I->N_pts + rowo_ + i_];                     \
+    if (DUAL) {                                                                         \
+      /* second row of the step: zz + 1 inside the item, else row zz again (zeroed) */  \
+      const int two_ = (zz) + 1 < z_end ? 1 : 0;                                        \
+      const float2* hrow2_ = I->h + (size_t)fg * I->Hp + (size_t)I->hrow[ey * I->n_rows + (zz) + two_] * I->HR; \
+      _Pragma("unroll") for (int j = 0; j < HPT; ++j) hreg2[j] = hrow2_[max(eidx[j], 0)]; \
+      const int XP1_ = 16 * I->S;                                                       \
+      gvalid = 0;                                                                       \
+      _Pragma("unroll") for (int j = 0; j < PPT; ++j) {                                 \
+        const int t_ = tid + 256 * j;                                                   \
+        const int r_ = t_ >= XP1_ ? 1 : 0;                                              \
+        const int x_ = t_ - r_ * XP1_;                                                  \
+        if (t_ < 2 * XP1_ && x_ < I->Nx && r_ <= two_) gvalid |= 1 << j;                \
+        const size_t o_ = (size_t)((zz) + (r_ & two_)) * I->Nx + min(x_, I->Nx - 1);    \
+        greg[j] = I->gm[(size_t)rr * I->N_pts + o_];                                    \
+        _Pragma("unroll") for (int c_ = 0; c_ < NC; ++c_)                               \
+          praw[j][c_] = I->P[(size_t)pr[c_] * I->N_pts + o_];                           \
+      }                                                                                 \
+    } else {                                                                            \
+      const size_t rowo_ = (size_t)(zz) * I->Nx;                                        \
+      const float2* grow_ = I->gm + (size_t)rr * I->N_pts + rowo_;                      \
+      _Pragma("unroll") for (int j = 0; j < PPT; ++j) {                                 \
+        const int i_ = min(tid + 256 * j, I->Nx - 1);                                   \
+        greg[j] = grow_[i_];                                                            \
+        _Pragma("unroll") for (int c_ = 0; c_ < NC; ++c_)                               \
+          praw[j][c_] = I->P[(size_t)pr[c_] * I->N_pts + rowo_ + i_];                   \
+      }                                                                                 \
     }                                                                                   \
   }
 
@@ -145,14 +172,17 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(const Inst* __restrict__ ins
 
   while (true) {
     // layout of the row about to be written to LDS
-    const int cNp = Np, cQL = QL, cHN = HN, cXP = XP;
-    float2* Hp = lds;              // [Np][QL] polyphase extended row
-    float2* Bs = lds + cHN;        // [XP][NCS]
+    const int cNp = Np, cQL = QL, cHN = HN, cXP1 = XP, cXP = ZS * XP;
+    float2* Hp = lds;              // [ZS][Np][QL] polyphase extended rows
+    float2* Bs = lds + ZS * cHN;   // [ZS * XP][NCS]
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < HPT; ++j) {
       const int idx = tid + 256 * j;
-      if (idx < cHN) Hp[idx] = eidx[j] >= 0 ? hreg[j] : make_float2(0.f, 0.f);
+      if (idx < cHN) {
+        Hp[idx] = eidx[j] >= 0 ? hreg[j] : make_float2(0.f, 0.f);
+        if (DUAL) Hp[cHN + idx] = eidx[j] >= 0 ? hreg2[j] : make_float2(0.f, 0.f);
+      }
     }
     {
       const bool xfull = cXP == n_Nx;   // no padded points in the B tile
@@ -160,7 +190,8 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(const Inst* __restrict__ ins
       for (int j = 0; j < PPT; ++j) {
         const int i = tid + 256 * j;
         if (i < cXP) {
-          const float2 g = (xfull || i < n_Nx) ? greg[j] : make_float2(0.f, 0.f);
+          const bool ok = DUAL ? ((gvalid >> j) & 1) != 0 : (xfull || i < n_Nx);
+          const float2 g = ok ? greg[j] : make_float2(0.f, 0.f);
 #pragma unroll
           for (int c = 0; c < NC; ++c) Bs[i * NCS + c] = cmulf(praw[j][c], g);
         }
@@ -173,9 +204,9 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(const Inst* __restrict__ ins
     const int item_cur = item;
     float2* const vpart_cur = Iv.vpart;
     const int ib_cur = Iv.item_begin;
-    const bool item_done = z + 1 >= z_end;
+    const bool item_done = z + ZS >= z_end;
     if (!item_done) {
-      ++z;
+      z += ZS;
     } else if (it + 1 >= n_my) {
       has_next = false;
     } else {
@@ -189,11 +220,15 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(const Inst* __restrict__ ins
     }
 
     // MFMA over this wave's points of the row in LDS
-    const int SW = cXP / (4 * PW);               // K-steps per wave and row
+    const int SW = cXP / (4 * PW);               // K-steps per wave and step
+    // DUAL: a wave's K-steps lie in one row (cXP1 is a multiple of cXP / PW);
+    // those of the second row read the second extended row
+    const int rw = DUAL && 4 * pg * SW >= cXP1 ? 1 : 0;
     int a_base, a_step;
     if (NP == 4 || NP == 2 || NP == 1) {
       a_base = (kk % cNp) * cQL + kk / cNp + 16 * MT - 1 - col - 16 * MTW * eg;
       a_step = 4 / cNp;
+      a_base += rw * (cHN - a_step * (cXP1 / 4));
     } else {
       a_base = 0;
       a_step = 0;
@@ -213,8 +248,8 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(const Inst* __restrict__ ins
       if (NP == 4 || NP == 2 || NP == 1) {
         ai = a_base + a_step * ks;
       } else {
-        const int i = 4 * ks + kk;
-        ai = (i % cNp) * cQL + i / cNp + 16 * MT - 1 - col - 16 * MTW * eg;
+        const int i = 4 * ks + kk - rw * cXP1;
+        ai = rw * cHN + (i % cNp) * cQL + i / cNp + 16 * MT - 1 - col - 16 * MTW * eg;
       }
 #pragma unroll
       for (int mt = 0; mt < MTW; ++mt) {
@@ -267,10 +302,10 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(const Inst* __restrict__ ins
 #undef BORN_FWD_INST
 }
 
-static size_t fwd_lds_bytes(int MT, int S, int Np) {
+static size_t fwd_lds_bytes(int MT, int S, int Np, bool dual = false) {
   const int XP = 16 * S;
   const int QL = (XP - 1) / Np + 16 * MT;
-  const size_t body = ((size_t)Np * QL + (size_t)XP * NCS) * sizeof(float2);
+  const size_t body = (dual ? 2 : 1) * ((size_t)Np * QL + (size_t)XP * NCS) * sizeof(float2);
   const size_t red = (size_t)4 * MT * 8 * 64 * sizeof(float);
   return std::max(body, red);
 }
@@ -288,18 +323,23 @@ int fwd_ppt(int S) {
   return XP <= 256 ? 1 : (XP <= 512 ? 2 : 4);
 }
 
+// Two rows per step when both fit the one-point-per-thread B fill.
+static bool fwd_dual(int S) { return 16 * S <= 128; }
+
 void launch_fwd_structured(const LaunchSet& L, int n_cu, hipStream_t s) {
   const int PPT = fwd_ppt(L.S_max);
-  const size_t lds = fwd_lds_bytes(L.MT, L.S_max, L.Np);
+  const bool dual = fwd_dual(L.S_max);
+  const size_t lds = fwd_lds_bytes(L.MT, L.S_max, L.Np, dual);
   const int grid = std::max(1, std::min(L.n_items, L.grid_mult * n_cu));
   const int Ne_pad = 16 * L.MT;
-#define BORN_FWD_LAUNCH(MTv, NPv, PPTv) \
-  hipLaunchKernelGGL((k_fwd_s2d<MTv, NPv, PPTv>), dim3(grid), dim3(256), lds, s, L.d_inst, (int)L.inst.size(), \
+#define BORN_FWD_LAUNCH(MTv, NPv, PPTv, Dv) \
+  hipLaunchKernelGGL((k_fwd_s2d<MTv, NPv, PPTv, Dv>), dim3(grid), dim3(256), lds, s, L.d_inst, (int)L.inst.size(), \
                      L.n_items, Ne_pad, L.d_chunks, L.d_chunk_pairs)
-#define BORN_FWD_PPT(MTv, NPv)                    \
-  if (PPT <= 1) BORN_FWD_LAUNCH(MTv, NPv, 1);     \
-  else if (PPT <= 2) BORN_FWD_LAUNCH(MTv, NPv, 2); \
-  else BORN_FWD_LAUNCH(MTv, NPv, 4);
+#define BORN_FWD_PPT(MTv, NPv)                           \
+  if (dual) BORN_FWD_LAUNCH(MTv, NPv, 1, true);          \
+  else if (PPT <= 1) BORN_FWD_LAUNCH(MTv, NPv, 1, false); \
+  else if (PPT <= 2) BORN_FWD_LAUNCH(MTv, NPv, 2, false); \
+  else BORN_FWD_LAUNCH(MTv, NPv, 4, false);
 #define BORN_FWD_NP(MTv)       \
   if (L.Np == 4) {             \
     BORN_FWD_PPT(MTv, 4)       \
@@ -591,9 +631,10 @@ void launch_adj_structured(const LaunchSet& L, const OutPtrs& outs, hipStream_t 
 template <int MT>
 static void set_max_lds_mt(int b) {
 #define BORN_SET(K) hipFuncSetAttribute((const void*)(K), hipFuncAttributeMaxDynamicSharedMemorySize, b)
-  BORN_SET((k_fwd_s2d<MT, 4, 1>)); BORN_SET((k_fwd_s2d<MT, 4, 2>)); BORN_SET((k_fwd_s2d<MT, 4, 4>));
-  BORN_SET((k_fwd_s2d<MT, 2, 1>)); BORN_SET((k_fwd_s2d<MT, 2, 2>)); BORN_SET((k_fwd_s2d<MT, 2, 4>));
-  BORN_SET((k_fwd_s2d<MT, 0, 1>)); BORN_SET((k_fwd_s2d<MT, 0, 2>)); BORN_SET((k_fwd_s2d<MT, 0, 4>));
+  BORN_SET((k_fwd_s2d<MT, 4, 1, false>)); BORN_SET((k_fwd_s2d<MT, 4, 2, false>)); BORN_SET((k_fwd_s2d<MT, 4, 4, false>));
+  BORN_SET((k_fwd_s2d<MT, 2, 1, false>)); BORN_SET((k_fwd_s2d<MT, 2, 2, false>)); BORN_SET((k_fwd_s2d<MT, 2, 4, false>));
+  BORN_SET((k_fwd_s2d<MT, 0, 1, false>)); BORN_SET((k_fwd_s2d<MT, 0, 2, false>)); BORN_SET((k_fwd_s2d<MT, 0, 4, false>));
+  BORN_SET((k_fwd_s2d<MT, 4, 1, true>)); BORN_SET((k_fwd_s2d<MT, 2, 1, true>)); BORN_SET((k_fwd_s2d<MT, 0, 1, true>));
   BORN_SET((k_adj_s2d<MT, 2, 4>)); BORN_SET((k_adj_s2d<MT, 4, 4>));
   BORN_SET((k_adj_s2d<MT, 2, 2>)); BORN_SET((k_adj_s2d<MT, 4, 2>));
   BORN_SET((k_adj_s2d<MT, 2, 0>)); BORN_SET((k_adj_s2d<MT, 4, 0>));
