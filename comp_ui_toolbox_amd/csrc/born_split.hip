@@ -107,7 +107,9 @@ __host__ __device__ constexpr int fwd_split_lap(int LA, int NP) {
   return ((LA + 12 + 63) / 64) * 64 + (NP == 4 ? 0 : 32);
 }
 
-template <int MT, bool EVEN, int NP, bool F16>
+// HALF (rows of at most 128 points): the MFMAs cover the segment's first 128
+// points only (its further points are padding), the waves splitting those.
+template <int MT, bool EVEN, int NP, bool F16, bool HALF>
 __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict__ inst, int n_inst, int n_items,
                                                           int Ne_pad, const Chunk* __restrict__ chunks,
                                                           const int* __restrict__ chunk_pairs) {
@@ -118,7 +120,7 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict
   constexpr int EW = MT >= 2 ? 2 : 1;
   constexpr int PW = 4 / EW;
   constexpr int MTW = MT / EW;
-  constexpr int KBW = (SEG / 32) / PW;
+  constexpr int KBW = (SEG / (HALF ? 64 : 32)) / PW;
   constexpr int LA = SEG + NP * (16 * MT - 1);
   constexpr int LAP = fwd_split_lap(LA, NP);
   constexpr int APT = 2 * ((LA + 511) / 512);   // window entries per thread: pairs (u, u + 1), u = 2 tid + 512 j
@@ -787,13 +789,20 @@ void launch_fwd_split(const LaunchSet& L, int n_cu, hipStream_t s) {
   for (const Inst& I : L.inst) even = even && (I.Nx % 2) == 0;
   const int n_inst = (int)L.inst.size();
   const int Ne_pad = 16 * L.MT;
-#define BORN_FWDS_L(MTv, EV, NPv)                                                                      \
-  if (L.f16)                                                                                         \
-    hipLaunchKernelGGL((k_fwd_s2d_split<MTv, EV, NPv, true>), dim3(grid), dim3(256), (fwd_split_lds_bytes<MTv, NPv>()), s, \
-                       L.d_inst, n_inst, L.n_items_s, Ne_pad, L.d_chunks, L.d_chunk_pairs);            \
-  else                                                                                               \
-    hipLaunchKernelGGL((k_fwd_s2d_split<MTv, EV, NPv, false>), dim3(grid), dim3(256), (fwd_split_lds_bytes<MTv, NPv>()), s, \
-                       L.d_inst, n_inst, L.n_items_s, Ne_pad, L.d_chunks, L.d_chunk_pairs)
+  const bool half = 16 * L.S_max <= 128;   // every instance's rows fit the first half of a segment
+#define BORN_FWDS_K(MTv, EV, NPv, F16v, Hv)                                                            \
+  hipLaunchKernelGGL((k_fwd_s2d_split<MTv, EV, NPv, F16v, Hv>), dim3(grid), dim3(256), (fwd_split_lds_bytes<MTv, NPv>()), s, \
+                     L.d_inst, n_inst, L.n_items_s, Ne_pad, L.d_chunks, L.d_chunk_pairs)
+#define BORN_FWDS_L(MTv, EV, NPv)                                        \
+  do {                                                                   \
+    if (L.f16) {                                                         \
+      if (half) BORN_FWDS_K(MTv, EV, NPv, true, true);                   \
+      else BORN_FWDS_K(MTv, EV, NPv, true, false);                       \
+    } else {                                                             \
+      if (half) BORN_FWDS_K(MTv, EV, NPv, false, true);                  \
+      else BORN_FWDS_K(MTv, EV, NPv, false, false);                      \
+    }                                                                    \
+  } while (0)
 #define BORN_FWDS_NP(MTv, EV)                 \
   if (L.Np == 2) BORN_FWDS_L(MTv, EV, 2);     \
   else BORN_FWDS_L(MTv, EV, 4);
@@ -811,6 +820,7 @@ void launch_fwd_split(const LaunchSet& L, int n_cu, hipStream_t s) {
 #undef BORN_FWDS
 #undef BORN_FWDS_NP
 #undef BORN_FWDS_L
+#undef BORN_FWDS_K
 }
 
 void launch_adj_split(const LaunchSet& L, const OutPtrs& outs, hipStream_t s) {
@@ -843,12 +853,16 @@ void launch_adj_split(const LaunchSet& L, const OutPtrs& outs, hipStream_t s) {
 template <int MT, int NP>
 static void set_max_lds_split_mt(int bytes) {
 #define BORN_SET(K) hipFuncSetAttribute((const void*)(K), hipFuncAttributeMaxDynamicSharedMemorySize, bytes)
-  BORN_SET((k_fwd_s2d_split<MT, true, NP, false>));
-  BORN_SET((k_fwd_s2d_split<MT, false, NP, false>));
+  BORN_SET((k_fwd_s2d_split<MT, true, NP, false, false>));
+  BORN_SET((k_fwd_s2d_split<MT, false, NP, false, false>));
+  BORN_SET((k_fwd_s2d_split<MT, true, NP, false, true>));
+  BORN_SET((k_fwd_s2d_split<MT, false, NP, false, true>));
   BORN_SET((k_adj_s2d_split<MT, NP, false, false>));
   BORN_SET((k_adj_s2d_split<MT, NP, true, false>));
-  BORN_SET((k_fwd_s2d_split<MT, true, NP, true>));
-  BORN_SET((k_fwd_s2d_split<MT, false, NP, true>));
+  BORN_SET((k_fwd_s2d_split<MT, true, NP, true, false>));
+  BORN_SET((k_fwd_s2d_split<MT, false, NP, true, false>));
+  BORN_SET((k_fwd_s2d_split<MT, true, NP, true, true>));
+  BORN_SET((k_fwd_s2d_split<MT, false, NP, true, true>));
   BORN_SET((k_adj_s2d_split<MT, NP, false, true>));
   BORN_SET((k_adj_s2d_split<MT, NP, true, true>));
 #undef BORN_SET
