@@ -460,7 +460,7 @@ struct AdjSplitLayout {
 };
 __device__ __forceinline__ int copy_offset(int d) { return d == 1 ? 16 : (d == 2 ? 28 : (d == 3 ? 40 : 0)); }
 
-template <int MT, int NP, bool Y3, bool F16>
+template <int MT, int NP, bool Y3, bool F16, bool HALF>
 __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict__ inst, int n_inst, OutPtrs outs,
                                                           const Chunk* __restrict__ chunks,
                                                           const int* __restrict__ chunk_pairs) {
@@ -622,7 +622,10 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict
 
   // ---------------------------------------------------------------- compute waves
   float2* __restrict__ g_out = outs.p[ki];
-  const int ph = w / WPP, t0 = 4 * (w % WPP);   // phase and first tile of this wave
+  // tiles of this wave: 4, or 2 with HALF (rows of at most 128 points fill
+  // only the first half of the segment, so the waves split that half)
+  constexpr int TPW = HALF ? 2 : 4;
+  const int ph = w / WPP, t0 = TPW * (w % WPP);   // phase and first tile of this wave
   // gamma_hat partials: tile t0 + t, v (0..3) -> point s = 16 (t0 + t) + 4 kk + v, i = NP s + ph
   float gx[4][4], gy[4][4];
 #pragma unroll
@@ -634,7 +637,7 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict
   const int abase = ph * CPB + dcp * LR + copy_offset(dcp) + (SPH - col + 8 * kk - dcp) + PADL - 16 * t0;
   v4f YR[4], YI[4];
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
+  for (int t = 0; t < TPW; ++t) {
     YR[t] = vz();
     YI[t] = vz();
   }
@@ -654,7 +657,7 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict
     // Y of the 4 tiles, summed over the chunk's slices: conj(H) W with conj
     // folded into the table (imag part stored negated):
     // Re = Ar Wr - Ai' Wi, Im = Ar Wi + Ai' Wr
-    constexpr int DMIN = -3, DMAX = 2 * (KB - 1);
+    constexpr int DMIN = -(TPW - 1), DMAX = 2 * (KB - 1);
     uint4 Ac[4], An[4];
 #define BORN_ADJS_LOAD_A(DST, DL)                                                                \
   _Pragma("unroll") for (int cp = 0; cp < 4; ++cp) {                                            \
@@ -667,7 +670,7 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict
       if (dl < DMAX) BORN_ADJS_LOAD_A(An, dl + 1)
       const uint4 nih = neg8(Ac[2]), nil = neg8(Ac[3]);
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
+      for (int t = 0; t < TPW; ++t) {
         const int k2 = dl + t;
         if (k2 >= 0 && (k2 & 1) == 0 && k2 / 2 < KB) {
           const int kb = k2 / 2;
@@ -695,7 +698,7 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict
       if (F16) {
         const float us = I->fscale[4];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
+        for (int t = 0; t < TPW; ++t) {
           YR[t] *= us;
           YI[t] *= us;
         }
@@ -703,7 +706,7 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict
       // gamma_hat += conj(p) Y; lane holds rows 4*kk + v of tile t0 + t, column col
       const float2* Ps = reinterpret_cast<const float2*>(ldsu + Lay::PS_OFF + (c & 1) * Lay::PS_U16);
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
+      for (int t = 0; t < TPW; ++t) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int s = 16 * (t0 + t) + 4 * kk + v;
@@ -721,7 +724,7 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict
         // same column mod col_rhs)
         const int o0 = I->col_rhs > 0 ? I->col_rhs : 1;
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
+        for (int t = 0; t < TPW; ++t) {
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
             float x = gx[t][v], y = gy[t][v];
@@ -737,7 +740,7 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict
           // lane col < n_rhs holds rhs col for every point of its rows
           if (col < I->n_rhs) {
 #pragma unroll
-            for (int t = 0; t < 4; ++t)
+            for (int t = 0; t < TPW; ++t)
 #pragma unroll
               for (int v = 0; v < 4; ++v) {
                 const int i = i_s + NP * (16 * (t0 + t) + 4 * kk + v) + ph;
@@ -747,7 +750,7 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict
         } else if (col < 4) {
           // lane col < 4 writes v = col for every tile
 #pragma unroll
-          for (int t = 0; t < 4; ++t) {
+          for (int t = 0; t < TPW; ++t) {
             float x = gx[t][0], y = gy[t][0];
 #pragma unroll
             for (int v = 1; v < 4; ++v) {
@@ -762,7 +765,7 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict
           }
         }
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
+        for (int t = 0; t < TPW; ++t)
 #pragma unroll
           for (int v = 0; v < 4; ++v) gx[t][v] = gy[t][v] = 0.f;
       }
@@ -828,13 +831,20 @@ void launch_adj_split(const LaunchSet& L, const OutPtrs& outs, hipStream_t s) {
   const int n_inst = (int)L.inst.size();
   bool y3 = false;
   for (const Inst& I : L.inst) y3 = y3 || I.Ney > 1;
-#define BORN_ADJS_L(MTv, NPv, Y3v)                                                                    \
-  if (L.f16)                                                                                         \
-    hipLaunchKernelGGL((k_adj_s2d_split<MTv, NPv, Y3v, true>), grid, dim3(512), (adj_split_lds_bytes<MTv, NPv>()), s, \
-                       L.d_inst, n_inst, outs, L.d_chunks, L.d_chunk_pairs);                           \
-  else                                                                                               \
-    hipLaunchKernelGGL((k_adj_s2d_split<MTv, NPv, Y3v, false>), grid, dim3(512), (adj_split_lds_bytes<MTv, NPv>()), s, \
-                       L.d_inst, n_inst, outs, L.d_chunks, L.d_chunk_pairs)
+  const bool half = 16 * L.S_max <= 128;   // every instance's rows fit the first half of a segment
+#define BORN_ADJS_K(MTv, NPv, Y3v, F16v, Hv)                                                           \
+  hipLaunchKernelGGL((k_adj_s2d_split<MTv, NPv, Y3v, F16v, Hv>), grid, dim3(512), (adj_split_lds_bytes<MTv, NPv>()), s, \
+                     L.d_inst, n_inst, outs, L.d_chunks, L.d_chunk_pairs)
+#define BORN_ADJS_L(MTv, NPv, Y3v)                                       \
+  do {                                                                   \
+    if (L.f16) {                                                         \
+      if (half) BORN_ADJS_K(MTv, NPv, Y3v, true, true);                  \
+      else BORN_ADJS_K(MTv, NPv, Y3v, true, false);                      \
+    } else {                                                             \
+      if (half) BORN_ADJS_K(MTv, NPv, Y3v, false, true);                 \
+      else BORN_ADJS_K(MTv, NPv, Y3v, false, false);                     \
+    }                                                                    \
+  } while (0)
 #define BORN_ADJS_NP(MTv)                                     \
   if (L.Np == 2) {                                            \
     if (y3) BORN_ADJS_L(MTv, 2, true); else BORN_ADJS_L(MTv, 2, false); \
@@ -848,6 +858,7 @@ void launch_adj_split(const LaunchSet& L, const OutPtrs& outs, hipStream_t s) {
   }
 #undef BORN_ADJS_NP
 #undef BORN_ADJS_L
+#undef BORN_ADJS_K
 }
 
 template <int MT, int NP>
@@ -857,14 +868,18 @@ static void set_max_lds_split_mt(int bytes) {
   BORN_SET((k_fwd_s2d_split<MT, false, NP, false, false>));
   BORN_SET((k_fwd_s2d_split<MT, true, NP, false, true>));
   BORN_SET((k_fwd_s2d_split<MT, false, NP, false, true>));
-  BORN_SET((k_adj_s2d_split<MT, NP, false, false>));
-  BORN_SET((k_adj_s2d_split<MT, NP, true, false>));
+  BORN_SET((k_adj_s2d_split<MT, NP, false, false, false>));
+  BORN_SET((k_adj_s2d_split<MT, NP, true, false, false>));
+  BORN_SET((k_adj_s2d_split<MT, NP, false, false, true>));
+  BORN_SET((k_adj_s2d_split<MT, NP, true, false, true>));
   BORN_SET((k_fwd_s2d_split<MT, true, NP, true, false>));
   BORN_SET((k_fwd_s2d_split<MT, false, NP, true, false>));
   BORN_SET((k_fwd_s2d_split<MT, true, NP, true, true>));
   BORN_SET((k_fwd_s2d_split<MT, false, NP, true, true>));
-  BORN_SET((k_adj_s2d_split<MT, NP, false, true>));
-  BORN_SET((k_adj_s2d_split<MT, NP, true, true>));
+  BORN_SET((k_adj_s2d_split<MT, NP, false, true, false>));
+  BORN_SET((k_adj_s2d_split<MT, NP, true, true, false>));
+  BORN_SET((k_adj_s2d_split<MT, NP, false, true, true>));
+  BORN_SET((k_adj_s2d_split<MT, NP, true, true, true>));
 #undef BORN_SET
 }
 
