@@ -12,10 +12,12 @@ are resident in HBM.  Prints one JSON line (see the keys below).
 `value` is pairs/s of the selected form (split BF16 by default); the plain
 FP32 form is timed beside it on the same plan and input, and the ratio and
 the relative L2 difference between the two forms' outputs are reported.
-`cpu_baseline` times the oracle's C FP64 loop nest (Phi materialized, as
-the reference's CPU path) on a sample of the config-1 pair, with the fused
-loop beside it; `config0_full` times config 0 in full on both the CPU loops
-and the device.
+`err_vs_fp64` gives each form's relative L2 and max-abs error against the
+FP64 oracle on a dense input at 3 of the frequencies (outside the timed
+region).  `cpu_baseline` times the oracle's C FP64 loop nest (Phi
+materialized, as the reference's CPU path) on a sample of the config-1 pair,
+with the fused loop beside it; `config0_full` times config 0 in full on both
+the CPU loops and the device.
 """
 import argparse
 import json
@@ -131,6 +133,40 @@ def cpu_baseline(op, spec, n_mixes: int, threads: int):
                   "sample": f"the oracle's fused FP64 loop (no Phi) on the same sample in {dt_f:.1f} s"},
         "host": host_cpu(),
     }
+
+
+def err_vs_fp64(op, spec, forms, f_sel=(0, 128, 256)):
+    """The checker: error of each numeric form against the FP64 oracle on a
+    dense input over the whole FOV, at 3 of the frequencies (first, middle,
+    last) -- the forward of the speckle phantom at those frequencies'
+    observations and the adjoint of random u supported on them -- through
+    the same device entry points and kernels as the timed pairs.  Runs
+    outside the timed region."""
+    import torch
+    from oracle import born_oracle as O
+    t0 = time.perf_counter()
+    f_sel = [i for i in f_sel if i < len(spec.frequencies)]
+    T, rows = O.frequency_tables(spec, np.asarray(f_sel))
+    g = spec_gamma(spec)
+    rng = np.random.Generator(np.random.PCG64(spec.seed + 200))
+    u_sub = rng.standard_normal(len(rows)) + 1j * rng.standard_normal(len(rows))
+    u_ref = O.forward_by_frequency(T, g)
+    g_ref = O.adjoint_by_frequency(T, u_sub)
+    t_oracle = time.perf_counter() - t0
+    x = torch.tensor(g.astype(np.complex64), device="cuda")
+    uu = torch.zeros(op.N_observations, dtype=torch.complex64, device="cuda")
+    uu[torch.from_numpy(rows).cuda()] = torch.from_numpy(u_sub.astype(np.complex64)).cuda()
+    out = {"sample": f"dense speckle phantom over all {spec.fov.N_points} points; frequencies {list(f_sel)} "
+                     f"({len(rows)} observations); FP64 oracle in {t_oracle:.1f} s"}
+    for form in forms:
+        op.set_precision(form)
+        u = op.forward_dev(x).cpu().numpy()[rows]
+        gh = op.adjoint_dev(uu).cpu().numpy()
+        out[form] = {"forward": O.rel_l2(u, u_ref), "adjoint": O.rel_l2(gh, g_ref),
+                     "max_abs": {"forward": O.max_abs(u, u_ref), "adjoint": O.max_abs(gh, g_ref)},
+                     "max_abs_rel": {"forward": O.max_abs(u, u_ref) / float(np.max(np.abs(u_ref))),
+                                     "adjoint": O.max_abs(gh, g_ref) / float(np.max(np.abs(g_ref)))}}
+    return out
 
 
 def config0_full(threads: int, steps: int):
@@ -433,6 +469,11 @@ def main():
         result["roofline_fp32"] = roofline(F, k32, "fp32", args.config)
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        if spec.fov.N_dimensions == 2 and n_rhs == 1:
+            try:
+                result["err_vs_fp64"] = err_vs_fp64(op, spec, forms)
+            except Exception as e:  # report, do not hide
+                result["err_vs_fp64"] = {"error": repr(e)}
         try:
             result["cpu_baseline"] = cpu_baseline(op, spec, args.cpu_mixes, threads)
         except Exception as e:  # report, do not hide

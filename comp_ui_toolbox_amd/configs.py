@@ -28,7 +28,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
-from typing import List, Optional, Sequence, Tuple
+from typing import Callable, List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -36,13 +36,30 @@ T_CLK = 1.0 / 80e6
 
 
 @dataclasses.dataclass
+class LensSpec:
+    """Acoustic lens on every element face (transducers/lens.m): lateral foci
+    at the axial distances `axial_focus_axis`, one per face axis (inf = no
+    focusing along that axis; +thickness/focus_axial.m), in a lossless lens
+    material of sound speed `c_lens` (absorption_models/none.m:83).  The
+    defaults are those of the L14-5/38 (parameters_L14_5_38.m:32-34)."""
+    axial_focus_axis: Tuple[float, ...] = (math.inf, 16e-3)
+    c_lens: float = 1540.0
+
+
+@dataclasses.dataclass
 class ArraySpec:
     """Regular planar orthogonal array; element centres are centred on the
-    origin (array_planar_regular.m:63-66)."""
+    origin (array_planar_regular.m:63-66).  Every element carries the same
+    face: `apodization` maps the face points' normalized positions (an
+    (N_face, N_axes) array with entries in [-1, 1]) to real weights (None =
+    uniform, +apodization/uniform.m), `lens` is an optional acoustic lens;
+    together they give the complex face weights of face.m:359-376."""
     N_elements_axis: Tuple[int, ...]        # (N_x,) in 2-D, (N_x, N_y) in 3-D
     pitch_axis: Tuple[float, ...]
     width_axis: Tuple[float, ...]
     N_face_points_axis: Tuple[int, ...]     # integration points per face axis
+    lens: Optional[LensSpec] = None
+    apodization: Optional[Callable[[np.ndarray], np.ndarray]] = None
 
     @property
     def N_dimensions(self) -> int:

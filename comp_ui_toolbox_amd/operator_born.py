@@ -142,6 +142,8 @@ class OperatorBorn:
         pitch = list(spec.array.pitch_axis) + [0.0] * (2 - spec.array.N_dimensions)
         nfp = list(spec.array.N_face_points_axis) + [1] * (2 - spec.array.N_dimensions)
         fdl = [w / n for w, n in zip(spec.array.width_axis, spec.array.N_face_points_axis)] + [0.0] * (2 - spec.array.N_dimensions)
+        fw = physics.face_weights(spec, spec.frequencies)
+        fw_ptr = None if fw is None else dptr(keeper.keep(_c128_colmajor(fw)))
         ntx = (C.c_int32 * spec.N_measurements)()
         txi = (C.POINTER(C.c_double) * spec.N_measurements)()
         vd = (C.POINTER(C.c_double) * spec.N_measurements)()
@@ -154,7 +156,7 @@ class OperatorBorn:
         d = _lib.GeometryDesc(
             nd, (C.c_int32 * 3)(*npa), (C.c_double * 3)(*dla), spec.fov.z0,
             (C.c_int32 * 2)(*nea), (C.c_double * 2)(*pitch), (C.c_int32 * 2)(*nfp), (C.c_double * 2)(*fdl),
-            None, len(f), dptr(f), dptr(k),
+            fw_ptr, len(f), dptr(f), dptr(k),
             _lib.FILTER_CODES[spec.filter_rx.kind], _lib.FILTER_CODES[spec.filter_tx.kind],
             spec.filter_rx.param, spec.filter_tx.param,
             spec.N_measurements, M, ntx, txi, vd, device, _lib.PRECISIONS[precision])

@@ -53,3 +53,42 @@ def normal_velocities(spec: SequenceSpec, m: int) -> np.ndarray:
     tau_q = np.round(np.asarray(wave.delays) / T_CLK) * T_CLK
     U = spec.excitation_spectrum(f)
     return U[:, None] * np.asarray(wave.apodization)[None, :] * np.exp(-2j * math.pi * f[:, None] * tau_q[None, :])
+
+
+def face_weights(spec: SequenceSpec, f: np.ndarray):
+    """Complex weights of the face integration points, N_face x N_f
+    (face.m:359-376): apodization(r) exp(-j k_lens(f) t(r)), r the points'
+    positions relative to the face centre normalized by the half widths
+    (face.m:219-226), t the lens thickness of +thickness/focus_axial.m
+    (sum over focused axes of (sqrt(1 + F^2) - sqrt(r^2 + F^2)) w/2 with
+    F = 2 focus / w), k_lens = 2 pi f / c_lens (lossless lens, none.m:83).
+    Returns None for uniform apodization without a lens (all weights 1).
+    The face must be mirror symmetric (face.m:290-347), as the shared
+    reference transfer function requires; otherwise ValueError."""
+    arr = spec.array
+    if arr.lens is None and arr.apodization is None:
+        return None
+    offs, _ = arr.face_offsets()
+    width = np.asarray(arr.width_axis, dtype=np.float64)
+    rel = 2.0 * offs / width[None, :]
+    apod = np.ones(rel.shape[0]) if arr.apodization is None else \
+        np.asarray(arr.apodization(rel), dtype=np.float64).reshape(-1)
+    thick = np.zeros(rel.shape[0])
+    k_lens = np.zeros(len(np.atleast_1d(f)))
+    if arr.lens is not None:
+        foc = np.asarray(arr.lens.axial_focus_axis, dtype=np.float64)
+        if foc.shape != width.shape:
+            raise ValueError("the lens needs one axial focus per face axis")
+        for d in np.nonzero(np.isfinite(foc))[0]:
+            Fn = 2.0 * foc[d] / width[d]
+            thick += (math.sqrt(1.0 + Fn * Fn) - np.sqrt(rel[:, d] ** 2 + Fn * Fn)) * width[d] / 2.0
+        k_lens = 2.0 * math.pi * np.asarray(f, dtype=np.float64).reshape(-1) / arr.lens.c_lens
+    # mirror symmetry of apodization and thickness along every face axis
+    grid = tuple(arr.N_face_points_axis)
+    for vals in (apod, thick):
+        v = vals.reshape(grid, order="F")
+        for ax in range(v.ndim):
+            if np.max(np.abs(v - np.flip(v, axis=ax)), initial=0.0) > 1e-12 * max(1.0, np.max(np.abs(v))):
+                raise ValueError("combined_quick_gpu:NoSymmetricGridsSetup: face apodization and lens must be "
+                                 "mirror symmetric")
+    return apod[:, None] * np.exp(-1j * thick[:, None] * k_lens[None, :])

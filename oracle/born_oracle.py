@@ -163,26 +163,77 @@ def filter_flags(spec, f: np.ndarray, element: int, positions: np.ndarray) -> np
 
 
 # ---------------------------------------------------------------------------
+# face weights (apodization and acoustic lens)
+# ---------------------------------------------------------------------------
+def lens_thickness(focus_axis, width_axis, rel_norm: np.ndarray) -> np.ndarray:
+    """Thickness profile of a lens focusing laterally at the axial distances
+    `focus_axis` (+thickness/focus_axial.m:21-36): for each face point with
+    normalized relative position rel_norm (rows, one column per face axis),
+    sum over the finite foci F of (sqrt(1 + Fn^2) - sqrt(r^2 + Fn^2)) w / 2,
+    Fn = 2 F / w.  All foci infinite gives zeros."""
+    rel_norm = np.atleast_2d(np.asarray(rel_norm, dtype=np.float64))
+    out = np.zeros(rel_norm.shape[0])
+    for d, (F, w) in enumerate(zip(focus_axis, width_axis)):
+        if math.isinf(F):
+            continue
+        Fn = 2.0 * F / w
+        peak = math.sqrt(1.0 + Fn ** 2)
+        for i in range(rel_norm.shape[0]):
+            out[i] += (peak - math.sqrt(rel_norm[i, d] ** 2 + Fn ** 2)) * w / 2.0
+    return out
+
+
+def face_weights(spec, f: np.ndarray) -> Optional[np.ndarray]:
+    """Complex apodization weights of the face integration points
+    (face.compute_weights_scalar, face.m:359-376): apodization(rel_norm) .*
+    exp(-1j * thickness * k_lens(f)), with the normalized positions
+    2 (r - r_ctr) / widths (face.m:219-226), the thickness of
+    lens_thickness and the wavenumbers of the lossless lens material,
+    2 pi f / c_lens (absorption_models/none.m:83).  (N_face, N_f); None for
+    a uniform face without a lens (every weight 1)."""
+    arr = spec.array
+    if getattr(arr, "lens", None) is None and getattr(arr, "apodization", None) is None:
+        return None
+    offs, _ = arr.face_offsets()
+    rel_norm = offs / (np.asarray(arr.width_axis, dtype=np.float64)[None, :] / 2.0)
+    f = np.atleast_1d(np.asarray(f, dtype=np.float64))
+    apod = np.ones(offs.shape[0]) if arr.apodization is None else np.asarray(arr.apodization(rel_norm), float)
+    W = np.empty((offs.shape[0], len(f)), dtype=np.complex128)
+    if arr.lens is None:
+        W[:] = apod.reshape(-1, 1)
+        return W
+    thick = lens_thickness(arr.lens.axial_focus_axis, arr.width_axis, rel_norm)
+    k_lens = 2.0 * math.pi * f / arr.lens.c_lens
+    for q in range(offs.shape[0]):
+        W[q] = apod.reshape(-1)[q] * np.exp(-1j * thick[q] * k_lens)
+    return W
+
+
+# ---------------------------------------------------------------------------
 # spatial transfer function, shift map
 # ---------------------------------------------------------------------------
 def transfer_function(spec, f: np.ndarray, element: int = 0, filt=None,
                       positions: Optional[np.ndarray] = None,
-                      face_weights: Optional[np.ndarray] = None) -> np.ndarray:
+                      face_weights_fq: Optional[np.ndarray] = None) -> np.ndarray:
     """h[f, p] = -2 dA sum_face w G(k_f, |r_p - r_face|) (setup.m:1216-1281),
     then the anti-aliasing taper of the same element (setup.m:902,
-    on.m:49-72).  `element` is 0-based; `positions` defaults to the FOV grid."""
+    on.m:49-72).  `element` is 0-based; `positions` defaults to the FOV grid.
+    The face weights w (N_face, N_f) default to face_weights(spec, f)
+    (setup.m:1241, :1262)."""
     if positions is None:
         positions = spec.fov.positions()
     nd = spec.fov.N_dimensions
     k = wavenumbers(f, spec.medium)
     offs, dA = spec.array.face_offsets()
     ctr = spec.array.positions_ctr()[element]
+    if face_weights_fq is None:
+        face_weights_fq = face_weights(spec, f)
     h = np.zeros((len(f), positions.shape[0]), dtype=np.complex128)
     for q in range(offs.shape[0]):
         fp = np.concatenate([ctr + offs[q], [0.0]])
         D = np.linalg.norm(positions - fp[None, :], axis=1)
         G = greens_function(k[:, None], D[None, :], nd)
-        w = 1.0 if face_weights is None else face_weights[q][:, None]
+        w = 1.0 if face_weights_fq is None else face_weights_fq[q][:, None]
         h -= 2.0 * dA * w * G
     if filt is not None and filt.kind != "off":
         flags = filter_flags(spec, f, element, positions)
@@ -610,6 +661,8 @@ def sampled_tables(spec, points: np.ndarray, f_sel: np.ndarray):
     from . import ref_loop
     if spec.fov.N_dimensions != 3:
         raise ValueError("sampled_tables: 3-D sequences only")
+    if face_weights(spec, spec.frequencies[:1]) is not None:
+        raise ValueError("sampled_tables: uniform faces without a lens only")
     N_e = spec.array.N_elements
     points = np.asarray(points)
     pos = spec.fov.positions()[points]
@@ -655,3 +708,145 @@ def sampled_tables(spec, points: np.ndarray, f_sel: np.ndarray):
                    tuple(spec.N_points_per_pitch_axis))
     return T, np.asarray(rows, dtype=np.int64)
 
+
+
+# ---------------------------------------------------------------------------
+# dense inputs over the whole FOV at a few frequencies
+# ---------------------------------------------------------------------------
+def frequency_tables(spec, f_sel: np.ndarray):
+    """The operator restricted to the global frequencies `f_sel` (0-based),
+    over every grid point, through the reference's own table layout: h_ref on
+    the FOV with the shift/mirror map (setup_grid_symmetric.m:95-241) and
+    p_inc from h_ref with the tx filter (sequence.m:825-840).  Every factor
+    of a row of A -- transfer function, taper, prefactor, normal velocity --
+    depends on that row's frequency alone, so these rows equal the full
+    operator's.  p_inc is formed per frequency as the product of the normal
+    velocities with the gathered element transfer functions.  Returns
+    (T, rows): rows holds the 0-based indices of T's observations in the full
+    observation vector."""
+    f_sel = np.asarray(sorted(set(int(i) for i in f_sel)), dtype=np.int64)
+    pos_of = {int(g): i for i, g in enumerate(f_sel)}
+    f = spec.frequencies[f_sel]
+    shift = shift_lateral(spec)
+    h_rx = transfer_function(spec, f, 0, spec.filter_rx)
+    same = (spec.filter_tx.kind, spec.filter_tx.param) == (spec.filter_rx.kind, spec.filter_rx.param)
+    h_tx = h_rx if same else transfer_function(spec, f, 0, spec.filter_tx)
+    pref_u = prefactors_unique(spec)
+    keeps = []
+    for m in range(spec.N_measurements):
+        fidx = spec.measurement_frequencies(m)
+        keeps.append(np.array([j for j, g in enumerate(fidx) if int(g) in pos_of], dtype=np.int64))
+    p_rows = [np.zeros((len(kp), spec.fov.N_points), dtype=np.complex128) for kp in keeps]
+    vds = [normal_velocities(spec, m) for m in range(spec.N_measurements)]
+    for i, g in enumerate(f_sel):
+        He = h_tx[i][shift.T - 1]                      # (N_e, N_pts): element e's transfer function
+        for m in range(spec.N_measurements):
+            fidx = spec.measurement_frequencies(m)
+            for r, j in enumerate(keeps[m]):
+                if int(fidx[j]) != int(g):
+                    continue
+                v = np.zeros(spec.array.N_elements, dtype=np.complex128)
+                v[np.asarray(spec.waves[m].indices_active)] = vds[m][j]
+                p_rows[m][r] = v @ He
+    meas, rows = [], []
+    obs = 0
+    for m in range(spec.N_measurements):
+        fidx = spec.measurement_frequencies(m)
+        sub_of = {int(j): r for r, j in enumerate(keeps[m])}
+        mixes = []
+        for mix in spec.mixes[m]:
+            fk = np.arange(len(fidx)) if mix.indices_f is None else np.asarray(mix.indices_f)
+            jj = [j for j in range(len(fk)) if int(fk[j]) in sub_of]
+            rows.extend(obs + j for j in jj)
+            fk_kept = np.array([fk[j] for j in jj], dtype=np.int64)
+            pref = pref_u[fidx[fk_kept]][:, None] * np.asarray(mix.weights)[None, :]
+            mixes.append(MixTables(np.array([sub_of[int(x)] for x in fk_kept], dtype=np.int64) + 1,
+                                   np.asarray(mix.indices_active) + 1, pref.astype(np.complex128)))
+            obs += len(fk)
+        meas.append(MeasTables(np.array([pos_of[int(fidx[j])] for j in keeps[m]], dtype=np.int64) + 1,
+                               p_rows[m], mixes))
+    T = BornTables(h_rx, shift, meas, tuple(spec.fov.N_points_axis), tuple(spec.array.N_elements_axis),
+                   tuple(spec.N_points_per_pitch_axis))
+    return T, np.asarray(rows, dtype=np.int64)
+
+
+def _rows_by_frequency(T: BornTables):
+    """For each measurement m and each of its unique frequencies j: the list
+    of (observation index, mix k, row i of the mix) reading it."""
+    out = []
+    obs = 0
+    for ms in T.meas:
+        per = [[] for _ in range(len(ms.idx_f_to_global))]
+        for k, mx in enumerate(ms.mixes):
+            for i, j1 in enumerate(mx.idx_f_to_meas):
+                per[j1 - 1].append((obs + i, k, i))
+            obs += len(mx.idx_f_to_meas)
+        out.append(per)
+    return out
+
+
+def forward_by_frequency(T: BornTables, gamma: np.ndarray) -> np.ndarray:
+    """u = A gamma over every grid point (no occupancy compaction), the loop
+    nest of forward_quick_cpu (operator_born.m:353-459) reordered frequency
+    first: for each global frequency, the element transfer functions H[e, p]
+    = h_ref[f, shift[p, e]] are gathered once and V_m = H (p_inc_m .* gamma)
+    holds every element's sum for measurement m; u(obs) = sum_a pref(i, a)
+    V_m[e_a].  For dense inputs at full FOV size.  (N_obs, N_rhs)."""
+    g = np.asarray(gamma, dtype=np.complex128)
+    squeeze = g.ndim == 1
+    if squeeze:
+        g = g[:, None]
+    out = np.zeros((T.N_observations(), g.shape[1]), dtype=np.complex128)
+    by_f = _rows_by_frequency(T)
+    for fg in range(T.h_ref.shape[0]):
+        users = [(m, j) for m, ms in enumerate(T.meas) for j in range(len(ms.idx_f_to_global))
+                 if ms.idx_f_to_global[j] - 1 == fg and by_f[m][j]]
+        if not users:
+            continue
+        H = T.h_ref[fg][T.shift.T - 1]
+        n_r = g.shape[1]
+        B = np.concatenate([T.meas[m].p_incident[j][:, None] * g for m, j in users], axis=1)
+        V_all = H @ B
+        for c, (m, j) in enumerate(users):
+            ms = T.meas[m]
+            V = V_all[:, c * n_r:(c + 1) * n_r]
+            for o, k, i in by_f[m][j]:
+                mx = ms.mixes[k]
+                out[o] = mx.prefactors[i] @ V[mx.idx_active - 1]
+    return out[:, 0] if squeeze else out
+
+
+def adjoint_by_frequency(T: BornTables, u: np.ndarray) -> np.ndarray:
+    """gamma_hat = A^H u over every grid point, the loop nest of
+    adjoint_quick_cpu (operator_born.m:511-593) reordered frequency first:
+    W_m[e] = sum over the rows reading (m, f) of conj(pref(i, a)) u(obs) for
+    e = e_a, then gamma_hat += conj(p_inc_m) .* (H^H W_m).  (N_pts, N_rhs)."""
+    uu = np.asarray(u, dtype=np.complex128)
+    squeeze = uu.ndim == 1
+    if squeeze:
+        uu = uu[:, None]
+    out = np.zeros((T.N_points, uu.shape[1]), dtype=np.complex128)
+    by_f = _rows_by_frequency(T)
+    for fg in range(T.h_ref.shape[0]):
+        users = [(m, j) for m, ms in enumerate(T.meas) for j in range(len(ms.idx_f_to_global))
+                 if ms.idx_f_to_global[j] - 1 == fg and by_f[m][j]]
+        if not users:
+            continue
+        H = T.h_ref[fg][T.shift.T - 1]
+        n_r = uu.shape[1]
+        W = np.zeros((T.N_elements, len(users) * n_r), dtype=np.complex128)
+        for c, (m, j) in enumerate(users):
+            ms = T.meas[m]
+            for o, k, i in by_f[m][j]:
+                mx = ms.mixes[k]
+                np.add.at(W[:, c * n_r:(c + 1) * n_r], mx.idx_active - 1,
+                          np.conj(mx.prefactors[i])[:, None] * uu[o][None, :])
+        Z = (W.conj().T @ H).conj()                    # (users * N_rhs, N_pts) = (H^H W)^T
+        for c, (m, j) in enumerate(users):
+            out += np.conj(T.meas[m].p_incident[j])[:, None] * Z[c * n_r:(c + 1) * n_r].T
+    return out[:, 0] if squeeze else out
+
+
+def max_abs(x: np.ndarray, ref: np.ndarray) -> float:
+    """Largest absolute deviation |x - ref| over all entries."""
+    return float(np.max(np.abs(np.asarray(x) - np.asarray(ref)), initial=0.0))

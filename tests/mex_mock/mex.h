@@ -28,6 +28,8 @@ bool mxIsDouble(const mxArray* a);
 bool mxIsComplex(const mxArray* a);
 bool mxIsNumeric(const mxArray* a);
 bool mxIsClass(const mxArray* a, const char* name);
+bool mxIsChar(const mxArray* a);
+int mxGetString(const mxArray* a, char* buf, mwSize buflen);
 size_t mxGetNumberOfElements(const mxArray* a);
 mwSize mxGetNumberOfDimensions(const mxArray* a);
 size_t mxGetM(const mxArray* a);

@@ -15,7 +15,8 @@
 #include <vector>
 
 struct mxArray_tag {
-  enum Kind { DOUBLE, CELL, OBJECT } kind = DOUBLE;
+  enum Kind { DOUBLE, CELL, OBJECT, CHAR } kind = DOUBLE;
+  std::string text;                                   // CHAR arrays
   std::string cls = "double";
   size_t M = 0, N = 0;
   bool cplx = false;
@@ -66,6 +67,12 @@ bool mxIsDouble(const mxArray* a) { return a && a->kind == mxArray::DOUBLE; }
 bool mxIsComplex(const mxArray* a) { return a && a->kind == mxArray::DOUBLE && a->cplx; }
 bool mxIsNumeric(const mxArray* a) { return a && a->kind == mxArray::DOUBLE; }
 bool mxIsClass(const mxArray* a, const char* name) { return a && a->cls == name; }
+bool mxIsChar(const mxArray* a) { return a && a->kind == mxArray::CHAR; }
+int mxGetString(const mxArray* a, char* buf, mwSize buflen) {
+  if (!a || a->kind != mxArray::CHAR || buflen == 0) return 1;
+  std::snprintf(buf, buflen, "%s", a->text.c_str());
+  return a->text.size() + 1 > buflen ? 1 : 0;
+}
 size_t mxGetNumberOfElements(const mxArray* a) {
   if (!a) return 0;
   if (a->kind == mxArray::OBJECT) return a->props.size();
@@ -98,6 +105,15 @@ mxArray* mxCreateDoubleMatrix(mwSize m, mwSize n, mxComplexity c) {
 mxArray* mock_double(size_t m, size_t n, int cplx, const double* data) {
   mxArray* a = mxCreateDoubleMatrix(m, n, cplx ? mxCOMPLEX : mxREAL);
   if (data) std::memcpy(a->data.data(), data, a->data.size() * sizeof(double));
+  return a;
+}
+mxArray* mock_char(const char* text) {
+  mxArray* a = make();
+  a->kind = mxArray::CHAR;
+  a->cls = "char";
+  a->text = text;
+  a->M = 1;
+  a->N = a->text.size();
   return a;
 }
 mxArray* mock_cell(size_t n) {

@@ -23,6 +23,8 @@ class Mex:
         L = C.CDLL(str(build_mock.LIB))
         L.mock_double.argtypes = [C.c_size_t, C.c_size_t, C.c_int, C.POINTER(C.c_double)]
         L.mock_double.restype = vp
+        L.mock_char.argtypes = [C.c_char_p]
+        L.mock_char.restype = vp
         L.mock_cell.argtypes = [C.c_size_t]
         L.mock_cell.restype = vp
         L.mock_cell_set.argtypes = [vp, C.c_size_t, vp]
@@ -48,6 +50,9 @@ class Mex:
             flat = np.ascontiguousarray(flat).view(np.float64)
         flat = np.ascontiguousarray(flat)
         return self.L.mock_double(a.shape[0], a.shape[1], int(cplx), flat.ctypes.data_as(C.POINTER(C.c_double)))
+
+    def char(self, text):
+        return self.L.mock_char(text.encode())
 
     def cell(self, items):
         c = self.L.mock_cell(len(items))
@@ -85,9 +90,12 @@ class Mex:
 
 
 def operator_object(mx, T, setup_cls="scattering.sequences.setups.setup_grid_symmetric",
-                    gpu_cls="scattering.options.gpu_active", h_ref=None):
-    """A scattering.operator_born with the properties the MEX reads."""
+                    gpu_cls="scattering.options.gpu_active", h_ref=None, sel=None):
+    """A scattering.operator_born with the properties the MEX reads; the
+    sequence holds every measurement of T, `sel` (1-based, default all) is
+    indices_measurement_sel (operator.m:92-96)."""
     n_meas = len(T.meas)
+    sel = np.arange(1, n_meas + 1, dtype=float) if sel is None else np.asarray(sel, dtype=float)
     fov_grid = mx.obj("math.grid_regular", [{"N_points_axis": mx.double(np.asarray(T.N_points_axis, float))}])
     setup = mx.obj(setup_cls, [{
         "indices_grid_FOV_shift": mx.double(np.asarray(T.shift, float)),
@@ -121,7 +129,7 @@ def operator_object(mx, T, setup_cls="scattering.sequences.setups.setup_grid_sym
                             [{"gpu": mx.obj(gpu_cls, [{"index": mx.double(0.0)}])}])}])
     return mx.obj("scattering.operator_born", [{
         "sequence": seq,
-        "indices_measurement_sel": mx.double(np.arange(1, n_meas + 1, dtype=float)),
+        "indices_measurement_sel": mx.double(sel),
         "options": options,
         "incident_waves": waves,
     }])
@@ -150,6 +158,13 @@ def test_argument_checks_and_error_identifiers(mex):
     assert mex.call([op, mex.double(np.ones((1, 2))), x]) == (None, "combined_quick_gpu:NoNumericScalarMode")
     real_h = operator_object(mex, T, h_ref=np.real(T.h_ref))
     assert mex.call([real_h, one, x]) == (None, "combined_quick_gpu:NoComplexDoubles")
+    # the optional fourth input names the numeric form; a fifth is too many
+    assert mex.call([op, one, x, mex.char("fp64")]) == (None, "combined_quick_gpu:NumberArguments")
+    assert mex.call([op, one, x, mex.double(7.0)]) == (None, "combined_quick_gpu:NumberArguments")
+    assert mex.call([op, one, x, mex.char("fp32"), one]) == (None, "combined_quick_gpu:NumberArguments")
+    # a selection outside the sequence's measurements
+    assert mex.call([operator_object(mex, T, sel=[3]), one, x]) == (None, "born:InvalidArgument")
+    assert mex.call([operator_object(mex, T, sel=[0]), one, x]) == (None, "born:InvalidArgument")
 
 
 def test_missing_property_is_reported(mex):
@@ -210,3 +225,91 @@ def test_integration_listing_is_the_shim_source():
     root = Path(__file__).resolve().parent.parent
     src = (root / "comp_ui_toolbox_amd" / "mex" / "combined_quick_gpu.cpp").read_text()
     assert "```cpp\n" + src + "```\n" in (root / "INTEGRATION.md").read_text()
+
+
+def three_measurement_tables():
+    """Three measurements of different waves and frequency subsets, mixes of
+    one to three active elements."""
+    from tests.test_oracle import tiny_spec
+    from comp_ui_toolbox_amd import configs as Cf
+    rng = np.random.Generator(np.random.PCG64(17))
+    spec = tiny_spec(freqs=np.array([2.0e6, 2.6e6, 3.1e6, 4.5e6]))
+    arr = spec.array
+    spec.waves = [Cf.qpw_wave(arr, a, 1500.0) for a in (0.1, -0.25, 0.3)]
+    spec.meas_f_indices = [np.array([0, 1, 3]), np.array([3, 2]), np.array([1, 2, 3, 0])]
+    spec.mixes = [
+        [Cf.MixSpec(np.array([0, 2]), np.array([1.0, -1.0])), Cf.MixSpec(np.array([3]), np.array([2.0]))],
+        [Cf.MixSpec(np.array([1, 2, 3]), rng.standard_normal(3) + 1j * rng.standard_normal(3))],
+        [Cf.MixSpec(np.array([0]), np.array([1.0]), np.array([3, 1])),
+         Cf.MixSpec(np.array([2, 3]), np.array([0.5, 1.0]))],
+    ]
+    return spec, O.build_tables(spec)
+
+
+def selected(T, sel):
+    """The operator tables of the 1-based selection `sel` (operator.m:92-96)."""
+    return O.BornTables(T.h_ref, T.shift, [T.meas[s - 1] for s in sel], T.N_points_axis, T.N_elements_axis,
+                        T.N_points_per_pitch_axis)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sel", [[2], [3, 1], [1, 2, 3]])
+def test_selected_measurements_read_their_own_incident_waves(mex, torch_cuda, sel):
+    """indices_measurement_sel picks measurements out of the sequence; each
+    reads incident_waves(index_measurement), its own settings, frequency map
+    and prefactors (cu:238-241, :807-823; operator_born.m:382, :542)."""
+    _, T = three_measurement_tables()
+    op = operator_object(mex, T, sel=sel)
+    Ts = selected(T, sel)
+    rng = np.random.Generator(np.random.PCG64(23))
+    g = rng.standard_normal((T.N_points, 2)) + 1j * rng.standard_normal((T.N_points, 2))
+    out, err = mex.call([op, mex.double(1.0), mex.double(g)])
+    assert err is None and out.shape == (Ts.N_observations(), 2)
+    assert O.rel_l2(out, O.forward(Ts, g)) < 1e-5
+    u = rng.standard_normal((Ts.N_observations(), 1)) + 1j * rng.standard_normal((Ts.N_observations(), 1))
+    out, err = mex.call([op, mex.double(2.0), mex.double(u)])
+    assert err is None and O.rel_l2(out, O.adjoint(Ts, u)) < 1e-5
+
+
+@pytest.mark.gpu
+def test_plan_cache_follows_every_input(mex, torch_cuda):
+    """The cached plan is rebuilt when an input it depends on changes: the
+    prefactors of one mix, the active set of one mix, the selection; and is
+    reused when nothing changed.  mode 1.5 truncates to 1 (forward), as
+    (int)mxGetScalar does (cu:404-409)."""
+    _, T = three_measurement_tables()
+    rng = np.random.Generator(np.random.PCG64(29))
+    g = rng.standard_normal((T.N_points, 1))
+    out, err = mex.call([operator_object(mex, T), mex.double(1.0), mex.double(g)])
+    assert err is None and O.rel_l2(out, O.forward(T, g)) < 1e-5
+    T2 = three_measurement_tables()[1]
+    T2.meas[1].mixes[0].prefactors = T2.meas[1].mixes[0].prefactors * (1.0 - 0.5j)
+    out, err = mex.call([operator_object(mex, T2), mex.double(1.0), mex.double(g)])
+    assert err is None and O.rel_l2(out, O.forward(T2, g)) < 1e-5
+    assert O.rel_l2(out, O.forward(T, g)) > 1e-2
+    T3 = three_measurement_tables()[1]
+    T3.meas[2].mixes[1].idx_active = np.array([1, 4])
+    out, err = mex.call([operator_object(mex, T3), mex.double(1.0), mex.double(g)])
+    assert err is None and O.rel_l2(out, O.forward(T3, g)) < 1e-5
+    out, err = mex.call([operator_object(mex, T, sel=[1, 3]), mex.double(1.5), mex.double(g)])
+    assert err is None and O.rel_l2(out, O.forward(selected(T, [1, 3]), g)) < 1e-5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form", ["split_bf16", "split_fp16", 1])
+def test_split_form_through_the_fourth_argument(mex, torch_cuda, form):
+    """combined_quick_gpu(op, mode, x, 'split_bf16') runs the split form,
+    within its 1e-4 bar against the FP64 oracle; without the fourth argument
+    the plain FP32 form is used again."""
+    _, T = three_measurement_tables()
+    op = operator_object(mex, T)
+    rng = np.random.Generator(np.random.PCG64(31))
+    g = rng.standard_normal((T.N_points, 2)) + 1j * rng.standard_normal((T.N_points, 2))
+    u = rng.standard_normal((T.N_observations(), 2)) + 1j * rng.standard_normal((T.N_observations(), 2))
+    arg = mex.char(form) if isinstance(form, str) else mex.double(float(form))
+    out, err = mex.call([op, mex.double(1.0), mex.double(g), arg])
+    assert err is None and O.rel_l2(out, O.forward(T, g)) < 1e-4
+    out, err = mex.call([op, mex.double(2.0), mex.double(u), arg])
+    assert err is None and O.rel_l2(out, O.adjoint(T, u)) < 1e-4
+    out, err = mex.call([op, mex.double(1.0), mex.double(g)])
+    assert err is None and O.rel_l2(out, O.forward(T, g)) < 1e-5

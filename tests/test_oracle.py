@@ -475,3 +475,222 @@ def test_prefactors_lossless_and_lossy():
     k = O.wavenumbers(f, spec.medium)
     assert np.all(p.imag > 0)
     assert np.allclose(p.imag, 2 * dx * dz * k.real * (-k.imag), rtol=1e-13)
+
+
+# ------------------------------------------------ taper argument, closed forms
+def _lossy_medium():
+    # strong loss, so that |k| and Re(k) differ by about 0.15 % at 4 MHz
+    return C.MediumSpec(absorption=20.0)
+
+
+def _point_at(ctr_lat, sin_axis, R):
+    """Point at distance R from an element centre on the array plane, whose
+    unit vector has the lateral components sin_axis (the axial one
+    positive)."""
+    s = np.asarray(sin_axis, dtype=np.float64)
+    return np.concatenate([np.asarray(ctr_lat) + R * s, [R * math.sqrt(1.0 - np.sum(s * s))]])
+
+
+def test_taper_argument_closed_forms_2d():
+    """The taper multiplies element e's transfer function by the raised
+    cosine of Re(k) pitch |sin(theta)|, theta the angle from the element's
+    axis (setup.m:993-1017): 1 straight below the element, exactly 0.5 where
+    that argument is 3 pi / 4 (roll-off 0.5), 0 beyond pi.  The lossy medium
+    and a width of 0.85 pitch make |k| for Re(k), or the width for the
+    pitch, change the 0.5 by 1e-3 or more."""
+    spec = tiny_spec(N_e=5, filt="raised_cosine")
+    spec.medium = _lossy_medium()
+    f = np.array([4.0e6])
+    kr = float(np.real(O.wavenumbers(f, spec.medium))[0])
+    pitch = spec.array.pitch_axis[0]
+    e = 3
+    ctr = spec.array.positions_ctr()[e]
+    filt = C.FilterSpec("raised_cosine", 0.5)
+    cases = [(0.0, 1.0), (3 * math.pi / 4 / (kr * pitch), 0.5), (-3 * math.pi / 4 / (kr * pitch), 0.5),
+             (0.6 * math.pi / (kr * pitch), 0.5 * (1 + math.cos(math.pi * (0.6 - 0.5) / 0.5))),
+             (min(0.99, 1.05 * math.pi / (kr * pitch)), 0.0)]
+    for s, expect in cases:
+        pt = _point_at(ctr, [s], 7e-3)[None, :]
+        with_t = O.transfer_function(spec, f, e, filt, positions=pt)[0, 0]
+        without = O.transfer_function(spec, f, e, None, positions=pt)[0, 0]
+        assert abs(with_t / without - expect) < 1e-12, (s, with_t / without, expect)
+    # the same point seen from another element has another angle
+    s = 3 * math.pi / 4 / (kr * pitch)
+    pt = _point_at(ctr, [s], 7e-3)[None, :]
+    ratio = (O.transfer_function(spec, f, 0, filt, positions=pt) /
+             O.transfer_function(spec, f, 0, None, positions=pt))[0, 0]
+    assert abs(ratio - 0.5) > 0.1
+
+
+def test_taper_argument_closed_forms_3d():
+    """3-D: one factor per lateral axis with more than one element,
+    Re(k) pitch_axis |e_axis|; the product of the two factors; an axis with a
+    single element is left out (setup.m:1000-1004)."""
+    arr = C.ArraySpec((4, 3), (3e-4, 2.5e-4), (2.7e-4, 2.2e-4), (2, 2))
+    fov = C.FovSpec((9, 7, 3), (1.5e-4, 1.25e-4, 1.5e-4), 2e-3)
+    spec = C.SequenceSpec("t3", arr, _lossy_medium(), fov, np.array([4.0e6]),
+                          [C.qsw_wave(arr, np.array([0.0, 0.0, -1e-2]), 1500.0)], [C.identity_mixes(12)])
+    f = spec.frequencies
+    kr = float(np.real(O.wavenumbers(f, spec.medium))[0])
+    filt = C.FilterSpec("raised_cosine", 0.5)
+    e = 5
+    ctr = arr.positions_ctr()[e]
+    sx = 3 * math.pi / 4 / (kr * 3e-4)
+    sy = 3 * math.pi / 4 / (kr * 2.5e-4)
+    for sins, expect in [((sx, 0.0), 0.5), ((0.0, sy), 0.5), ((sx, -sy), 0.25), ((0.0, 0.0), 1.0)]:
+        pt = _point_at(ctr, sins, 6e-3)[None, :]
+        r = (O.transfer_function(spec, f, e, filt, positions=pt) /
+             O.transfer_function(spec, f, e, None, positions=pt))[0, 0]
+        assert abs(r - expect) < 1e-12, (sins, r, expect)
+    # a single row of elements: the y direction does not taper
+    arr1 = C.ArraySpec((4, 1), (3e-4, 2.5e-4), (2.7e-4, 2.2e-4), (2, 2))
+    spec1 = C.SequenceSpec("t31", arr1, _lossy_medium(), fov, f,
+                           [C.qsw_wave(arr1, np.array([0.0, 0.0, -1e-2]), 1500.0)], [C.identity_mixes(4)])
+    ctr1 = arr1.positions_ctr()[1]
+    pt = _point_at(ctr1, (sx, sy), 6e-3)[None, :]
+    r = (O.transfer_function(spec1, f, 1, filt, positions=pt) /
+         O.transfer_function(spec1, f, 1, None, positions=pt))[0, 0]
+    assert abs(r - 0.5) < 1e-12
+
+
+# --------------------------------------------------------- face weights, lens
+def test_lens_thickness_closed_form():
+    """focus_axial's normalized formula equals the path-length difference
+    sqrt(F^2 + (w/2)^2) - sqrt(F^2 + x^2) at the physical position x; zero at
+    the edges, its maximum at the centre, zero everywhere for no focus."""
+    w, F = 4e-3, 16e-3
+    r = np.array([-1.0, -0.5, 0.0, 0.3, 1.0])
+    t = O.lens_thickness((F,), (w,), r[:, None])
+    x = r * w / 2
+    assert np.max(np.abs(t - (math.hypot(F, w / 2) - np.hypot(F, x)))) < 1e-16      # 1e-12 of t(0)
+    assert abs(t[0]) < 1e-16 and abs(t[-1]) < 1e-16 and np.argmax(t) == 2
+    t2 = O.lens_thickness((math.inf, F), (2e-4, w), np.stack([r, r], axis=1))
+    assert np.allclose(t2, t, rtol=0, atol=1e-16)
+    assert np.all(O.lens_thickness((math.inf,), (w,), r[:, None]) == 0)
+
+
+def _lens_spec_3d(lens, c=1540.0, nfx=1, nfy=8, absorption=0.0):
+    arr = C.ArraySpec((3, 2), (3e-4, 4.2e-3), (2.7e-4, 4e-3), (nfx, nfy), lens=lens)
+    fov = C.FovSpec((6, 4, 3), (1.5e-4, 2.1e-3, 1.5e-4), 2e-3)
+    med = C.MediumSpec(c_ref=c, c_avg=c, absorption=absorption, dispersion=absorption > 0)
+    return C.SequenceSpec("lens3", arr, med, fov, np.array([3.0e6, 5.0e6]),
+                          [C.qsw_wave(arr, np.array([0.0, 0.0, -1e-2]), c)], [C.identity_mixes(6)])
+
+
+def test_lens_focuses_at_its_focal_distance():
+    """With c_lens equal to the lossless medium's sound speed, the lens delay
+    exp(-j k t(y)) makes every face point's contribution arrive in phase at
+    the elevation focus straight in front of the element:
+    |sum_q w_q G_q| = sum_q |G_q|.  A sign slip in the exponent, a thickness
+    in normalized units or the focus on the wrong axis breaks it."""
+    F = 12e-3
+    spec = _lens_spec_3d(C.LensSpec((math.inf, F), 1540.0))
+    e = 4
+    ctr = spec.array.positions_ctr()[e]
+    pt = np.array([[ctr[0], ctr[1], F]])
+    f = spec.frequencies
+    h = O.transfer_function(spec, f, e, None, positions=pt)[:, 0]
+    offs, dA = spec.array.face_offsets()
+    D = np.linalg.norm(pt[0][None, :] - np.concatenate([ctr[None, :] + offs, np.zeros((len(offs), 1))], axis=1), axis=1)
+    coherent = 2 * dA * np.sum(1.0 / (4 * math.pi * D))
+    assert np.max(np.abs(np.abs(h) - coherent)) < 1e-12 * coherent
+    spec0 = _lens_spec_3d(None)
+    h0 = O.transfer_function(spec0, f, e, None, positions=pt)[:, 0]
+    assert np.all(np.abs(h0) < 0.99 * coherent)
+    bad = _lens_spec_3d(C.LensSpec((F, math.inf), 1540.0))
+    hb = O.transfer_function(bad, f, e, None, positions=pt)[:, 0]
+    assert np.all(np.abs(hb) < 0.99 * coherent)
+
+
+def test_face_weights_closed_form_and_product_copy():
+    """Weights of the lens at the face centre: exp(-j 2 pi f t(0) / c_lens)
+    with t(0) = sqrt(F^2 + (w/2)^2) - F; at the face edges t -> 0.  An
+    apodization multiplies the weights.  The product's copy
+    (physics.face_weights) agrees."""
+    from comp_ui_toolbox_amd import physics
+    F, c_l = 10e-3, 1000.0
+    apod = lambda r: 1.0 - 0.5 * r[:, 1] ** 2
+    spec = _lens_spec_3d(C.LensSpec((math.inf, F), c_l), nfy=5)
+    spec.array.apodization = apod
+    f = spec.frequencies
+    W = O.face_weights(spec, f)
+    offs, _ = spec.array.face_offsets()
+    w_y = spec.array.width_axis[1]
+    centre = np.nonzero(np.abs(offs[:, 1]) < 1e-15)[0]
+    t0 = math.hypot(F, w_y / 2) - F
+    assert np.max(np.abs(W[centre] - np.exp(-2j * math.pi * f * t0 / c_l)[None, :])) < 1e-12
+    ty = math.hypot(F, w_y / 2) - np.hypot(F, offs[:, 1])
+    a = 1.0 - 0.5 * (offs[:, 1] / (w_y / 2)) ** 2
+    expect = a[:, None] * np.exp(-2j * math.pi * ty[:, None] * f[None, :] / c_l)
+    assert np.max(np.abs(W - expect)) < 1e-12
+    assert np.max(np.abs(physics.face_weights(spec, f) - W)) < 1e-14
+    assert O.face_weights(_lens_spec_3d(None), f) is None and physics.face_weights(_lens_spec_3d(None), f) is None
+    lop = _lens_spec_3d(None)
+    lop.array.apodization = lambda r: 1.0 + 0.3 * r[:, 1]     # not mirror symmetric
+    with pytest.raises(ValueError, match="NoSymmetricGridsSetup"):
+        physics.face_weights(lop, f)
+
+
+def test_face_weighted_transfer_function_by_face_points():
+    """h = -2 dA sum_q w_q G(k, D_q) with the face points placed explicitly:
+    3 points across a 2-D element at the centre and +-width/3, weights
+    (0.2, 1, 0.2) exp(-j k_lens t)."""
+    from scipy.special import hankel2
+    spec = tiny_spec(N_e=3, nfp=3, filt="off")
+    spec.array.apodization = lambda r: np.where(np.abs(r[:, 0]) > 0.1, 0.2, 1.0)
+    spec.array.lens = C.LensSpec((3e-3,), 1200.0)
+    e = 2
+    f = spec.frequencies
+    pos = spec.fov.positions()[::7]
+    h = O.transfer_function(spec, f, e, None, positions=pos)
+    w = spec.array.width_axis[0]
+    xc = spec.array.positions_ctr()[e][0]
+    k = O.wavenumbers(f, spec.medium)
+    acc = np.zeros_like(h)
+    for xq, a in ((xc - w / 3, 0.2), (xc, 1.0), (xc + w / 3, 0.2)):
+        t = math.hypot(3e-3, w / 2) - math.hypot(3e-3, xq - xc)
+        wq = a * np.exp(-2j * math.pi * f * t / 1200.0)
+        D = np.hypot(pos[:, 0] - xq, pos[:, 1])
+        acc += wq[:, None] * 0.25j * hankel2(0, k[:, None] * D[None, :])
+    assert O.rel_l2(h, -2 * (w / 3) * acc) < 1e-13
+
+
+# ------------------------------------------- dense inputs, frequency first
+def test_by_frequency_loop_order_equals_loop_nest():
+    """forward_by_frequency / adjoint_by_frequency (frequency-first order)
+    equal the measurement -> mix -> element loop nest, itself pinned by the
+    dense matrix, on mixed tables with frequency subsets, a repeated
+    frequency and several active elements per mix."""
+    spec, T = mixed_tiny_tables()
+    rng = np.random.Generator(np.random.PCG64(12))
+    g = rng.standard_normal((T.N_points, 2)) + 1j * rng.standard_normal((T.N_points, 2))
+    u = rng.standard_normal((T.N_observations(), 2)) + 1j * rng.standard_normal((T.N_observations(), 2))
+    assert O.rel_l2(O.forward_by_frequency(T, g), O.forward(T, g)) < 1e-13
+    assert O.rel_l2(O.adjoint_by_frequency(T, u), O.adjoint(T, u)) < 1e-13
+    assert O.rel_l2(O.forward_by_frequency(T, g[:, 0]), O.forward(T, g[:, 0])) < 1e-13
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_frequency_tables_are_rows_of_the_full_operator(mixed):
+    """frequency_tables(spec, f_sel) is the operator's block at the sampled
+    frequencies' observations over every point: its forward equals the rows
+    of the full operator, its adjoint the full adjoint of u supported on
+    those rows (with frequency subsets and +-1 mixing when mixed)."""
+    if mixed:
+        spec, T_full = mixed_tiny_tables()
+        spec.mixes[1][0] = C.MixSpec(np.array([0, 1, 2, 3]), np.array([1.0, -1.0, -1.0, 1.0]))
+        T_full = O.build_tables(spec)
+        f_sel = np.array([1, 4])
+    else:
+        spec = tiny_spec()
+        T_full = O.build_tables(spec)
+        f_sel = np.array([0, 2])
+    T_s, rows = O.frequency_tables(spec, f_sel)
+    assert T_s.N_observations() == len(rows) > 0
+    rng = np.random.Generator(np.random.PCG64(13))
+    g = rng.standard_normal((T_full.N_points, 2)) + 1j * rng.standard_normal((T_full.N_points, 2))
+    assert O.rel_l2(O.forward_by_frequency(T_s, g), O.forward(T_full, g)[rows]) < 1e-13
+    u = np.zeros((T_full.N_observations(), 2), dtype=np.complex128)
+    us = rng.standard_normal((len(rows), 2)) + 1j * rng.standard_normal((len(rows), 2))
+    u[rows] = us
+    assert O.rel_l2(O.adjoint_by_frequency(T_s, us), O.adjoint(T_full, u)) < 1e-13

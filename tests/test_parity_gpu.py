@@ -200,16 +200,24 @@ def c1():
     return spec, op
 
 
-def test_config1_forward_sparse_against_oracle(torch_cuda, c1):
+@pytest.mark.parametrize("sparse", [0, -1])
+def test_config1_forward_points_against_oracle(torch_cuda, c1, sparse):
+    """Host forward of gamma on 24 points: through the grid contraction
+    (sparse forward off) and through the gather path that the plan picks for
+    so few occupied points (the MEX's occupancy compaction)."""
     spec, op = c1
     rng = np.random.Generator(np.random.PCG64(21))
     pts = np.sort(rng.choice(spec.fov.N_points, 24, replace=False))
     R = O.restricted_tables(spec, pts)
     g = np.zeros(spec.fov.N_points)
     g[pts] = rng.standard_normal(len(pts))
-    u = op.forward(g)
+    op.set_sparse_forward(sparse)
+    try:
+        u = op.forward(g)
+    finally:
+        op.set_sparse_forward(-1)
     e = O.rel_l2(u, O.forward(R, g[pts]))
-    log("config1 forward (24 points) rel-L2 %.2e" % e)
+    log("config1 forward (24 points, %s) rel-L2 %.2e" % ("contraction" if sparse == 0 else "gather", e))
     assert e < TOL_FP32
 
 
@@ -334,6 +342,7 @@ def test_split_config1(torch_cuda, c1, form):
     g[pts] = rng.standard_normal(len(pts))
     u = rnd(op.N_observations, 82)
     op.set_precision(form)
+    op.set_sparse_forward(0)          # the contraction of this form, not the gather path
     try:
         e_f = O.rel_l2(op.forward(g), O.forward(R, g[pts]))
         e_a = O.rel_l2(op.adjoint(u)[pts], O.adjoint(R, u))
@@ -345,6 +354,7 @@ def test_split_config1(torch_cuda, c1, form):
         e_d = ((us - u32).norm() / u32.norm()).item()
     finally:
         op.set_precision("fp32")
+        op.set_sparse_forward(-1)
     log("%s config1: forward rel-L2 %.2e adjoint rel-L2 %.2e dense vs fp32 %.2e" % (form, e_f, e_a, e_d))
     assert e_f < TOL_SPLIT and e_a < TOL_SPLIT and e_d < TOL_SPLIT
 
@@ -415,20 +425,139 @@ def c2(torch_cuda):
     return spec, op, pts, g, u, O.forward(R, g[pts]), O.adjoint(R, u)
 
 
-@pytest.mark.parametrize("precision,tol", [("fp32", TOL_FP32), ("split_bf16", TOL_SPLIT)])
+@pytest.mark.parametrize("precision,tol", [("fp32", TOL_FP32), ("split_bf16", TOL_SPLIT), ("split_fp16", TOL_SPLIT)])
 def test_config2_against_oracle(torch_cuda, c2, precision, tol):
     """32 random waves (+-1 apodization, delays U[0, 5 us]) compounded into 16
     mixes of all 128 elements with +-1 weights: sampled points against tables
-    the oracle restricts exactly to them."""
+    the oracle restricts exactly to them, through the grid contraction."""
     spec, op, pts, g, u, ref_f, ref_a = c2
     op.set_precision(precision)
+    op.set_sparse_forward(0)
     try:
         e_f = O.rel_l2(op.forward(g), ref_f)
         e_a = O.rel_l2(op.adjoint(u)[pts], ref_a)
     finally:
         op.set_precision("fp32")
+        op.set_sparse_forward(-1)
     log("config2 %s: forward rel-L2 %.2e adjoint rel-L2 %.2e" % (precision, e_f, e_a))
     assert e_f < tol and e_a < tol
+
+
+# ------------------------------- dense inputs over the full FOV, against FP64
+C12_FREQS = np.array([0, 128, 256])      # first, middle and last of the 257 bins
+
+
+def dense_case(spec, seed):
+    """The FP64 oracle of a dense input over the whole FOV at 3 of the 257
+    frequencies: the forward of the speckle phantom at those frequencies'
+    observations, and the adjoint of random u supported on them."""
+    T, rows = O.frequency_tables(spec, C12_FREQS)
+    g = C.phantom_speckle(spec)
+    u_sub = rnd(len(rows), seed)
+    return dict(rows=rows, g=g, u_sub=u_sub, u_ref=O.forward_by_frequency(T, g),
+                g_ref=O.adjoint_by_frequency(T, u_sub))
+
+
+def dense_errors(op, d, precision):
+    """Forward and adjoint of the dense case through the device entry points
+    (the contraction kernels the benchmark times) in one numeric form:
+    relative L2 and max-abs error per output, max-abs also relative to the
+    largest reference entry."""
+    import torch
+    op.set_precision(precision)
+    try:
+        x = torch.tensor(d["g"].astype(np.complex64), device="cuda")
+        u = op.forward_dev(x).cpu().numpy()[d["rows"]]
+        uu = torch.zeros(op.N_observations, dtype=torch.complex64, device="cuda")
+        uu[torch.from_numpy(d["rows"]).cuda()] = torch.from_numpy(d["u_sub"].astype(np.complex64)).cuda()
+        gh = op.adjoint_dev(uu).cpu().numpy()
+    finally:
+        op.set_precision("fp32")
+    out = {}
+    for name, x, ref in (("forward", u, d["u_ref"]), ("adjoint", gh, d["g_ref"])):
+        m = O.max_abs(x, ref)
+        out[name] = (O.rel_l2(x, ref), m, m / np.max(np.abs(ref)))
+    return out
+
+
+@pytest.fixture(scope="module")
+def c1_dense(c1):
+    log("config1: dense oracle at 3 frequencies over 262144 points")
+    return dense_case(c1[0], 111)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", TOL_FP32), ("split_bf16", TOL_SPLIT), ("split_fp16", TOL_SPLIT)])
+def test_config1_dense_against_fp64(torch_cuda, c1, c1_dense, precision, tol):
+    """Config 1 at full size: the speckle phantom over all 262144 points (and
+    u on every observation of 3 frequencies) through the forward and adjoint
+    contraction kernels, against the FP64 oracle."""
+    errs = dense_errors(c1[1], c1_dense, precision)
+    for name, (e, m, mr) in errs.items():
+        log("config1 dense %s %s: rel-L2 %.2e max-abs %.3e (%.2e of max |ref|)" % (precision, name, e, m, mr))
+        assert e < tol and mr < tol
+
+
+@pytest.fixture(scope="module")
+def c2_dense(c2):
+    log("config2: dense oracle at 3 frequencies over 262144 points")
+    return dense_case(c2[0], 112)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", TOL_FP32), ("split_bf16", TOL_SPLIT), ("split_fp16", TOL_SPLIT)])
+def test_config2_dense_against_fp64(torch_cuda, c2, c2_dense, precision, tol):
+    """Config 2 at full size (32 random waves, 16 +-1 mixes of 128 elements):
+    dense forward and adjoint against the FP64 oracle at 3 frequencies."""
+    errs = dense_errors(c2[1], c2_dense, precision)
+    for name, (e, m, mr) in errs.items():
+        log("config2 dense %s %s: rel-L2 %.2e max-abs %.3e (%.2e of max |ref|)" % (precision, name, e, m, mr))
+        assert e < tol and mr < tol
+
+
+# ------------------------------------------------- face weights (apodization, lens)
+def lens_specs():
+    """A 2-D array whose faces carry a symmetric apodization and a lens
+    focusing along x, and a 3-D matrix array with an elevation lens (the
+    L14-5/38 kind, focus along y)."""
+    from tests.test_oracle import tiny_spec
+    s2 = tiny_spec(N_e=20, Nx=80, Nz=4, npp=4, nfp=4)
+    s2.array.apodization = lambda r: 1.0 - 0.4 * r[:, 0] ** 2
+    s2.array.lens = C.LensSpec((3e-3,), 1200.0)
+    arr = C.ArraySpec((20, 2), (3e-4, 3e-4), (2.5e-4, 2.5e-4), (2, 4), lens=C.LensSpec((math.inf, 4e-3), 1000.0))
+    fov = C.FovSpec((40, 4, 3), (1.5e-4, 1.5e-4, 1.5e-4), 2e-3)
+    s3 = C.SequenceSpec("lens3", arr, C.MediumSpec(), fov, np.array([2.5e6, 4e6, 5e6]),
+                        [C.qsw_wave(arr, np.array([0, 0, -1e-2]), 1500.0),
+                         C.qsw_wave(arr, np.array([3e-4, 0, -1e-2]), 1500.0)],
+                        [C.identity_mixes(40), C.identity_mixes(40)])
+    return {"2d": s2, "3d": s3}
+
+
+@pytest.mark.parametrize("dims", ["2d", "3d"])
+def test_device_tables_with_face_weights(torch_cuda, dims):
+    """Complex, non-unit face weights (face.m:359-376) passed to the device
+    table build: h_ref and every p_incident against the oracle's mirrored
+    tables, and the operator pair in plain FP32 and split BF16 against every
+    element's own weighted transfer function (no shift map)."""
+    spec = lens_specs()[dims]
+    T = O.build_tables(spec)
+    D = O.direct_tables(spec)
+    ref_plain = O.transfer_function(spec, spec.frequencies, 0, spec.filter_rx,
+                                    face_weights_fq=np.ones((len(spec.array.face_offsets()[0]),
+                                                             len(spec.frequencies))))
+    assert O.rel_l2(T.h_ref, ref_plain) > 1e-2          # the weights do change the table
+    op = OperatorBorn.from_sequence(spec)
+    e_h = O.rel_l2(op.table("h_ref"), T.h_ref)
+    e_p = max(O.rel_l2(op.table("p_incident", m), T.meas[m].p_incident) for m in range(spec.N_measurements))
+    log("face weights %s: device h_ref rel-L2 %.2e p_inc rel-L2 %.2e" % (dims, e_h, e_p))
+    assert e_h < TOL_TABLE and e_p < TOL_TABLE
+    g = rnd((T.N_points, 2), 121)
+    u = rnd((T.N_observations(), 2), 122)
+    for form, tol in (("fp32", TOL_FP32), ("split_bf16", TOL_SPLIT)):
+        op.set_precision(form)
+        op.set_sparse_forward(0)
+        e_f = O.rel_l2(op.forward(g), O.forward(D, g))
+        e_a = O.rel_l2(op.adjoint(u), O.adjoint(D, u))
+        log("face weights %s %s: forward rel-L2 %.2e adjoint rel-L2 %.2e" % (dims, form, e_f, e_a))
+        assert e_f < tol and e_a < tol
 
 
 # ---------------------------------------------------------------- config 4 (3-D)
