@@ -227,13 +227,13 @@ def test_integration_listing_is_the_shim_source():
     assert "```cpp\n" + src + "```\n" in (root / "INTEGRATION.md").read_text()
 
 
-def three_measurement_tables():
+def three_measurement_tables(N_e=4, Nx=16):
     """Three measurements of different waves and frequency subsets, mixes of
     one to three active elements."""
     from tests.test_oracle import tiny_spec
     from comp_ui_toolbox_amd import configs as Cf
     rng = np.random.Generator(np.random.PCG64(17))
-    spec = tiny_spec(freqs=np.array([2.0e6, 2.6e6, 3.1e6, 4.5e6]))
+    spec = tiny_spec(N_e=N_e, Nx=Nx, freqs=np.array([2.0e6, 2.6e6, 3.1e6, 4.5e6]))
     arr = spec.array
     spec.waves = [Cf.qpw_wave(arr, a, 1500.0) for a in (0.1, -0.25, 0.3)]
     spec.meas_f_indices = [np.array([0, 1, 3]), np.array([3, 2]), np.array([1, 2, 3, 0])]
@@ -300,8 +300,9 @@ def test_plan_cache_follows_every_input(mex, torch_cuda):
 def test_split_form_through_the_fourth_argument(mex, torch_cuda, form):
     """combined_quick_gpu(op, mode, x, 'split_bf16') runs the split form,
     within its 1e-4 bar against the FP64 oracle; without the fourth argument
-    the plain FP32 form is used again."""
-    _, T = three_measurement_tables()
+    the plain FP32 form is used again.  The split form needs 17 or more
+    elements per row."""
+    _, T = three_measurement_tables(N_e=20, Nx=80)
     op = operator_object(mex, T)
     rng = np.random.Generator(np.random.PCG64(31))
     g = rng.standard_normal((T.N_points, 2)) + 1j * rng.standard_normal((T.N_points, 2))
