@@ -34,7 +34,7 @@ EXPORTED = (
     "born_batch_get_timing", "born_tgc_set", "born_tgc_apply", "born_tgc_apply_dev",
     "born_energy_rx_tgc", "born_energy_rx_tgc_dev", "born_debug_fail_allocation",
     "born_debug_live_allocations", "born_columns", "born_columns_dev",
-    "born_plan_set_sparse_forward",
+    "born_plan_set_sparse_forward", "born_apply_dev_noalloc",
 )
 
 
@@ -114,6 +114,7 @@ def lib() -> C.CDLL:
     L.born_adjoint.argtypes = [vp, pd, C.c_int32, pd]
     L.born_forward_dev.argtypes = [vp, vp, C.c_int32, vp, vp]
     L.born_adjoint_dev.argtypes = [vp, vp, C.c_int32, vp, vp]
+    L.born_apply_dev_noalloc.argtypes = [vp, C.c_int32, vp, C.c_int32, vp, vp]
     L.born_plan_get_table.argtypes = [vp, C.c_int32, C.c_int32, vp, C.c_int64]
     L.born_plan_table_shape.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.born_energy_rx.argtypes = [vp, C.c_int32, pd]

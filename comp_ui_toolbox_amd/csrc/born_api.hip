@@ -16,6 +16,7 @@
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -126,6 +127,12 @@ Plan::~Plan() {
   hipSetDevice(device);
   if (stream) hipStreamSynchronize(stream);
   free_workspace(*this);
+  dfree(na.d_chunks);
+  dfree(na.d_col_pair);
+  dfree(na.d_chunk_pairs);
+  dfree(na.d_pr_chunk);
+  dfree(na.d_pr_pos);
+  dfree(na.ls.d_inst);
   dfree(ls.d_inst);
   dfree(ls_fwd.d_inst);
   dfree(d_fscale);
@@ -412,11 +419,10 @@ static void ensure_shift_on_device(Plan& P) {
 struct Blocking {
   int NZB, ZR, NZB_s, ZR_s, nseg_s, vblk;
 };
-static Blocking blocking_for(const Plan& P, int n_chunks) {
+static Blocking blocking_for(const Plan& P, int n_chunks, int target) {
   Blocking b;
   const int Ney = P.nd == 3 ? P.Ney : 1;
   const int rows = P.n_rows > 0 ? P.n_rows : P.Nz;
-  const int target = P.items_target;
   int NZB = std::max(1, std::min(rows, target / std::max(1, n_chunks * Ney)));
   b.ZR = (rows + NZB - 1) / NZB;
   b.NZB = (rows + b.ZR - 1) / b.ZR;
@@ -427,6 +433,7 @@ static Blocking blocking_for(const Plan& P, int n_chunks) {
   b.vblk = Ney * std::max(b.NZB, b.NZB_s * b.nseg_s);   // partial-sum blocks per chunk
   return b;
 }
+static Blocking blocking_for(const Plan& P, int n_chunks) { return blocking_for(P, n_chunks, P.items_target); }
 static void set_blocking(Plan& P) {
   const Blocking b = blocking_for(P, (int)P.chunks.size());
   P.NZB = b.NZB;
@@ -755,6 +762,169 @@ static void build_workspace(Plan& P, int N_rhs) {
   ++P.ws_gen;
 }
 
+// ------------------------------------------------ allocation-free apply
+// The scratch of born_apply_dev_noalloc: one static array per device, held
+// until the process ends.  Host code never uses its name as an address:
+// scratch_base() looks the device address up once per device (with that
+// device current) and every launch takes that pointer.
+static __device__ __align__(256) unsigned char scratch[BORN_NOALLOC_SCRATCH_BYTES];
+
+static void* scratch_base(int device) {
+  static std::mutex mu;
+  static std::vector<void*> base;
+  std::lock_guard<std::mutex> lock(mu);
+  if ((int)base.size() <= device) base.resize(device + 1, nullptr);
+  if (!base[device]) {
+    int cur = -1;
+    HIP_CHECK(hipGetDevice(&cur));
+    if (cur != device) throw Error(BORN_ERR_DEVICE, "scratch lookup on a device that is not current");
+    void* ptr = nullptr;
+    HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw Error(BORN_ERR_DEVICE, "no device address for the scratch array");
+    base[device] = ptr;
+  }
+  return base[device];
+}
+
+static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+// Scratch bytes of the one-rhs layout with `n_chunks` chunks and blocking b:
+// masked gamma, V, Wc, forward partial sums, each 256-byte aligned.
+static size_t noalloc_bytes(const Plan& P, int n_chunks, const Blocking& b) {
+  const int Ney = P.nd == 3 ? P.Ney : 1;
+  return align256((size_t)P.N_pts * sizeof(float2)) + align256((size_t)P.N_pairs * P.N_e * sizeof(double2)) +
+         align256((size_t)n_chunks * Ney * P.Ne_pad * NC * sizeof(float2)) +
+         align256((size_t)n_chunks * b.vblk * P.Ne_pad * NC * sizeof(float2));
+}
+
+// The one-rhs chunk layout of the allocation-free apply, built at plan
+// creation (structured plans).  Its blocking takes the largest work-item
+// count, up to the plan's, whose scratch fits the static array; a plan that
+// does not fit even with one z-block per chunk keeps na.ok = false.
+static void build_noalloc_layout(Plan& P) {
+  Plan::NoAlloc& A = P.na;
+  if (!P.structured) {
+    A.why = "the allocation-free apply needs the structured path";
+    return;
+  }
+  A.chunks.clear();
+  A.col_pair.clear();
+  std::vector<int> pr_chunk((size_t)P.N_pairs, -1), pr_pos((size_t)P.N_pairs, -1);
+  for (int fg = 0; fg < P.N_fu; ++fg) {
+    const int b = P.fg_pair_off[fg], e = P.fg_pair_off[fg + 1];
+    for (int j = b; j < e; j += NC) {
+      Chunk c;
+      c.fg = fg;
+      c.r = 0;
+      c.col_start = (int)A.col_pair.size();
+      c.ncols = std::min(NC, e - j);
+      for (int q = 0; q < c.ncols; ++q) {
+        const int pr = P.fg_pairs[j + q];
+        pr_chunk[pr] = (int)A.chunks.size();
+        pr_pos[pr] = q;
+        A.col_pair.push_back(pr);
+      }
+      A.chunks.push_back(c);
+    }
+  }
+  const int n_chunks = (int)A.chunks.size();
+  int target = P.items_target;
+  Blocking bl = blocking_for(P, n_chunks, target);
+  while (noalloc_bytes(P, n_chunks, bl) > BORN_NOALLOC_SCRATCH_BYTES && target > 1) {
+    target /= 2;
+    bl = blocking_for(P, n_chunks, target);
+  }
+  A.bytes = noalloc_bytes(P, n_chunks, bl);
+  if (A.bytes > BORN_NOALLOC_SCRATCH_BYTES) {
+    A.why = "the plan needs " + std::to_string(A.bytes) + " bytes of scratch; the allocation-free apply holds " +
+            std::to_string((size_t)BORN_NOALLOC_SCRATCH_BYTES);
+    return;
+  }
+  A.NZB = bl.NZB;
+  A.ZR = bl.ZR;
+  A.NZB_s = bl.NZB_s;
+  A.ZR_s = bl.ZR_s;
+  A.nseg_s = bl.nseg_s;
+  A.vblk = bl.vblk;
+  unsigned char* base = static_cast<unsigned char*>(scratch_base(P.device));
+  const int Ney = P.nd == 3 ? P.Ney : 1;
+  size_t off = 0;
+  A.d_gm = reinterpret_cast<float2*>(base + off);
+  off += align256((size_t)P.N_pts * sizeof(float2));
+  A.d_V = reinterpret_cast<double2*>(base + off);
+  off += align256((size_t)P.N_pairs * P.N_e * sizeof(double2));
+  A.d_Wc = reinterpret_cast<float2*>(base + off);
+  A.Wc_n = (size_t)n_chunks * Ney * P.Ne_pad * NC;
+  off += align256(A.Wc_n * sizeof(float2));
+  A.d_vpart = reinterpret_cast<float2*>(base + off);
+  std::vector<int> chunk_pairs((size_t)n_chunks * NC, P.N_pairs);  // unused: the zero row
+  for (int c = 0; c < n_chunks; ++c)
+    for (int q = 0; q < A.chunks[c].ncols; ++q) chunk_pairs[(size_t)c * NC + q] = A.col_pair[A.chunks[c].col_start + q];
+  A.d_chunks = dupload(A.chunks, P.stream);
+  A.d_col_pair = dupload(A.col_pair, P.stream);
+  A.d_chunk_pairs = dupload(chunk_pairs, P.stream);
+  A.d_pr_chunk = dupload(pr_chunk, P.stream);
+  A.d_pr_pos = dupload(pr_pos, P.stream);
+  Inst I = make_inst(P);
+  I.gm = (decltype(I.gm))A.d_gm;
+  I.Wc = (decltype(I.Wc))A.d_Wc;
+  I.vpart = (decltype(I.vpart))A.d_vpart;
+  I.n_chunks = n_chunks;
+  I.NZB = A.NZB;
+  I.ZR = A.ZR;
+  I.NZB_s = A.NZB_s;
+  I.ZR_s = A.ZR_s;
+  I.nseg_s = A.nseg_s;
+  A.ls.inst.assign(1, I);
+  A.ls.MT = P.Ne_pad / 16;
+  A.ls.Np = P.Np;
+  A.ls.adj_mw = adj_mw_default(P.Nx);
+  A.ls.d_chunks = A.d_chunks;
+  A.ls.d_chunk_pairs = A.d_chunk_pairs;
+  finalize_launchset(A.ls);
+  A.ok = true;
+}
+
+// Swaps the plan's per-rhs workspace with its allocation-free layout for the
+// duration of one apply (and back, also on an exception); event timing is
+// off meanwhile, since it creates events.
+struct NoAllocView {
+  Plan& P;
+  bool timing;
+  explicit NoAllocView(Plan& P_) : P(P_), timing(P_.timing) {
+    swap_all();
+    P.timing = false;
+  }
+  ~NoAllocView() {
+    swap_all();
+    P.timing = timing;
+  }
+  void swap_all() {
+    Plan::NoAlloc& A = P.na;
+    std::swap(P.chunks, A.chunks);
+    std::swap(P.col_pair, A.col_pair);
+    std::swap(P.d_chunks, A.d_chunks);
+    std::swap(P.d_col_pair, A.d_col_pair);
+    std::swap(P.d_chunk_pairs, A.d_chunk_pairs);
+    std::swap(P.d_pr_chunk, A.d_pr_chunk);
+    std::swap(P.d_pr_pos, A.d_pr_pos);
+    std::swap(P.NZB, A.NZB);
+    std::swap(P.ZR, A.ZR);
+    std::swap(P.NZB_s, A.NZB_s);
+    std::swap(P.ZR_s, A.ZR_s);
+    std::swap(P.nseg_s, A.nseg_s);
+    std::swap(P.vblk, A.vblk);
+    std::swap(P.d_gm, A.d_gm);
+    std::swap(P.d_V, A.d_V);
+    std::swap(P.d_Wc, A.d_Wc);
+    std::swap(P.d_vpart, A.d_vpart);
+    std::swap(P.ls, A.ls);
+    std::swap(P.ws_rhs, A.ws_rhs);
+    std::swap(P.fwd_expand, A.fwd_expand);
+    std::swap(P.fwd_table_expanded, A.fwd_table_expanded);
+  }
+};
+
 static void ensure_io(Plan& P, size_t in_bytes, size_t out_bytes) {
   const size_t need = std::max(in_bytes, out_bytes);
   if (P.io_bytes >= need) return;
@@ -792,6 +962,8 @@ static void common_dims(Plan& P, int nd, const int32_t* Npa, const int32_t* Nea,
   P.n_rows = P.Ny * P.Nz;
 }
 
+static void build_noalloc_layout(Plan& P);
+
 static void finish_plan(Plan& P, bool structure_known = false) {
   int ncu = 0;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, P.device) == hipSuccess && ncu > 0)
@@ -803,6 +975,7 @@ static void finish_plan(Plan& P, bool structure_known = false) {
   P.d_f16part = dalloc<float>(256);
   set_max_lds(160 * 1024);
   set_max_lds_split(160 * 1024);
+  build_noalloc_layout(P);
   HIP_CHECK(hipStreamSynchronize(P.stream));
 }
 
@@ -1664,6 +1837,35 @@ int born_adjoint_dev(born_plan_t plan, const void* u_c64, int32_t N_rhs, void* g
     HIP_CHECK(hipSetDevice(P.device));
     ensure_workspace(P, N_rhs);
     adjoint_device(P, (const float2*)u_c64, N_rhs, (float2*)gamma_hat_c64, (hipStream_t)stream);
+  });
+}
+
+int born_apply_dev_noalloc(born_plan_t plan, int32_t mode, const void* in_c64, int32_t N_rhs, void* out_c64,
+                           void* stream) {
+  return guarded([&] {
+    if (!plan) throw Error(BORN_ERR_NO_OPERATOR_BORN, "plan is NULL");
+    if (!in_c64 || !out_c64 || N_rhs <= 0) throw Error(BORN_ERR_NO_VALID_DOUBLE_MATRIX, "invalid device buffers");
+    Plan& P = *reinterpret_cast<Plan*>(plan);
+    check_precision(P);
+    if (!use_structured(P)) throw Error(BORN_ERR_UNSUPPORTED, "the allocation-free apply needs the structured path");
+    if (!P.na.ok) throw Error(BORN_ERR_UNSUPPORTED, P.na.why);
+    HIP_CHECK(hipSetDevice(P.device));
+    hipStream_t s = (hipStream_t)stream;
+    const bool forward = mode == 1;
+    const size_t rows_in = forward ? (size_t)P.N_pts : (size_t)P.N_obs;
+    const size_t rows_out = forward ? (size_t)P.N_obs : (size_t)P.N_pts;
+    NoAllocView view(P);
+    for (int r = 0; r < N_rhs; ++r) {   // one right-hand side at a time through the one-rhs layout
+      const float2* in = static_cast<const float2*>(in_c64) + r * rows_in;
+      float2* out = static_cast<float2*>(out_c64) + r * rows_out;
+      if (forward) {
+        launch_mask_gamma(in, P.N_pts, 1, P.d_gm, s);
+        forward_device(P, P.d_gm, 1, out, false, s);
+      } else {
+        launch_zero_f2(P.d_Wc, P.na.Wc_n, s);   // unused chunk columns and element padding read as zero
+        adjoint_device(P, in, 1, out, s);
+      }
+    }
   });
 }
 

@@ -210,6 +210,33 @@ struct Plan {
   void* d_io_out = nullptr;
   hipStream_t stream = nullptr;   // host-path stream
 
+  // Allocation-free apply (born_apply_dev_noalloc): a one-rhs chunk layout
+  // built at plan creation.  Its scratch -- masked gamma, per-element sums
+  // V, un-mixed weights Wc, forward partial sums -- lives in the library's
+  // static device array; the members carry the names of the plan's own
+  // workspace so that an apply can swap the two.
+  struct NoAlloc {
+    bool ok = false;
+    std::string why;                // the reason when !ok
+    size_t bytes = 0;               // scratch bytes the layout needs
+    size_t Wc_n = 0;                // float2 entries of Wc (cleared on every adjoint)
+    std::vector<Chunk> chunks;
+    std::vector<int> col_pair;
+    Chunk* d_chunks = nullptr;
+    int* d_col_pair = nullptr;
+    int* d_chunk_pairs = nullptr;
+    int* d_pr_chunk = nullptr;
+    int* d_pr_pos = nullptr;
+    int NZB = 1, ZR = 1, NZB_s = 1, ZR_s = 1, nseg_s = 1, vblk = 1;
+    float2* d_gm = nullptr;         // in the scratch array
+    double2* d_V = nullptr;
+    float2* d_Wc = nullptr;
+    float2* d_vpart = nullptr;
+    LaunchSet ls;
+    int ws_rhs = 1;
+    bool fwd_expand = false, fwd_table_expanded = false;
+  } na;
+
   // optional event timing of the contraction kernels: (start, stop, kind)
   bool timing = false;
   struct Timed {
@@ -225,6 +252,7 @@ struct Plan {
 void launch_convert_transpose(const double2* src, int rows, int cols, float2* dst, hipStream_t s);
 void launch_f64_to_f32(const double2* src, size_t n, float2* dst, hipStream_t s);
 void launch_mask_gamma(const float2* g, int N_pts, int N_rhs, float2* gm, hipStream_t s);
+void launch_zero_f2(float2* x, size_t n, hipStream_t s);
 
 void launch_fwd_structured(const LaunchSet& L, int n_cu, hipStream_t s);
 void launch_fwd_general(const Plan& P, const float2* gm, int N_rhs, hipStream_t s);

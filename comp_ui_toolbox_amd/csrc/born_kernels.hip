@@ -76,6 +76,18 @@ __global__ void k_mask_gamma(const float2* __restrict__ g, int N_pts, int N_rhs,
   }
 }
 
+// Zero fill by a kernel (no memset node, so the clear is safe to capture).
+__global__ void k_zero_f2(float2* __restrict__ x, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    x[i] = make_float2(0.f, 0.f);
+}
+
+void launch_zero_f2(float2* x, size_t n, hipStream_t s) {
+  if (n == 0) return;
+  const size_t blocks = std::min<size_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_zero_f2, dim3((unsigned)blocks), dim3(256), 0, s, x, n);
+}
+
 void launch_mask_gamma(const float2* g, int N_pts, int N_rhs, float2* gm, hipStream_t s) {
   hipLaunchKernelGGL(k_mask_gamma, dim3((N_pts + 255) / 256), dim3(256), 0, s, g, N_pts, N_rhs, gm);
 }

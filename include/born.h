@@ -167,6 +167,27 @@ int born_adjoint(born_plan_t plan, const double* u, int32_t N_rhs, double* gamma
 int born_forward_dev(born_plan_t plan, const void* gamma_c64, int32_t N_rhs, void* u_c64, void* stream);
 int born_adjoint_dev(born_plan_t plan, const void* u_c64, int32_t N_rhs, void* gamma_hat_c64, void* stream);
 
+/* The operator pair applied without allocating: mode 1 is the forward of
+ * born_forward_dev, any other value the adjoint of born_adjoint_dev (the
+ * MEX's mode rule, cu:404-409), same buffers and layouts, computed by the
+ * same kernels.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory -- code under HIP graph capture, serving
+ * loops, harnesses that forbid it.  It calls no hipMalloc or hipFree:
+ * instead of the plan's per-rhs workspace it uses a one-rhs layout built at
+ * plan creation and a static device array of BORN_NOALLOC_SCRATCH_BYTES that
+ * the library holds on every device it runs on until the process ends.  The
+ * right-hand sides run one after another.  Every call on a device shares
+ * that array, so calls that overlap on one device (other streams, other
+ * threads, other plans) race and must be serialized by the caller.  A plan
+ * off the structured path, or whose one-rhs layout needs more scratch than
+ * the array holds, gets BORN_ERR_UNSUPPORTED with the sizes in
+ * born_last_error().  Event timing (born_plan_set_timing) does not cover
+ * these calls.  Results agree with born_forward_dev / born_adjoint_dev within
+ * FP32 rounding (the forward's partial sums may be blocked differently). */
+#define BORN_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int born_apply_dev_noalloc(born_plan_t plan, int32_t mode, const void* in_c64, int32_t N_rhs, void* out_c64,
+                           void* stream);
+
 /* Received energies of one pulse-echo measurement, canonical basis, no TGC:
  * energy_rx_canonical( op ) (operator_born.m:718-859) -- for every grid
  * point p, sum over the measurement's mixes and their frequencies of
