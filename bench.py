@@ -199,10 +199,12 @@ _GAMMA = {}
 
 
 def spec_gamma(spec):
-    """Config 0: 30 occupied points; the others: the speckle phantom."""
+    """The workload's own phantom (configs.phantom): config 0, 30 occupied
+    points; configs 1 and 2, speckle with 6 disks and 20 bright points;
+    config 4, speckle with 4 anechoic spheres."""
     from comp_ui_toolbox_amd import configs as C
     if spec.name not in _GAMMA:
-        _GAMMA[spec.name] = C.phantom_points(spec) if spec.name == "config0" else C.phantom_speckle(spec)
+        _GAMMA[spec.name] = C.phantom(spec)
     return _GAMMA[spec.name]
 
 

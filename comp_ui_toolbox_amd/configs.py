@@ -431,3 +431,16 @@ def phantom_speckle(spec: SequenceSpec, n_disks: int = 6, radius: float = 2e-3,
         idx = rng.choice(spec.fov.N_points, size=n_bright, replace=False)
         g[idx] = amp_bright
     return g
+
+
+def phantom(spec: SequenceSpec) -> np.ndarray:
+    """The phantom each named workload is stated with: config 0, 30 point
+    scatterers; configs 1 and 2 (and config-3 instances), speckle with 6
+    anechoic disks of radius 2 mm and 20 bright points of amplitude 10;
+    config 4, speckle with 4 anechoic spheres of radius 2 mm and no bright
+    points."""
+    if spec.name == "config0":
+        return phantom_points(spec, 30)
+    if spec.name == "config4":
+        return phantom_speckle(spec, n_disks=4, n_bright=0)
+    return phantom_speckle(spec)

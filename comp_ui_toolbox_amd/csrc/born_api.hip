@@ -427,7 +427,9 @@ static Blocking blocking_for(const Plan& P, int n_chunks, int target) {
   b.ZR = (rows + NZB - 1) / NZB;
   b.NZB = (rows + b.ZR - 1) / b.ZR;
   b.nseg_s = (P.Nx + 255) / 256;
-  NZB = std::max(1, std::min(rows, target / std::max(1, n_chunks * Ney * b.nseg_s)));
+  // the split forward takes sgrp y slices per work item
+  const int sgrp = fwd_split_slice_group(P.nd, P.Nx, P.Ne_pad, Ney, P.Np);
+  NZB = std::max(1, std::min(rows, target / std::max(1, n_chunks * (Ney / sgrp) * b.nseg_s)));
   b.ZR_s = (rows + NZB - 1) / NZB;
   b.NZB_s = (rows + b.ZR_s - 1) / b.ZR_s;
   b.vblk = Ney * std::max(b.NZB, b.NZB_s * b.nseg_s);   // partial-sum blocks per chunk
@@ -473,6 +475,7 @@ static Inst make_inst(const Plan& P) {
   I.NZB_s = P.NZB_s;
   I.ZR_s = P.ZR_s;
   I.nseg_s = P.nseg_s;
+  I.sgrp = fwd_split_slice_group(P.nd, P.Nx, P.Ne_pad, I.Ney, P.Np);
   return I;
 }
 
@@ -480,11 +483,17 @@ static Inst make_inst(const Plan& P) {
 static void finalize_launchset(LaunchSet& L) {
   int it = 0, its = 0, wg = 0, wgs = 0;
   L.S_max = 1;
+  // the several-slice split forward serves a launch only when every
+  // instance takes the same slice group
+  bool same_group = true;
+  for (const Inst& I : L.inst) same_group = same_group && I.sgrp == L.inst[0].sgrp;
+  if (!same_group)
+    for (Inst& I : L.inst) I.sgrp = 1;
   for (Inst& I : L.inst) {
     I.item_begin = it;
     it += I.n_chunks * I.Ney * I.NZB;
     I.item_begin_s = its;
-    its += I.n_chunks * I.Ney * I.NZB_s * I.nseg_s;
+    its += I.n_chunks * (I.Ney / std::max(1, I.sgrp)) * I.NZB_s * I.nseg_s;
     I.adj_segs = (I.Nx + 64 * L.adj_mw - 1) / (64 * L.adj_mw);
     I.wg_begin = wg;
     wg += I.adj_segs * I.n_rows;
@@ -975,6 +984,7 @@ static void finish_plan(Plan& P, bool structure_known = false) {
   P.d_f16part = dalloc<float>(256);
   set_max_lds(160 * 1024);
   set_max_lds_split(160 * 1024);
+  set_max_lds_split3(160 * 1024);
   build_noalloc_layout(P);
   HIP_CHECK(hipStreamSynchronize(P.stream));
 }

@@ -43,6 +43,7 @@ struct Inst {
   BORN_GLOBAL const int* hrow;   // [Ney][n_rows] table row of (slice, row)
   int NZB, ZR, S;           // plain forward: z-blocks of ZR rows, S = ceil(Nx / 16)
   int NZB_s, ZR_s, nseg_s;  // split forward: z-blocks x 256-point segments
+  int sgrp;                 // split forward: y slices per work item (k_fwd_split_slices when > 1)
   int item_begin, item_begin_s;  // first forward work item of the instance (plain, split)
   int wg_begin, wg_begin_s;      // first adjoint workgroup of the instance (plain, split)
   int adj_segs, adj_segs_s;      // adjoint segments per row (plain, split)
@@ -253,6 +254,10 @@ void launch_convert_transpose(const double2* src, int rows, int cols, float2* ds
 void launch_f64_to_f32(const double2* src, size_t n, float2* dst, hipStream_t s);
 void launch_mask_gamma(const float2* g, int N_pts, int N_rhs, float2* gm, hipStream_t s);
 void launch_zero_f2(float2* x, size_t n, hipStream_t s);
+// 3-D split forward over several y slices per work item (born_split3d.hip)
+int fwd_split_slice_group(int nd, int Nx, int Ne_pad, int Ney, int Np);
+void launch_fwd_split_slices(const LaunchSet& L, int n_cu, hipStream_t s);
+void set_max_lds_split3(int bytes);
 
 void launch_fwd_structured(const LaunchSet& L, int n_cu, hipStream_t s);
 void launch_fwd_general(const Plan& P, const float2* gm, int N_rhs, hipStream_t s);

@@ -469,7 +469,10 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict
   constexpr int NEP = Lay::NEP, SPH = Lay::SPH, LQ = Lay::LQ, PADL = Lay::PADL;
   constexpr int LR = Lay::LR, CPB = Lay::CPB, WS = Lay::WS, PSS = Lay::PSS;
   constexpr int KB = MT / 2;                    // K-blocks of 32 elements
-  constexpr int L = SEG + NP * (NEP - 1);       // extended-row window entries
+  // extended-row window entries the compute waves read: with HALF only the
+  // segment's first 128 points have tiles, which read reversed positions
+  // m >= SPH - 64 + 1, i.e. window entries u < 128 + NP (NEP - 1)
+  constexpr int L = (HALF ? SEG / 2 : SEG) + NP * (NEP - 1);
   // window fill: thread (p, r) = (ft mod NP, ft / NP) holds the entries
   // u = NP q + p, q = 4 r + i (i = 0..3): four consecutive reversed positions
   // of one phase row
@@ -787,6 +790,10 @@ bool fwd_split_supported(const Plan& P) {
 }
 
 void launch_fwd_split(const LaunchSet& L, int n_cu, hipStream_t s) {
+  if (!L.inst.empty() && L.inst[0].sgrp > 1) {   // 3-D short rows: several y slices per item
+    launch_fwd_split_slices(L, n_cu, s);
+    return;
+  }
   const int grid = std::max(1, std::min(L.n_items_s, L.grid_mult * n_cu));
   bool even = true;
   for (const Inst& I : L.inst) even = even && (I.Nx % 2) == 0;

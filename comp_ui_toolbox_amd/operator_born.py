@@ -116,6 +116,7 @@ class OperatorBorn:
         return cls._wrap(plan, device, {"rows_h": np.asarray(T.h_ref).shape[0],
                                         "rows_p": [len(ms.idx_f_to_global) for ms in T.meas],
                                         "n_mix_signals": sum(len(ms.mixes) for ms in T.meas),
+                                        "meas_n_mix": [len(ms.mixes) for ms in T.meas],
                                         "mix_n_obs": [len(mx.idx_f_to_meas) for ms in T.meas for mx in ms.mixes]})
 
     @classmethod
@@ -166,6 +167,7 @@ class OperatorBorn:
                                         "rows_p": [len(spec.measurement_frequencies(m))
                                                    for m in range(spec.N_measurements)],
                                         "n_mix_signals": sum(len(mm) for mm in spec.mixes),
+                                        "meas_n_mix": [len(mm) for mm in spec.mixes],
                                         "mix_n_obs": [len(mx.idx_f_to_meas) for ms in meas for mx in ms.mixes]})
 
     @classmethod
@@ -256,6 +258,50 @@ class OperatorBorn:
         fn = _lib.lib().born_energy_rx_tgc_dev if tgc else _lib.lib().born_energy_rx_dev
         check(fn(self._plan, int(measurement), C.c_void_p(out.data_ptr()), C.c_void_p(s)))
         return out
+
+    def measurement_rows(self, measurement: int) -> range:
+        """Observation indices of one measurement (0-based among the plan's
+        measurements): observations are measurement-major (cu:773, :978)."""
+        n_mix = self.meta["meas_n_mix"]
+        if not 0 <= measurement < len(n_mix):
+            raise BornError(10, "born:InvalidArgument", "measurement index out of range")
+        first = sum(n_mix[:measurement])
+        obs = self.meta["mix_n_obs"]
+        start = sum(obs[:first])
+        return range(start, start + sum(obs[first:first + n_mix[measurement]]))
+
+    def energy_rx_arbitrary(self, dictionary, measurements=None, tgc: bool = False, batch: int = 64):
+        """Received energies of the coefficients of a dictionary
+        (energy_rx_arbitrary, operator_born.m:606-713): for every coefficient
+        j, E[j] = ||TGC(A Psi^T e_j)||^2 over the rows of the measurements,
+        summed over the measurements as energy_rx_scalar does (:136-151; all
+        of the plan's by default).  As in the reference, the unit coefficient
+        vectors go through the forward in batches (of `batch` right-hand sides
+        here; N_objects = 1024 there, :609): synthesis by
+        dictionary.adjoint_transform, the device forward with one right-hand
+        side per atom, the plan's TGC when `tgc` (operator_born.m:222-236),
+        and the squared norms of the measurements' rows, accumulated in FP64
+        on the device.  `dictionary` needs adjoint_transform on device
+        tensors of shape (N_pts, n) (e.g. wavelet.D4Dictionary) and
+        N_coefficients = N_points.  (N_points,) float64 numpy array."""
+        import torch
+        if measurements is None:
+            measurements = range(len(self.meta["meas_n_mix"]))
+        rows = torch.tensor(np.concatenate([np.arange(r.start, r.stop) for r in map(self.measurement_rows, measurements)]),
+                            dtype=torch.int64, device=f"cuda:{self.device}")
+        n = self.N_points
+        E = torch.empty(n, dtype=torch.float64, device=f"cuda:{self.device}")
+        for j0 in range(0, n, batch):
+            nb = min(batch, n - j0)
+            theta = torch.zeros((n, nb), dtype=torch.float64, device=f"cuda:{self.device}")
+            theta[torch.arange(j0, j0 + nb, device=theta.device), torch.arange(nb, device=theta.device)] = 1.0
+            atoms = dictionary.adjoint_transform(theta)                 # (N_pts, nb)
+            g = atoms.T.to(torch.complex64).contiguous()
+            u = self.forward_dev(g)
+            if tgc:
+                u = self.tgc_dev(u, mode=1)
+            E[j0:j0 + nb] = (u.index_select(1, rows).to(torch.complex128).abs() ** 2).sum(dim=1)
+        return E.cpu().numpy()
 
     def time_signals(self, u, frequencies, delta_f: float = 0.0, delta_t: float = 0.0, stream=None):
         """Time-domain mixed voltage signals (format_voltages + signal,

@@ -561,13 +561,16 @@ def restricted_tables(spec, points: np.ndarray) -> BornTables:
     positions), p_inc only at `points` (sequence.m:825-840).  Makes
     full-size checks affordable on the CPU."""
     points = np.asarray(points)
-    shift = shift_lateral(spec)
+    if mirror_map_defined(spec):
+        shift, grid_pos = shift_lateral(spec), spec.fov.positions()
+    else:
+        shift, grid_pos = extended_shift(spec)
     need = np.unique(shift[points, :] - 1)
-    pos = spec.fov.positions()[need]
+    pos = grid_pos[need]
     h_sub = transfer_function(spec, spec.frequencies, 0, spec.filter_rx, positions=pos)
     same = (spec.filter_tx.kind, spec.filter_tx.param) == (spec.filter_rx.kind, spec.filter_rx.param)
     htx_sub = h_sub if same else transfer_function(spec, spec.frequencies, 0, spec.filter_tx, positions=pos)
-    remap = np.full(spec.fov.N_points, -1, dtype=np.int64)
+    remap = np.full(grid_pos.shape[0], -1, dtype=np.int64)
     remap[need] = np.arange(len(need))
     shift_sub = remap[shift[points, :] - 1] + 1
     pref_u = prefactors_unique(spec)
@@ -727,10 +730,14 @@ def frequency_tables(spec, f_sel: np.ndarray):
     f_sel = np.asarray(sorted(set(int(i) for i in f_sel)), dtype=np.int64)
     pos_of = {int(g): i for i, g in enumerate(f_sel)}
     f = spec.frequencies[f_sel]
-    shift = shift_lateral(spec)
-    h_rx = transfer_function(spec, f, 0, spec.filter_rx)
+    if mirror_map_defined(spec):
+        shift = shift_lateral(spec)
+        pos = None
+    else:
+        shift, pos = extended_shift(spec)
+    h_rx = transfer_function(spec, f, 0, spec.filter_rx, positions=pos)
     same = (spec.filter_tx.kind, spec.filter_tx.param) == (spec.filter_rx.kind, spec.filter_rx.param)
-    h_tx = h_rx if same else transfer_function(spec, f, 0, spec.filter_tx)
+    h_tx = h_rx if same else transfer_function(spec, f, 0, spec.filter_tx, positions=pos)
     pref_u = prefactors_unique(spec)
     keeps = []
     for m in range(spec.N_measurements):
@@ -768,6 +775,47 @@ def frequency_tables(spec, f_sel: np.ndarray):
     T = BornTables(h_rx, shift, meas, tuple(spec.fov.N_points_axis), tuple(spec.array.N_elements_axis),
                    tuple(spec.N_points_per_pitch_axis))
     return T, np.asarray(rows, dtype=np.int64)
+
+
+def mirror_map_defined(spec) -> bool:
+    """The mirror of setup_grid_symmetric.m:199-228 stays inside the FOV only
+    when every lateral axis holds at least (N_elements - 1) pitches of
+    points."""
+    npp = spec.N_points_per_pitch_axis
+    return all(spec.fov.N_points_axis[d] >= (spec.array.N_elements_axis[d] - 1) * npp[d]
+               for d in range(spec.fov.N_dimensions - 1))
+
+
+def extended_shift(spec):
+    """Shift map into the FOV grid extended laterally by (N_elements_axis - 1)
+    pitches before it, for FOVs narrower than the aperture: by translation
+    invariance of the face integral and the taper, element e sees point
+    (i_x, [i_y,] z) where element 1 sees (i_x - (e_x - 1) N_p, ...), which
+    lies on the extended grid.  Returns the 1-based N_pts x N_e map and the
+    positions of the extended grid (MATLAB order, x fastest)."""
+    npp = spec.N_points_per_pitch_axis
+    N_axis = list(spec.fov.N_points_axis)
+    nd = len(N_axis)
+    Ne_axis = list(spec.array.N_elements_axis)
+    ext_axis = [N_axis[d] + (Ne_axis[d] - 1) * npp[d] for d in range(nd - 1)] + [N_axis[-1]]
+    axes = []
+    for d in range(nd):
+        if d < nd - 1:
+            j = np.arange(ext_axis[d]) - (Ne_axis[d] - 1) * npp[d]
+            axes.append((j - (N_axis[d] - 1) / 2.0) * spec.fov.delta_axis[d])
+        else:
+            axes.append(spec.fov.z0 + np.arange(N_axis[d]) * spec.fov.delta_axis[d])
+    mesh = np.meshgrid(*axes, indexing="ij")
+    pos = np.stack([m.reshape(-1, order="F") for m in mesh], axis=1)
+    sub = np.stack(np.unravel_index(np.arange(spec.fov.N_points), N_axis, order="F"), axis=1)
+    e_sub = np.stack(np.unravel_index(np.arange(spec.array.N_elements), Ne_axis, order="F"), axis=1)
+    shift = np.empty((spec.fov.N_points, spec.array.N_elements), dtype=np.int64)
+    for e in range(spec.array.N_elements):
+        s = sub.copy()
+        for d in range(nd - 1):
+            s[:, d] = sub[:, d] - e_sub[e, d] * npp[d] + (Ne_axis[d] - 1) * npp[d]
+        shift[:, e] = np.ravel_multi_index(tuple(s.T), ext_axis, order="F") + 1
+    return shift, pos
 
 
 def _rows_by_frequency(T: BornTables):

@@ -119,3 +119,132 @@ def test_batch_autograd_matches_single(torch_cuda):
         e = float((g.grad - x.grad).norm() / x.grad.norm())
         assert e < 1e-6, e
     log("batched autograd: gradients equal the per-instance ones")
+
+
+# ------------------------------------------------------------ config-3 scale
+@pytest.fixture(scope="module")
+def c3_loss(torch_cuda):
+    """The config-3 loss over its 24 uneven instances through one batch
+    (born_batch_apply): per instance 0.5 ||A gamma - u||^2 + lambda
+    ||Psi gamma||_1 with Psi the D4 wavelet at full depth, u = A gamma_true +
+    complex Gaussian noise at 20 dB SNR (gamma_true: the config-1 speckle
+    phantom on the instance's FOV), lambda = 1e-3 ||Re A^H u||_inf.  The
+    gradient is taken at gamma_0 = Psi^T c with every |c| >= 0.5, so no
+    finite-difference step crosses the kink of |.|."""
+    torch = torch_cuda
+    from comp_ui_toolbox_amd.autograd import born_batch_apply
+    specs = C.config3_instances()
+    ops = [OperatorBorn.from_sequence(s) for s in specs]
+    B = OperatorBatch(ops)
+    rng = np.random.Generator(np.random.PCG64(404))
+    clean = B.forward_dev([torch.tensor(C.phantom(s).astype(np.complex64), device="cuda") for s in specs])
+    inst = []
+    for s, op, c in zip(specs, ops, clean):
+        Nx, Nz = s.fov.N_points_axis
+        lv = W.max_levels((Nz, Nx))
+        c64 = c.cpu().numpy().astype(np.complex128)
+        noise = rng.standard_normal(c64.shape) + 1j * rng.standard_normal(c64.shape)
+        noise *= 0.1 * np.linalg.norm(c64) / np.linalg.norm(noise)
+        u = c64 + noise
+        lam = 1e-3 * float(op.adjoint_dev(torch.tensor(u.astype(np.complex64), device="cuda")).real.abs().max())
+        coef = np.sign(rng.standard_normal((Nz, Nx))) * (0.5 + np.abs(rng.standard_normal((Nz, Nx))))
+        g0 = W.idwt2(torch.tensor(coef), lv).numpy().reshape(-1)
+        inst.append(dict(spec=s, op=op, u=u, lam=lam, lv=lv, g0=g0, Nx=Nx, Nz=Nz))
+    xs = [torch.tensor(d["g0"].astype(np.float32), device="cuda", requires_grad=True) for d in inst]
+    outs = born_batch_apply(B, xs)
+    loss = 0
+    for d, x, o in zip(inst, xs, outs):
+        ut = torch.tensor(d["u"].astype(np.complex64), device="cuda")
+        loss = loss + 0.5 * (o - ut).abs().pow(2).sum() + d["lam"] * W.dwt2(x.view(d["Nz"], d["Nx"]), d["lv"]).abs().sum()
+    loss.backward()
+    for d, x, o in zip(inst, xs, outs):
+        d["grad"] = x.grad.double().cpu().numpy()
+        d["r_gpu"] = o.detach().cpu().numpy().astype(np.complex128) - d["u"]
+    log("config3 loss over 24 instances: %.6e" % loss.item())
+    return inst
+
+
+def l1_gradient(d, x):
+    """lambda Psi^T sign(Psi x) in FP64."""
+    torch = __import__("torch")
+    Nz, Nx, lv = d["Nz"], d["Nx"], d["lv"]
+    s = np.sign(W.dwt2(torch.tensor(x.reshape(Nz, Nx)), lv).numpy())
+    return d["lam"] * W.idwt2(torch.tensor(s), lv).numpy().reshape(-1)
+
+
+@pytest.fixture(scope="module")
+def c3_fp64_tables(c3_loss):
+    """FP64 tables of the full operator of each 128 x 128 instance (every
+    frequency; the FOV is narrower than the aperture, so they index the
+    laterally extended grid)."""
+    out = {}
+    for d in c3_loss:
+        if d["Nx"] == 128:
+            s = d["spec"]
+            T, rows = O.frequency_tables(s, np.arange(len(s.frequencies)))
+            assert np.array_equal(rows, np.arange(d["op"].N_observations))
+            out[s.name] = T
+    return out
+
+
+def test_config3_gradient_against_fp64_at_128(c3_loss, c3_fp64_tables):
+    """The three 128 x 128 instances: the whole gradient against the FP64
+    oracle's (residual and adjoint of the full operator in FP64)."""
+    for d in c3_loss:
+        if d["Nx"] != 128:
+            continue
+        s = d["spec"]
+        T = c3_fp64_tables[s.name]
+        r64 = O.forward_by_frequency(T, d["g0"]) - d["u"]
+        g64 = O.adjoint_by_frequency(T, r64).real + l1_gradient(d, d["g0"])
+        e = np.linalg.norm(d["grad"] - g64) / np.linalg.norm(g64)
+        log("%s (%d f, %d waves): gradient vs FP64 rel %.2e" % (s.name, len(s.frequencies), s.N_measurements, e))
+        assert e < 1e-5
+
+
+def test_config3_gradient_against_fp64_finite_differences(c3_loss, c3_fp64_tables):
+    """The smallest 128 x 128 instance: the gradient against FP64 central
+    differences of the loss (relative step 1e-3) on 16 random directions.
+    The data term's central difference is evaluated exactly as
+    Re <A gamma_0 - u, A v> (the term is quadratic), with A in FP64."""
+    torch = __import__("torch")
+    d = min((d for d in c3_loss if d["Nx"] == 128),
+            key=lambda d: len(d["spec"].frequencies) * d["spec"].N_measurements)
+    s = d["spec"]
+    T = c3_fp64_tables[s.name]
+    rng = np.random.Generator(np.random.PCG64(405))
+    V = rng.standard_normal((d["g0"].size, 16))
+    AV = O.forward_by_frequency(T, np.concatenate([d["g0"][:, None], V], axis=1))
+    r64 = AV[:, 0] - d["u"]
+    Nz, Nx, lv = d["Nz"], d["Nx"], d["lv"]
+
+    def l1(x):
+        return d["lam"] * np.abs(W.dwt2(torch.tensor(x.reshape(Nz, Nx)), lv).numpy()).sum()
+
+    fd = np.empty(16)
+    for k in range(16):
+        h = 1e-3 * np.linalg.norm(d["g0"]) / np.linalg.norm(V[:, k])
+        fd[k] = np.vdot(r64, AV[:, k + 1]).real + (l1(d["g0"] + h * V[:, k]) - l1(d["g0"] - h * V[:, k])) / (2 * h)
+    gd = V.T @ d["grad"]
+    e = np.linalg.norm(gd - fd) / np.linalg.norm(fd)
+    log("%s: 16 directions vs FP64 central differences rel %.2e" % (s.name, e))
+    assert e < 1e-5
+
+
+def test_config3_gradient_against_reference_adjoint_all_instances(c3_loss):
+    """Every one of the 24 instances: the data term of the gradient (the
+    gradient minus the FP64 l1 term) at 4 sampled points against the FP64
+    oracle's adjoint of the residual, with tables restricted exactly to
+    those points."""
+    rng = np.random.Generator(np.random.PCG64(406))
+    worst = 0.0
+    for d in c3_loss:
+        s = d["spec"]
+        pts = np.sort(rng.choice(s.fov.N_points, 4, replace=False))
+        R = O.restricted_tables(s, pts)
+        ref = O.adjoint(R, d["r_gpu"]).real
+        got = (d["grad"] - l1_gradient(d, d["g0"]))[pts]
+        e = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+        worst = max(worst, e)
+        assert e < 1e-5, (s.name, e)
+    log("config3: data-term gradient at 4 points of each of the 24 instances vs FP64 adjoint, worst rel %.2e" % worst)

@@ -36,3 +36,18 @@ def test_qpw_delays_closed_form():
     w = C.qpw_wave(arr, 0.2, 1500.0)
     x = arr.positions_ctr()[:, 0]
     assert np.allclose(w.delays, (x - x.min()) * math.sin(0.2) / 1500.0, rtol=0, atol=1e-18)
+
+
+def test_config4_phantom_has_four_spheres_and_no_bright_points():
+    """The config-4 phantom: N(0,1) speckle scaled by 0.05 inside 4 spheres of
+    radius 2 mm, no bright points; configs 1 and 2 keep their 20 points of 10."""
+    import numpy as np
+    from comp_ui_toolbox_amd import configs as C
+    s4 = C.config4()
+    g = C.phantom(s4)
+    assert g.shape == (s4.fov.N_points,) and np.max(np.abs(g)) < 8.0
+    assert np.array_equal(g, C.phantom_speckle(s4, n_disks=4, radius=2e-3, n_bright=0))
+    # N(0,1) puts 1.6 % of the points below 0.02; the damped spheres add about 0.4 %
+    assert 0.017 < np.mean(np.abs(g) < 0.02) < 0.025
+    g1 = C.phantom(C.config1())
+    assert np.sum(g1 == 10.0) == 20

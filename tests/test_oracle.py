@@ -694,3 +694,37 @@ def test_frequency_tables_are_rows_of_the_full_operator(mixed):
     us = rng.standard_normal((len(rows), 2)) + 1j * rng.standard_normal((len(rows), 2))
     u[rows] = us
     assert O.rel_l2(O.adjoint_by_frequency(T_s, us), O.adjoint(T_full, u)) < 1e-13
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_frequency_tables_narrow_fov_equal_direct_tables(dims):
+    """A FOV narrower than the aperture (no mirror map): frequency_tables'
+    extended-grid map gives the operator of every element's own transfer
+    function (direct_tables) at the sampled frequencies."""
+    if dims == 2:
+        spec = tiny_spec(N_e=6, Nx=9, Nz=3, npp=4)
+    else:
+        arr = C.ArraySpec((4, 3), (3e-4, 3e-4), (2.7e-4, 2.7e-4), (2, 2))
+        fov = C.FovSpec((5, 3, 2), (1.5e-4, 1.5e-4, 1.5e-4), 2e-3)
+        spec = C.SequenceSpec("narrow3", arr, C.MediumSpec(), fov, np.array([2.5e6, 3.7e6, 5.5e6]),
+                              [C.qsw_wave(arr, np.array([x, 0.0, -1e-2]), 1500.0) for x in (0.0, 6e-4)],
+                              [C.identity_mixes(12)] * 2)
+    assert not O.mirror_map_defined(spec)
+    D = O.direct_tables(spec)
+    T_s, rows = O.frequency_tables(spec, np.array([0, 2]))
+    rng = np.random.Generator(np.random.PCG64(14))
+    g = rng.standard_normal((D.N_points, 2)) + 1j * rng.standard_normal((D.N_points, 2))
+    assert O.rel_l2(O.forward_by_frequency(T_s, g), O.forward(D, g)[rows]) < 1e-12
+    u = np.zeros((D.N_observations(), 2), dtype=np.complex128)
+    us = rng.standard_normal((len(rows), 2)) + 1j * rng.standard_normal((len(rows), 2))
+    u[rows] = us
+    assert O.rel_l2(O.adjoint_by_frequency(T_s, us), O.adjoint(D, u)) < 1e-12
+
+
+def test_restricted_tables_narrow_fov():
+    """restricted_tables on a FOV narrower than the aperture keeps those
+    columns of the operator of every element's own transfer function."""
+    spec = tiny_spec(N_e=6, Nx=9, Nz=3, npp=4)
+    pts = np.array([0, 4, 13, 26])
+    A = O.assemble_matrix(O.direct_tables(spec))
+    assert O.rel_l2(O.assemble_matrix(O.restricted_tables(spec, pts)), A[:, pts]) < 1e-12

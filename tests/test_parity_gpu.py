@@ -901,3 +901,41 @@ def test_op_A_with_dictionary_and_tgc(torch_cuda):
         assert abs(lhs - rhs) < TOL_FP32 * abs(lhs)
     log("op_A with D4 dictionary: forward %.2e adjoint %.2e" % (e_f, e_a))
     assert e_f < TOL_FP32 and e_a < TOL_FP32
+
+
+class _Identity:
+    """The canonical basis as a dictionary (linear_transforms.identity)."""
+
+    def adjoint_transform(self, x):
+        return x
+
+
+def test_energy_rx_arbitrary(torch_cuda):
+    """Received energies of dictionary coefficients (energy_rx_arbitrary,
+    operator_born.m:606-713): with the D4 wavelet, the squared column norms
+    of the measurement's rows of A Psi^T (A assembled entry by entry in
+    FP64), per measurement and summed over both (energy_rx_scalar), with and
+    without TGC; with the canonical basis, the canonical energies
+    (energy_rx_canonical).  Batches of 48 atoms leave a partial last batch."""
+    from comp_ui_toolbox_amd.wavelet import D4Dictionary
+    spec = tiny_spec(N_e=4, Nx=16, Nz=8, npp=4)
+    T = O.build_tables(spec)
+    op = OperatorBorn.from_tables(T)
+    D = D4Dictionary((16, 8))
+    A = O.assemble_matrix(T)
+    Psi_T = D.adjoint_transform(np.eye(T.N_points))          # columns: the atoms
+    APt = A @ Psi_T
+    kernels = [np.array([0.1 - 0.05j, 1.0, 0.1 + 0.05j])] * len(op.meta["mix_n_obs"])
+    APt_tgc = O.tgc_apply(kernels, op.meta["mix_n_obs"], APt, 1)
+    for m in (0, 1):
+        rows = np.asarray(op.measurement_rows(m))
+        ref = np.sum(np.abs(APt[rows]) ** 2, axis=0)
+        e = O.rel_l2(op.energy_rx_arbitrary(D, [m], batch=48), ref)
+        e_c = O.rel_l2(op.energy_rx_arbitrary(_Identity(), [m], batch=48), O.energy_rx(T, m))
+        log("energy_rx_arbitrary measurement %d: D4 rel-L2 %.2e, canonical %.2e" % (m, e, e_c))
+        assert e < TOL_FP32 and e_c < TOL_FP32
+    e_s = O.rel_l2(op.energy_rx_arbitrary(D), np.sum(np.abs(APt) ** 2, axis=0))
+    op.set_tgc(kernels)
+    e_t = O.rel_l2(op.energy_rx_arbitrary(D, tgc=True), np.sum(np.abs(APt_tgc) ** 2, axis=0))
+    log("energy_rx_arbitrary summed over measurements %.2e, with TGC %.2e" % (e_s, e_t))
+    assert e_s < TOL_FP32 and e_t < TOL_FP32
