@@ -325,6 +325,32 @@ def bench_batched(args):
     kern = {k: v[0] / max(v[1], 1) for k, v in tm.items()}
     F = sum(algorithmic_flops(sp, 1) for sp in specs)
     Bb = sum(algorithmic_bytes(sp, 1) for sp in specs)
+    per_instance = None
+    if args.per_instance:
+        # every instance alone (its own launches, event timing), beside the batch
+        per_instance = []
+        for sp, op, x in zip(specs, ops, xs):
+            u1 = torch.empty(op.N_observations, dtype=torch.complex64, device="cuda")
+            g1 = torch.empty(op.N_points, dtype=torch.complex64, device="cuda")
+            for _ in range(2):
+                op.forward_dev(x, out=u1)
+                op.adjoint_dev(u1, out=g1)
+            torch.cuda.synchronize()
+            op.get_timing()
+            op.set_timing(True)
+            for _ in range(3):
+                op.forward_dev(x, out=u1)
+                op.adjoint_dev(u1, out=g1)
+            torch.cuda.synchronize()
+            op.set_timing(False)
+            t1 = op.get_timing()
+            f_ms = t1["forward"][0] / max(t1["forward"][1], 1)
+            a_ms = t1["adjoint"][0] / max(t1["adjoint"][1], 1)
+            Fi = algorithmic_flops(sp, 1)
+            per_instance.append({"name": sp.name, "fov": sp.fov.N_points_axis[0], "n_f": len(sp.frequencies),
+                                 "waves": sp.N_measurements, "gflop": Fi / 1e9, "forward_ms": f_ms,
+                                 "adjoint_ms": a_ms, "forward_tflops": Fi / f_ms / 1e9,
+                                 "adjoint_tflops": Fi / a_ms / 1e9})
     result = {
         "metric": "batched operator pairs/s (24 uneven instances, forward+adjoint of all per pair)",
         "value": args.steps / elapsed,
@@ -351,6 +377,8 @@ def bench_batched(args):
         "roofline": roofline(F, kern, prec, "config3"),
         "cpu_baseline": None,
     }
+    if per_instance is not None:
+        result["per_instance"] = per_instance
     print(json.dumps(result), flush=True)
 
 
@@ -373,6 +401,8 @@ def main():
     ap.add_argument("--cpu-mixes", type=int, default=128)
     ap.add_argument("--batched-split", action="store_true",
                     help="config3: run the batch in split BF16 (default: plain FP32)")
+    ap.add_argument("--per-instance", action="store_true",
+                    help="config3: also time every instance alone and report its kernel times")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (calibrates HBM byte counters) and exit")
     args = ap.parse_args()

@@ -257,6 +257,7 @@ void launch_zero_f2(float2* x, size_t n, hipStream_t s);
 // 3-D split forward over several y slices per work item (born_split3d.hip)
 int fwd_split_slice_group(int nd, int Nx, int Ne_pad, int Ney, int Np);
 void launch_fwd_split_slices(const LaunchSet& L, int n_cu, hipStream_t s);
+void launch_adj_split_slices(const LaunchSet& L, const OutPtrs& outs, hipStream_t s);
 void set_max_lds_split3(int bytes);
 
 void launch_fwd_structured(const LaunchSet& L, int n_cu, hipStream_t s);

@@ -834,6 +834,10 @@ void launch_fwd_split(const LaunchSet& L, int n_cu, hipStream_t s) {
 }
 
 void launch_adj_split(const LaunchSet& L, const OutPtrs& outs, hipStream_t s) {
+  if (!L.inst.empty() && L.inst[0].sgrp == 4 && L.MT == 2) {   // 3-D short rows, 32 elements per slice
+    launch_adj_split_slices(L, outs, s);
+    return;
+  }
   const dim3 grid(L.n_wg_s);
   const int n_inst = (int)L.inst.size();
   bool y3 = false;

@@ -381,6 +381,328 @@ static size_t fwd_split3_lds_bytes() {
   return std::max(body, red);
 }
 
+// ======================================================================
+// adjoint over several y slices per step
+// ======================================================================
+// One workgroup = (instance, row (y, z)) of at most 128 points.  For one
+// chunk the adjoint sums over the y slices ey of the Toeplitz products of
+// k_adj_s2d_split (born_split.hip):
+//     Y[s][c] = sum_ey sum_e conj(H_ey(NP s + ph - NP e + off)) W[ey][e][c],
+// then gamma_hat += conj(p_inc) Y.  With 32 elements per slice one slice is a
+// single K-block of 32, and a step of one slice is too short for the loads
+// two steps ahead to arrive: the steps wait on memory latency.  Here a step
+// is (chunk, group of G slices): G windows and G weight blocks per buffer,
+// the compute waves walk the slices of the group, so every latency-bound step
+// carries G times the MFMAs.
+//
+// Layout per buffer (u16): G slices of {Ar[comp][ph][CPB], Wt[comp][c][WS]}
+// with the reversed-row copies of the one-slice kernel, here for a 128-point
+// segment; then two p_inc buffers Ps[c][PSS] (float2) by chunk parity.
+template <int NP>
+struct AdjSplit3Layout {
+  static constexpr int G = 4;                                   // slices per step
+  static constexpr int NEP = 32;                                // elements per slice (two tiles)
+  static constexpr int SPH = SEGH / NP;                         // points per phase
+  static constexpr int LQ = SPH + NEP;                          // reversed phase-row entries
+  static constexpr int PADL = 4;                                // lead pad of every copy
+  static constexpr int LR = ((LQ + PADL - 4 + 63) / 64) * 64 + 4;   // copy stride, 4 mod 64
+  static constexpr int CPB = ((4 * LR + 40 - 16 + 63) / 64) * 64 + 16;  // (comp, phase) block, 16 mod 64
+  static constexpr int WS = NEP + 8;                            // padded W row
+  static constexpr int PSS = SEGH + 1;                          // padded p_inc row (float2)
+  static constexpr int AR_U16 = 4 * NP * CPB;
+  static constexpr int SL_U16 = AR_U16 + 4 * NC * WS;           // one slice: window copies + weights
+  static constexpr int AW_U16 = ((G * SL_U16 + 7) / 8) * 8;     // one buffer
+  static constexpr int PS_U16 = ((NC * PSS * 4 + 7) / 8) * 8;   // one p_inc buffer
+  static constexpr int PS_OFF = 2 * AW_U16;
+  static constexpr size_t bytes = (size_t)(2 * AW_U16 + 2 * PS_U16) * 2;
+};
+__device__ __forceinline__ int copy_offset3(int d) { return d == 1 ? 16 : (d == 2 ? 28 : (d == 3 ? 40 : 0)); }
+__device__ __forceinline__ uint4 join3(uint2 a, uint2 b) { return make_uint4(a.x, a.y, b.x, b.y); }
+
+template <int NP, bool F16>
+__global__ __launch_bounds__(512, 1) void k_adj_split_slices(const Inst* __restrict__ inst, int n_inst, OutPtrs outs,
+                                                             const Chunk* __restrict__ chunks,
+                                                             const int* __restrict__ chunk_pairs) {
+  extern __shared__ unsigned short ldsu[];
+  typedef AdjSplit3Layout<NP> Lay;
+  constexpr int G = Lay::G, NEP = Lay::NEP, SPH = Lay::SPH, LQ = Lay::LQ, PADL = Lay::PADL;
+  constexpr int LR = Lay::LR, CPB = Lay::CPB, WS = Lay::WS, PSS = Lay::PSS;
+  constexpr int L = SEGH + NP * (NEP - 1);      // window entries of one slice
+  constexpr int HPT = 4;
+  constexpr int WG4 = ((L + NP - 1) / NP + 3) / 4;   // 4-entry groups per phase
+  constexpr int TPSW = 256 / G;                      // fill threads per slice window
+  static_assert(NP * WG4 <= TPSW, "window groups exceed the fill threads of a slice");
+  static_assert(LQ % 4 == 0 && LR % 4 == 0 && CPB % 4 == 0, "copy alignment classes assume multiples of 4");
+  constexpr int WPT = NEP * NC / 256;           // W entries per fill thread and slice (2)
+  constexpr int PPT = NC * SEGH / 256;          // p_inc entries per fill thread (8)
+  constexpr int WPP = 4 / NP;                   // compute waves per phase
+  constexpr int TPW = 2;                        // tiles per compute wave (64 points of 4 phase tiles)
+
+  const int tid = threadIdx.x, lane = tid & 63, w = (tid >> 6) & 3;
+  const bool filler = tid >= 256;
+  const int ft = tid & 255;
+  const int kk = lane >> 4, col = lane & 15;
+  int ki = 0;
+  while (ki + 1 < n_inst && (int)blockIdx.x >= inst[ki + 1].wg_begin_s) ++ki;
+  const Inst Iv = inst[ki];
+  const Inst* I = &Iv;
+  const int z = (int)blockIdx.x - I->wg_begin_s;   // one 128-point segment per row
+  const int NSG = I->Ney / G;
+  const int n_steps = I->n_chunks * NSG;
+
+  if (filler) {
+    const int t_shift = -(NEP - I->N_e) * NP;   // window entry u holds Hext[u + t_shift]
+    const float s_h = F16 ? I->fscale[0] : 1.f, s_w = F16 ? I->fscale[3] : 1.f;
+    const int fsl = ft / TPSW, fw = ft % TPSW;  // window role: slice of the group, thread within it
+    const bool wact = fw < NP * WG4;
+    const int wp = fw % NP, wr = fw / NP;
+    int eidx[HPT];
+#pragma unroll
+    for (int j = 0; j < HPT; ++j) {
+      const int u = NP * (4 * wr + j) + wp;
+      const int t = u + t_shift;
+      eidx[j] = (wact && u < L && t >= 0 && t < I->N_ext) ? I->ext[t] : -1;
+    }
+    const int pt = ft & (SEGH - 1), pcg = ft >> 7;   // p_inc role: point, column group of 8
+    const int gi = min(pt, I->Nx - 1);
+    float2 hA[HPT], wA[G][WPT], pA[PPT], hB[HPT], wB[G][WPT], pB[PPT];
+#define BORN_AS3_PREFETCH(ST, H, WV, PV)                                                        \
+    {                                                                                           \
+      const int st_ = (ST);                                                                     \
+      const int c_ = st_ / NSG, sg_ = st_ - c_ * NSG;                                           \
+      const Chunk ch_ = chunks[I->chunk_base + c_];                                             \
+      const int row_ = I->hrow[(sg_ * G + fsl) * I->n_rows + z];                                \
+      const float2* hrow_ = I->h + (size_t)ch_.fg * I->Hp + (size_t)row_ * I->HR;               \
+      _Pragma("unroll") for (int j = 0; j < HPT; ++j) H[j] = hrow_[max(eidx[j], 0)];            \
+      _Pragma("unroll") for (int s_ = 0; s_ < G; ++s_) {                                       \
+        const float2* wblk_ = I->Wc + ((size_t)c_ * I->Ney + sg_ * G + s_) * NEP * NC;           \
+        _Pragma("unroll") for (int j = 0; j < WPT; ++j)                                         \
+          WV[s_][j] = wblk_[(2 * (ft >> 4) + j) * NC + (ft & 15)];                              \
+      }                                                                                         \
+      if (sg_ == 0) {                                                                           \
+        const int* prs_ = chunk_pairs + (size_t)(I->chunk_base + c_) * NC + 8 * pcg;             \
+        _Pragma("unroll") for (int j = 0; j < PPT; ++j)                                         \
+          PV[j] = I->P[(size_t)prs_[j] * I->N_pts + (size_t)z * I->Nx + gi];                    \
+      }                                                                                         \
+    }
+#define BORN_AS3_FILL(ST, H, WV, PV)                                                            \
+    {                                                                                           \
+      const int st_ = (ST);                                                                     \
+      const int c_ = st_ / NSG, sg_ = st_ - c_ * NSG;                                           \
+      unsigned short* buf_ = ldsu + (st_ & 1) * Lay::AW_U16;                                    \
+      if (wact) {                                                                               \
+        unsigned short* Ar = buf_ + fsl * Lay::SL_U16;                                          \
+        float xr[4], xi[4];   /* position order: index i holds q = 4 wr + 3 - i */              \
+        _Pragma("unroll") for (int i = 0; i < 4; ++i) {                                         \
+          float2 hv = H[3 - i];                                                                 \
+          if (eidx[3 - i] < 0) hv = make_float2(0.f, 0.f);                                      \
+          if (F16) hv = make_float2(hv.x * s_h, hv.y * s_h);                                    \
+          xr[i] = hv.x;                                                                         \
+          xi[i] = -hv.y; /* conj */                                                             \
+        }                                                                                       \
+        unsigned int pr[4][2];                                                                  \
+        split_pair_s<F16>(xr[0], xr[1], pr[0][0], pr[1][0]);                                    \
+        split_pair_s<F16>(xr[2], xr[3], pr[0][1], pr[1][1]);                                    \
+        split_pair_s<F16>(xi[0], xi[1], pr[2][0], pr[3][0]);                                    \
+        split_pair_s<F16>(xi[2], xi[3], pr[2][1], pr[3][1]);                                    \
+        _Pragma("unroll") for (int cp = 0; cp < 4; ++cp) {                                      \
+          unsigned short* P0 = Ar + (cp * NP + wp) * CPB + PADL + (LQ - 4 - 4 * wr);            \
+          const unsigned int A_ = pr[cp][0], B_ = pr[cp][1];                                    \
+          const unsigned int mid_ = (A_ >> 16) | (B_ << 16);                                    \
+          *reinterpret_cast<uint2*>(P0) = make_uint2(A_, B_);                                   \
+          unsigned short* P1 = P0 + LR + 16 - 1;                                                \
+          P1[0] = (unsigned short)A_;                                                           \
+          *reinterpret_cast<unsigned int*>(P1 + 1) = mid_;                                      \
+          P1[3] = (unsigned short)(B_ >> 16);                                                   \
+          unsigned short* P2 = P0 + 2 * LR + 28 - 2;                                            \
+          *reinterpret_cast<unsigned int*>(P2) = A_;                                            \
+          *reinterpret_cast<unsigned int*>(P2 + 2) = B_;                                        \
+          unsigned short* P3 = P0 + 3 * LR + 40 - 3;                                            \
+          P3[0] = (unsigned short)A_;                                                           \
+          *reinterpret_cast<unsigned int*>(P3 + 1) = mid_;                                      \
+          P3[3] = (unsigned short)(B_ >> 16);                                                   \
+        }                                                                                       \
+      }                                                                                         \
+      /* weights: elements k, k + 1 of column ft mod 16, one 32-bit word per plane */           \
+      _Pragma("unroll") for (int s_ = 0; s_ < G; ++s_) {                                       \
+        unsigned short* Wt = buf_ + s_ * Lay::SL_U16 + Lay::AR_U16;                             \
+        const int k = 2 * (ft >> 4), wc = ft & 15;                                              \
+        unsigned int rh, rl, ih, il;                                                            \
+        const float wsc_ = F16 ? s_w : 1.f;                                                     \
+        split_pair_s<F16>(WV[s_][0].x * wsc_, WV[s_][1].x * wsc_, rh, rl);                      \
+        split_pair_s<F16>(WV[s_][0].y * wsc_, WV[s_][1].y * wsc_, ih, il);                      \
+        unsigned int* wd = reinterpret_cast<unsigned int*>(&Wt[wc * WS + k]);                   \
+        wd[0 * NC * WS / 2] = rh;                                                               \
+        wd[1 * NC * WS / 2] = rl;                                                               \
+        wd[2 * NC * WS / 2] = ih;                                                               \
+        wd[3 * NC * WS / 2] = il;                                                               \
+      }                                                                                         \
+      if (sg_ == 0) {                                                                           \
+        float2* Ps = reinterpret_cast<float2*>(ldsu + Lay::PS_OFF + (c_ & 1) * Lay::PS_U16);    \
+        _Pragma("unroll") for (int j = 0; j < PPT; ++j) Ps[(8 * pcg + j) * PSS + pt] = PV[j];   \
+      }                                                                                         \
+    }
+    const int last = n_steps - 1;
+    BORN_AS3_PREFETCH(0, hA, wA, pA)
+    BORN_AS3_FILL(0, hA, wA, pA)
+    BORN_AS3_PREFETCH(min(1, last), hB, wB, pB)
+    BORN_AS3_PREFETCH(min(2, last), hA, wA, pA)
+    __syncthreads();
+    for (int st = 0; st < n_steps; st += 2) {
+      if (st + 1 < n_steps) BORN_AS3_FILL(st + 1, hB, wB, pB)
+      BORN_AS3_PREFETCH(min(st + 3, last), hB, wB, pB)
+      __syncthreads();
+      if (st + 1 >= n_steps) break;
+      if (st + 2 < n_steps) BORN_AS3_FILL(st + 2, hA, wA, pA)
+      BORN_AS3_PREFETCH(min(st + 4, last), hA, wA, pA)
+      __syncthreads();
+    }
+#undef BORN_AS3_PREFETCH
+#undef BORN_AS3_FILL
+    return;
+  }
+
+  // ---------------------------------------------------------------- compute waves
+  float2* __restrict__ g_out = outs.p[ki];
+  const int ph = w / WPP, t0 = TPW * (w % WPP);
+  float gx[TPW][4], gy[TPW][4];
+#pragma unroll
+  for (int a = 0; a < TPW; ++a)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) gx[a][v] = gy[a][v] = 0.f;
+  const int dcp = (4 - (col & 3)) & 3;
+  const int abase = ph * CPB + dcp * LR + copy_offset3(dcp) + (SPH - col + 8 * kk - dcp) + PADL - 16 * t0;
+  v4f YR[TPW], YI[TPW];
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) {
+    YR[t] = vzs();
+    YI[t] = vzs();
+  }
+  __syncthreads();
+  for (int st = 0; st < n_steps; ++st) {
+    const int c = st / NSG, sg = st - c * NSG;
+    // the group's slices: tile t at diagonal dl reads the window at
+    // 16 (2 kb - t) with kb = 0, so (t, dl) = (1, -1) and (0, 0)
+#pragma unroll
+    for (int s = 0; s < G; ++s) {
+      const unsigned short* Ar = ldsu + (st & 1) * Lay::AW_U16 + s * Lay::SL_U16;
+      const unsigned short* Wt = Ar + Lay::AR_U16;
+      uint4 W4[4];
+#pragma unroll
+      for (int cp = 0; cp < 4; ++cp) W4[cp] = *reinterpret_cast<const uint4*>(&Wt[(cp * NC + col) * WS + 8 * kk]);
+#pragma unroll
+      for (int dl = -1; dl <= 0; ++dl) {
+        uint4 Ac[4];
+#pragma unroll
+        for (int cp = 0; cp < 4; ++cp) {
+          const unsigned short* a_ = Ar + cp * NP * CPB + abase + 16 * dl;
+          Ac[cp] = join3(*reinterpret_cast<const uint2*>(a_), *reinterpret_cast<const uint2*>(a_ + 4));
+        }
+        const uint4 nih = neg8s(Ac[2]), nil = neg8s(Ac[3]);
+        const int t = -dl;
+        YR[t] = mfma32s<F16>(Ac[0], W4[0], YR[t]);
+        YR[t] = mfma32s<F16>(Ac[0], W4[1], YR[t]);
+        YR[t] = mfma32s<F16>(Ac[1], W4[0], YR[t]);
+        YR[t] = mfma32s<F16>(nih, W4[2], YR[t]);
+        YR[t] = mfma32s<F16>(nih, W4[3], YR[t]);
+        YR[t] = mfma32s<F16>(nil, W4[2], YR[t]);
+        YI[t] = mfma32s<F16>(Ac[0], W4[2], YI[t]);
+        YI[t] = mfma32s<F16>(Ac[0], W4[3], YI[t]);
+        YI[t] = mfma32s<F16>(Ac[1], W4[2], YI[t]);
+        YI[t] = mfma32s<F16>(Ac[2], W4[0], YI[t]);
+        YI[t] = mfma32s<F16>(Ac[2], W4[1], YI[t]);
+        YI[t] = mfma32s<F16>(Ac[3], W4[0], YI[t]);
+      }
+    }
+    if (sg == NSG - 1) {
+      if (F16) {
+        const float us = I->fscale[4];
+#pragma unroll
+        for (int t = 0; t < TPW; ++t) {
+          YR[t] *= us;
+          YI[t] *= us;
+        }
+      }
+      // gamma_hat += conj(p) Y; lane holds rows 4*kk + v of tile t0 + t, column col
+      const float2* Ps = reinterpret_cast<const float2*>(ldsu + Lay::PS_OFF + (c & 1) * Lay::PS_U16);
+#pragma unroll
+      for (int t = 0; t < TPW; ++t) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int sp = 16 * (t0 + t) + 4 * kk + v;
+          const float2 p = Ps[col * PSS + NP * sp + ph];
+          gx[t][v] += p.x * YR[t][v] + p.y * YI[t][v];
+          gy[t][v] += p.x * YI[t][v] - p.y * YR[t][v];
+        }
+        YR[t] = vzs();
+        YI[t] = vzs();
+      }
+      const int r = chunks[I->chunk_base + c].r;
+      const bool has_next = c + 1 < I->n_chunks;
+      if (!has_next || chunks[I->chunk_base + c + 1].r != r) {
+        const int o0 = I->col_rhs > 0 ? I->col_rhs : 1;
+#pragma unroll
+        for (int t = 0; t < TPW; ++t) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            float x = gx[t][v], y = gy[t][v];
+            for (int o = o0; o < 16; o <<= 1) {
+              x += __shfl_xor(x, o, 64);
+              y += __shfl_xor(y, o, 64);
+            }
+            gx[t][v] = x;
+            gy[t][v] = y;
+          }
+        }
+        if (I->col_rhs > 0) {
+          if (col < I->n_rhs) {
+#pragma unroll
+            for (int t = 0; t < TPW; ++t)
+#pragma unroll
+              for (int v = 0; v < 4; ++v) {
+                const int i = NP * (16 * (t0 + t) + 4 * kk + v) + ph;
+                if (i < I->Nx) g_out[(size_t)col * I->N_pts + (size_t)z * I->Nx + i] = make_float2(gx[t][v], gy[t][v]);
+              }
+          }
+        } else if (col < 4) {
+#pragma unroll
+          for (int t = 0; t < TPW; ++t) {
+            float x = gx[t][0], y = gy[t][0];
+#pragma unroll
+            for (int v = 1; v < 4; ++v) {
+              if (col == v) {
+                x = gx[t][v];
+                y = gy[t][v];
+              }
+            }
+            const int i = NP * (16 * (t0 + t) + 4 * kk + col) + ph;
+            if (i < I->Nx) g_out[(size_t)r * I->N_pts + (size_t)z * I->Nx + i] = make_float2(x, y);
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < TPW; ++t)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) gx[t][v] = gy[t][v] = 0.f;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+void launch_adj_split_slices(const LaunchSet& L, const OutPtrs& outs, hipStream_t s) {
+  const dim3 grid(L.n_wg_s);
+  const int n_inst = (int)L.inst.size();
+#define BORN_AS3_K(NPv, F16v)                                                                              \
+  hipLaunchKernelGGL((k_adj_split_slices<NPv, F16v>), grid, dim3(512), AdjSplit3Layout<NPv>::bytes, s, L.d_inst, \
+                     n_inst, outs, L.d_chunks, L.d_chunk_pairs)
+  if (L.Np == 2) {
+    if (L.f16) BORN_AS3_K(2, true); else BORN_AS3_K(2, false);
+  } else {
+    if (L.f16) BORN_AS3_K(4, true); else BORN_AS3_K(4, false);
+  }
+#undef BORN_AS3_K
+}
+
 // Slices per work item for a plan: 3-D, rows of at most 128 points, 16 or
 // 32 padded elements per slice (G * MTS = 8 tiles), a slice count G
 // divides; 1 = the one-slice kernel.
@@ -436,6 +758,10 @@ static void set_max_lds_split3_mt(int bytes) {
 }
 
 void set_max_lds_split3(int bytes) {
+  hipFuncSetAttribute((const void*)(k_adj_split_slices<2, false>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  hipFuncSetAttribute((const void*)(k_adj_split_slices<2, true>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  hipFuncSetAttribute((const void*)(k_adj_split_slices<4, false>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  hipFuncSetAttribute((const void*)(k_adj_split_slices<4, true>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   set_max_lds_split3_mt<2, 4, 2>(bytes);
   set_max_lds_split3_mt<2, 4, 4>(bytes);
   set_max_lds_split3_mt<1, 8, 2>(bytes);
