@@ -423,7 +423,9 @@ static Blocking blocking_for(const Plan& P, int n_chunks, int target) {
   Blocking b;
   const int Ney = P.nd == 3 ? P.Ney : 1;
   const int rows = P.n_rows > 0 ? P.n_rows : P.Nz;
-  int NZB = std::max(1, std::min(rows, target / std::max(1, n_chunks * Ney)));
+  // the plain forward takes sgrp_p y slices per work item
+  const int sgrp_p = fwd_plain_slice_group(P.nd, P.Nx, P.Ne_pad, Ney, P.Np);
+  int NZB = std::max(1, std::min(rows, target / std::max(1, n_chunks * (Ney / sgrp_p))));
   b.ZR = (rows + NZB - 1) / NZB;
   b.NZB = (rows + b.ZR - 1) / b.ZR;
   b.nseg_s = (P.Nx + 255) / 256;
@@ -476,6 +478,7 @@ static Inst make_inst(const Plan& P) {
   I.ZR_s = P.ZR_s;
   I.nseg_s = P.nseg_s;
   I.sgrp = fwd_split_slice_group(P.nd, P.Nx, P.Ne_pad, I.Ney, P.Np);
+  I.sgrp_p = fwd_plain_slice_group(P.nd, P.Nx, P.Ne_pad, I.Ney, P.Np);
   return I;
 }
 
@@ -483,15 +486,20 @@ static Inst make_inst(const Plan& P) {
 static void finalize_launchset(LaunchSet& L) {
   int it = 0, its = 0, wg = 0, wgs = 0;
   L.S_max = 1;
-  // the several-slice split forward serves a launch only when every
-  // instance takes the same slice group
-  bool same_group = true;
-  for (const Inst& I : L.inst) same_group = same_group && I.sgrp == L.inst[0].sgrp;
-  if (!same_group)
-    for (Inst& I : L.inst) I.sgrp = 1;
+  // the several-slice forwards serve a launch only when every instance
+  // takes the same slice group
+  bool same_group = true, same_group_p = true;
+  for (const Inst& I : L.inst) {
+    same_group = same_group && I.sgrp == L.inst[0].sgrp;
+    same_group_p = same_group_p && I.sgrp_p == L.inst[0].sgrp_p;
+  }
+  for (Inst& I : L.inst) {
+    if (!same_group) I.sgrp = 1;
+    if (!same_group_p) I.sgrp_p = 1;
+  }
   for (Inst& I : L.inst) {
     I.item_begin = it;
-    it += I.n_chunks * I.Ney * I.NZB;
+    it += I.n_chunks * (I.Ney / std::max(1, I.sgrp_p)) * I.NZB;
     I.item_begin_s = its;
     its += I.n_chunks * (I.Ney / std::max(1, I.sgrp)) * I.NZB_s * I.nseg_s;
     I.adj_segs = (I.Nx + 64 * L.adj_mw - 1) / (64 * L.adj_mw);

@@ -44,6 +44,7 @@ struct Inst {
   int NZB, ZR, S;           // plain forward: z-blocks of ZR rows, S = ceil(Nx / 16)
   int NZB_s, ZR_s, nseg_s;  // split forward: z-blocks x 256-point segments
   int sgrp;                 // split forward: y slices per work item (k_fwd_split_slices when > 1)
+  int sgrp_p;               // plain forward: y slices per work item (k_fwd_s3d when > 1)
   int item_begin, item_begin_s;  // first forward work item of the instance (plain, split)
   int wg_begin, wg_begin_s;      // first adjoint workgroup of the instance (plain, split)
   int adj_segs, adj_segs_s;      // adjoint segments per row (plain, split)
@@ -57,6 +58,29 @@ struct Inst {
   // unscale 1 / ([0] [3])
   BORN_GLOBAL const float* fscale;
 };
+
+// Work items of the 3-D forwards over several y slices per item: instance-
+// local index lid -> (chunk ci, slice group sg of NSG, row block zb of NZB).
+// The NSG items of one (chunk, row block) read the same p_inc * gamma rows;
+// they lie XCDS apart in item order, so on a persistent grid that is a
+// multiple of XCDS (workgroups are dispatched round-robin over the XCDs)
+// they run at the same time on one XCD and share its L2.  Row blocks past
+// the last multiple of XCDS follow with their slice groups adjacent.
+constexpr int XCDS = 8;
+__device__ __forceinline__ void slice_item(int lid, int NSG, int NZB, int& ci, int& sg, int& zb) {
+  ci = lid / (NSG * NZB);
+  const int r = lid - ci * (NSG * NZB);
+  const int NZB8 = NZB - NZB % XCDS;
+  if (r < NSG * NZB8) {
+    const int t = r / XCDS;
+    sg = t % NSG;
+    zb = (t / NSG) * XCDS + (r - t * XCDS);
+  } else {
+    const int r2 = r - NSG * NZB8;
+    sg = r2 % NSG;
+    zb = NZB8 + r2 / NSG;
+  }
+}
 
 constexpr int MAX_BATCH = 64;
 struct OutPtrs {  // adjoint outputs, passed by value
@@ -254,8 +278,11 @@ void launch_convert_transpose(const double2* src, int rows, int cols, float2* ds
 void launch_f64_to_f32(const double2* src, size_t n, float2* dst, hipStream_t s);
 void launch_mask_gamma(const float2* g, int N_pts, int N_rhs, float2* gm, hipStream_t s);
 void launch_zero_f2(float2* x, size_t n, hipStream_t s);
-// 3-D split forward over several y slices per work item (born_split3d.hip)
+// 3-D forwards over several y slices per work item (born_split3d.hip,
+// born_structured3d.hip)
 int fwd_split_slice_group(int nd, int Nx, int Ne_pad, int Ney, int Np);
+int fwd_plain_slice_group(int nd, int Nx, int Ne_pad, int Ney, int Np);
+void launch_fwd_s3d(const LaunchSet& L, int n_cu, hipStream_t s);
 void launch_fwd_split_slices(const LaunchSet& L, int n_cu, hipStream_t s);
 void launch_adj_split_slices(const LaunchSet& L, const OutPtrs& outs, hipStream_t s);
 void set_max_lds_split3(int bytes);

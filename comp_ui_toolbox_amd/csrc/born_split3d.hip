@@ -124,7 +124,7 @@ __global__ __launch_bounds__(256, 2) void k_fwd_split_slices(const Inst* __restr
   const int bi = 2 * (tid & 63);
 
   int it = 0, item = blockIdx.x;
-  int ci, sg, z, z_end, fg, rr, lid;
+  int ci, sg, zb, z, z_end, fg, rr, lid;
   bool hfull;
   unsigned int hoff[2 * APT];        // window gather byte offsets within a table row
   unsigned int hval = 0;             // bit j: window entry j inside the table
@@ -139,11 +139,8 @@ __global__ __launch_bounds__(256, 2) void k_fwd_split_slices(const Inst* __restr
     const int tpad_ = (NEPS - I->N_e) * NP;                                                 \
     const int NSG_ = I->Ney / G;                                                            \
     lid = I->n_chunks * NSG_ * I->NZB_s - 1 - ((itm) - I->item_begin_s);                    \
-    ci = lid / (NSG_ * I->NZB_s);                                                           \
-    const int rem_ = lid - ci * (NSG_ * I->NZB_s);                                          \
-    sg = rem_ / I->NZB_s;                                                                   \
-    const int zb_ = rem_ - sg * I->NZB_s;                                                   \
-    z = zb_ * I->ZR_s;                                                                      \
+    slice_item(lid, NSG_, I->NZB_s, ci, sg, zb);                                            \
+    z = zb * I->ZR_s;                                                                       \
     z_end = min(I->n_rows, z + I->ZR_s);                                                    \
     const Chunk c_ = chunks[I->chunk_base + ci];                                            \
     fg = c_.fg;                                                                             \
@@ -271,11 +268,10 @@ __global__ __launch_bounds__(256, 2) void k_fwd_split_slices(const Inst* __restr
     }
     __syncthreads();
 
-    const int lid_done = lid, ci_done = ci, sg_done = sg, zb_done = (lid % I->NZB_s);
+    const int ci_done = ci, sg_done = sg, zb_done = zb;
     float2* const vpart_done = Iv.vpart;
     const int Ney_done = I->Ney, NZB_done = I->NZB_s * I->nseg_s;
     const float unscale_done = F16 ? Iv.fscale[2] : 1.f;
-    (void)lid_done;
     const bool item_done = z + 1 >= z_end;
     bool has_next = true;
     if (!item_done) {

@@ -327,6 +327,10 @@ int fwd_ppt(int S) {
 static bool fwd_dual(int S) { return 16 * S <= 128; }
 
 void launch_fwd_structured(const LaunchSet& L, int n_cu, hipStream_t s) {
+  if (!L.inst.empty() && L.inst[0].sgrp_p > 1) {   // 3-D short rows: several y slices per item
+    launch_fwd_s3d(L, n_cu, s);
+    return;
+  }
   const int PPT = fwd_ppt(L.S_max);
   const bool dual = fwd_dual(L.S_max);
   const size_t lds = fwd_lds_bytes(L.MT, L.S_max, L.Np, dual);

@@ -98,17 +98,18 @@ def test_3d_tables(torch_cuda, general):
     assert e_f < TOL_FP32 and e_a < TOL_FP32
 
 
-@pytest.mark.parametrize("precision,tol,nex", [("fp32", TOL_FP32, 8), ("split_bf16", 1e-4, 32),
-                                               ("split_fp16", 1e-4, 32)])
-@pytest.mark.parametrize("shape", [(20, 12, 5), (10, 7, 3)])
-def test_3d_geometry_structured(torch_cuda, shape, precision, tol, nex):
+@pytest.mark.parametrize("precision,tol,nex", [("fp32", TOL_FP32, 8), ("fp32", TOL_FP32, 32),
+                                               ("split_bf16", 1e-4, 32), ("split_fp16", 1e-4, 32)])
+@pytest.mark.parametrize("shape,div", [((20, 12, 5), 2), ((10, 7, 3), 2), ((40, 12, 3), 4), ((100, 9, 2), 4)])
+def test_3d_geometry_structured(torch_cuda, shape, div, precision, tol, nex):
     """3-D plans built from geometry on the structured path: an nex x 4
-    matrix array at pitch/2 over FOVs as wide as the aperture (mirrored
+    matrix array at pitch/div over FOVs as wide as the aperture (mirrored
     tables) or narrower (tables on the extended grid), against every
     element's own transfer function.  The split form needs 17 or more
-    elements per x row."""
+    elements per x row; with 32 elements per x row and 4 y rows the
+    forwards take the four y slices in one work item."""
     arr = C.ArraySpec((nex, 4), (3e-4, 3e-4), (2.7e-4, 2.7e-4), (2, 2))
-    fov = C.FovSpec(shape, (1.5e-4, 1.5e-4, 1.5e-4), 2e-3)
+    fov = C.FovSpec(shape, (3e-4 / div,) * 3, 2e-3)
     waves = [C.qsw_wave(arr, np.array([x, 0.0, -1e-2]), 1500.0) for x in (0.0, 6e-4)]
     spec = C.SequenceSpec("t3g", arr, C.MediumSpec(), fov, np.array([2.5e6, 4e6, 5.5e6]), waves,
                           [C.identity_mixes(4 * nex) for _ in waves])
@@ -119,7 +120,7 @@ def test_3d_geometry_structured(torch_cuda, shape, precision, tol, nex):
     u = rnd((T.N_observations(), 2), 10)
     e_f = O.rel_l2(op.forward(g), O.forward(T, g))
     e_a = O.rel_l2(op.adjoint(u), O.adjoint(T, u))
-    log("3-D geometry %s %s: forward %.2e adjoint %.2e" % (shape, precision, e_f, e_a))
+    log("3-D geometry %s pitch/%d %s %d: forward %.2e adjoint %.2e" % (shape, div, precision, nex, e_f, e_a))
     assert e_f < tol and e_a < tol
 
 
