@@ -16,7 +16,7 @@ ROOT = PKG.parent
 LIB = PKG / "libborn.so"
 SOURCES = [PKG / "csrc" / "born_api.hip", PKG / "csrc" / "born_kernels.hip", PKG / "csrc" / "born_structured.hip",
            PKG / "csrc" / "born_split.hip", PKG / "csrc" / "born_split3d.hip",
-           PKG / "csrc" / "born_structured3d.hip"]
+           PKG / "csrc" / "born_structured3d.hip", PKG / "csrc" / "born_fewcol.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("BORN_OFFLOAD_ARCH", "gfx950")
 

@@ -104,6 +104,7 @@ static void free_workspace(Plan& P) {
   dfree(P.d_pr_chunk);
   dfree(P.d_pr_pos);
   dfree(P.d_vpart);
+  dfree(P.d_gpart);
   dfree(P.d_V);
   dfree(P.d_W);
   dfree(P.d_Wc);
@@ -448,6 +449,12 @@ static void set_blocking(Plan& P) {
   P.vblk = b.vblk;
 }
 
+static int chunk_max_cols(const std::vector<Chunk>& ch) {
+  int m = 0;
+  for (const Chunk& c : ch) m = std::max(m, c.ncols);
+  return m;
+}
+
 // The record of a plan inside a structured launch (workspace must exist).
 static Inst make_inst(const Plan& P) {
   Inst I;
@@ -479,6 +486,10 @@ static Inst make_inst(const Plan& P) {
   I.nseg_s = P.nseg_s;
   I.sgrp = fwd_split_slice_group(P.nd, P.Nx, P.Ne_pad, I.Ney, P.Np);
   I.sgrp_p = fwd_plain_slice_group(P.nd, P.Nx, P.Ne_pad, I.Ney, P.Np);
+  I.max_cols = chunk_max_cols(P.chunks);
+  I.gpart = (decltype(I.gpart))P.d_gpart;
+  I.n_out = 1;
+  for (const Chunk& c : P.chunks) I.n_out = std::max(I.n_out, c.r + 1);
   return I;
 }
 
@@ -497,6 +508,27 @@ static void finalize_launchset(LaunchSet& L) {
     if (!same_group) I.sgrp = 1;
     if (!same_group_p) I.sgrp_p = 1;
   }
+  // chunks of at most 4 columns: the few-column kernels (one work item per
+  // (chunk, y slice, row block), like the one-slice plain forward)
+  int max_cols = 0;
+  for (const Inst& I : L.inst) max_cols = std::max(max_cols, I.max_cols);
+  L.fc_w = L.inst.empty() || !L.fc_allowed ? 0 : fewcol_width(max_cols);
+  for (const Inst& I : L.inst)
+    if (!I.gpart || !fewcol_fits(I.Nx, I.Np, 16 * L.MT, L.fc_w, I.n_rows)) L.fc_w = 0;
+  if (L.fc_w > 0)
+    for (Inst& I : L.inst) I.sgrp_p = 1;
+  // adjoint: task blocks x chunk groups, about 2048 workgroups per instance
+  // and at least 2 chunks per group
+  int wgf = 0;
+  for (Inst& I : L.inst) {
+    I.wg_begin_f = wgf;
+    I.fc_g = 1;
+    if (L.fc_w == 0) continue;
+    const int nb = fewcol_adj_blocks(I.Nx, I.Np, I.n_rows, 16 * L.MT, L.fc_w);
+    I.fc_g = std::max(1, std::min({FC_GROUPS, (2048 + nb - 1) / nb, I.n_chunks / 2}));
+    wgf += nb * I.fc_g;
+  }
+  L.n_wg_f = wgf;
   for (Inst& I : L.inst) {
     I.item_begin = it;
     it += I.n_chunks * (I.Ney / std::max(1, I.sgrp_p)) * I.NZB;
@@ -621,6 +653,7 @@ static void setup_expanded_forward(Plan& P, int N_rhs) {
   I.gm = (decltype(I.gm))P.d_ones;
   I.vpart = (decltype(I.vpart))P.d_fvpart;
   I.n_chunks = n_f;
+  I.max_cols = chunk_max_cols(fch);
   I.NZB = bl.NZB;
   I.ZR = bl.ZR;
   I.NZB_s = bl.NZB_s;
@@ -696,6 +729,7 @@ static void setup_column_rhs_adjoint(Plan& P, int N_rhs) {
   Inst I = make_inst(P);
   I.col_rhs = k;
   I.n_rhs = N_rhs;
+  I.n_out = N_rhs;
   P.ls.inst.assign(1, I);
   P.ls.d_chunks = P.d_chunks;
   P.ls.d_chunk_pairs = P.d_chunk_pairs;
@@ -761,6 +795,10 @@ static void build_workspace(Plan& P, int N_rhs) {
     HIP_CHECK(hipMemset(P.d_Wc, 0, n * sizeof(float2)));  // padding columns/elements stay zero
   }
   P.d_gm = dalloc<float2>((size_t)N_rhs * P.N_pts);
+  // partial sums of the few-column adjoint (plans with at most 4 pairs per frequency)
+  int ppf = 0;
+  for (int fg = 0; fg < P.N_fu; ++fg) ppf = std::max(ppf, P.fg_pair_off[fg + 1] - P.fg_pair_off[fg]);
+  if (P.structured && ppf <= 4) P.d_gpart = dalloc<float2>((size_t)FC_GROUPS * N_rhs * P.N_pts);
   if (P.structured) {
     P.ls.inst.assign(1, make_inst(P));
     P.ls.MT = P.Ne_pad / 16;
@@ -887,12 +925,14 @@ static void build_noalloc_layout(Plan& P) {
   I.Wc = (decltype(I.Wc))A.d_Wc;
   I.vpart = (decltype(I.vpart))A.d_vpart;
   I.n_chunks = n_chunks;
+  I.max_cols = chunk_max_cols(A.chunks);
   I.NZB = A.NZB;
   I.ZR = A.ZR;
   I.NZB_s = A.NZB_s;
   I.ZR_s = A.ZR_s;
   I.nseg_s = A.nseg_s;
   A.ls.inst.assign(1, I);
+  A.ls.fc_allowed = false;   // its scratch holds no few-column partials
   A.ls.MT = P.Ne_pad / 16;
   A.ls.Np = P.Np;
   A.ls.adj_mw = adj_mw_default(P.Nx);
@@ -993,6 +1033,7 @@ static void finish_plan(Plan& P, bool structure_known = false) {
   set_max_lds(160 * 1024);
   set_max_lds_split(160 * 1024);
   set_max_lds_split3(160 * 1024);
+  set_max_lds_fc(160 * 1024);
   build_noalloc_layout(P);
   HIP_CHECK(hipStreamSynchronize(P.stream));
 }
@@ -1281,6 +1322,30 @@ static void fp16_scales(Plan& P, bool forward, int N_rhs, hipStream_t s) {
   }
 }
 
+// Contraction launches of a launch set in a numeric form.  Few-column sets
+// run on the FP32 vector kernels in the FP32 form, and in the split forms
+// when every chunk has one column (where the split MFMA kernels fill one
+// column in 16 and the FP32 kernels are both faster and more accurate).
+static bool fc_path(const LaunchSet& L, int precision) {
+  return L.fc_w > 0 && (precision == BORN_PRECISION_FP32 || L.fc_w == 1);
+}
+static void fwd_contract(const LaunchSet& L, int precision, int n_cu, hipStream_t s) {
+  if (L.inst.empty()) return;
+  if (fc_path(L, precision)) launch_fwd_fc(L, s);
+  else if (precision == BORN_PRECISION_FP32) launch_fwd_structured(L, n_cu, s);
+  else launch_fwd_split(L, n_cu, s);
+}
+static void adj_contract(const LaunchSet& L, int precision, const OutPtrs& outs, hipStream_t s) {
+  if (L.inst.empty()) return;
+  if (fc_path(L, precision)) launch_adj_fc(L, outs, s);
+  else if (precision == BORN_PRECISION_FP32) launch_adj_structured(L, outs, s);
+  else launch_adj_split(L, outs, s);
+}
+// Row-block partial sums per (chunk, y slice) that the forward of L wrote.
+static int fwd_blocks(const LaunchSet& L, int precision, const Inst& I) {
+  return fc_path(L, precision) || precision == BORN_PRECISION_FP32 ? I.NZB : I.NZB_s * I.nseg_s;
+}
+
 static void forward_device(Plan& P, const float2* gm, int N_rhs, void* u, bool out_f64, hipStream_t s) {
   if (use_structured(P)) {
     P.ls.f16 = P.ls_fwd.f16 = P.precision == BORN_PRECISION_SPLIT_FP16;
@@ -1290,23 +1355,16 @@ static void forward_device(Plan& P, const float2* gm, int N_rhs, void* u, bool o
     {
       TimedLaunch t(P, s, 0);
       if (P.fwd_table_expanded) launch_expand_pg(P, s);
-      if (P.precision == BORN_PRECISION_FP32)
-        launch_fwd_structured(P.ls_fwd, P.n_cu, s);
-      else
-        launch_fwd_split(P.ls_fwd, P.n_cu, s);
+      fwd_contract(P.ls_fwd, P.precision, P.n_cu, s);
     }
-    const Inst& I = P.ls_fwd.inst[0];
-    launch_fwd_reduce_expanded(P, P.precision == BORN_PRECISION_FP32 ? I.NZB : I.NZB_s * I.nseg_s, s);
+    launch_fwd_reduce_expanded(P, fwd_blocks(P.ls_fwd, P.precision, P.ls_fwd.inst[0]), s);
   } else if (use_structured(P)) {
     (void)gm;   // the launch record points at the masked input P.d_gm
     {
       TimedLaunch t(P, s, 0);
-      if (P.precision == BORN_PRECISION_FP32)
-        launch_fwd_structured(P.ls, P.n_cu, s);
-      else
-        launch_fwd_split(P.ls, P.n_cu, s);
+      fwd_contract(P.ls, P.precision, P.n_cu, s);
     }
-    launch_fwd_reduce(P, N_rhs, P.precision == BORN_PRECISION_FP32 ? P.NZB : P.NZB_s * P.nseg_s, s);  // per slice
+    launch_fwd_reduce(P, N_rhs, fwd_blocks(P.ls, P.precision, P.ls.inst[0]), s);  // per slice
   } else {
     ensure_shift_on_device(P);
     TimedLaunch t(P, s, 0);
@@ -1326,10 +1384,7 @@ static void adjoint_device(Plan& P, const float2* u, int N_rhs, float2* g, hipSt
   if (use_structured(P)) {
     OutPtrs outs;
     outs.p[0] = g;
-    if (P.precision == BORN_PRECISION_FP32)
-      launch_adj_structured(P.ls, outs, s);
-    else
-      launch_adj_split(P.ls, outs, s);
+    adj_contract(P.ls, P.precision, outs, s);
   } else {
     ensure_shift_on_device(P);
     launch_adj_general(P, N_rhs, g, s);
@@ -1468,9 +1523,16 @@ struct Batch {
   int precision = 0;
   std::vector<Plan*> plans;
   std::vector<uint64_t> gen;     // workspace generation of each plan when the records were made
-  LaunchSet ls;
-  Chunk* d_chunks = nullptr;
-  int* d_chunk_pairs = nullptr;
+  // instances whose chunks hold at most 1, 2 or 4 columns run in a launch of
+  // their own per width on the few-column kernels (sets 1-3), the others in
+  // one MFMA launch (set 0, ls)
+  static constexpr int N_SETS = 4;
+  LaunchSet ls, ls_fc[N_SETS - 1];
+  std::vector<int> set_of;         // per plan: its launch set
+  std::vector<int> slot;           // per plan: its index in its launch set
+  Chunk* d_chunks[N_SETS] = {};
+  int* d_chunk_pairs[N_SETS] = {};
+  LaunchSet& set(int k) { return k == 0 ? ls : ls_fc[k - 1]; }
   bool graph = false;
   struct Captured {
     hipGraph_t g = nullptr;
@@ -1502,9 +1564,11 @@ Batch::~Batch() {
     hipEventDestroy(t.a);
     hipEventDestroy(t.b);
   }
-  dfree(ls.d_inst);
-  dfree(d_chunks);
-  dfree(d_chunk_pairs);
+  for (int k = 0; k < N_SETS; ++k) {
+    dfree(set(k).d_inst);
+    dfree(d_chunks[k]);
+    dfree(d_chunk_pairs[k]);
+  }
   if (ev_in) hipEventDestroy(ev_in);
   if (ev_out) hipEventDestroy(ev_out);
   if (own) {
@@ -1513,36 +1577,63 @@ Batch::~Batch() {
   }
 }
 
-// Instance records, concatenated chunk tables and ranges of the batch.
-static void build_batch_launch(Batch& B) {
-  B.ls.inst.clear();
-  B.gen.clear();
+// Instance records, concatenated chunk tables and ranges of the batch's
+// launch sets.
+static void build_batch_set(Batch& B, int set) {
+  LaunchSet& L = B.set(set);
+  Chunk*& d_chunks = B.d_chunks[set];
+  int*& d_chunk_pairs = B.d_chunk_pairs[set];
+  L.inst.clear();
   size_t n_chunks = 0;
-  for (Plan* P : B.plans) n_chunks += P->chunks.size();
-  dfree(B.d_chunks);
-  dfree(B.d_chunk_pairs);
-  B.d_chunks = dalloc<Chunk>(n_chunks);
-  B.d_chunk_pairs = dalloc<int>(n_chunks * NC);
+  for (size_t i = 0; i < B.plans.size(); ++i)
+    if (B.set_of[i] == set) n_chunks += B.plans[i]->chunks.size();
+  dfree(d_chunks);
+  dfree(d_chunk_pairs);
+  d_chunks = nullptr;
+  d_chunk_pairs = nullptr;
+  if (n_chunks == 0) {
+    L.fc_w = 0;
+    return;
+  }
+  d_chunks = dalloc<Chunk>(n_chunks);
+  d_chunk_pairs = dalloc<int>(n_chunks * NC);
   size_t base = 0;
   bool small = true;
-  for (Plan* P : B.plans) {
+  const Plan* first = nullptr;
+  for (size_t i = 0; i < B.plans.size(); ++i) {
+    if (B.set_of[i] != set) continue;
+    Plan* P = B.plans[i];
     const size_t nc = P->chunks.size();
-    HIP_CHECK(hipMemcpy(B.d_chunks + base, P->d_chunks, nc * sizeof(Chunk), hipMemcpyDeviceToDevice));
-    HIP_CHECK(hipMemcpy(B.d_chunk_pairs + base * NC, P->d_chunk_pairs, nc * NC * sizeof(int),
+    HIP_CHECK(hipMemcpy(d_chunks + base, P->d_chunks, nc * sizeof(Chunk), hipMemcpyDeviceToDevice));
+    HIP_CHECK(hipMemcpy(d_chunk_pairs + base * NC, P->d_chunk_pairs, nc * NC * sizeof(int),
                         hipMemcpyDeviceToDevice));
     Inst I = make_inst(*P);
     I.chunk_base = (int)base;
-    B.ls.inst.push_back(I);
-    B.gen.push_back(P->ws_gen);
+    B.slot[i] = (int)L.inst.size();
+    L.inst.push_back(I);
     base += nc;
     small = small && P->Nx <= 128;
+    if (!first) first = P;
   }
-  B.ls.d_chunks = B.d_chunks;
-  B.ls.d_chunk_pairs = B.d_chunk_pairs;
-  B.ls.MT = B.plans[0]->Ne_pad / 16;
-  B.ls.Np = B.plans[0]->Np;
-  if (B.ls.adj_mw != 2 && B.ls.adj_mw != 4) B.ls.adj_mw = small ? 2 : 4;
-  finalize_launchset(B.ls);
+  L.d_chunks = d_chunks;
+  L.d_chunk_pairs = d_chunk_pairs;
+  L.MT = first->Ne_pad / 16;
+  L.Np = first->Np;
+  if (L.adj_mw != 2 && L.adj_mw != 4) L.adj_mw = small ? 2 : 4;
+  finalize_launchset(L);
+}
+
+static void build_batch_launch(Batch& B) {
+  B.gen.clear();
+  B.set_of.assign(B.plans.size(), 0);
+  B.slot.assign(B.plans.size(), 0);
+  for (size_t i = 0; i < B.plans.size(); ++i) {
+    const Plan& P = *B.plans[i];
+    const int w = fewcol_width(chunk_max_cols(P.chunks));
+    if (w > 0 && P.d_gpart && fewcol_fits(P.Nx, P.Np, P.Ne_pad, w, P.n_rows)) B.set_of[i] = w == 1 ? 1 : (w == 2 ? 2 : 3);
+    B.gen.push_back(P.ws_gen);
+  }
+  for (int k = 0; k < Batch::N_SETS; ++k) build_batch_set(B, k);
   for (auto& c : B.cap) drop_graph(c);
 }
 
@@ -1552,48 +1643,43 @@ static void batch_forward_enqueue(Batch& B, const void* const* in, void* const* 
     launch_mask_gamma((const float2*)in[i], P.N_pts, 1, P.d_gm, s);
     fp16_scales(P, true, 1, s);
   }
-  B.ls.f16 = B.precision == BORN_PRECISION_SPLIT_FP16;
+  for (int k = 0; k < Batch::N_SETS; ++k) B.set(k).f16 = B.precision == BORN_PRECISION_SPLIT_FP16;
   hipEvent_t a = nullptr, b = nullptr;
   if (B.timing) {
     HIP_CHECK(hipEventCreate(&a));
     HIP_CHECK(hipEventCreate(&b));
     HIP_CHECK(hipEventRecord(a, s));
   }
-  if (B.precision == BORN_PRECISION_FP32)
-    launch_fwd_structured(B.ls, B.n_cu, s);
-  else
-    launch_fwd_split(B.ls, B.n_cu, s);
+  for (int k = 0; k < Batch::N_SETS; ++k) fwd_contract(B.set(k), B.precision, B.n_cu, s);
   if (B.timing) {
     HIP_CHECK(hipEventRecord(b, s));
     B.timed.push_back({a, b, 0});
   }
   for (size_t i = 0; i < B.plans.size(); ++i) {
     Plan& P = *B.plans[i];
-    launch_fwd_reduce(P, 1, B.precision == BORN_PRECISION_FP32 ? P.NZB : P.NZB_s * P.nseg_s, s);
+    const LaunchSet& L = B.set(B.set_of[i]);
+    launch_fwd_reduce(P, 1, fwd_blocks(L, B.precision, L.inst[B.slot[i]]), s);
     launch_fwd_mix(P, 1, out[i], false, s);
   }
   HIP_CHECK(hipGetLastError());
 }
 
 static void batch_adjoint_enqueue(Batch& B, const void* const* in, void* const* out, hipStream_t s) {
-  OutPtrs outs;
+  OutPtrs outs[Batch::N_SETS];
   for (size_t i = 0; i < B.plans.size(); ++i) {
     Plan& P = *B.plans[i];
     launch_adj_unmix(P, (const float2*)in[i], 1, true, s);
     fp16_scales(P, false, 1, s);
-    outs.p[i] = (float2*)out[i];
+    outs[B.set_of[i]].p[B.slot[i]] = (float2*)out[i];
   }
-  B.ls.f16 = B.precision == BORN_PRECISION_SPLIT_FP16;
+  for (int k = 0; k < Batch::N_SETS; ++k) B.set(k).f16 = B.precision == BORN_PRECISION_SPLIT_FP16;
   hipEvent_t a = nullptr, b = nullptr;
   if (B.timing) {
     HIP_CHECK(hipEventCreate(&a));
     HIP_CHECK(hipEventCreate(&b));
     HIP_CHECK(hipEventRecord(a, s));
   }
-  if (B.precision == BORN_PRECISION_FP32)
-    launch_adj_structured(B.ls, outs, s);
-  else
-    launch_adj_split(B.ls, outs, s);
+  for (int k = 0; k < Batch::N_SETS; ++k) adj_contract(B.set(k), B.precision, outs[k], s);
   if (B.timing) {
     HIP_CHECK(hipEventRecord(b, s));
     B.timed.push_back({a, b, 1});
@@ -1713,8 +1799,12 @@ static TuneResult autotune(std::vector<Plan*>& plans, LaunchSet& L, int precisio
   OutPtrs outs;
   std::vector<float2*> scratch;
   try {
+    // instance i of L need not be plan i (a batch splits its plans over two
+    // launch sets): every buffer holds the largest plan
+    int max_pts = 0;
+    for (const Plan* P : plans) max_pts = std::max(max_pts, P->N_pts);
     for (size_t i = 0; i < plans.size(); ++i) {
-      float2* g = dalloc<float2>(plans[i]->N_pts);
+      float2* g = dalloc<float2>(max_pts);
       scratch.push_back(g);
       outs.p[i] = g;
     }
@@ -1723,10 +1813,7 @@ static TuneResult autotune(std::vector<Plan*>& plans, LaunchSet& L, int precisio
       rebuild();
       for (int gm : grid_c) {
         L.grid_mult = gm;
-        const float t = time_launches([&] {
-          if (precision == BORN_PRECISION_FP32) launch_fwd_structured(L, n_cu, s);
-          else launch_fwd_split(L, n_cu, s);
-        }, s, 3);
+        const float t = time_launches([&] { fwd_contract(L, precision, n_cu, s); }, s, 3);
         if (t < best.fwd_ms) {
           best.fwd_ms = t;
           best.items = items;
@@ -1741,10 +1828,7 @@ static TuneResult autotune(std::vector<Plan*>& plans, LaunchSet& L, int precisio
       if (precision != BORN_PRECISION_FP32 && mw != 4) continue;   // the split adjoint has one width
       L.adj_mw = mw;
       rebuild();
-      const float t = time_launches([&] {
-        if (precision == BORN_PRECISION_FP32) launch_adj_structured(L, outs, s);
-        else launch_adj_split(L, outs, s);
-      }, s, 3);
+      const float t = time_launches([&] { adj_contract(L, precision, outs, s); }, s, 3);
       if (t < best.adj_ms) {
         best.adj_ms = t;
         best.adj_mw = mw;

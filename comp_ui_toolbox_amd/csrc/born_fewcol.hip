@@ -1,0 +1,532 @@
+// Structured contractions for chunks of at most 4 columns, on the vector ALUs
+// (gfx950).
+//
+// A chunk holds the (wave, frequency) pairs of one frequency, so a plan with
+// w plane waves fills w of the 16 MFMA columns of the contraction kernels
+// (born_structured.hip, born_split.hip): with one wave, 15/16 of every MFMA
+// is padding.  For w <= 4 the Toeplitz products run here as FP32 FMAs
+// instead, W = 1, 2 or 4 columns per chunk, with the same inputs and outputs
+// as k_fwd_s2d / k_adj_s2d: the forward writes the same row-block partial
+// sums (summed in FP64 by k_fwd_reduce), the adjoint writes gamma_hat once
+// per right-hand side.
+//
+// Work split: a task is one phase ph of a row (points i = Np q + ph) and S
+// consecutive q, against a block of E consecutive elements.  Element e sees
+// point i at extended-row entry t = i - e Np + (N_e - 1) Np (the lateral map
+// of setup_grid_symmetric.shift_lateral, setup_grid_symmetric.m:171-233).
+// Rows are staged in LDS as in k_fwd_s2d: per phase, Hp[ph][q'] = Hext at
+// t = (q' - (Ne_pad - N_e)) Np + ph, so element e at point q of phase ph
+// reads Hp[ph][q + Ne_pad - 1 - e], and over a task's (s, j) = (q - S qb,
+// e - E eb) these are S + E - 1 consecutive entries, held in registers for
+// S x E x W complex products.  Complex products use three real
+// multiplications (P1 = ar br, P2 = ai bi, P3 = (ar + ai)(br + bi)): three
+// FMAs per complex multiply-add, all operands from registers.
+#include "born_internal.h"
+
+namespace born {
+
+namespace {
+
+// Geometry of the staged rows of one instance for a task shape (E, S).
+struct FcGeom {
+  int QN;    // points per phase (ceil(Nx / Np))
+  int NQB;   // q-blocks of S per phase
+  int QLF;   // staged entries per phase: S NQB + Ne_pad - 1
+  int XPF;   // staged B points per row: Np S NQB
+  int NEB;   // element blocks of E
+  int tpr;   // tasks per row: Np NQB NEB
+};
+__host__ __device__ inline FcGeom fc_geom(int Nx, int Np, int Ne_pad, int E, int S) {
+  FcGeom g;
+  g.QN = (Nx + Np - 1) / Np;
+  g.NQB = (g.QN + S - 1) / S;
+  g.QLF = S * g.NQB + Ne_pad - 1;
+  g.XPF = Np * S * g.NQB;
+  g.NEB = Ne_pad / E;
+  g.tpr = Np * g.NQB * g.NEB;
+  return g;
+}
+
+// Forward: rows per step R (the smallest power of 2, at most 8, whose tasks
+// fill the 256 threads); row stride in LDS Np QLF + XPF W entries.
+__host__ __device__ inline int fc_fwd_rows(const FcGeom& g) {
+  int R = 1;
+  while (R < 8 && R * g.tpr < 256) R <<= 1;
+  return R;
+}
+// Adjoint: a workgroup owns 256 consecutive tasks (row, phase, q-block,
+// element block) of its instance and stages the rows they touch: the most
+// rows any block of the instance touches.
+inline int fc_adj_rows_max(const FcGeom& g, int n_rows) {
+  if (g.tpr % 256 == 0) return 1;
+  const long T = (long)n_rows * g.tpr;
+  int m = 1;
+  for (long t0 = 0; t0 < T; t0 += 256)
+    m = std::max(m, (int)((std::min(T, t0 + 256) - 1) / g.tpr - t0 / g.tpr + 1));
+  return m;
+}
+
+// Task shapes: forward W = 1: E 16 x S 16, 2: 8 x 16, 4: 8 x 8 (sums of E x
+// W); adjoint E 16 x S 8 (W = 4: S 4) (sums of S x W).
+constexpr int FC_LDS_MAX = 78 * 1024;   // two workgroups per compute unit
+void fc_fwd_shape(int fc_w, int& E, int& S) {
+  E = fc_w == 1 ? 16 : 8;
+  S = fc_w == 4 ? 8 : 16;
+}
+void fc_adj_shape(int fc_w, int& E, int& S) {
+  E = 16;
+  S = fc_w == 4 ? 4 : 8;
+}
+size_t fc_fwd_lds(int Nx, int Np, int Ne_pad, int fc_w) {
+  int E, S;
+  fc_fwd_shape(fc_w, E, S);
+  const FcGeom g = fc_geom(Nx, Np, Ne_pad, E, S);
+  const size_t rows = (size_t)fc_fwd_rows(g) * (Np * g.QLF + g.XPF * fc_w);
+  // staged rows (then the final reduction), window slot offsets, row table
+  return std::max(rows, (size_t)4 * 128 * fc_w) * sizeof(float2) + ((size_t)Np * g.QLF + 8) * sizeof(int);
+}
+// staged entries of one adjoint step
+size_t fc_adj_entries(int Nx, int Np, int Ne_pad, int fc_w, int n_rows) {
+  int E, S;
+  fc_adj_shape(fc_w, E, S);
+  const FcGeom g = fc_geom(Nx, Np, Ne_pad, E, S);
+  const int NR = fc_adj_rows_max(g, n_rows);
+  return (size_t)NR * (Np * g.QLF + g.XPF * fc_w) + (size_t)Ne_pad * fc_w;
+}
+size_t fc_adj_lds(int Nx, int Np, int Ne_pad, int fc_w, int n_rows) {
+  int E, S;
+  fc_adj_shape(fc_w, E, S);
+  const FcGeom g = fc_geom(Nx, Np, Ne_pad, E, S);
+  return fc_adj_entries(Nx, Np, Ne_pad, fc_w, n_rows) * sizeof(float2) + ((size_t)Np * g.QLF + 16) * sizeof(int);
+}
+
+}  // namespace
+
+int fewcol_width(int max_cols) { return max_cols <= 1 ? 1 : (max_cols <= 2 ? 2 : (max_cols <= 4 ? 4 : 0)); }
+
+bool fewcol_fits(int Nx, int Np, int Ne_pad, int fc_w, int n_rows) {
+  if (fc_w <= 0 || !(Ne_pad == 16 || Ne_pad == 32 || Ne_pad == 64 || Ne_pad == 128)) return false;
+  int E, S;
+  fc_adj_shape(fc_w, E, S);
+  if (fc_geom(Nx, Np, Ne_pad, E, S).tpr < 32) return false;
+  if (fc_adj_lds(Nx, Np, Ne_pad, fc_w, n_rows) > FC_LDS_MAX) return false;
+  return fc_fwd_lds(Nx, Np, Ne_pad, fc_w) <= FC_LDS_MAX;
+}
+
+// ------------------------------------------------------------------ forward
+// One workgroup per work item (instance, chunk, y slice, row block) -- the
+// items of k_fwd_s2d (item_begin, NZB, ZR).  A step stages R rows; thread
+// tid takes the tasks tid, tid + 256, ... of the step (element block
+// eb = tid mod NEB throughout), and its sums over all steps are reduced over
+// the threads of its element block at the end.
+template <int W, int E, int S>
+__global__ __launch_bounds__(256, 2) void k_fwd_fc(const Inst* __restrict__ inst, int n_inst, int Ne_pad,
+                                                   const Chunk* __restrict__ chunks,
+                                                   const int* __restrict__ chunk_pairs) {
+  extern __shared__ float2 lds[];
+  const int item = blockIdx.x;
+  int k = 0;
+  while (k + 1 < n_inst && item >= inst[k + 1].item_begin) ++k;
+  const Inst Iv = inst[k];
+  const Inst* I = &Iv;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+
+  const int li = item - I->item_begin;   // (chunk * Ney + ey) * NZB + zb
+  const int ci = li / (I->Ney * I->NZB);
+  const int rem = li - ci * (I->Ney * I->NZB);
+  const int ey = rem / I->NZB;
+  const int z0 = (rem - ey * I->NZB) * I->ZR;
+  const int z1 = min(I->n_rows, z0 + I->ZR);
+  const Chunk ch = chunks[I->chunk_base + ci];
+  const int* cpr = chunk_pairs + (I->chunk_base + ci) * NC;
+
+  const int Np = I->Np, Nx = I->Nx, N_pts = I->N_pts;
+  const FcGeom g = fc_geom(Nx, Np, Ne_pad, E, S);
+  const int R = fc_fwd_rows(g);
+  const int HS = Np * g.QLF, RS = HS + g.XPF * W;   // staged row: Hp, then B[i][c]
+  const int Qoff = Ne_pad - I->N_e;
+  const float2* hf = I->h + (size_t)ch.fg * I->Hp;
+  const float2* gmr = I->gm + (size_t)ch.r * N_pts;
+  const int eb = tid % g.NEB;
+  int pr[W];
+#pragma unroll
+  for (int c = 0; c < W; ++c) pr[c] = cpr[c];
+  // table x index of each staged window slot (-1: outside the table), and
+  // the table row of each staged row of the step (-1: past the block)
+  const size_t rows_bytes = std::max((size_t)R * RS, (size_t)4 * 128 * W) * sizeof(float2);
+  int* dh = reinterpret_cast<int*>(reinterpret_cast<char*>(lds) + rows_bytes);
+  int* hr = dh + HS;
+  for (int o = tid; o < HS; o += 256) {
+    const int ph = o / g.QLF, q = o - ph * g.QLF;
+    const int t = (q - Qoff) * Np + ph;
+    dh[o] = t >= 0 && t < I->N_ext ? I->ext[t] : -1;
+  }
+
+  float a1[E][W], a2[E][W], a3[E][W];
+#pragma unroll
+  for (int j = 0; j < E; ++j)
+#pragma unroll
+    for (int c = 0; c < W; ++c) a1[j][c] = a2[j][c] = a3[j][c] = 0.f;
+
+  for (int z = z0; z < z1; z += R) {
+    if (tid < R) hr[tid] = z + tid < z1 ? I->hrow[ey * I->n_rows + z + tid] : -1;
+    __syncthreads();
+    for (int r = 0; r < R; ++r) {
+      const int row = hr[r], zr = z + r;
+      float2* dst = lds + r * RS;
+      for (int o = tid; o < HS; o += 256) {
+        const int e = dh[o];
+        dst[o] = row >= 0 && e >= 0 ? hf[(size_t)row * I->HR + e] : make_float2(0.f, 0.f);
+      }
+      for (int o = tid; o < g.XPF * W; o += 256) {
+        const int i = o / W, c = o - i * W;
+        float2 v = make_float2(0.f, 0.f);
+        if (row >= 0 && i < Nx) {
+          int q = pr[0];
+#pragma unroll
+          for (int cc = 1; cc < W; ++cc)
+            if (c == cc) q = pr[cc];
+          const size_t p = (size_t)zr * Nx + i;
+          const float2 a = I->P[(size_t)q * N_pts + p], b = gmr[p];
+          v = make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+        }
+        dst[HS + o] = v;
+      }
+    }
+    __syncthreads();
+    for (int tk = tid; tk < R * g.tpr; tk += 256) {
+      const int r = tk / g.tpr, t2 = (tk - r * g.tpr) / g.NEB;
+      const int ph = t2 / g.NQB, qb = t2 - ph * g.NQB;
+      const float2* Hr = lds + r * RS + ph * g.QLF + S * qb + Ne_pad - E * eb - E;
+      const float2* Br = lds + r * RS + HS + ph * W;
+      float wr[S + E - 1], wi[S + E - 1], ws[S + E - 1];
+#pragma unroll
+      for (int u = 0; u < S + E - 1; ++u) {
+        const float2 a = Hr[u];
+        wr[u] = a.x;
+        wi[u] = a.y;
+        ws[u] = a.x + a.y;
+      }
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        float br[W], bi[W], bs[W];
+#pragma unroll
+        for (int c = 0; c < W; ++c) {
+          const float2 b = Br[(Np * (S * qb + s)) * W + c];
+          br[c] = b.x;
+          bi[c] = b.y;
+          bs[c] = b.x + b.y;
+        }
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+          const int u = s - j + E - 1;
+#pragma unroll
+          for (int c = 0; c < W; ++c) {
+            a1[j][c] = fmaf(wr[u], br[c], a1[j][c]);
+            a2[j][c] = fmaf(wi[u], bi[c], a2[j][c]);
+            a3[j][c] = fmaf(ws[u], bs[c], a3[j][c]);
+          }
+        }
+      }
+    }
+  }
+
+  // Re = P1 - P2, Im = P3 - P1 - P2; sum over the lanes of one element block
+  // (lane bits from NEB up), then over the 4 waves through LDS
+  __syncthreads();
+  float2* red = lds;   // [wave][element][column]
+#pragma unroll
+  for (int j = 0; j < E; ++j)
+#pragma unroll
+    for (int c = 0; c < W; ++c) {
+      float x = a1[j][c] - a2[j][c], y = a3[j][c] - a1[j][c] - a2[j][c];
+      for (int o = g.NEB; o < 64; o <<= 1) {
+        x += __shfl_xor(x, o, 64);
+        y += __shfl_xor(y, o, 64);
+      }
+      if (lane < g.NEB) red[(w * 128 + E * lane + j) * W + c] = make_float2(x, y);
+    }
+  __syncthreads();
+  float2* out = I->vpart + (size_t)li * Ne_pad * NC;   // [element][column]
+  for (int idx = tid; idx < Ne_pad * W; idx += 256) {
+    const int e = idx / W, c = idx - e * W;
+    float2 v = red[e * W + c];
+#pragma unroll
+    for (int ww = 1; ww < 4; ++ww) {
+      const float2 u = red[(ww * 128 + e) * W + c];
+      v.x += u.x;
+      v.y += u.y;
+    }
+    out[(size_t)e * NC + c] = v;
+  }
+}
+
+// ------------------------------------------------------------------ adjoint
+// Workgroups of instance k: [wg_begin_f, wg_begin_f + blocks * G); workgroup
+// wl owns the tasks [256 b, 256 b + 256) of the instance's list (row, phase,
+// q-block, element block), b = wl / G, one per thread, and the chunks
+// g, g + G, g + 2G, ... (g = wl mod G; chunks are rhs-major, then frequency).
+// Per chunk and y slice it stages the rows its tasks touch, the weights and
+// (first slice) the p_inc rows:
+//   Y[s][c]   = sum_e conj(Hext[t(s, e)]) W[e][c]      (its E elements)
+//   Gh[r][s] += sum_c conj(p_inc[pair_c][i_s]) Y[s][c]  (r = rhs of column c)
+// The NEB element-block lanes of a task (adjacent lanes) are summed when a
+// right-hand side is complete within the group, and written to the group's
+// partial gpart[g][r][point]; k_adj_fc_reduce sums the G partials in order
+// (no atomics: bitwise repeatable).  KR = 1: every column belongs to the
+// chunk's rhs; KR = col_rhs (2 or 4): column c belongs to rhs c mod KR.
+template <int W, int E, int S, int KR>
+__global__ __launch_bounds__(256, 2) void k_adj_fc(const Inst* __restrict__ inst, int n_inst, int Ne_pad,
+                                                   const Chunk* __restrict__ chunks,
+                                                   const int* __restrict__ chunk_pairs) {
+  extern __shared__ float2 lds[];
+  int k = 0;
+  while (k + 1 < n_inst && (int)blockIdx.x >= inst[k + 1].wg_begin_f) ++k;
+  const Inst Iv = inst[k];
+  const Inst* I = &Iv;
+  const int tid = threadIdx.x;
+
+  const int Np = I->Np, Nx = I->Nx, N_pts = I->N_pts, G = I->fc_g;
+  const FcGeom g = fc_geom(Nx, Np, Ne_pad, E, S);
+  const int T = I->n_rows * g.tpr;
+  const int wl = (int)blockIdx.x - I->wg_begin_f;
+  const int grp = wl % G;
+  const int t_first = (wl / G) * 256;
+  const int t_last = min(T, t_first + 256) - 1;
+  const int r_first = t_first / g.tpr, NR = t_last / g.tpr - r_first + 1;
+  const int HS = Np * g.QLF, PS = g.XPF * W;
+  const int nH = NR * HS, nW = Ne_pad * W, nP = NR * PS;
+  float2* Hs = lds;              // [NR][Np][QLF]
+  float2* Ws = lds + nH;         // [Ne_pad][W]
+  float2* Ps = Ws + nW;          // [NR][XPF][W]
+  const int Qoff = Ne_pad - I->N_e;
+
+  const int tk = t_first + tid;
+  const bool live = tk < T;
+  const int tkc = live ? tk : t_last;
+  const int zr = tkc / g.tpr, eb = tkc % g.NEB;
+  const int t2 = (tkc - zr * g.tpr) / g.NEB;
+  const int ph = t2 / g.NQB, qb = t2 - ph * g.NQB;
+  const int r = zr - r_first;
+  const float2* Hr = Hs + r * HS + ph * g.QLF + S * qb + Ne_pad - E * eb - E;
+  const float2* Pr = Ps + r * PS + ph * W;
+  const float2* Wr = Ws + E * eb * W;
+
+  // table x index of each staged window slot (-1: outside the table), and
+  // the table row of each staged row of the step
+  int* dh = reinterpret_cast<int*>(Ps + nP);
+  int* hr = dh + HS;
+  for (int o = tid; o < HS; o += 256) {
+    const int p2 = o / g.QLF, q = o - p2 * g.QLF;
+    const int t = (q - Qoff) * Np + p2;
+    dh[o] = t >= 0 && t < I->N_ext ? I->ext[t] : -1;
+  }
+
+  const int Ney = I->Ney, n_chunks = I->n_chunks;
+  float gr[KR][S], gi[KR][S];
+#pragma unroll
+  for (int q = 0; q < KR; ++q)
+#pragma unroll
+    for (int s = 0; s < S; ++s) gr[q][s] = gi[q][s] = 0.f;
+  float y1[S][W], y2[S][W], y3[S][W];
+
+  for (int cix = grp, ey = 0; cix < n_chunks;) {
+    // stage the windows of the rows, the slice's weights and, with the
+    // first slice, the chunk's p_inc rows (all addresses valid or skipped)
+    const Chunk ch = chunks[I->chunk_base + cix];
+    const float2* hf = I->h + (size_t)ch.fg * I->Hp;
+    const int* cpr = chunk_pairs + (I->chunk_base + cix) * NC;
+    if (tid < NR) hr[tid] = I->hrow[ey * I->n_rows + r_first + tid];
+    __syncthreads();
+    for (int rr = 0; rr < NR; ++rr) {
+      const int row = hr[rr];
+      for (int o = tid; o < HS; o += 256) {
+        const int e = dh[o];
+        Hs[rr * HS + o] = e >= 0 ? hf[(size_t)row * I->HR + e] : make_float2(0.f, 0.f);
+      }
+    }
+    const float2* wc = I->Wc + ((size_t)cix * Ney + ey) * Ne_pad * NC;
+    for (int o = tid; o < nW; o += 256) {
+      const int e = o / W, c = o - e * W;
+      Ws[o] = wc[e * NC + c];
+    }
+    if (ey == 0) {
+      int pr[W];
+#pragma unroll
+      for (int c = 0; c < W; ++c) pr[c] = cpr[c];
+      for (int rr = 0; rr < NR; ++rr) {
+        const size_t rowo = (size_t)(r_first + rr) * Nx;
+        for (int o = tid; o < PS; o += 256) {
+          const int i = o / W, c = o - i * W;
+          int q = pr[0];
+#pragma unroll
+          for (int cc = 1; cc < W; ++cc)
+            if (c == cc) q = pr[cc];
+          float2 v = make_float2(0.f, 0.f);
+          if (i < Nx) v = I->P[(size_t)q * N_pts + rowo + i];
+          Ps[rr * PS + o] = v;
+        }
+      }
+    }
+    __syncthreads();
+
+    if (ey == 0) {
+#pragma unroll
+      for (int s = 0; s < S; ++s)
+#pragma unroll
+        for (int c = 0; c < W; ++c) y1[s][c] = y2[s][c] = y3[s][c] = 0.f;
+    }
+    // conj(A) W: P1 = Ar Wr, P2 = Ai Wi, P3 = (Ar - Ai)(Wr + Wi)
+    {
+      float wr[S + E - 1], wi[S + E - 1], wd[S + E - 1];
+#pragma unroll
+      for (int u = 0; u < S + E - 1; ++u) {
+        const float2 a = Hr[u];
+        wr[u] = a.x;
+        wi[u] = a.y;
+        wd[u] = a.x - a.y;
+      }
+#pragma unroll
+      for (int j = 0; j < E; ++j) {
+        float vr[W], vi[W], vs[W];
+#pragma unroll
+        for (int c = 0; c < W; ++c) {
+          const float2 v = Wr[j * W + c];
+          vr[c] = v.x;
+          vi[c] = v.y;
+          vs[c] = v.x + v.y;
+        }
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+          const int u = s - j + E - 1;
+#pragma unroll
+          for (int c = 0; c < W; ++c) {
+            y1[s][c] = fmaf(wr[u], vr[c], y1[s][c]);
+            y2[s][c] = fmaf(wi[u], vi[c], y2[s][c]);
+            y3[s][c] = fmaf(wd[u], vs[c], y3[s][c]);
+          }
+        }
+      }
+    }
+    if (ey + 1 < Ney) {
+      ++ey;
+      continue;
+    }
+    // Re = P1 + P2, Im = P3 - P1 + P2; gamma_hat += conj(p) Y
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+#pragma unroll
+      for (int c = 0; c < W; ++c) {
+        const float2 pv = Pr[(Np * (S * qb + s)) * W + c];
+        const float yr = y1[s][c] + y2[s][c], yi = y3[s][c] - y1[s][c] + y2[s][c];
+        gr[c % KR][s] += pv.x * yr + pv.y * yi;
+        gi[c % KR][s] += pv.x * yi - pv.y * yr;
+      }
+    }
+    // a right-hand side is complete within the group after its last chunk
+    // there (KR = 1), all of them after the group's last chunk (KR > 1)
+    const int rhs = chunks[I->chunk_base + cix].r;
+    const bool last = cix + G >= n_chunks;
+    const bool flush = last || (KR == 1 && chunks[I->chunk_base + cix + G].r != rhs);
+    if (flush) {
+#pragma unroll
+      for (int q = 0; q < KR; ++q) {
+        if (KR > 1 && q >= I->n_rhs) continue;
+        float2* out = I->gpart + ((size_t)grp * I->n_out + (KR > 1 ? q : rhs)) * N_pts + (size_t)zr * Nx;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+          float x = gr[q][s], y = gi[q][s];
+          for (int o = 1; o < g.NEB; o <<= 1) {
+            x += __shfl_xor(x, o, 64);
+            y += __shfl_xor(y, o, 64);
+          }
+          const int i = Np * (S * qb + s) + ph;
+          if (live && eb == 0 && i < Nx) out[i] = make_float2(x, y);
+          gr[q][s] = gi[q][s] = 0.f;
+        }
+      }
+    }
+    cix += G;
+    ey = 0;
+  }
+}
+
+// gamma_hat[r][p] = sum over the G groups of gpart[g][r][p] (in group order).
+__global__ void k_adj_fc_reduce(const float2* __restrict__ gpart, int G, size_t n, float2* __restrict__ out) {
+  const size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  float2 v = gpart[idx];
+  for (int q = 1; q < G; ++q) {
+    const float2 u = gpart[(size_t)q * n + idx];
+    v.x += u.x;
+    v.y += u.y;
+  }
+  out[idx] = v;
+}
+
+// ------------------------------------------------------------------ launch
+void launch_fwd_fc(const LaunchSet& L, hipStream_t s) {
+  const int Ne_pad = 16 * L.MT;
+  size_t lds = 0;
+  for (const Inst& I : L.inst) lds = std::max(lds, fc_fwd_lds(I.Nx, I.Np, Ne_pad, L.fc_w));
+  const dim3 grid(L.n_items);
+  const int n_inst = (int)L.inst.size();
+#define BORN_FC_F(Wv, Ev, Sv)                                                                            \
+  hipLaunchKernelGGL((k_fwd_fc<Wv, Ev, Sv>), grid, dim3(256), lds, s, L.d_inst, n_inst, Ne_pad, L.d_chunks, \
+                     L.d_chunk_pairs)
+  if (L.fc_w == 1) BORN_FC_F(1, 16, 16);
+  else if (L.fc_w == 2) BORN_FC_F(2, 8, 16);
+  else BORN_FC_F(4, 8, 8);
+#undef BORN_FC_F
+}
+
+void launch_adj_fc(const LaunchSet& L, const OutPtrs& outs, hipStream_t s) {
+  const int Ne_pad = 16 * L.MT;
+  size_t lds = 0;
+  for (const Inst& I : L.inst) lds = std::max(lds, fc_adj_lds(I.Nx, I.Np, Ne_pad, L.fc_w, I.n_rows));
+  const int n_inst = (int)L.inst.size();
+  // the partials start at zero: a group writes only the right-hand sides it has chunks of
+  for (const Inst& I : L.inst) launch_zero_f2(I.gpart, (size_t)I.fc_g * I.n_out * I.N_pts, s);
+  const int kr = L.inst.empty() ? 0 : L.inst[0].col_rhs;
+#define BORN_FC_A(Wv, Ev, Sv, KRv)                                                                      \
+  hipLaunchKernelGGL((k_adj_fc<Wv, Ev, Sv, KRv>), dim3(L.n_wg_f), dim3(256), lds, s, L.d_inst, n_inst, Ne_pad, \
+                     L.d_chunks, L.d_chunk_pairs)
+  if (L.fc_w == 1) BORN_FC_A(1, 16, 8, 1);
+  else if (L.fc_w == 2 && kr == 2) BORN_FC_A(2, 16, 8, 2);
+  else if (L.fc_w == 2) BORN_FC_A(2, 16, 8, 1);
+  else if (kr == 4) BORN_FC_A(4, 16, 4, 4);
+  else if (kr == 2) BORN_FC_A(4, 16, 4, 2);
+  else BORN_FC_A(4, 16, 4, 1);
+#undef BORN_FC_A
+  for (int k = 0; k < n_inst; ++k) {
+    const Inst& I = L.inst[k];
+    const size_t n = (size_t)I.n_out * I.N_pts;
+    hipLaunchKernelGGL(k_adj_fc_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, I.gpart, I.fc_g, n,
+                       outs.p[k]);
+  }
+}
+
+// Adjoint task blocks (of 256 tasks) of one instance.
+int fewcol_adj_blocks(int Nx, int Np, int n_rows, int Ne_pad, int fc_w) {
+  int E, S;
+  fc_adj_shape(fc_w, E, S);
+  const FcGeom g = fc_geom(Nx, Np, Ne_pad, E, S);
+  return (int)(((long)n_rows * g.tpr + 255) / 256);
+}
+
+// Allow the few-column kernels their dynamic LDS.
+void set_max_lds_fc(int bytes) {
+#define BORN_SET(K) (void)hipFuncSetAttribute((const void*)(K), hipFuncAttributeMaxDynamicSharedMemorySize, bytes)
+  BORN_SET((k_fwd_fc<1, 16, 16>));
+  BORN_SET((k_fwd_fc<2, 8, 16>));
+  BORN_SET((k_fwd_fc<4, 8, 8>));
+  BORN_SET((k_adj_fc<1, 16, 8, 1>));
+  BORN_SET((k_adj_fc<2, 16, 8, 1>));
+  BORN_SET((k_adj_fc<2, 16, 8, 2>));
+  BORN_SET((k_adj_fc<4, 16, 4, 1>));
+  BORN_SET((k_adj_fc<4, 16, 4, 2>));
+  BORN_SET((k_adj_fc<4, 16, 4, 4>));
+#undef BORN_SET
+}
+
+}  // namespace born

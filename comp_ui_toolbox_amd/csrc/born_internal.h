@@ -38,6 +38,7 @@ struct Inst {
   BORN_GLOBAL float2* vpart;
   int Nx, Nz, N_pts, N_e, Np, N_ext, Hp, HR;
   int n_chunks;
+  int max_cols;             // widest chunk of the instance (columns)
   int chunk_base;           // first chunk of the instance in the launch's chunk tables
   int Ney, n_rows;          // y-element slices, rows (Ny * Nz)
   BORN_GLOBAL const int* hrow;   // [Ney][n_rows] table row of (slice, row)
@@ -48,6 +49,10 @@ struct Inst {
   int item_begin, item_begin_s;  // first forward work item of the instance (plain, split)
   int wg_begin, wg_begin_s;      // first adjoint workgroup of the instance (plain, split)
   int adj_segs, adj_segs_s;      // adjoint segments per row (plain, split)
+  int wg_begin_f;                // first adjoint workgroup of the instance (few-column kernels)
+  int fc_g;                      // few-column adjoint: chunk groups (partial sums summed after)
+  int n_out;                     // right-hand sides the adjoint writes
+  BORN_GLOBAL float2* gpart;     // few-column adjoint partials [fc_g][n_out][N_pts]
   // adjoint chunk columns: col_rhs = 0: every column of a chunk belongs to
   // the chunk's rhs; col_rhs = k (power of 2): column c belongs to rhs
   // c mod k (< n_rhs), and gamma_hat of all right-hand sides is written once
@@ -103,6 +108,9 @@ struct LaunchSet {
   int n_wg = 0, n_wg_s = 0;
   int grid_mult = 2;              // persistent forward workgroups per CU
   bool f16 = false;               // split kernels: FP16 (true) or BF16 hi/lo operands
+  int fc_w = 0;                   // > 0: chunks of at most fc_w columns run on the few-column kernels
+  bool fc_allowed = true;         // false: MFMA kernels only (the allocation-free layout)
+  int n_wg_f = 0;                 // adjoint workgroups of the few-column kernel
 };
 
 // Device tables and index maps.  All device arrays are owned by the plan.
@@ -189,6 +197,7 @@ struct Plan {
   int vblk = 1;                   // partial-sum blocks per chunk allocated in vpart
   int Ne_pad = 16;
   float2* d_vpart = nullptr;      // [n_chunks][vblk][Ne_pad][NC]
+  float2* d_gpart = nullptr;      // few-column adjoint partials [FC_GROUPS][N_rhs][N_pts]
   double2* d_V = nullptr;         // [N_pairs][N_e][N_rhs]
   float2* d_W = nullptr;          // [N_pairs][N_e][N_rhs] (general path)
   float2* d_Wc = nullptr;         // [n_chunks][Ne_pad][NC] (structured path; unused slots stay zero)
@@ -288,6 +297,14 @@ void launch_adj_split_slices(const LaunchSet& L, const OutPtrs& outs, hipStream_
 void set_max_lds_split3(int bytes);
 
 void launch_fwd_structured(const LaunchSet& L, int n_cu, hipStream_t s);
+// chunks of at most 4 columns on the vector ALUs (born_fewcol.hip)
+int fewcol_width(int max_cols);
+bool fewcol_fits(int Nx, int Np, int Ne_pad, int fc_w, int n_rows);
+int fewcol_adj_blocks(int Nx, int Np, int n_rows, int Ne_pad, int fc_w);
+constexpr int FC_GROUPS = 8;    // most chunk groups of the few-column adjoint
+void set_max_lds_fc(int bytes);
+void launch_fwd_fc(const LaunchSet& L, hipStream_t s);
+void launch_adj_fc(const LaunchSet& L, const OutPtrs& outs, hipStream_t s);
 void launch_fwd_general(const Plan& P, const float2* gm, int N_rhs, hipStream_t s);
 void launch_fwd_reduce(const Plan& P, int N_rhs, int nblk, hipStream_t s);
 void launch_fwd_reduce_expanded(const Plan& P, int nblk, hipStream_t s);

@@ -940,3 +940,47 @@ def test_energy_rx_arbitrary(torch_cuda):
     e_t = O.rel_l2(op.energy_rx_arbitrary(D, tgc=True), np.sum(np.abs(APt_tgc) ** 2, axis=0))
     log("energy_rx_arbitrary summed over measurements %.2e, with TGC %.2e" % (e_s, e_t))
     assert e_s < TOL_FP32 and e_t < TOL_FP32
+
+
+# ------------------------------------------------------ few-column chunks
+@pytest.mark.parametrize("precision,tol", [("fp32", TOL_FP32), ("split_bf16", TOL_SPLIT)])
+@pytest.mark.parametrize("n_waves,n_rhs", [(1, 1), (1, 2), (1, 4), (3, 1), (4, 1), (2, 2), (1, 3)])
+@pytest.mark.parametrize("dims", ["2d", "3d"])
+def test_few_column_kernels(torch_cuda, dims, n_waves, n_rhs, precision, tol):
+    """Chunks of at most 4 columns (1-4 waves per frequency; with several
+    right-hand sides packed per column) run on the FP32 vector kernels: one
+    column in both forms, 2-4 in the FP32 form.  Rows whose points per phase
+    are not a multiple of the task length, 2-D at pitch/4 and 3-D at pitch/2
+    (20 x 2 elements over a narrower FOV: extended tables), host and device
+    entry points, against the oracle."""
+    import torch
+    med = C.MediumSpec()
+    if dims == "2d":
+        spec = tiny_spec(N_e=40, Nx=170, Nz=5, npp=4)
+        spec.waves = [C.qpw_wave(spec.array, a, med.c_avg) for a in np.linspace(-0.25, 0.25, n_waves)]
+        spec.mixes = [C.identity_mixes(40) for _ in spec.waves]
+        T = O.build_tables(spec)
+    else:
+        arr = C.ArraySpec((20, 2), (3e-4, 3e-4), (2.7e-4, 2.7e-4), (2, 2))
+        fov = C.FovSpec((21, 10, 3), (1.5e-4, 1.5e-4, 1.5e-4), 2e-3)
+        waves = [C.qsw_wave(arr, np.array([x, 0.0, -1e-2]), med.c_avg) for x in np.linspace(-6e-4, 6e-4, n_waves)]
+        spec = C.SequenceSpec("fc3", arr, med, fov, np.array([2.5e6, 4e6, 5.5e6]), waves,
+                              [C.identity_mixes(40) for _ in waves])
+        T = O.direct_tables(spec)
+    if dims == "2d":
+        op = OperatorBorn.from_tables(T, precision=precision)
+    else:
+        op = OperatorBorn.from_sequence(spec, precision=precision)
+        assert op.path == "structured"
+    g = rnd((T.N_points, n_rhs), 111)
+    u = rnd((T.N_observations(), n_rhs), 112)
+    e_f = O.rel_l2(op.forward(g), O.forward(T, g))
+    e_a = O.rel_l2(op.adjoint(u), O.adjoint(T, u))
+    gd = torch.from_numpy(g.T.astype(np.complex64)).cuda().contiguous()
+    ud = torch.from_numpy(u.T.astype(np.complex64)).cuda().contiguous()
+    e_fd = O.rel_l2(op.forward_dev(gd).cpu().numpy().T, O.forward(T, g.astype(np.complex64)))
+    e_ad = O.rel_l2(op.adjoint_dev(ud).cpu().numpy().T, O.adjoint(T, u.astype(np.complex64)))
+    log("few columns %s, %d waves, %d rhs, %s: forward %.2e adjoint %.2e (device %.2e %.2e)"
+        % (dims, n_waves, n_rhs, precision, e_f, e_a, e_fd, e_ad))
+    assert max(e_f, e_a, e_fd, e_ad) < tol
+    op.close()

@@ -18,6 +18,13 @@ from tests.test_oracle import tiny_spec
 pytestmark = pytest.mark.gpu
 
 TOL = {"fp32": 1e-5, "split_bf16": 1e-4, "split_fp16": 1e-4}
+# Against the allocating entry points: the same arithmetic in FP32.  In the
+# split forms the allocating entry points pack several right-hand sides into
+# wider chunks, and a one-wave plan's one-column chunks (the layout of the
+# allocation-free apply) run on the FP32 vector kernels, the wider ones on
+# the split MFMA kernels: the two then differ by the split form's own error
+# (about 5e-6 in BF16), well inside its 1e-4 bar against FP64.
+AGREE = {"fp32": 1e-6, "split_bf16": 2e-5, "split_fp16": 2e-5}
 
 
 def rnd(shape, seed):
@@ -67,7 +74,7 @@ def test_noalloc_matches_the_allocating_entry_points(torch_cuda, precision):
         assert L.born_debug_live_allocations() == live
         e_u, e_g = rel(u, u_ref), rel(g, g_ref)
         print(f"  against born_forward_dev {e_u:.2e}, born_adjoint_dev {e_g:.2e}", flush=True)
-        assert e_u < 1e-6 and e_g < 1e-6
+        assert e_u < AGREE[precision] and e_g < AGREE[precision]
         if op is small:
             x64 = x.cpu().numpy().T.astype(np.complex128)
             y64 = y.cpu().numpy().T.astype(np.complex128)
