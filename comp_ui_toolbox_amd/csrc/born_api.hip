@@ -427,12 +427,17 @@ static Blocking blocking_for(const Plan& P, int n_chunks, int target) {
   // the plain forward takes sgrp_p y slices per work item
   const int sgrp_p = fwd_plain_slice_group(P.nd, P.Nx, P.Ne_pad, Ney, P.Np);
   int NZB = std::max(1, std::min(rows, target / std::max(1, n_chunks * (Ney / sgrp_p))));
+  // several slices per item: whole groups of XCDS row blocks (from XCDS / 2
+  // up), so that the slice groups of a row block run together on one XCD
+  // (slice_item)
+  if (sgrp_p > 1 && NZB >= XCDS / 2) NZB = std::min(rows, (NZB + XCDS - 1) / XCDS * XCDS);
   b.ZR = (rows + NZB - 1) / NZB;
   b.NZB = (rows + b.ZR - 1) / b.ZR;
   b.nseg_s = (P.Nx + 255) / 256;
   // the split forward takes sgrp y slices per work item
   const int sgrp = fwd_split_slice_group(P.nd, P.Nx, P.Ne_pad, Ney, P.Np);
   NZB = std::max(1, std::min(rows, target / std::max(1, n_chunks * (Ney / sgrp) * b.nseg_s)));
+  if (sgrp > 1 && NZB >= XCDS / 2) NZB = std::min(rows, (NZB + XCDS - 1) / XCDS * XCDS);
   b.ZR_s = (rows + NZB - 1) / NZB;
   b.NZB_s = (rows + b.ZR_s - 1) / b.ZR_s;
   b.vblk = Ney * std::max(b.NZB, b.NZB_s * b.nseg_s);   // partial-sum blocks per chunk
