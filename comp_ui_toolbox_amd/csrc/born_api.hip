@@ -520,8 +520,17 @@ static void finalize_launchset(LaunchSet& L) {
   L.fc_w = L.inst.empty() || !L.fc_allowed ? 0 : fewcol_width(max_cols);
   for (const Inst& I : L.inst)
     if (!I.gpart || !fewcol_fits(I.Nx, I.Np, 16 * L.MT, L.fc_w, I.n_rows)) L.fc_w = 0;
+  // few-column forward items of at least 4 steps of R staged rows (fewer,
+  // longer items than the MFMA kernels' blocking)
   if (L.fc_w > 0)
-    for (Inst& I : L.inst) I.sgrp_p = 1;
+    for (Inst& I : L.inst) {
+      I.sgrp_p = 1;
+      const int R = fewcol_fwd_rows(I.Nx, I.Np, 16 * L.MT, L.fc_w);
+      if (I.ZR < 4 * R) {
+        I.ZR = std::min(I.n_rows, 4 * R);
+        I.NZB = (I.n_rows + I.ZR - 1) / I.ZR;
+      }
+    }
   // adjoint: task blocks x chunk groups, about 2048 workgroups per instance
   // and at least 2 chunks per group
   int wgf = 0;

@@ -102,6 +102,12 @@ size_t fc_adj_lds(int Nx, int Np, int Ne_pad, int fc_w, int n_rows) {
 
 }  // namespace
 
+int fewcol_fwd_rows(int Nx, int Np, int Ne_pad, int fc_w) {
+  int E, S;
+  fc_fwd_shape(fc_w, E, S);
+  return fc_fwd_rows(fc_geom(Nx, Np, Ne_pad, E, S));
+}
+
 int fewcol_width(int max_cols) { return max_cols <= 1 ? 1 : (max_cols <= 2 ? 2 : (max_cols <= 4 ? 4 : 0)); }
 
 bool fewcol_fits(int Nx, int Np, int Ne_pad, int fc_w, int n_rows) {

@@ -301,6 +301,7 @@ void launch_fwd_structured(const LaunchSet& L, int n_cu, hipStream_t s);
 int fewcol_width(int max_cols);
 bool fewcol_fits(int Nx, int Np, int Ne_pad, int fc_w, int n_rows);
 int fewcol_adj_blocks(int Nx, int Np, int n_rows, int Ne_pad, int fc_w);
+int fewcol_fwd_rows(int Nx, int Np, int Ne_pad, int fc_w);
 constexpr int FC_GROUPS = 8;    // most chunk groups of the few-column adjoint
 void set_max_lds_fc(int bytes);
 void launch_fwd_fc(const LaunchSet& L, hipStream_t s);

@@ -951,8 +951,8 @@ def test_few_column_kernels(torch_cuda, dims, n_waves, n_rhs, precision, tol):
     right-hand sides packed per column) run on the FP32 vector kernels: one
     column in both forms, 2-4 in the FP32 form.  Rows whose points per phase
     are not a multiple of the task length, 2-D at pitch/4 and 3-D at pitch/2
-    (20 x 2 elements over a narrower FOV: extended tables), host and device
-    entry points, against the oracle."""
+    (20 x 2 elements, 130-point rows), host and device entry points, against
+    the oracle."""
     import torch
     med = C.MediumSpec()
     if dims == "2d":
@@ -962,7 +962,7 @@ def test_few_column_kernels(torch_cuda, dims, n_waves, n_rhs, precision, tol):
         T = O.build_tables(spec)
     else:
         arr = C.ArraySpec((20, 2), (3e-4, 3e-4), (2.7e-4, 2.7e-4), (2, 2))
-        fov = C.FovSpec((21, 10, 3), (1.5e-4, 1.5e-4, 1.5e-4), 2e-3)
+        fov = C.FovSpec((130, 6, 2), (1.5e-4, 1.5e-4, 1.5e-4), 2e-3)
         waves = [C.qsw_wave(arr, np.array([x, 0.0, -1e-2]), med.c_avg) for x in np.linspace(-6e-4, 6e-4, n_waves)]
         spec = C.SequenceSpec("fc3", arr, med, fov, np.array([2.5e6, 4e6, 5.5e6]), waves,
                               [C.identity_mixes(40) for _ in waves])
