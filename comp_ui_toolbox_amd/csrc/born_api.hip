@@ -520,6 +520,11 @@ static void finalize_launchset(LaunchSet& L) {
   L.fc_w = L.inst.empty() || !L.fc_allowed ? 0 : fewcol_width(max_cols);
   for (const Inst& I : L.inst)
     if (!I.gpart || !fewcol_fits(I.Nx, I.Np, 16 * L.MT, L.fc_w, I.n_rows)) L.fc_w = 0;
+  for (const Inst& I : L.inst) {
+    int bt = 0, nr = 0;
+    if (L.fc_w > 0) fewcol_adj_block(I.Nx, I.Np, 16 * L.MT, L.fc_w, I.col_rhs, I.n_rows, bt, nr);
+    if (bt == 0) L.fc_w = 0;
+  }
   // few-column forward items of at least 4 steps of R staged rows (fewer,
   // longer items than the MFMA kernels' blocking)
   if (L.fc_w > 0)
@@ -537,9 +542,15 @@ static void finalize_launchset(LaunchSet& L) {
   for (Inst& I : L.inst) {
     I.wg_begin_f = wgf;
     I.fc_g = 1;
+    I.fc_bt = 256;
+    I.fc_nr = 1;
+    I.fc_cb = 1;
     if (L.fc_w == 0) continue;
-    const int nb = fewcol_adj_blocks(I.Nx, I.Np, I.n_rows, 16 * L.MT, L.fc_w);
+    fewcol_adj_block(I.Nx, I.Np, 16 * L.MT, L.fc_w, I.col_rhs, I.n_rows, I.fc_bt, I.fc_nr);
+    const long T = (long)I.n_rows * fewcol_adj_tpr(I.Nx, I.Np, 16 * L.MT, L.fc_w);
+    const int nb = (int)((T + I.fc_bt - 1) / I.fc_bt);
     I.fc_g = std::max(1, std::min({FC_GROUPS, (2048 + nb - 1) / nb, I.n_chunks / 2}));
+    I.fc_cb = fewcol_adj_cb(I.Nx, I.Np, 16 * L.MT, L.fc_w, I.col_rhs, I.fc_nr, (I.n_chunks + I.fc_g - 1) / I.fc_g);
     wgf += nb * I.fc_g;
   }
   L.n_wg_f = wgf;

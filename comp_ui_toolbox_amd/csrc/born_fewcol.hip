@@ -54,29 +54,61 @@ __host__ __device__ inline int fc_fwd_rows(const FcGeom& g) {
   while (R < 8 && R * g.tpr < 256) R <<= 1;
   return R;
 }
-// Adjoint: a workgroup owns 256 consecutive tasks (row, phase, q-block,
-// element block) of its instance and stages the rows they touch: the most
-// rows any block of the instance touches.
-inline int fc_adj_rows_max(const FcGeom& g, int n_rows) {
-  if (g.tpr % 256 == 0) return 1;
-  const long T = (long)n_rows * g.tpr;
-  int m = 1;
-  for (long t0 = 0; t0 < T; t0 += 256)
-    m = std::max(m, (int)((std::min(T, t0 + 256) - 1) / g.tpr - t0 / g.tpr + 1));
-  return m;
-}
-
 // Task shapes: forward W = 1: E 16 x S 16, 2: 8 x 16, 4: 8 x 8 (sums of E x
-// W); adjoint E 16 x S 8 (W = 4: S 4) (sums of S x W).
+// W); adjoint W = 1: E 8 x S 8, 2: 16 x 8, 4: 16 x 4 with at most fc_km
+// tasks per thread (sums of S x W per task).
 constexpr int FC_LDS_MAX = 78 * 1024;   // two workgroups per compute unit
+constexpr int FC_ROWS_MAX = 16;         // rows an adjoint workgroup stages (its row table in LDS)
 void fc_fwd_shape(int fc_w, int& E, int& S) {
   E = fc_w == 1 ? 16 : 8;
   S = fc_w == 4 ? 8 : 16;
 }
 void fc_adj_shape(int fc_w, int& E, int& S) {
-  E = 16;
+  E = fc_w == 1 ? 8 : 16;
   S = fc_w == 4 ? 4 : 8;
 }
+// tasks per thread: W = 1 one (4 waves per SIMD), W = 2, 4 two; one with
+// right-hand sides packed per column (registers)
+inline int fc_km(int fc_w, int kr) { return kr > 1 || fc_w == 1 ? 1 : 2; }
+// cb staged units (windows, weights, p_inc of R rows), window offsets, row table
+size_t fc_adj_lds_rows(const FcGeom& g, int Np, int Ne_pad, int fc_w, int R, int cb = 1) {
+  return (size_t)cb * ((size_t)R * (Np * g.QLF + g.XPF * fc_w) + (size_t)Ne_pad * fc_w) * sizeof(float2) +
+         ((size_t)Np * g.QLF + FC_ROWS_MAX) * sizeof(int);
+}
+// Adjoint work per workgroup, in tasks (BT) of the instance's list (row,
+// phase, q-block, element block): R whole rows when their tasks fill at
+// least 3/4 of at most fc_km rounds of 256 threads (the best filled, then
+// the fewest rows), else 256 fc_km consecutive tasks (row segments);
+// NR = the most rows a block touches (at most FC_ROWS_MAX), within the LDS
+// budget.  BT = 0: none.
+struct FcAdjBlock {
+  int BT, NR;
+};
+FcAdjBlock fc_adj_block(int Nx, int Np, int Ne_pad, int fc_w, int kr, int n_rows) {
+  int E, S;
+  fc_adj_shape(fc_w, E, S);
+  const FcGeom g = fc_geom(Nx, Np, Ne_pad, E, S);
+  const int km = fc_km(fc_w, kr);
+  FcAdjBlock b{0, 0};
+  double best_u = 0.0;
+  for (int R = 1; R <= 8; ++R) {
+    const int K = (R * g.tpr + 255) / 256;
+    if (K > km || fc_adj_lds_rows(g, Np, Ne_pad, fc_w, R) > FC_LDS_MAX) continue;
+    const double u = (double)R * g.tpr / (256.0 * K);
+    if (u > best_u + 1e-9) {
+      best_u = u;
+      b = FcAdjBlock{R * g.tpr, R};
+    }
+  }
+  if (best_u >= 0.75) return b;
+  const long T = (long)n_rows * g.tpr;
+  const int BT = 256 * km;
+  int NR = 1;
+  for (long t0 = 0; t0 < T; t0 += BT) NR = std::max(NR, (int)((std::min(T, t0 + BT) - 1) / g.tpr - t0 / g.tpr + 1));
+  if (NR > FC_ROWS_MAX || fc_adj_lds_rows(g, Np, Ne_pad, fc_w, NR) > FC_LDS_MAX) return b;
+  return FcAdjBlock{BT, NR};
+}
+
 size_t fc_fwd_lds(int Nx, int Np, int Ne_pad, int fc_w) {
   int E, S;
   fc_fwd_shape(fc_w, E, S);
@@ -85,19 +117,10 @@ size_t fc_fwd_lds(int Nx, int Np, int Ne_pad, int fc_w) {
   // staged rows (then the final reduction), window slot offsets, row table
   return std::max(rows, (size_t)4 * 128 * fc_w) * sizeof(float2) + ((size_t)Np * g.QLF + 8) * sizeof(int);
 }
-// staged entries of one adjoint step
-size_t fc_adj_entries(int Nx, int Np, int Ne_pad, int fc_w, int n_rows) {
+size_t fc_adj_lds(int Nx, int Np, int Ne_pad, int fc_w, int NR, int cb) {
   int E, S;
   fc_adj_shape(fc_w, E, S);
-  const FcGeom g = fc_geom(Nx, Np, Ne_pad, E, S);
-  const int NR = fc_adj_rows_max(g, n_rows);
-  return (size_t)NR * (Np * g.QLF + g.XPF * fc_w) + (size_t)Ne_pad * fc_w;
-}
-size_t fc_adj_lds(int Nx, int Np, int Ne_pad, int fc_w, int n_rows) {
-  int E, S;
-  fc_adj_shape(fc_w, E, S);
-  const FcGeom g = fc_geom(Nx, Np, Ne_pad, E, S);
-  return fc_adj_entries(Nx, Np, Ne_pad, fc_w, n_rows) * sizeof(float2) + ((size_t)Np * g.QLF + 16) * sizeof(int);
+  return fc_adj_lds_rows(fc_geom(Nx, Np, Ne_pad, E, S), Np, Ne_pad, fc_w, NR, cb);
 }
 
 }  // namespace
@@ -112,10 +135,7 @@ int fewcol_width(int max_cols) { return max_cols <= 1 ? 1 : (max_cols <= 2 ? 2 :
 
 bool fewcol_fits(int Nx, int Np, int Ne_pad, int fc_w, int n_rows) {
   if (fc_w <= 0 || !(Ne_pad == 16 || Ne_pad == 32 || Ne_pad == 64 || Ne_pad == 128)) return false;
-  int E, S;
-  fc_adj_shape(fc_w, E, S);
-  if (fc_geom(Nx, Np, Ne_pad, E, S).tpr < 32) return false;
-  if (fc_adj_lds(Nx, Np, Ne_pad, fc_w, n_rows) > FC_LDS_MAX) return false;
+  if (fc_adj_block(Nx, Np, Ne_pad, fc_w, 0, n_rows).BT == 0) return false;
   return fc_fwd_lds(Nx, Np, Ne_pad, fc_w) <= FC_LDS_MAX;
 }
 
@@ -269,20 +289,25 @@ __global__ __launch_bounds__(256, 2) void k_fwd_fc(const Inst* __restrict__ inst
 
 // ------------------------------------------------------------------ adjoint
 // Workgroups of instance k: [wg_begin_f, wg_begin_f + blocks * G); workgroup
-// wl owns the tasks [256 b, 256 b + 256) of the instance's list (row, phase,
-// q-block, element block), b = wl / G, one per thread, and the chunks
-// g, g + G, g + 2G, ... (g = wl mod G; chunks are rhs-major, then frequency).
-// Per chunk and y slice it stages the rows its tasks touch, the weights and
-// (first slice) the p_inc rows:
+// wl owns the tasks [BT b, BT b + BT) of the instance's list (row, phase,
+// q-block, element block; b = wl / G, BT = fc_bt: whole rows or a row
+// segment) and the chunks g, g + G, g + 2G, ... (g = wl mod G; chunks are
+// rhs-major, then frequency).  Its tasks go to the threads in rounds of 256,
+// at most KM per thread.  Per chunk and y slice the workgroup stages the
+// windows of the rows its tasks touch,
+// the slice's weights and, with the first slice, the p_inc rows; then every
+// task computes
 //   Y[s][c]   = sum_e conj(Hext[t(s, e)]) W[e][c]      (its E elements)
 //   Gh[r][s] += sum_c conj(p_inc[pair_c][i_s]) Y[s][c]  (r = rhs of column c)
-// The NEB element-block lanes of a task (adjacent lanes) are summed when a
-// right-hand side is complete within the group, and written to the group's
-// partial gpart[g][r][point]; k_adj_fc_reduce sums the G partials in order
-// (no atomics: bitwise repeatable).  KR = 1: every column belongs to the
-// chunk's rhs; KR = col_rhs (2 or 4): column c belongs to rhs c mod KR.
-template <int W, int E, int S, int KR>
-__global__ __launch_bounds__(256, 2) void k_adj_fc(const Inst* __restrict__ inst, int n_inst, int Ne_pad,
+// (p_inc does not depend on the slice, so each slice's Y is applied as it
+// is done).  The NEB element-block lanes of a task (adjacent lanes) are
+// summed when a right-hand side is complete within the group, and written
+// to the group's partial gpart[g][r][point]; k_adj_fc_reduce sums the G
+// partials in order (no atomics: bitwise repeatable).  KR = 1: every column
+// belongs to the chunk's rhs; KR = col_rhs (2 or 4): column c belongs to rhs
+// c mod KR, and all are written after the group's last chunk.
+template <int W, int E, int S, int KR, int KM>
+__global__ __launch_bounds__(256, (KM == 1 && W == 1 ? 4 : 2)) void k_adj_fc(const Inst* __restrict__ inst, int n_inst, int Ne_pad,
                                                    const Chunk* __restrict__ chunks,
                                                    const int* __restrict__ chunk_pairs) {
   extern __shared__ float2 lds[];
@@ -292,168 +317,191 @@ __global__ __launch_bounds__(256, 2) void k_adj_fc(const Inst* __restrict__ inst
   const Inst* I = &Iv;
   const int tid = threadIdx.x;
 
-  const int Np = I->Np, Nx = I->Nx, N_pts = I->N_pts, G = I->fc_g;
+  const int Np = I->Np, Nx = I->Nx, N_pts = I->N_pts, G = I->fc_g, BT = I->fc_bt;
   const FcGeom g = fc_geom(Nx, Np, Ne_pad, E, S);
-  const int T = I->n_rows * g.tpr;
   const int wl = (int)blockIdx.x - I->wg_begin_f;
   const int grp = wl % G;
-  const int t_first = (wl / G) * 256;
-  const int t_last = min(T, t_first + 256) - 1;
-  const int r_first = t_first / g.tpr, NR = t_last / g.tpr - r_first + 1;
+  const int T = I->n_rows * g.tpr;
+  const int t_first = (wl / G) * BT, t_end = min(T, t_first + BT);
+  const int r_first = t_first / g.tpr, NR = (t_end - 1) / g.tpr - r_first + 1;
+  const int t_off = t_first - r_first * g.tpr;   // first task's position in its row
   const int HS = Np * g.QLF, PS = g.XPF * W;
   const int nH = NR * HS, nW = Ne_pad * W, nP = NR * PS;
-  float2* Hs = lds;              // [NR][Np][QLF]
-  float2* Ws = lds + nH;         // [Ne_pad][W]
-  float2* Ps = Ws + nW;          // [NR][XPF][W]
+  // staged units, one per chunk of a step: windows [NR][Np][QLF], weights
+  // [Ne_pad][W], p_inc [NR][XPF][W]; then the window offsets and row table
+  const int unit = nH + nW + nP;
+  int* dh = reinterpret_cast<int*>(lds + I->fc_cb * unit);   // window slot -> table x index (-1: outside)
+  int* hr = dh + HS;                           // staged row -> table row (NR <= FC_ROWS_MAX)
   const int Qoff = Ne_pad - I->N_e;
+  const int K = (BT + 255) / 256;              // task rounds (<= KM)
+  const int ntask = t_end - t_first;
+  const int eb = tid % g.NEB;
 
-  const int tk = t_first + tid;
-  const bool live = tk < T;
-  const int tkc = live ? tk : t_last;
-  const int zr = tkc / g.tpr, eb = tkc % g.NEB;
-  const int t2 = (tkc - zr * g.tpr) / g.NEB;
-  const int ph = t2 / g.NQB, qb = t2 - ph * g.NQB;
-  const int r = zr - r_first;
-  const float2* Hr = Hs + r * HS + ph * g.QLF + S * qb + Ne_pad - E * eb - E;
-  const float2* Pr = Ps + r * PS + ph * W;
-  const float2* Wr = Ws + E * eb * W;
-
-  // table x index of each staged window slot (-1: outside the table), and
-  // the table row of each staged row of the step
-  int* dh = reinterpret_cast<int*>(Ps + nP);
-  int* hr = dh + HS;
+  // the thread's tasks: window, p_inc and output offsets
+  int hoff[KM], poff[KM], oofs[KM], xk[KM];
+  bool live[KM];
+#pragma unroll
+  for (int kk = 0; kk < KM; ++kk) {
+    const int tk = tid + 256 * kk;
+    live[kk] = kk < K && tk < ntask;
+    const int tkc = (live[kk] ? tk : 0) + t_off;
+    const int r = tkc / g.tpr, t2 = (tkc - r * g.tpr) / g.NEB;
+    const int ph = t2 / g.NQB, qb = t2 - ph * g.NQB;
+    hoff[kk] = r * HS + ph * g.QLF + S * qb + Ne_pad - E * eb - E;
+    poff[kk] = r * PS + (Np * S * qb + ph) * W;
+    xk[kk] = Np * S * qb + ph;                          // x of s = 0
+    oofs[kk] = (r_first + r) * Nx + xk[kk];
+  }
   for (int o = tid; o < HS; o += 256) {
     const int p2 = o / g.QLF, q = o - p2 * g.QLF;
     const int t = (q - Qoff) * Np + p2;
     dh[o] = t >= 0 && t < I->N_ext ? I->ext[t] : -1;
   }
 
-  const int Ney = I->Ney, n_chunks = I->n_chunks;
-  float gr[KR][S], gi[KR][S];
+  float gr[KM][KR][S], gi[KM][KR][S];
 #pragma unroll
-  for (int q = 0; q < KR; ++q)
+  for (int kk = 0; kk < KM; ++kk)
 #pragma unroll
-    for (int s = 0; s < S; ++s) gr[q][s] = gi[q][s] = 0.f;
-  float y1[S][W], y2[S][W], y3[S][W];
+    for (int q = 0; q < KR; ++q)
+#pragma unroll
+      for (int s = 0; s < S; ++s) gr[kk][q][s] = gi[kk][q][s] = 0.f;
 
-  for (int cix = grp, ey = 0; cix < n_chunks;) {
-    // stage the windows of the rows, the slice's weights and, with the
-    // first slice, the chunk's p_inc rows (all addresses valid or skipped)
-    const Chunk ch = chunks[I->chunk_base + cix];
-    const float2* hf = I->h + (size_t)ch.fg * I->Hp;
-    const int* cpr = chunk_pairs + (I->chunk_base + cix) * NC;
-    if (tid < NR) hr[tid] = I->hrow[ey * I->n_rows + r_first + tid];
-    __syncthreads();
-    for (int rr = 0; rr < NR; ++rr) {
-      const int row = hr[rr];
-      for (int o = tid; o < HS; o += 256) {
-        const int e = dh[o];
-        Hs[rr * HS + o] = e >= 0 ? hf[(size_t)row * I->HR + e] : make_float2(0.f, 0.f);
-      }
-    }
-    const float2* wc = I->Wc + ((size_t)cix * Ney + ey) * Ne_pad * NC;
-    for (int o = tid; o < nW; o += 256) {
-      const int e = o / W, c = o - e * W;
-      Ws[o] = wc[e * NC + c];
-    }
-    if (ey == 0) {
-      int pr[W];
+  const int Ney = I->Ney, n_chunks = I->n_chunks, cb = I->fc_cb;
+  // a step stages cb of the group's chunks (2-D; 3-D: one chunk and slice)
+  for (int m = 0; grp + G * m < n_chunks; m += cb) {
+    const int nck = min(cb, (n_chunks - grp - G * m + G - 1) / G);
+    for (int ey = 0; ey < Ney; ++ey) {
+      if (tid < NR) hr[tid] = I->hrow[ey * I->n_rows + r_first + tid];
+      __syncthreads();
+      for (int j = 0; j < nck; ++j) {
+        const int cix = grp + G * (m + j);
+        const Chunk ch = chunks[I->chunk_base + cix];
+        const float2* hf = I->h + (size_t)ch.fg * I->Hp;
+        float2* Hs = lds + j * unit;
+        float2* Ws = Hs + nH;
+        float2* Ps = Ws + nW;
+        for (int rr = 0; rr < NR; ++rr) {
+          const int row = hr[rr];
+          for (int o = tid; o < HS; o += 256) {
+            const int e = dh[o];
+            Hs[rr * HS + o] = e >= 0 ? hf[(size_t)row * I->HR + e] : make_float2(0.f, 0.f);
+          }
+        }
+        const float2* wc = I->Wc + ((size_t)cix * Ney + ey) * Ne_pad * NC;
+        for (int o = tid; o < nW; o += 256) {
+          const int e = o / W, c = o - e * W;
+          Ws[o] = wc[e * NC + c];
+        }
+        if (ey == 0) {
+          const int* cpr = chunk_pairs + (I->chunk_base + cix) * NC;
+          int pr[W];
 #pragma unroll
-      for (int c = 0; c < W; ++c) pr[c] = cpr[c];
-      for (int rr = 0; rr < NR; ++rr) {
-        const size_t rowo = (size_t)(r_first + rr) * Nx;
-        for (int o = tid; o < PS; o += 256) {
-          const int i = o / W, c = o - i * W;
-          int q = pr[0];
+          for (int c = 0; c < W; ++c) pr[c] = cpr[c];
+          for (int rr = 0; rr < NR; ++rr) {
+            const size_t rowo = (size_t)(r_first + rr) * Nx;
+            for (int o = tid; o < PS; o += 256) {
+              const int i = o / W, c = o - i * W;
+              int q = pr[0];
 #pragma unroll
-          for (int cc = 1; cc < W; ++cc)
-            if (c == cc) q = pr[cc];
-          float2 v = make_float2(0.f, 0.f);
-          if (i < Nx) v = I->P[(size_t)q * N_pts + rowo + i];
-          Ps[rr * PS + o] = v;
+              for (int cc = 1; cc < W; ++cc)
+                if (c == cc) q = pr[cc];
+              float2 v = make_float2(0.f, 0.f);
+              if (i < Nx) v = I->P[(size_t)q * N_pts + rowo + i];
+              Ps[rr * PS + o] = v;
+            }
+          }
         }
       }
-    }
-    __syncthreads();
+      __syncthreads();
 
-    if (ey == 0) {
+      for (int j = 0; j < nck; ++j) {
+        const float2* Hs = lds + j * unit;
+        const float2* Ws = Hs + nH;
+        const float2* Ps = Ws + nW;
 #pragma unroll
-      for (int s = 0; s < S; ++s)
+        for (int kk = 0; kk < KM; ++kk) {
+          if (kk >= K) continue;
+          // conj(A) W: P1 = Ar Wr, P2 = Ai Wi, P3 = (Ar - Ai)(Wr + Wi)
+          float y1[S][W], y2[S][W], y3[S][W];
 #pragma unroll
-        for (int c = 0; c < W; ++c) y1[s][c] = y2[s][c] = y3[s][c] = 0.f;
-    }
-    // conj(A) W: P1 = Ar Wr, P2 = Ai Wi, P3 = (Ar - Ai)(Wr + Wi)
-    {
-      float wr[S + E - 1], wi[S + E - 1], wd[S + E - 1];
+          for (int s = 0; s < S; ++s)
 #pragma unroll
-      for (int u = 0; u < S + E - 1; ++u) {
-        const float2 a = Hr[u];
-        wr[u] = a.x;
-        wi[u] = a.y;
-        wd[u] = a.x - a.y;
-      }
+            for (int c = 0; c < W; ++c) y1[s][c] = y2[s][c] = y3[s][c] = 0.f;
+          {
+            const float2* Hr = Hs + hoff[kk];
+            const float2* Wr = Ws + E * eb * W;
+            float wr[S + E - 1], wi[S + E - 1], wd[S + E - 1];
 #pragma unroll
-      for (int j = 0; j < E; ++j) {
-        float vr[W], vi[W], vs[W];
+            for (int u = 0; u < S + E - 1; ++u) {
+              const float2 a = Hr[u];
+              wr[u] = a.x;
+              wi[u] = a.y;
+              wd[u] = a.x - a.y;
+            }
 #pragma unroll
-        for (int c = 0; c < W; ++c) {
-          const float2 v = Wr[j * W + c];
-          vr[c] = v.x;
-          vi[c] = v.y;
-          vs[c] = v.x + v.y;
+            for (int jj = 0; jj < E; ++jj) {
+              float vr[W], vi[W], vs[W];
+#pragma unroll
+              for (int c = 0; c < W; ++c) {
+                const float2 v = Wr[jj * W + c];
+                vr[c] = v.x;
+                vi[c] = v.y;
+                vs[c] = v.x + v.y;
+              }
+#pragma unroll
+              for (int s = 0; s < S; ++s) {
+                const int u = s - jj + E - 1;
+#pragma unroll
+                for (int c = 0; c < W; ++c) {
+                  y1[s][c] = fmaf(wr[u], vr[c], y1[s][c]);
+                  y2[s][c] = fmaf(wi[u], vi[c], y2[s][c]);
+                  y3[s][c] = fmaf(wd[u], vs[c], y3[s][c]);
+                }
+              }
+            }
+          }
+          // Re = P1 + P2, Im = P3 - P1 + P2; gamma_hat += conj(p) Y
+          const float2* Pr = Ps + poff[kk];
+#pragma unroll
+          for (int s = 0; s < S; ++s) {
+#pragma unroll
+            for (int c = 0; c < W; ++c) {
+              const float2 pv = Pr[Np * s * W + c];
+              const float yr = y1[s][c] + y2[s][c], yi = y3[s][c] - y1[s][c] + y2[s][c];
+              gr[kk][c % KR][s] += pv.x * yr + pv.y * yi;
+              gi[kk][c % KR][s] += pv.x * yi - pv.y * yr;
+            }
+          }
         }
+        if (ey + 1 < Ney) continue;
+        // a right-hand side is complete within the group after its last chunk
+        // there (KR = 1), all of them after the group's last chunk (KR > 1)
+        const int cix = grp + G * (m + j);
+        const int rhs = chunks[I->chunk_base + cix].r;
+        const bool last = cix + G >= n_chunks;
+        const bool flush = last || (KR == 1 && chunks[I->chunk_base + cix + G].r != rhs);
+        if (!flush) continue;
 #pragma unroll
-        for (int s = 0; s < S; ++s) {
-          const int u = s - j + E - 1;
+        for (int kk = 0; kk < KM; ++kk) {
+          if (kk >= K) continue;
 #pragma unroll
-          for (int c = 0; c < W; ++c) {
-            y1[s][c] = fmaf(wr[u], vr[c], y1[s][c]);
-            y2[s][c] = fmaf(wi[u], vi[c], y2[s][c]);
-            y3[s][c] = fmaf(wd[u], vs[c], y3[s][c]);
+          for (int q = 0; q < KR; ++q) {
+            if (KR > 1 && q >= I->n_rhs) continue;
+            float2* out = I->gpart + ((size_t)grp * I->n_out + (KR > 1 ? q : rhs)) * N_pts + oofs[kk];
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+              float x = gr[kk][q][s], y = gi[kk][q][s];
+              for (int o = 1; o < g.NEB; o <<= 1) {
+                x += __shfl_xor(x, o, 64);
+                y += __shfl_xor(y, o, 64);
+              }
+              if (live[kk] && eb == 0 && xk[kk] + Np * s < Nx) out[Np * s] = make_float2(x, y);
+              gr[kk][q][s] = gi[kk][q][s] = 0.f;
+            }
           }
         }
       }
     }
-    if (ey + 1 < Ney) {
-      ++ey;
-      continue;
-    }
-    // Re = P1 + P2, Im = P3 - P1 + P2; gamma_hat += conj(p) Y
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-#pragma unroll
-      for (int c = 0; c < W; ++c) {
-        const float2 pv = Pr[(Np * (S * qb + s)) * W + c];
-        const float yr = y1[s][c] + y2[s][c], yi = y3[s][c] - y1[s][c] + y2[s][c];
-        gr[c % KR][s] += pv.x * yr + pv.y * yi;
-        gi[c % KR][s] += pv.x * yi - pv.y * yr;
-      }
-    }
-    // a right-hand side is complete within the group after its last chunk
-    // there (KR = 1), all of them after the group's last chunk (KR > 1)
-    const int rhs = chunks[I->chunk_base + cix].r;
-    const bool last = cix + G >= n_chunks;
-    const bool flush = last || (KR == 1 && chunks[I->chunk_base + cix + G].r != rhs);
-    if (flush) {
-#pragma unroll
-      for (int q = 0; q < KR; ++q) {
-        if (KR > 1 && q >= I->n_rhs) continue;
-        float2* out = I->gpart + ((size_t)grp * I->n_out + (KR > 1 ? q : rhs)) * N_pts + (size_t)zr * Nx;
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-          float x = gr[q][s], y = gi[q][s];
-          for (int o = 1; o < g.NEB; o <<= 1) {
-            x += __shfl_xor(x, o, 64);
-            y += __shfl_xor(y, o, 64);
-          }
-          const int i = Np * (S * qb + s) + ph;
-          if (live && eb == 0 && i < Nx) out[i] = make_float2(x, y);
-          gr[q][s] = gi[q][s] = 0.f;
-        }
-      }
-    }
-    cix += G;
-    ey = 0;
   }
 }
 
@@ -489,20 +537,20 @@ void launch_fwd_fc(const LaunchSet& L, hipStream_t s) {
 void launch_adj_fc(const LaunchSet& L, const OutPtrs& outs, hipStream_t s) {
   const int Ne_pad = 16 * L.MT;
   size_t lds = 0;
-  for (const Inst& I : L.inst) lds = std::max(lds, fc_adj_lds(I.Nx, I.Np, Ne_pad, L.fc_w, I.n_rows));
+  for (const Inst& I : L.inst) lds = std::max(lds, fc_adj_lds(I.Nx, I.Np, Ne_pad, L.fc_w, I.fc_nr, I.fc_cb));
   const int n_inst = (int)L.inst.size();
   // the partials start at zero: a group writes only the right-hand sides it has chunks of
   for (const Inst& I : L.inst) launch_zero_f2(I.gpart, (size_t)I.fc_g * I.n_out * I.N_pts, s);
   const int kr = L.inst.empty() ? 0 : L.inst[0].col_rhs;
-#define BORN_FC_A(Wv, Ev, Sv, KRv)                                                                      \
-  hipLaunchKernelGGL((k_adj_fc<Wv, Ev, Sv, KRv>), dim3(L.n_wg_f), dim3(256), lds, s, L.d_inst, n_inst, Ne_pad, \
-                     L.d_chunks, L.d_chunk_pairs)
-  if (L.fc_w == 1) BORN_FC_A(1, 16, 8, 1);
-  else if (L.fc_w == 2 && kr == 2) BORN_FC_A(2, 16, 8, 2);
-  else if (L.fc_w == 2) BORN_FC_A(2, 16, 8, 1);
-  else if (kr == 4) BORN_FC_A(4, 16, 4, 4);
-  else if (kr == 2) BORN_FC_A(4, 16, 4, 2);
-  else BORN_FC_A(4, 16, 4, 1);
+#define BORN_FC_A(Wv, Ev, Sv, KRv, KMv)                                                                   \
+  hipLaunchKernelGGL((k_adj_fc<Wv, Ev, Sv, KRv, KMv>), dim3(L.n_wg_f), dim3(256), lds, s, L.d_inst, n_inst, \
+                     Ne_pad, L.d_chunks, L.d_chunk_pairs)
+  if (L.fc_w == 1) BORN_FC_A(1, 8, 8, 1, 1);
+  else if (L.fc_w == 2 && kr == 2) BORN_FC_A(2, 16, 8, 2, 1);
+  else if (L.fc_w == 2) BORN_FC_A(2, 16, 8, 1, 2);
+  else if (kr == 4) BORN_FC_A(4, 16, 4, 4, 1);
+  else if (kr == 2) BORN_FC_A(4, 16, 4, 2, 1);
+  else BORN_FC_A(4, 16, 4, 1, 2);
 #undef BORN_FC_A
   for (int k = 0; k < n_inst; ++k) {
     const Inst& I = L.inst[k];
@@ -512,12 +560,29 @@ void launch_adj_fc(const LaunchSet& L, const OutPtrs& outs, hipStream_t s) {
   }
 }
 
-// Adjoint task blocks (of 256 tasks) of one instance.
-int fewcol_adj_blocks(int Nx, int Np, int n_rows, int Ne_pad, int fc_w) {
+// Adjoint tasks per workgroup and the most rows a workgroup stages (BT = 0: none fits).
+void fewcol_adj_block(int Nx, int Np, int Ne_pad, int fc_w, int col_rhs, int n_rows, int& BT, int& NR) {
+  const FcAdjBlock b = fc_adj_block(Nx, Np, Ne_pad, fc_w, col_rhs, n_rows);
+  BT = b.BT;
+  NR = b.NR;
+}
+// Chunks an adjoint workgroup stages per step: up to 4 (at most per_group),
+// within the LDS that keeps the kernel's workgroups per compute unit (4 for
+// W = 1 with one task per thread, else 2).
+int fewcol_adj_cb(int Nx, int Np, int Ne_pad, int fc_w, int col_rhs, int NR, int per_group) {
   int E, S;
   fc_adj_shape(fc_w, E, S);
   const FcGeom g = fc_geom(Nx, Np, Ne_pad, E, S);
-  return (int)(((long)n_rows * g.tpr + 255) / 256);
+  const size_t budget = fc_w == 1 && fc_km(fc_w, col_rhs) == 1 ? FC_LDS_MAX / 2 : FC_LDS_MAX;
+  int cb = 1;
+  while (cb < 4 && cb < per_group && fc_adj_lds_rows(g, Np, Ne_pad, fc_w, NR, cb + 1) <= budget) ++cb;
+  return cb;
+}
+// Tasks per row of the adjoint.
+int fewcol_adj_tpr(int Nx, int Np, int Ne_pad, int fc_w) {
+  int E, S;
+  fc_adj_shape(fc_w, E, S);
+  return fc_geom(Nx, Np, Ne_pad, E, S).tpr;
 }
 
 // Allow the few-column kernels their dynamic LDS.
@@ -526,12 +591,12 @@ void set_max_lds_fc(int bytes) {
   BORN_SET((k_fwd_fc<1, 16, 16>));
   BORN_SET((k_fwd_fc<2, 8, 16>));
   BORN_SET((k_fwd_fc<4, 8, 8>));
-  BORN_SET((k_adj_fc<1, 16, 8, 1>));
-  BORN_SET((k_adj_fc<2, 16, 8, 1>));
-  BORN_SET((k_adj_fc<2, 16, 8, 2>));
-  BORN_SET((k_adj_fc<4, 16, 4, 1>));
-  BORN_SET((k_adj_fc<4, 16, 4, 2>));
-  BORN_SET((k_adj_fc<4, 16, 4, 4>));
+  BORN_SET((k_adj_fc<1, 8, 8, 1, 1>));
+  BORN_SET((k_adj_fc<2, 16, 8, 1, 2>));
+  BORN_SET((k_adj_fc<2, 16, 8, 2, 1>));
+  BORN_SET((k_adj_fc<4, 16, 4, 1, 2>));
+  BORN_SET((k_adj_fc<4, 16, 4, 2, 1>));
+  BORN_SET((k_adj_fc<4, 16, 4, 4, 1>));
 #undef BORN_SET
 }
 

@@ -51,6 +51,8 @@ struct Inst {
   int adj_segs, adj_segs_s;      // adjoint segments per row (plain, split)
   int wg_begin_f;                // first adjoint workgroup of the instance (few-column kernels)
   int fc_g;                      // few-column adjoint: chunk groups (partial sums summed after)
+  int fc_bt, fc_nr;              // few-column adjoint: tasks per workgroup, most rows it stages
+  int fc_cb;                     // few-column adjoint: chunks staged per step
   int n_out;                     // right-hand sides the adjoint writes
   BORN_GLOBAL float2* gpart;     // few-column adjoint partials [fc_g][n_out][N_pts]
   // adjoint chunk columns: col_rhs = 0: every column of a chunk belongs to
@@ -300,7 +302,9 @@ void launch_fwd_structured(const LaunchSet& L, int n_cu, hipStream_t s);
 // chunks of at most 4 columns on the vector ALUs (born_fewcol.hip)
 int fewcol_width(int max_cols);
 bool fewcol_fits(int Nx, int Np, int Ne_pad, int fc_w, int n_rows);
-int fewcol_adj_blocks(int Nx, int Np, int n_rows, int Ne_pad, int fc_w);
+void fewcol_adj_block(int Nx, int Np, int Ne_pad, int fc_w, int col_rhs, int n_rows, int& BT, int& NR);
+int fewcol_adj_tpr(int Nx, int Np, int Ne_pad, int fc_w);
+int fewcol_adj_cb(int Nx, int Np, int Ne_pad, int fc_w, int col_rhs, int NR, int per_group);
 int fewcol_fwd_rows(int Nx, int Np, int Ne_pad, int fc_w);
 constexpr int FC_GROUPS = 8;    // most chunk groups of the few-column adjoint
 void set_max_lds_fc(int bytes);
