@@ -5,11 +5,14 @@
 // rounded remainder) and each real product taken as
 //     x*y ~ x_hi*y_hi + x_hi*y_lo + x_lo*y_hi
 // on 16x16x32 MFMA with FP32 accumulation.  The dropped x_lo*y_lo term and
-// the roundings leave a relative error of about 2^-16 per product.  A complex
-// multiply-add is four real products (Re = Ar Br - Ai Bi, Im = Ar Bi + Ai Br),
-// i.e. 12 MFMAs of K = 32 with 4 hi/lo vectors per operand: fewer LDS bytes
-// per MFMA than the three-product form, which matters because the operands
-// stream from LDS.
+// the roundings leave a relative error of about 2^-16 per product.
+// The forward takes a complex multiply-add as three real products,
+//     P1 = Ar Br, P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi),
+//     Re = P1 - P2, Im = P3 - P1 - P2,
+// i.e. 9 MFMAs of K = 32 with 6 hi/lo vectors per operand (its time follows
+// the MFMA count).  The adjoint, whose compute waves stream their operands
+// from LDS at a higher rate, keeps the four-product form (12 MFMAs, 4 hi/lo
+// vectors per operand: Re = Ar Br - Ai Bi, Im = Ar Bi + Ai Br).
 #include "born_internal.h"
 
 namespace born {
@@ -29,9 +32,6 @@ __device__ __forceinline__ v4f mfma32(uint4 a, uint4 b, v4f c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(v8h, a), __builtin_bit_cast(v8h, b), c, 0, 0, 0);
   else
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a), __builtin_bit_cast(v8bf, b), c, 0, 0, 0);
-}
-__device__ __forceinline__ uint4 neg8(uint4 v) {
-  return make_uint4(v.x ^ 0x80008000u, v.y ^ 0x80008000u, v.z ^ 0x80008000u, v.w ^ 0x80008000u);
 }
 __device__ __forceinline__ uint4 join(uint2 a, uint2 b) { return make_uint4(a.x, a.y, b.x, b.y); }
 
@@ -94,10 +94,11 @@ __device__ __forceinline__ const void* byte_offset(const void* base, unsigned in
 //
 // LDS (u16 units):
 //   Aw[comp][copy][LAP]  window of the extended row (comp = re_hi, re_lo,
-//                        im_hi, im_lo); copy c is copy 0 shifted by NP*c
-//                        entries (8 / NP copies) so every lane's 8-entry read
-//                        is 16-byte aligned
-//   Bt[comp][c][SEGP]    p_inc*gamma of the segment, points contiguous
+//                        im_hi, im_lo, sum_hi, sum_lo with sum = re + im);
+//                        copy c is copy 0 shifted by NP*c entries (8 / NP
+//                        copies) so every lane's 8-entry read is 16-byte aligned
+//   Bt[comp][c][SEGP]    p_inc*gamma of the segment in the same 6 planes,
+//                        points contiguous
 // Copy length of the forward's window planes: at least LA + 12 entries,
 // 16-byte multiple, and a stride between copies that puts the 16 lanes of
 // one 16-byte read group on distinct banks of gfx950's 64-bank LDS (0 mod 64
@@ -124,8 +125,8 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict
   constexpr int LA = SEG + NP * (16 * MT - 1);
   constexpr int LAP = fwd_split_lap(LA, NP);
   constexpr int APT = 2 * ((LA + 511) / 512);   // window entries per thread: pairs (u, u + 1), u = 2 tid + 512 j
-  unsigned short* Aw = ldsu;                     // [4][NCOPY][LAP]
-  unsigned short* Bt = ldsu + 4 * NCOPY * LAP;   // [4][NC][SEGP]
+  unsigned short* Aw = ldsu;                     // [6][NCOPY][LAP]
+  unsigned short* Bt = ldsu + 6 * NCOPY * LAP;   // [6][NC][SEGP]
   unsigned int* Bt32 = reinterpret_cast<unsigned int*>(Bt);
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -224,11 +225,12 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict
   BORN_SPLIT_ITEM(item)
   BORN_SPLIT_PREFETCH(z)
 
-  v4f CR[MTW], CI[MTW];
+  v4f C1[MTW], C2[MTW], C3[MTW];   // Ar Br, Ai Bi, (Ar + Ai)(Br + Bi)
 #pragma unroll
   for (int m = 0; m < MTW; ++m) {
-    CR[m] = vz();
-    CI[m] = vz();
+    C1[m] = vz();
+    C2[m] = vz();
+    C3[m] = vz();
   }
 
   // lane A position: e = 16*(eg*MTW + m) + col, u = 32*k + 8*kk + NP*(16*MT - 1 - e);
@@ -260,11 +262,12 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict
             h0 = make_float2(h0.x * s_h, h0.y * s_h);
             h1 = make_float2(h1.x * s_h, h1.y * s_h);
           }
-          unsigned int v[4];   // re_hi, re_lo, im_hi, im_lo; entry u in the low half
+          unsigned int v[6];   // re, im, re + im as hi, lo; entry u in the low half
           split_pair<F16>(h0.x, h1.x, v[0], v[1]);
           split_pair<F16>(h0.y, h1.y, v[2], v[3]);
+          split_pair<F16>(h0.x + h0.y, h1.x + h1.y, v[4], v[5]);
 #pragma unroll
-          for (int cp = 0; cp < 4; ++cp) {
+          for (int cp = 0; cp < 6; ++cp) {
 #pragma unroll
             for (int c = 0; c < NCOPY; ++c)
               if (u >= NP * c) Aw32[((cp * NCOPY + c) * LAP + u - NP * c) >> 1] = v[cp];
@@ -290,14 +293,20 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict
           re1 *= s_b;
           im1 *= s_b;
         }
-        unsigned int rh, rl, ih, il;
+        unsigned int rh, rl, ih, il, sh, sl;
         split_pair<F16>(re0, re1, rh, rl);
         split_pair<F16>(im0, im1, ih, il);
+        // FP16: |re|, |im| < 2^15 after scaling, so the sum is halved to stay
+        // in range (P3 doubled when the partial sums are written)
+        const float hs = F16 ? 0.5f : 1.f;
+        split_pair<F16>((re0 + im0) * hs, (re1 + im1) * hs, sh, sl);
         const int o = ((bc0 + q) * SEGP + bi) >> 1;
         Bt32[0 * (NC * SEGP / 2) + o] = rh;
         Bt32[1 * (NC * SEGP / 2) + o] = rl;
         Bt32[2 * (NC * SEGP / 2) + o] = ih;
         Bt32[3 * (NC * SEGP / 2) + o] = il;
+        Bt32[4 * (NC * SEGP / 2) + o] = sh;
+        Bt32[5 * (NC * SEGP / 2) + o] = sl;
       }
     }
     __syncthreads();
@@ -322,51 +331,46 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict
     // the same data as tile 0 at K-block kb - DSTEP*m.  Walk the diagonals
     // d = kb - DSTEP*m, holding the wave's B blocks in registers and
     // streaming one A set per diagonal (double-buffered).
-    uint4 B4[KBW][4], NB[KBW][2];
+    uint4 B6[KBW][6];
 #pragma unroll
     for (int kb = 0; kb < KBW; ++kb) {
       const int k = pg * KBW + kb;
 #pragma unroll
-      for (int cp = 0; cp < 4; ++cp)
-        B4[kb][cp] = *reinterpret_cast<const uint4*>(&Bt[cp * NC * SEGP + bbase + 32 * k]);
-      NB[kb][0] = neg8(B4[kb][2]);
-      NB[kb][1] = neg8(B4[kb][3]);
+      for (int cp = 0; cp < 6; ++cp)
+        B6[kb][cp] = *reinterpret_cast<const uint4*>(&Bt[cp * NC * SEGP + bbase + 32 * k]);
     }
     constexpr int D0 = -DSTEP * (MTW - 1);
-    uint4 Acur[4], Anext[4];
+    uint4 Acur[6], Anext[6];
     {
       const int ao = abase + 32 * (pg * KBW + D0);
 #pragma unroll
-      for (int cp = 0; cp < 4; ++cp) Acur[cp] = *reinterpret_cast<const uint4*>(&Aw[cp * NCOPY * LAP + ao]);
+      for (int cp = 0; cp < 6; ++cp) Acur[cp] = *reinterpret_cast<const uint4*>(&Aw[cp * NCOPY * LAP + ao]);
     }
 #pragma unroll
     for (int d = D0; d < KBW; ++d) {
       if (d + 1 < KBW) {
         const int ao = abase + 32 * (pg * KBW + d + 1);
 #pragma unroll
-        for (int cp = 0; cp < 4; ++cp) Anext[cp] = *reinterpret_cast<const uint4*>(&Aw[cp * NCOPY * LAP + ao]);
+        for (int cp = 0; cp < 6; ++cp) Anext[cp] = *reinterpret_cast<const uint4*>(&Aw[cp * NCOPY * LAP + ao]);
       }
 #pragma unroll
       for (int m = 0; m < MTW; ++m) {
         const int kb = d + DSTEP * m;
         if (kb >= 0 && kb < KBW) {
-          CR[m] = mfma32<F16>(Acur[0], B4[kb][0], CR[m]);
-          CR[m] = mfma32<F16>(Acur[0], B4[kb][1], CR[m]);
-          CR[m] = mfma32<F16>(Acur[1], B4[kb][0], CR[m]);
-          CR[m] = mfma32<F16>(Acur[2], NB[kb][0], CR[m]);
-          CR[m] = mfma32<F16>(Acur[2], NB[kb][1], CR[m]);
-          CR[m] = mfma32<F16>(Acur[3], NB[kb][0], CR[m]);
-          CI[m] = mfma32<F16>(Acur[0], B4[kb][2], CI[m]);
-          CI[m] = mfma32<F16>(Acur[0], B4[kb][3], CI[m]);
-          CI[m] = mfma32<F16>(Acur[1], B4[kb][2], CI[m]);
-          CI[m] = mfma32<F16>(Acur[2], B4[kb][0], CI[m]);
-          CI[m] = mfma32<F16>(Acur[2], B4[kb][1], CI[m]);
-          CI[m] = mfma32<F16>(Acur[3], B4[kb][0], CI[m]);
+          C1[m] = mfma32<F16>(Acur[0], B6[kb][0], C1[m]);
+          C1[m] = mfma32<F16>(Acur[0], B6[kb][1], C1[m]);
+          C1[m] = mfma32<F16>(Acur[1], B6[kb][0], C1[m]);
+          C2[m] = mfma32<F16>(Acur[2], B6[kb][2], C2[m]);
+          C2[m] = mfma32<F16>(Acur[2], B6[kb][3], C2[m]);
+          C2[m] = mfma32<F16>(Acur[3], B6[kb][2], C2[m]);
+          C3[m] = mfma32<F16>(Acur[4], B6[kb][4], C3[m]);
+          C3[m] = mfma32<F16>(Acur[4], B6[kb][5], C3[m]);
+          C3[m] = mfma32<F16>(Acur[5], B6[kb][4], C3[m]);
         }
       }
       if (d + 1 < KBW) {
 #pragma unroll
-        for (int cp = 0; cp < 4; ++cp) Acur[cp] = Anext[cp];
+        for (int cp = 0; cp < 6; ++cp) Acur[cp] = Anext[cp];
       }
     }
 
@@ -377,11 +381,12 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict
       for (int m = 0; m < MTW; ++m) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          red[((w * MTW + m) * 8 + v) * 64 + lane] = CR[m][v];
-          red[((w * MTW + m) * 8 + 4 + v) * 64 + lane] = CI[m][v];
+          red[((w * MTW + m) * 8 + v) * 64 + lane] = C1[m][v] - C2[m][v];
+          red[((w * MTW + m) * 8 + 4 + v) * 64 + lane] = (F16 ? 2.f : 1.f) * C3[m][v] - C1[m][v] - C2[m][v];
         }
-        CR[m] = vz();
-        CI[m] = vz();
+        C1[m] = vz();
+        C2[m] = vz();
+        C3[m] = vz();
       }
       __syncthreads();
       // vpart block of the item: [chunk][zb * nseg + xs] = local item index
@@ -408,7 +413,7 @@ static size_t fwd_split_lds_bytes() {
   constexpr int SEGP = SEG + 8;
   constexpr int LA = SEG + NP * (16 * MT - 1);
   constexpr int LAP = fwd_split_lap(LA, NP);
-  const size_t body = ((size_t)4 * (8 / NP) * LAP + (size_t)4 * NC * SEGP) * sizeof(unsigned short);
+  const size_t body = ((size_t)6 * (8 / NP) * LAP + (size_t)6 * NC * SEGP) * sizeof(unsigned short);
   const size_t red = (size_t)4 * (MT >= 2 ? MT / 2 : MT) * 8 * 64 * sizeof(float);
   return std::max(body, red);
 }
@@ -638,11 +643,12 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict
   // lane's A read position (relative diagonal 0, comp 0): copy d = (-r) mod 4 of phase ph
   const int dcp = (4 - (col & 3)) & 3;
   const int abase = ph * CPB + dcp * LR + copy_offset(dcp) + (SPH - col + 8 * kk - dcp) + PADL - 16 * t0;
-  v4f YR[4], YI[4];
+  v4f YR[4], YI[4], YQ[4];
 #pragma unroll
   for (int t = 0; t < TPW; ++t) {
     YR[t] = vz();
     YI[t] = vz();
+    YQ[t] = vz();
   }
   __syncthreads();
   for (int st = 0; st < n_steps; ++st) {
@@ -659,7 +665,9 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict
         W4[kb][cp] = *reinterpret_cast<const uint4*>(&Wt[(cp * NC + col) * WS + 32 * kb + 8 * kk]);
     // Y of the 4 tiles, summed over the chunk's slices: conj(H) W with conj
     // folded into the table (imag part stored negated):
-    // Re = Ar Wr - Ai' Wi, Im = Ar Wi + Ai' Wr
+    // Re = Ar Wr - Ai' Wi (the Ai' Wi products in YQ, subtracted once per
+    // chunk), Im = Ar Wi + Ai' Wr.  In 2-D each step starts a tile's sums
+    // from zero with its first product.
     constexpr int DMIN = -(TPW - 1), DMAX = 2 * (KB - 1);
     uint4 Ac[4], An[4];
 #define BORN_ADJS_LOAD_A(DST, DL)                                                                \
@@ -671,19 +679,20 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict
 #pragma unroll
     for (int dl = DMIN; dl <= DMAX; ++dl) {
       if (dl < DMAX) BORN_ADJS_LOAD_A(An, dl + 1)
-      const uint4 nih = neg8(Ac[2]), nil = neg8(Ac[3]);
 #pragma unroll
       for (int t = 0; t < TPW; ++t) {
         const int k2 = dl + t;
         if (k2 >= 0 && (k2 & 1) == 0 && k2 / 2 < KB) {
           const int kb = k2 / 2;
-          YR[t] = mfma32<F16>(Ac[0], W4[kb][0], YR[t]);
+          // 2-D: the tile's first product of the step starts from zero
+          const bool z0 = !Y3 && dl == -t;
+          YR[t] = mfma32<F16>(Ac[0], W4[kb][0], z0 ? vz() : YR[t]);
           YR[t] = mfma32<F16>(Ac[0], W4[kb][1], YR[t]);
           YR[t] = mfma32<F16>(Ac[1], W4[kb][0], YR[t]);
-          YR[t] = mfma32<F16>(nih, W4[kb][2], YR[t]);
-          YR[t] = mfma32<F16>(nih, W4[kb][3], YR[t]);
-          YR[t] = mfma32<F16>(nil, W4[kb][2], YR[t]);
-          YI[t] = mfma32<F16>(Ac[0], W4[kb][2], YI[t]);
+          YQ[t] = mfma32<F16>(Ac[2], W4[kb][2], z0 ? vz() : YQ[t]);
+          YQ[t] = mfma32<F16>(Ac[2], W4[kb][3], YQ[t]);
+          YQ[t] = mfma32<F16>(Ac[3], W4[kb][2], YQ[t]);
+          YI[t] = mfma32<F16>(Ac[0], W4[kb][2], z0 ? vz() : YI[t]);
           YI[t] = mfma32<F16>(Ac[0], W4[kb][3], YI[t]);
           YI[t] = mfma32<F16>(Ac[1], W4[kb][2], YI[t]);
           YI[t] = mfma32<F16>(Ac[2], W4[kb][0], YI[t]);
@@ -698,6 +707,11 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict
     }
 #undef BORN_ADJS_LOAD_A
     if (ey == Ney - 1) {
+#pragma unroll
+      for (int t = 0; t < TPW; ++t) {
+        YR[t] -= YQ[t];
+        if (Y3) YQ[t] = vz();
+      }
       if (F16) {
         const float us = I->fscale[4];
 #pragma unroll
@@ -717,8 +731,10 @@ __global__ __launch_bounds__(512, 1) void k_adj_s2d_split(const Inst* __restrict
           gx[t][v] += p.x * YR[t][v] + p.y * YI[t][v];
           gy[t][v] += p.x * YI[t][v] - p.y * YR[t][v];
         }
-        YR[t] = vz();
-        YI[t] = vz();
+        if (Y3) {
+          YR[t] = vz();
+          YI[t] = vz();
+        }
       }
       const int r = chunks[I->chunk_base + c].r;
       const bool has_next = c + 1 < I->n_chunks;
