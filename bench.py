@@ -253,8 +253,11 @@ def roofline(F, kern, precision, config):
     r = {"bound": "mfma", "kernel": f"{dom} contraction ({precision})", "achieved": achieved, "peak": peak,
          "unit": unit, "frac": achieved / peak, "traffic": traffic}
     if precision != "fp32":
-        # hi*hi + hi*lo + lo*hi: three MFMA products per algorithmic product
-        r["mfma_work_frac"] = 3.0 * achieved / peak
+        # hi*hi + hi*lo + lo*hi per real product: 12 MFMAs per complex
+        # (tile, K-block) in the adjoints and the 3-D slice forward (3x the
+        # algorithmic flops), 9 in the 2-D forward's three-product form (2.25x)
+        factor = 2.25 if dom == "forward" and config != "config4" else 3.0
+        r["mfma_work_frac"] = factor * achieved / peak
     return r
 
 

@@ -568,14 +568,11 @@ __global__ __launch_bounds__(512, 1) void k_adj_split_slices(const Inst* __restr
     for (int v = 0; v < 4; ++v) gx[a][v] = gy[a][v] = 0.f;
   const int dcp = (4 - (col & 3)) & 3;
   const int abase = ph * CPB + dcp * LR + copy_offset3(dcp) + (SPH - col + 8 * kk - dcp) + PADL - 16 * t0;
-  // Re = Ar Wr - Ai' Wi with the Ai' Wi products in YQ (subtracted once per
-  // chunk), Im = Ar Wi + Ai' Wr
-  v4f YR[TPW], YI[TPW], YQ[TPW];
+  v4f YR[TPW], YI[TPW];
 #pragma unroll
   for (int t = 0; t < TPW; ++t) {
     YR[t] = vzs();
     YI[t] = vzs();
-    YQ[t] = vzs();
   }
   __syncthreads();
   for (int st = 0; st < n_steps; ++st) {
@@ -597,13 +594,14 @@ __global__ __launch_bounds__(512, 1) void k_adj_split_slices(const Inst* __restr
           const unsigned short* a_ = Ar + cp * NP * CPB + abase + 16 * dl;
           Ac[cp] = join3(*reinterpret_cast<const uint2*>(a_), *reinterpret_cast<const uint2*>(a_ + 4));
         }
+        const uint4 nih = neg8s(Ac[2]), nil = neg8s(Ac[3]);
         const int t = -dl;
         YR[t] = mfma32s<F16>(Ac[0], W4[0], YR[t]);
         YR[t] = mfma32s<F16>(Ac[0], W4[1], YR[t]);
         YR[t] = mfma32s<F16>(Ac[1], W4[0], YR[t]);
-        YQ[t] = mfma32s<F16>(Ac[2], W4[2], YQ[t]);
-        YQ[t] = mfma32s<F16>(Ac[2], W4[3], YQ[t]);
-        YQ[t] = mfma32s<F16>(Ac[3], W4[2], YQ[t]);
+        YR[t] = mfma32s<F16>(nih, W4[2], YR[t]);
+        YR[t] = mfma32s<F16>(nih, W4[3], YR[t]);
+        YR[t] = mfma32s<F16>(nil, W4[2], YR[t]);
         YI[t] = mfma32s<F16>(Ac[0], W4[2], YI[t]);
         YI[t] = mfma32s<F16>(Ac[0], W4[3], YI[t]);
         YI[t] = mfma32s<F16>(Ac[1], W4[2], YI[t]);
@@ -613,11 +611,6 @@ __global__ __launch_bounds__(512, 1) void k_adj_split_slices(const Inst* __restr
       }
     }
     if (sg == NSG - 1) {
-#pragma unroll
-      for (int t = 0; t < TPW; ++t) {
-        YR[t] -= YQ[t];
-        YQ[t] = vzs();
-      }
       if (F16) {
         const float us = I->fscale[4];
 #pragma unroll
