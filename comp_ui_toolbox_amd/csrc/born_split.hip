@@ -296,10 +296,10 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict
         unsigned int rh, rl, ih, il, sh, sl;
         split_pair<F16>(re0, re1, rh, rl);
         split_pair<F16>(im0, im1, ih, il);
-        // FP16: |re|, |im| < 2^15 after scaling, so the sum is halved to stay
-        // in range (P3 doubled when the partial sums are written)
-        const float hs = F16 ? 0.5f : 1.f;
-        split_pair<F16>((re0 + im0) * hs, (re1 + im1) * hs, sh, sl);
+        // FP16 range: with every component of p_inc below 2^ep and of gamma
+        // below 2^ex, the scale 2^(14 - ep - ex) keeps |p_inc gamma| below
+        // 2^15, so |re + im| <= sqrt(2) |p_inc gamma| < 46341 < 65504
+        split_pair<F16>(re0 + im0, re1 + im1, sh, sl);
         const int o = ((bc0 + q) * SEGP + bi) >> 1;
         Bt32[0 * (NC * SEGP / 2) + o] = rh;
         Bt32[1 * (NC * SEGP / 2) + o] = rl;
@@ -382,7 +382,7 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d_split(const Inst* __restrict
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           red[((w * MTW + m) * 8 + v) * 64 + lane] = C1[m][v] - C2[m][v];
-          red[((w * MTW + m) * 8 + 4 + v) * 64 + lane] = (F16 ? 2.f : 1.f) * C3[m][v] - C1[m][v] - C2[m][v];
+          red[((w * MTW + m) * 8 + 4 + v) * 64 + lane] = C3[m][v] - C1[m][v] - C2[m][v];
         }
         C1[m] = vz();
         C2[m] = vz();
