@@ -550,7 +550,7 @@ static void finalize_launchset(LaunchSet& L) {
     const long T = (long)I.n_rows * fewcol_adj_tpr(I.Nx, I.Np, 16 * L.MT, L.fc_w);
     const int nb = (int)((T + I.fc_bt - 1) / I.fc_bt);
     I.fc_g = std::max(1, std::min({FC_GROUPS, (2048 + nb - 1) / nb, I.n_chunks / 2}));
-    I.fc_cb = fewcol_adj_cb(I.Nx, I.Np, 16 * L.MT, L.fc_w, I.col_rhs, I.fc_nr, (I.n_chunks + I.fc_g - 1) / I.fc_g);
+    I.fc_cb = fewcol_adj_cb(I.Nx, I.Np, 16 * L.MT, L.fc_w, I.col_rhs, I.fc_nr, (I.n_chunks + I.fc_g - 1) / I.fc_g, I.Ney);
     wgf += nb * I.fc_g;
   }
   L.n_wg_f = wgf;

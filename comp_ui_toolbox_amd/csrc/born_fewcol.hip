@@ -367,7 +367,8 @@ __global__ __launch_bounds__(256, (KM == 1 && W == 1 ? 4 : 2)) void k_adj_fc(con
       for (int s = 0; s < S; ++s) gr[kk][q][s] = gi[kk][q][s] = 0.f;
 
   const int Ney = I->Ney, n_chunks = I->n_chunks, cb = I->fc_cb;
-  // a step stages cb of the group's chunks (2-D; 3-D: one chunk and slice)
+  // a step stages cb of the group's chunks (2-D; 3-D: one chunk and slice,
+  // cb = 1)
   for (int m = 0; grp + G * m < n_chunks; m += cb) {
     const int nck = min(cb, (n_chunks - grp - G * m + G - 1) / G);
     for (int ey = 0; ey < Ney; ++ey) {
@@ -568,8 +569,12 @@ void fewcol_adj_block(int Nx, int Np, int Ne_pad, int fc_w, int col_rhs, int n_r
 }
 // Chunks an adjoint workgroup stages per step: up to 4 (at most per_group),
 // within the LDS that keeps the kernel's workgroups per compute unit (4 for
-// W = 1 with one task per thread, else 2).
-int fewcol_adj_cb(int Nx, int Np, int Ne_pad, int fc_w, int col_rhs, int NR, int per_group) {
+// W = 1 with one task per thread, else 2).  3-D plans (Ney > 1) take one: a
+// step walks its chunks slice by slice, and the slices of one chunk must all
+// be summed before the next chunk (possibly another right-hand side) adds to
+// the same accumulators.
+int fewcol_adj_cb(int Nx, int Np, int Ne_pad, int fc_w, int col_rhs, int NR, int per_group, int Ney) {
+  if (Ney > 1) return 1;
   int E, S;
   fc_adj_shape(fc_w, E, S);
   const FcGeom g = fc_geom(Nx, Np, Ne_pad, E, S);

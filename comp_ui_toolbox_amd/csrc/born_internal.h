@@ -304,7 +304,7 @@ int fewcol_width(int max_cols);
 bool fewcol_fits(int Nx, int Np, int Ne_pad, int fc_w, int n_rows);
 void fewcol_adj_block(int Nx, int Np, int Ne_pad, int fc_w, int col_rhs, int n_rows, int& BT, int& NR);
 int fewcol_adj_tpr(int Nx, int Np, int Ne_pad, int fc_w);
-int fewcol_adj_cb(int Nx, int Np, int Ne_pad, int fc_w, int col_rhs, int NR, int per_group);
+int fewcol_adj_cb(int Nx, int Np, int Ne_pad, int fc_w, int col_rhs, int NR, int per_group, int Ney);
 int fewcol_fwd_rows(int Nx, int Np, int Ne_pad, int fc_w);
 constexpr int FC_GROUPS = 8;    // most chunk groups of the few-column adjoint
 void set_max_lds_fc(int bytes);

@@ -944,7 +944,7 @@ def test_energy_rx_arbitrary(torch_cuda):
 
 # ------------------------------------------------------ few-column chunks
 @pytest.mark.parametrize("precision,tol", [("fp32", TOL_FP32), ("split_bf16", TOL_SPLIT)])
-@pytest.mark.parametrize("n_waves,n_rhs", [(1, 1), (1, 2), (1, 4), (3, 1), (4, 1), (2, 2), (1, 3)])
+@pytest.mark.parametrize("n_waves,n_rhs", [(1, 1), (1, 2), (1, 4), (3, 1), (4, 1), (2, 2), (1, 3), (1, 20)])
 @pytest.mark.parametrize("dims", ["2d", "3d"])
 def test_few_column_kernels(torch_cuda, dims, n_waves, n_rhs, precision, tol):
     """Chunks of at most 4 columns (1-4 waves per frequency; with several
@@ -952,7 +952,9 @@ def test_few_column_kernels(torch_cuda, dims, n_waves, n_rhs, precision, tol):
     column in both forms, 2-4 in the FP32 form.  Rows whose points per phase
     are not a multiple of the task length, 2-D at pitch/4 and 3-D at pitch/2
     (20 x 2 elements, 130-point rows), host and device entry points, against
-    the oracle."""
+    the oracle.  20 right-hand sides are too many to pack per column: each
+    chunk then holds one rhs, and a chunk group walks chunks of several rhs
+    (in 3-D slice by slice)."""
     import torch
     med = C.MediumSpec()
     if dims == "2d":
