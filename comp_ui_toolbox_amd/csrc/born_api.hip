@@ -1550,14 +1550,17 @@ struct Batch {
   std::vector<uint64_t> gen;     // workspace generation of each plan when the records were made
   // instances whose chunks hold at most 1, 2 or 4 columns run in a launch of
   // their own per width on the few-column kernels (sets 1-3), the others in
-  // one MFMA launch (set 0, ls)
-  static constexpr int N_SETS = 4;
-  LaunchSet ls, ls_fc[N_SETS - 1];
+  // MFMA launches: set 0 (ls), and in the FP32 form set 4 for the instances
+  // whose rows fill 128-point adjoint segments better than 256-point ones
+  // (rows of at most 128 points; 384 points: 3 segments of 128 instead of 2
+  // of 256, one half empty)
+  static constexpr int N_SETS = 5;
+  LaunchSet ls, ls_fc[3], ls_mw2;
   std::vector<int> set_of;         // per plan: its launch set
   std::vector<int> slot;           // per plan: its index in its launch set
   Chunk* d_chunks[N_SETS] = {};
   int* d_chunk_pairs[N_SETS] = {};
-  LaunchSet& set(int k) { return k == 0 ? ls : ls_fc[k - 1]; }
+  LaunchSet& set(int k) { return k == 0 ? ls : (k == 4 ? ls_mw2 : ls_fc[k - 1]); }
   bool graph = false;
   struct Captured {
     hipGraph_t g = nullptr;
@@ -1644,7 +1647,8 @@ static void build_batch_set(Batch& B, int set) {
   L.d_chunk_pairs = d_chunk_pairs;
   L.MT = first->Ne_pad / 16;
   L.Np = first->Np;
-  if (L.adj_mw != 2 && L.adj_mw != 4) L.adj_mw = small ? 2 : 4;
+  if (set == 4) L.adj_mw = 2;
+  else if (L.adj_mw != 2 && L.adj_mw != 4) L.adj_mw = small ? 2 : 4;
   finalize_launchset(L);
 }
 
@@ -1656,6 +1660,7 @@ static void build_batch_launch(Batch& B) {
     const Plan& P = *B.plans[i];
     const int w = fewcol_width(chunk_max_cols(P.chunks));
     if (w > 0 && P.d_gpart && fewcol_fits(P.Nx, P.Np, P.Ne_pad, w, P.n_rows)) B.set_of[i] = w == 1 ? 1 : (w == 2 ? 2 : 3);
+    else if (B.precision == BORN_PRECISION_FP32 && adj_mw_default(P.Nx) == 2) B.set_of[i] = 4;
     B.gen.push_back(P.ws_gen);
   }
   for (int k = 0; k < Batch::N_SETS; ++k) build_batch_set(B, k);

@@ -600,7 +600,13 @@ bool adj_structured_fits(int Nx, int Ne_pad, int Np) {
   return L <= 256 * HPT_MAX && Ne_pad <= 16 * WPT_MAX && adj_lds_bytes(4, Ne_pad, Np) <= 160 * 1024;
 }
 
-int adj_mw_default(int Nx) { return Nx <= 128 ? 2 : 4; }
+// Adjoint segment of 128 points for rows of at most 128 points and for rows
+// that 128-point segments pad less than 256-point ones (384 points: 3 x 128
+// against 2 x 256), else 256.
+int adj_mw_default(int Nx) {
+  const int pad2 = (Nx + 127) / 128 * 128, pad4 = (Nx + 255) / 256 * 256;
+  return Nx <= 128 || pad2 < pad4 ? 2 : 4;
+}
 
 void launch_adj_structured(const LaunchSet& L, const OutPtrs& outs, hipStream_t s) {
   const int MW = L.adj_mw;
