@@ -70,10 +70,15 @@ void fc_adj_shape(int fc_w, int& E, int& S) {
 // tasks per thread: W = 1 one (4 waves per SIMD), W = 2, 4 two; one with
 // right-hand sides packed per column (registers)
 inline int fc_km(int fc_w, int kr) { return kr > 1 || fc_w == 1 ? 1 : 2; }
-// cb staged units (windows, weights, p_inc of R rows), window offsets, row table
+// cb staged units (windows, weights, p_inc of R rows, each padded against
+// bank conflicts as the adjoint kernel lays them out), window offsets, row table
 size_t fc_adj_lds_rows(const FcGeom& g, int Np, int Ne_pad, int fc_w, int R, int cb = 1) {
-  return (size_t)cb * ((size_t)R * (Np * g.QLF + g.XPF * fc_w) + (size_t)Ne_pad * fc_w) * sizeof(float2) +
-         ((size_t)Np * g.QLF + FC_ROWS_MAX) * sizeof(int);
+  int E, S;
+  fc_adj_shape(fc_w, E, S);
+  const size_t HS = (size_t)Np * ((g.QLF + S - 1) / S * S);
+  const size_t row = HS + HS / S + (size_t)g.XPF * fc_w + g.NQB;
+  const size_t unit = (size_t)R * row + (size_t)Ne_pad * fc_w + Ne_pad / E;
+  return (size_t)cb * unit * sizeof(float2) + (HS + FC_ROWS_MAX) * sizeof(int);
 }
 // Adjoint work per workgroup, in tasks (BT) of the instance's list (row,
 // phase, q-block, element block): R whole rows when their tasks fill at
@@ -113,9 +118,12 @@ size_t fc_fwd_lds(int Nx, int Np, int Ne_pad, int fc_w) {
   int E, S;
   fc_fwd_shape(fc_w, E, S);
   const FcGeom g = fc_geom(Nx, Np, Ne_pad, E, S);
-  const size_t rows = (size_t)fc_fwd_rows(g) * (Np * g.QLF + g.XPF * fc_w);
-  // staged rows (then the final reduction), window slot offsets, row table
-  return std::max(rows, (size_t)4 * 128 * fc_w) * sizeof(float2) + ((size_t)Np * g.QLF + 8) * sizeof(int);
+  // padded staged rows as the kernel lays them out (then the final
+  // reduction), window slot offsets, row table
+  const int PW = std::min(S, E);
+  const size_t HS = (size_t)Np * ((g.QLF + PW - 1) / PW * PW);
+  const size_t rows = (size_t)fc_fwd_rows(g) * (HS + HS / PW + (size_t)g.XPF * fc_w + g.NQB);
+  return std::max(rows, (size_t)4 * 128 * fc_w) * sizeof(float2) + (HS + 8) * sizeof(int);
 }
 size_t fc_adj_lds(int Nx, int Np, int Ne_pad, int fc_w, int NR, int cb) {
   int E, S;
@@ -169,7 +177,15 @@ __global__ __launch_bounds__(256, 2) void k_fwd_fc(const Inst* __restrict__ inst
   const int Np = I->Np, Nx = I->Nx, N_pts = I->N_pts;
   const FcGeom g = fc_geom(Nx, Np, Ne_pad, E, S);
   const int R = fc_fwd_rows(g);
-  const int HS = Np * g.QLF, RS = HS + g.XPF * W;   // staged row: Hp, then B[i][c]
+  // Staged row: the phase windows Hp [Np][QLS], then B[i][c], padded against
+  // LDS bank conflicts as in the adjoint: the tasks of a wave read windows at
+  // offsets S qb - E eb (multiples of PW = min(S, E) entries) and B at
+  // (Np S W) qb, so one pad entry follows every PW window entries (phase rows
+  // QLS = QLF rounded up to PW) and every Np S W entries of B.
+  constexpr int PW = S < E ? S : E;
+  const int QLS = (g.QLF + PW - 1) / PW * PW;
+  const int HS = Np * QLS, BS = g.XPF * W;            // logical lengths
+  const int HSP = HS + HS / PW, RS = HSP + BS + g.NQB; // padded row
   const int Qoff = Ne_pad - I->N_e;
   const float2* hf = I->h + (size_t)ch.fg * I->Hp;
   const float2* gmr = I->gm + (size_t)ch.r * N_pts;
@@ -183,9 +199,9 @@ __global__ __launch_bounds__(256, 2) void k_fwd_fc(const Inst* __restrict__ inst
   int* dh = reinterpret_cast<int*>(reinterpret_cast<char*>(lds) + rows_bytes);
   int* hr = dh + HS;
   for (int o = tid; o < HS; o += 256) {
-    const int ph = o / g.QLF, q = o - ph * g.QLF;
+    const int ph = o / QLS, q = o - ph * QLS;
     const int t = (q - Qoff) * Np + ph;
-    dh[o] = t >= 0 && t < I->N_ext ? I->ext[t] : -1;
+    dh[o] = q < g.QLF && t >= 0 && t < I->N_ext ? I->ext[t] : -1;
   }
 
   float a1[E][W], a2[E][W], a3[E][W];
@@ -202,9 +218,9 @@ __global__ __launch_bounds__(256, 2) void k_fwd_fc(const Inst* __restrict__ inst
       float2* dst = lds + r * RS;
       for (int o = tid; o < HS; o += 256) {
         const int e = dh[o];
-        dst[o] = row >= 0 && e >= 0 ? hf[(size_t)row * I->HR + e] : make_float2(0.f, 0.f);
+        dst[o + o / PW] = row >= 0 && e >= 0 ? hf[(size_t)row * I->HR + e] : make_float2(0.f, 0.f);
       }
-      for (int o = tid; o < g.XPF * W; o += 256) {
+      for (int o = tid; o < BS; o += 256) {
         const int i = o / W, c = o - i * W;
         float2 v = make_float2(0.f, 0.f);
         if (row >= 0 && i < Nx) {
@@ -216,19 +232,20 @@ __global__ __launch_bounds__(256, 2) void k_fwd_fc(const Inst* __restrict__ inst
           const float2 a = I->P[(size_t)q * N_pts + p], b = gmr[p];
           v = make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
         }
-        dst[HS + o] = v;
+        dst[HSP + o + o / (Np * S * W)] = v;
       }
     }
     __syncthreads();
     for (int tk = tid; tk < R * g.tpr; tk += 256) {
       const int r = tk / g.tpr, t2 = (tk - r * g.tpr) / g.NEB;
       const int ph = t2 / g.NQB, qb = t2 - ph * g.NQB;
-      const float2* Hr = lds + r * RS + ph * g.QLF + S * qb + Ne_pad - E * eb - E;
-      const float2* Br = lds + r * RS + HS + ph * W;
+      const int hl = ph * QLS + S * qb + Ne_pad - E * eb - E;   // a multiple of PW
+      const float2* Hr = lds + r * RS + hl + hl / PW;
+      const float2* Br = lds + r * RS + HSP + qb * (Np * S * W + 1) + ph * W;
       float wr[S + E - 1], wi[S + E - 1], ws[S + E - 1];
 #pragma unroll
       for (int u = 0; u < S + E - 1; ++u) {
-        const float2 a = Hr[u];
+        const float2 a = Hr[u + u / PW];
         wr[u] = a.x;
         wi[u] = a.y;
         ws[u] = a.x + a.y;
@@ -238,7 +255,7 @@ __global__ __launch_bounds__(256, 2) void k_fwd_fc(const Inst* __restrict__ inst
         float br[W], bi[W], bs[W];
 #pragma unroll
         for (int c = 0; c < W; ++c) {
-          const float2 b = Br[(Np * (S * qb + s)) * W + c];
+          const float2 b = Br[Np * s * W + c];
           br[c] = b.x;
           bi[c] = b.y;
           bs[c] = b.x + b.y;
@@ -325,10 +342,20 @@ __global__ __launch_bounds__(256, (KM == 1 && W == 1 ? 4 : 2)) void k_adj_fc(con
   const int t_first = (wl / G) * BT, t_end = min(T, t_first + BT);
   const int r_first = t_first / g.tpr, NR = (t_end - 1) / g.tpr - r_first + 1;
   const int t_off = t_first - r_first * g.tpr;   // first task's position in its row
-  const int HS = Np * g.QLF, PS = g.XPF * W;
-  const int nH = NR * HS, nW = Ne_pad * W, nP = NR * PS;
-  // staged units, one per chunk of a step: windows [NR][Np][QLF], weights
-  // [Ne_pad][W], p_inc [NR][XPF][W]; then the window offsets and row table
+  // Staged layouts, padded against LDS bank conflicts: the tasks of a wave
+  // read their windows at offsets S qb - E eb (multiples of S entries), their
+  // weights at E W eb and their p_inc at (Np S W) qb, strides that fall on
+  // one or two bank positions.  One pad entry after every S window entries
+  // (phase rows QLS = QLF rounded up to S, so every task's window starts at a
+  // multiple of S), every E W weight entries and every Np S W p_inc entries
+  // spreads them over the banks with fixed read offsets.
+  const int QLS = (g.QLF + S - 1) / S * S;
+  const int HS = Np * QLS, PS = g.XPF * W;             // logical row lengths
+  const int HSP = HS + HS / S, PSP = PS + g.NQB;       // padded
+  const int nH = NR * HSP, nW = Ne_pad * W + Ne_pad / E, nP = NR * PSP;
+  // staged units, one per chunk of a step: windows [NR][Np][QLS], weights
+  // [Ne_pad][W], p_inc [NR][XPF][W] (padded); then the window offsets and
+  // row table
   const int unit = nH + nW + nP;
   int* dh = reinterpret_cast<int*>(lds + I->fc_cb * unit);   // window slot -> table x index (-1: outside)
   int* hr = dh + HS;                           // staged row -> table row (NR <= FC_ROWS_MAX)
@@ -347,15 +374,16 @@ __global__ __launch_bounds__(256, (KM == 1 && W == 1 ? 4 : 2)) void k_adj_fc(con
     const int tkc = (live[kk] ? tk : 0) + t_off;
     const int r = tkc / g.tpr, t2 = (tkc - r * g.tpr) / g.NEB;
     const int ph = t2 / g.NQB, qb = t2 - ph * g.NQB;
-    hoff[kk] = r * HS + ph * g.QLF + S * qb + Ne_pad - E * eb - E;
-    poff[kk] = r * PS + (Np * S * qb + ph) * W;
+    const int hl = r * HS + ph * QLS + S * qb + Ne_pad - E * eb - E;   // a multiple of S
+    hoff[kk] = hl + hl / S;
+    poff[kk] = r * PSP + qb * (Np * S * W + 1) + ph * W;
     xk[kk] = Np * S * qb + ph;                          // x of s = 0
     oofs[kk] = (r_first + r) * Nx + xk[kk];
   }
   for (int o = tid; o < HS; o += 256) {
-    const int p2 = o / g.QLF, q = o - p2 * g.QLF;
+    const int p2 = o / QLS, q = o - p2 * QLS;
     const int t = (q - Qoff) * Np + p2;
-    dh[o] = t >= 0 && t < I->N_ext ? I->ext[t] : -1;
+    dh[o] = q < g.QLF && t >= 0 && t < I->N_ext ? I->ext[t] : -1;
   }
 
   float gr[KM][KR][S], gi[KM][KR][S];
@@ -385,13 +413,13 @@ __global__ __launch_bounds__(256, (KM == 1 && W == 1 ? 4 : 2)) void k_adj_fc(con
           const int row = hr[rr];
           for (int o = tid; o < HS; o += 256) {
             const int e = dh[o];
-            Hs[rr * HS + o] = e >= 0 ? hf[(size_t)row * I->HR + e] : make_float2(0.f, 0.f);
+            Hs[rr * HSP + o + o / S] = e >= 0 ? hf[(size_t)row * I->HR + e] : make_float2(0.f, 0.f);
           }
         }
         const float2* wc = I->Wc + ((size_t)cix * Ney + ey) * Ne_pad * NC;
-        for (int o = tid; o < nW; o += 256) {
+        for (int o = tid; o < Ne_pad * W; o += 256) {
           const int e = o / W, c = o - e * W;
-          Ws[o] = wc[e * NC + c];
+          Ws[o + o / (E * W)] = wc[e * NC + c];
         }
         if (ey == 0) {
           const int* cpr = chunk_pairs + (I->chunk_base + cix) * NC;
@@ -408,7 +436,7 @@ __global__ __launch_bounds__(256, (KM == 1 && W == 1 ? 4 : 2)) void k_adj_fc(con
                 if (c == cc) q = pr[cc];
               float2 v = make_float2(0.f, 0.f);
               if (i < Nx) v = I->P[(size_t)q * N_pts + rowo + i];
-              Ps[rr * PS + o] = v;
+              Ps[rr * PSP + o + o / (Np * S * W)] = v;
             }
           }
         }
@@ -430,11 +458,11 @@ __global__ __launch_bounds__(256, (KM == 1 && W == 1 ? 4 : 2)) void k_adj_fc(con
             for (int c = 0; c < W; ++c) y1[s][c] = y2[s][c] = y3[s][c] = 0.f;
           {
             const float2* Hr = Hs + hoff[kk];
-            const float2* Wr = Ws + E * eb * W;
+            const float2* Wr = Ws + (E * W + 1) * eb;
             float wr[S + E - 1], wi[S + E - 1], wd[S + E - 1];
 #pragma unroll
             for (int u = 0; u < S + E - 1; ++u) {
-              const float2 a = Hr[u];
+              const float2 a = Hr[u + u / S];
               wr[u] = a.x;
               wi[u] = a.y;
               wd[u] = a.x - a.y;
