@@ -55,8 +55,10 @@ __host__ __device__ inline int fc_fwd_rows(const FcGeom& g) {
   return R;
 }
 // Task shapes: forward W = 1: E 16 x S 16, 2: 8 x 16, 4: 8 x 8 (sums of E x
-// W); adjoint W = 1: E 8 x S 8, 2: 16 x 8, 4: 16 x 4 with at most fc_km
-// tasks per thread (sums of S x W per task).
+// W); adjoint W = 1: E 16 x S 8, 2: 16 x 8, 4: 16 x 4 with at most fc_km
+// tasks per thread (sums of S x W per task).  (W = 1 keeps four workgroups
+// per compute unit at the cost of a 16-byte register spill: measured faster
+// than E 8, and than three workgroups without the spill.)
 constexpr int FC_LDS_MAX = 78 * 1024;   // two workgroups per compute unit
 constexpr int FC_ROWS_MAX = 16;         // rows an adjoint workgroup stages (its row table in LDS)
 void fc_fwd_shape(int fc_w, int& E, int& S) {
@@ -64,7 +66,7 @@ void fc_fwd_shape(int fc_w, int& E, int& S) {
   S = fc_w == 4 ? 8 : 16;
 }
 void fc_adj_shape(int fc_w, int& E, int& S) {
-  E = fc_w == 1 ? 8 : 16;
+  E = 16;
   S = fc_w == 4 ? 4 : 8;
 }
 // tasks per thread: W = 1 one (4 waves per SIMD), W = 2, 4 two; one with
@@ -574,7 +576,7 @@ void launch_adj_fc(const LaunchSet& L, const OutPtrs& outs, hipStream_t s) {
 #define BORN_FC_A(Wv, Ev, Sv, KRv, KMv)                                                                   \
   hipLaunchKernelGGL((k_adj_fc<Wv, Ev, Sv, KRv, KMv>), dim3(L.n_wg_f), dim3(256), lds, s, L.d_inst, n_inst, \
                      Ne_pad, L.d_chunks, L.d_chunk_pairs)
-  if (L.fc_w == 1) BORN_FC_A(1, 8, 8, 1, 1);
+  if (L.fc_w == 1) BORN_FC_A(1, 16, 8, 1, 1);
   else if (L.fc_w == 2 && kr == 2) BORN_FC_A(2, 16, 8, 2, 1);
   else if (L.fc_w == 2) BORN_FC_A(2, 16, 8, 1, 2);
   else if (kr == 4) BORN_FC_A(4, 16, 4, 4, 1);
@@ -624,7 +626,7 @@ void set_max_lds_fc(int bytes) {
   BORN_SET((k_fwd_fc<1, 16, 16>));
   BORN_SET((k_fwd_fc<2, 8, 16>));
   BORN_SET((k_fwd_fc<4, 8, 8>));
-  BORN_SET((k_adj_fc<1, 8, 8, 1, 1>));
+  BORN_SET((k_adj_fc<1, 16, 8, 1, 1>));
   BORN_SET((k_adj_fc<2, 16, 8, 1, 2>));
   BORN_SET((k_adj_fc<2, 16, 8, 2, 1>));
   BORN_SET((k_adj_fc<4, 16, 4, 1, 2>));
