@@ -10,9 +10,10 @@
 //     P1 = Ar Br, P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi),
 //     Re = P1 - P2, Im = P3 - P1 - P2,
 // i.e. 9 MFMAs of K = 32 with 6 hi/lo vectors per operand (its time follows
-// the MFMA count).  The adjoint, whose compute waves stream their operands
-// from LDS at a higher rate, keeps the four-product form (12 MFMAs, 4 hi/lo
-// vectors per operand: Re = Ar Br - Ai Bi, Im = Ar Bi + Ai Br).
+// the MFMA count).  The adjoint keeps the four-product form (12 MFMAs, 4 hi/lo
+// vectors per operand: Re = Ar Br - Ai Bi, Im = Ar Bi + Ai Br): its
+// double-buffered operand planes and p_inc rows already fill the LDS, with
+// no room for six planes.
 #include "born_internal.h"
 
 namespace born {
