@@ -54,15 +54,15 @@ __host__ __device__ inline int fc_fwd_rows(const FcGeom& g) {
   while (R < 8 && R * g.tpr < 256) R <<= 1;
   return R;
 }
-// Task shapes: forward W = 1: E 16 x S 16, 2: 8 x 16, 4: 8 x 8 (sums of E x
-// W); adjoint W = 1: E 16 x S 8, 2: 16 x 8, 4: 16 x 4 with at most fc_km
+// Task shapes: forward W = 1, 2: E 8 x S 16, 4: 8 x 8 (sums of E x W);
+// adjoint W = 1: E 16 x S 8, 2: 16 x 8, 4: 16 x 4 with at most fc_km
 // tasks per thread (sums of S x W per task).  (W = 1 keeps four workgroups
 // per compute unit at the cost of a 16-byte register spill: measured faster
 // than E 8, and than three workgroups without the spill.)
 constexpr int FC_LDS_MAX = 78 * 1024;   // two workgroups per compute unit
 constexpr int FC_ROWS_MAX = 16;         // rows an adjoint workgroup stages (its row table in LDS)
 void fc_fwd_shape(int fc_w, int& E, int& S) {
-  E = fc_w == 1 ? 16 : 8;
+  E = 8;
   S = fc_w == 4 ? 8 : 16;
 }
 void fc_adj_shape(int fc_w, int& E, int& S) {
@@ -559,7 +559,7 @@ void launch_fwd_fc(const LaunchSet& L, hipStream_t s) {
 #define BORN_FC_F(Wv, Ev, Sv)                                                                            \
   hipLaunchKernelGGL((k_fwd_fc<Wv, Ev, Sv>), grid, dim3(256), lds, s, L.d_inst, n_inst, Ne_pad, L.d_chunks, \
                      L.d_chunk_pairs)
-  if (L.fc_w == 1) BORN_FC_F(1, 16, 16);
+  if (L.fc_w == 1) BORN_FC_F(1, 8, 16);
   else if (L.fc_w == 2) BORN_FC_F(2, 8, 16);
   else BORN_FC_F(4, 8, 8);
 #undef BORN_FC_F
@@ -623,7 +623,7 @@ int fewcol_adj_tpr(int Nx, int Np, int Ne_pad, int fc_w) {
 // Allow the few-column kernels their dynamic LDS.
 void set_max_lds_fc(int bytes) {
 #define BORN_SET(K) (void)hipFuncSetAttribute((const void*)(K), hipFuncAttributeMaxDynamicSharedMemorySize, bytes)
-  BORN_SET((k_fwd_fc<1, 16, 16>));
+  BORN_SET((k_fwd_fc<1, 8, 16>));
   BORN_SET((k_fwd_fc<2, 8, 16>));
   BORN_SET((k_fwd_fc<4, 8, 8>));
   BORN_SET((k_adj_fc<1, 16, 8, 1, 1>));
