@@ -2478,7 +2478,17 @@ int born_batch_autotune(born_batch_t batch, void* stream, double* fwd_ms, double
     hipStream_t s = (hipStream_t)stream;
     batch_refresh(B);
     HIP_CHECK(hipDeviceSynchronize());
-    const TuneResult r = autotune(B.plans, B.ls, B.precision, B.n_cu, [&] { build_batch_launch(B); }, s);
+    TuneResult r = autotune(B.plans, B.ls, B.precision, B.n_cu, [&] { build_batch_launch(B); }, s);
+    // the MFMA set with 128-point adjoint segments gets its own work-item
+    // count and grid (its plans only; its adjoint width stays 128)
+    std::vector<Plan*> p4;
+    for (size_t i = 0; i < B.plans.size(); ++i)
+      if (B.set_of[i] == 4) p4.push_back(B.plans[i]);
+    if (!p4.empty()) {
+      const TuneResult r4 = autotune(p4, B.ls_mw2, B.precision, B.n_cu, [&] { build_batch_launch(B); }, s);
+      r.fwd_ms += r4.fwd_ms;
+      r.adj_ms += r4.adj_ms;
+    }
     if (fwd_ms) *fwd_ms = r.fwd_ms;
     if (adj_ms) *adj_ms = r.adj_ms;
   });

@@ -321,7 +321,8 @@ int born_batch_adjoint_dev(born_batch_t batch, const void* const* u_c64, void* c
 int born_batch_set_graph(born_batch_t batch, int32_t on);
 /* Autotuner: times the candidate work-item counts and persistent grid sizes
  * of the forward contraction and the segment widths of the adjoint on the
- * batch's own instances and keeps the fastest; returns their times (ms). */
+ * batch's own instances and keeps the fastest, for each MFMA launch set of
+ * the batch; returns their times (ms, summed over the sets). */
 int born_batch_autotune(born_batch_t batch, void* stream, double* fwd_ms, double* adj_ms);
 int born_plan_autotune(born_plan_t plan, void* stream, double* fwd_ms, double* adj_ms);
 /* Event timing of the batched contraction launches (as born_plan_set_timing). */
