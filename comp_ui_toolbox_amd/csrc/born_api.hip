@@ -518,6 +518,8 @@ static void finalize_launchset(LaunchSet& L) {
   int max_cols = 0;
   for (const Inst& I : L.inst) max_cols = std::max(max_cols, I.max_cols);
   L.fc_w = L.inst.empty() || !L.fc_allowed ? 0 : fewcol_width(max_cols);
+  // the plain MFMA kernels take the 8-column stacked form when every chunk fits
+  L.stk = max_cols > 0 && max_cols <= NC / 2;
   for (const Inst& I : L.inst)
     if (!I.gpart || !fewcol_fits(I.Nx, I.Np, 16 * L.MT, L.fc_w, I.n_rows)) L.fc_w = 0;
   for (const Inst& I : L.inst) {
@@ -1550,17 +1552,19 @@ struct Batch {
   std::vector<uint64_t> gen;     // workspace generation of each plan when the records were made
   // instances whose chunks hold at most 1, 2 or 4 columns run in a launch of
   // their own per width on the few-column kernels (sets 1-3), the others in
-  // MFMA launches: set 0 (ls), and in the FP32 form set 4 for the instances
-  // whose rows fill 128-point adjoint segments better than 256-point ones
-  // (rows of at most 128 points; 384 points: 3 segments of 128 instead of 2
-  // of 256, one half empty)
-  static constexpr int N_SETS = 5;
-  LaunchSet ls, ls_fc[3], ls_mw2;
+  // MFMA launches: set 0, and in the FP32 form set 4 for the instances whose
+  // rows fill 128-point adjoint segments better than 256-point ones (rows of
+  // at most 128 points; 384 points: 3 segments of 128 instead of 2 of 256,
+  // one half empty); FP32 instances of at most 8 columns per chunk take the
+  // stacked kernels in sets 5 (like 0) and 6 (like 4)
+  static constexpr int N_SETS = 7;
+  LaunchSet sets[N_SETS];
   std::vector<int> set_of;         // per plan: its launch set
   std::vector<int> slot;           // per plan: its index in its launch set
   Chunk* d_chunks[N_SETS] = {};
   int* d_chunk_pairs[N_SETS] = {};
-  LaunchSet& set(int k) { return k == 0 ? ls : (k == 4 ? ls_mw2 : ls_fc[k - 1]); }
+  LaunchSet& set(int k) { return sets[k]; }
+  static bool mfma_set(int k) { return k == 0 || k >= 4; }
   bool graph = false;
   struct Captured {
     hipGraph_t g = nullptr;
@@ -1647,7 +1651,7 @@ static void build_batch_set(Batch& B, int set) {
   L.d_chunk_pairs = d_chunk_pairs;
   L.MT = first->Ne_pad / 16;
   L.Np = first->Np;
-  if (set == 4) L.adj_mw = 2;
+  if (set == 4 || set == 6) L.adj_mw = 2;
   else if (L.adj_mw != 2 && L.adj_mw != 4) L.adj_mw = small ? 2 : 4;
   finalize_launchset(L);
 }
@@ -1660,7 +1664,10 @@ static void build_batch_launch(Batch& B) {
     const Plan& P = *B.plans[i];
     const int w = fewcol_width(chunk_max_cols(P.chunks));
     if (w > 0 && P.d_gpart && fewcol_fits(P.Nx, P.Np, P.Ne_pad, w, P.n_rows)) B.set_of[i] = w == 1 ? 1 : (w == 2 ? 2 : 3);
-    else if (B.precision == BORN_PRECISION_FP32 && adj_mw_default(P.Nx) == 2) B.set_of[i] = 4;
+    else if (B.precision == BORN_PRECISION_FP32) {
+      const bool mw2 = adj_mw_default(P.Nx) == 2, stk = chunk_max_cols(P.chunks) <= NC / 2;
+      B.set_of[i] = stk ? (mw2 ? 6 : 5) : (mw2 ? 4 : 0);
+    }
     B.gen.push_back(P.ws_gen);
   }
   for (int k = 0; k < Batch::N_SETS; ++k) build_batch_set(B, k);
@@ -2478,16 +2485,18 @@ int born_batch_autotune(born_batch_t batch, void* stream, double* fwd_ms, double
     hipStream_t s = (hipStream_t)stream;
     batch_refresh(B);
     HIP_CHECK(hipDeviceSynchronize());
-    TuneResult r = autotune(B.plans, B.ls, B.precision, B.n_cu, [&] { build_batch_launch(B); }, s);
-    // the MFMA set with 128-point adjoint segments gets its own work-item
-    // count and grid (its plans only; its adjoint width stays 128)
-    std::vector<Plan*> p4;
-    for (size_t i = 0; i < B.plans.size(); ++i)
-      if (B.set_of[i] == 4) p4.push_back(B.plans[i]);
-    if (!p4.empty()) {
-      const TuneResult r4 = autotune(p4, B.ls_mw2, B.precision, B.n_cu, [&] { build_batch_launch(B); }, s);
-      r.fwd_ms += r4.fwd_ms;
-      r.adj_ms += r4.adj_ms;
+    // each MFMA launch set over its own plans: work items and grid (and the
+    // adjoint width where the set does not fix it at 128 points)
+    TuneResult r{0, 0, 0, 0.f, 0.f};
+    for (int k = 0; k < Batch::N_SETS; ++k) {
+      if (!Batch::mfma_set(k)) continue;
+      std::vector<Plan*> pk;
+      for (size_t i = 0; i < B.plans.size(); ++i)
+        if (B.set_of[i] == k) pk.push_back(B.plans[i]);
+      if (pk.empty()) continue;
+      const TuneResult rk = autotune(pk, B.set(k), B.precision, B.n_cu, [&] { build_batch_launch(B); }, s);
+      r.fwd_ms += rk.fwd_ms;
+      r.adj_ms += rk.adj_ms;
     }
     if (fwd_ms) *fwd_ms = r.fwd_ms;
     if (adj_ms) *adj_ms = r.adj_ms;

@@ -106,6 +106,7 @@ struct LaunchSet {
   int MT = 1, Np = 4;
   int S_max = 1;                  // largest S (plain forward LDS and registers)
   int adj_mw = 4;                 // plain adjoint segment = 64 * adj_mw points
+  bool stk = false;               // plain kernels: chunks of at most 8 columns (stacked B operand)
   int n_items = 0, n_items_s = 0;
   int n_wg = 0, n_wg_s = 0;
   int grid_mult = 2;              // persistent forward workgroups per CU

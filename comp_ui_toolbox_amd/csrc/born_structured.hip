@@ -52,7 +52,14 @@ __device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
 // B tiles side by side (points [0, XP) and [XP, 2 XP)) and their two
 // extended h rows one after the other, so the point groups of the waves
 // split at the row boundary and each barrier pair serves twice the MFMAs.
-template <int MT, int NP, int PPT, bool DUAL>
+//
+// STK (chunks of at most 8 columns): the B operand holds the real parts of
+// the 8 columns in lanes 0-7 of each K row and their imaginary parts in lanes
+// 8-15, so two MFMAs per tile and K-step, A_re [B_re | B_im] and
+// A_im [B_re | B_im], give all four real products of 8 columns (three MFMAs
+// over 16 columns otherwise); the item's sums are combined across lanes
+// c and c + 8 when it is written.  Only the 8 columns are loaded and filled.
+template <int MT, int NP, int PPT, bool DUAL, bool STK>
 __global__ __launch_bounds__(256, 2) void k_fwd_s2d(const Inst* __restrict__ inst, int n_inst, int n_items,
                                                     int Ne_pad, const Chunk* __restrict__ chunks,
                                                     const int* __restrict__ chunk_pairs) {
@@ -62,6 +69,7 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(const Inst* __restrict__ ins
   constexpr int EW = MT >= 2 ? 2 : 1;        // element groups
   constexpr int PW = 4 / EW;                 // point groups
   constexpr int MTW = MT / EW;               // element tiles per wave
+  constexpr int NCF = STK ? NC / 2 : NC;     // columns loaded and filled
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int eg = w % EW, pg = w / EW;
   const int kk = lane >> 4, col = lane & 15;
@@ -117,7 +125,7 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(const Inst* __restrict__ ins
 
   float2 hreg[HPT];
   float2 hreg2[DUAL ? HPT : 1];   // the step's second h row (DUAL)
-  float2 praw[PPT][NC];
+  float2 praw[PPT][NCF];
   float2 greg[PPT];
   int gvalid = 0;                 // bit j: B point tid + 256 j inside its row (DUAL)
 
@@ -141,7 +149,7 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(const Inst* __restrict__ ins
         if (t_ < 2 * XP1_ && x_ < I->Nx && r_ <= two_) gvalid |= 1 << j;                \
         const size_t o_ = (size_t)((zz) + (r_ & two_)) * I->Nx + min(x_, I->Nx - 1);    \
         greg[j] = I->gm[(size_t)rr * I->N_pts + o_];                                    \
-        _Pragma("unroll") for (int c_ = 0; c_ < NC; ++c_)                               \
+        _Pragma("unroll") for (int c_ = 0; c_ < NCF; ++c_)                              \
           praw[j][c_] = I->P[(size_t)pr[c_] * I->N_pts + o_];                           \
       }                                                                                 \
     } else {                                                                            \
@@ -150,7 +158,7 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(const Inst* __restrict__ ins
       _Pragma("unroll") for (int j = 0; j < PPT; ++j) {                                 \
         const int i_ = min(tid + 256 * j, I->Nx - 1);                                   \
         greg[j] = grow_[i_];                                                            \
-        _Pragma("unroll") for (int c_ = 0; c_ < NC; ++c_)                               \
+        _Pragma("unroll") for (int c_ = 0; c_ < NCF; ++c_)                              \
           praw[j][c_] = I->P[(size_t)pr[c_] * I->N_pts + rowo_ + i_];                   \
       }                                                                                 \
     }                                                                                   \
@@ -167,7 +175,7 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(const Inst* __restrict__ ins
     SR[mt] = vzero();
     SI[mt] = vzero();
   }
-  const int b_base = kk * NCS + col;
+  const int b_base = kk * NCS + (STK ? (col & 7) : col);
   int it = 0;
 
   while (true) {
@@ -193,7 +201,7 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(const Inst* __restrict__ ins
           const bool ok = DUAL ? ((gvalid >> j) & 1) != 0 : (xfull || i < n_Nx);
           const float2 g = ok ? greg[j] : make_float2(0.f, 0.f);
 #pragma unroll
-          for (int c = 0; c < NC; ++c) Bs[i * NCS + c] = cmulf(praw[j][c], g);
+          for (int c = 0; c < NCF; ++c) Bs[i * NCS + c] = cmulf(praw[j][c], g);
         }
       }
     }
@@ -233,17 +241,18 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(const Inst* __restrict__ ins
       a_base = 0;
       a_step = 0;
     }
-    v4f P1[MTW], P2[MTW], P3[MTW];
+    v4f P1[MTW], P2[MTW], P3[STK ? 1 : MTW];
 #pragma unroll
     for (int mt = 0; mt < MTW; ++mt) {
       P1[mt] = vzero();
       P2[mt] = vzero();
-      P3[mt] = vzero();
+      if constexpr (!STK) P3[mt] = vzero();
     }
     for (int s = 0; s < SW; ++s) {
       const int ks = pg * SW + s;
       const float2 b = Bs[b_base + 4 * ks * NCS];
       const float bsum = b.x + b.y;
+      const float bst = col < 8 ? b.x : b.y;   // STK: [B_re | B_im]
       int ai;
       if (NP == 4 || NP == 2 || NP == 1) {
         ai = a_base + a_step * ks;
@@ -254,17 +263,28 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(const Inst* __restrict__ ins
 #pragma unroll
       for (int mt = 0; mt < MTW; ++mt) {
         const float2 a = Hp[ai - 16 * mt];
-        const float asum = a.x + a.y;
-        P1[mt] = mfma16(a.x, b.x, P1[mt]);
-        P2[mt] = mfma16(a.y, b.y, P2[mt]);
-        P3[mt] = mfma16(asum, bsum, P3[mt]);
+        if constexpr (STK) {
+          P1[mt] = mfma16(a.x, bst, P1[mt]);
+          P2[mt] = mfma16(a.y, bst, P2[mt]);
+        } else {
+          const float asum = a.x + a.y;
+          P1[mt] = mfma16(a.x, b.x, P1[mt]);
+          P2[mt] = mfma16(a.y, b.y, P2[mt]);
+          P3[mt] = mfma16(asum, bsum, P3[mt]);
+        }
       }
     }
     // row done: Re = P1 - P2, Im = P3 - P1 - P2, folded into the item sums
+    // (STK: the raw products, combined when the item is written)
 #pragma unroll
     for (int mt = 0; mt < MTW; ++mt) {
-      SR[mt] += P1[mt] - P2[mt];
-      SI[mt] += P3[mt] - P1[mt] - P2[mt];
+      if constexpr (STK) {
+        SR[mt] += P1[mt];
+        SI[mt] += P2[mt];
+      } else {
+        SR[mt] += P1[mt] - P2[mt];
+        SI[mt] += P3[mt] - P1[mt] - P2[mt];
+      }
     }
 
     if (item_done) {
@@ -275,8 +295,19 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(const Inst* __restrict__ ins
       for (int mt = 0; mt < MTW; ++mt) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          red[((w * MTW + mt) * 8 + v) * 64 + lane] = SR[mt][v];
-          red[((w * MTW + mt) * 8 + 4 + v) * 64 + lane] = SI[mt][v];
+          float re = SR[mt][v], im = SI[mt][v];
+          if constexpr (STK) {
+            // lane c < 8: A_re B_re, A_im B_re of column c; lane c + 8:
+            // A_re B_im, A_im B_im.  Re = A_re B_re - A_im B_im,
+            // Im = A_re B_im + A_im B_re; lanes 8-15 write zero columns.
+            const float re_p = __shfl_xor(re, 8, 64), im_p = __shfl_xor(im, 8, 64);
+            const bool lo = col < 8;
+            const float r2 = lo ? re - im_p : 0.f, i2 = lo ? re_p + im : 0.f;
+            re = r2;
+            im = i2;
+          }
+          red[((w * MTW + mt) * 8 + v) * 64 + lane] = re;
+          red[((w * MTW + mt) * 8 + 4 + v) * 64 + lane] = im;
         }
         SR[mt] = vzero();
         SI[mt] = vzero();
@@ -336,9 +367,14 @@ void launch_fwd_structured(const LaunchSet& L, int n_cu, hipStream_t s) {
   const size_t lds = fwd_lds_bytes(L.MT, L.S_max, L.Np, dual);
   const int grid = std::max(1, std::min(L.n_items, L.grid_mult * n_cu));
   const int Ne_pad = 16 * L.MT;
-#define BORN_FWD_LAUNCH(MTv, NPv, PPTv, Dv) \
-  hipLaunchKernelGGL((k_fwd_s2d<MTv, NPv, PPTv, Dv>), dim3(grid), dim3(256), lds, s, L.d_inst, (int)L.inst.size(), \
+#define BORN_FWD_LAUNCH2(MTv, NPv, PPTv, Dv, Sv)                                                              \
+  hipLaunchKernelGGL((k_fwd_s2d<MTv, NPv, PPTv, Dv, Sv>), dim3(grid), dim3(256), lds, s, L.d_inst, (int)L.inst.size(), \
                      L.n_items, Ne_pad, L.d_chunks, L.d_chunk_pairs)
+#define BORN_FWD_LAUNCH(MTv, NPv, PPTv, Dv)                                \
+  do {                                                                     \
+    if (NPv != 0 && L.stk) BORN_FWD_LAUNCH2(MTv, NPv, PPTv, Dv, (NPv != 0)); \
+    else BORN_FWD_LAUNCH2(MTv, NPv, PPTv, Dv, false);                      \
+  } while (0)
 #define BORN_FWD_PPT(MTv, NPv)                           \
   if (dual) BORN_FWD_LAUNCH(MTv, NPv, 1, true);          \
   else if (PPT <= 1) BORN_FWD_LAUNCH(MTv, NPv, 1, false); \
@@ -361,6 +397,7 @@ void launch_fwd_structured(const LaunchSet& L, int n_cu, hipStream_t s) {
 #undef BORN_FWD_NP
 #undef BORN_FWD_PPT
 #undef BORN_FWD_LAUNCH
+#undef BORN_FWD_LAUNCH2
 }
 
 // ------------------------------------------------------------------ adjoint
@@ -370,14 +407,21 @@ void launch_fwd_structured(const LaunchSet& L, int n_cu, hipStream_t s) {
 // output in registers: no atomics, bitwise repeatable.  Per chunk
 //   Y[i][c] = sum_e conj(Hext[i - e*Np + off]) W[e][c]       (MFMA, A = points x elements)
 //   gamma_hat[i] += sum_c conj(p_inc[pair_c][i]) Y[i][c]     (registers)
-template <int MT, int MW, int NP>
+// STK (chunks of at most 8 columns): the B operand holds W_re of the 8
+// columns in lanes 0-7 and W_im in lanes 8-15, two MFMAs per tile and
+// K-step (A_re B, A_im B) instead of three over 16 columns; lane c < 8 then
+// holds A_re W_re, A_im W_re of column c and lane c + 8 A_re W_im, A_im W_im,
+// and each lane's share of conj(p) Y needs only its own products (the sum
+// over the column lanes adds the two halves).  Only 8 p_inc rows are loaded.
+template <int MT, int MW, int NP, bool STK>
 __global__ __launch_bounds__(256, 2) void k_adj_s2d(const Inst* __restrict__ inst, int n_inst, OutPtrs outs,
                                                     const Chunk* __restrict__ chunks,
                                                     const int* __restrict__ chunk_pairs) {
   extern __shared__ float2 lds[];
   constexpr int SEG = 64 * MW;
   constexpr int PS = SEG + 4;                 // padded stride of the p_inc tile
-  constexpr int PPT = NC * SEG / 256;         // p_inc tile entries per thread
+  constexpr int NCP = STK ? NC / 2 : NC;      // p_inc rows loaded
+  constexpr int PPT = NCP * SEG / 256;        // p_inc tile entries per thread
   constexpr int NEP = 16 * MT;                // padded element count
   constexpr int HPT = NP ? (SEG + (NEP - 1) * NP + 255) / 256 : HPT_MAX;
   constexpr int WPT = MT;                     // W tile entries per thread
@@ -394,6 +438,7 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d(const Inst* __restrict__ ins
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int kk = lane >> 4, col = lane & 15;
+  const int ccol = STK ? (col & 7) : col;     // column whose W and p_inc the lane reads
   const int lw = (int)blockIdx.x - I->wg_begin;
   const int z = lw / I->adj_segs;
   const int i_s = (lw - z * I->adj_segs) * SEG;
@@ -459,7 +504,7 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d(const Inst* __restrict__ ins
     for (int v = 0; v < 4; ++v) gx[mw][v] = gy[mw][v] = 0.f;
 
   const int u_base = ib - i_s + col + (NEP - 1) * Np;
-  v4f P1[MW], P2[MW], P3[MW];
+  v4f P1[MW], P2[MW], P3[STK ? 1 : MW];
   const int n_steps = I->n_chunks * Ney;
 
   for (int st = 0; st < n_steps; ++st) {
@@ -501,33 +546,48 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d(const Inst* __restrict__ ins
       for (int mw = 0; mw < MW; ++mw) {
         P1[mw] = vzero();
         P2[mw] = vzero();
-        P3[mw] = vzero();
+        if constexpr (!STK) P3[mw] = vzero();
       }
     }
 #pragma unroll 4
     for (int k0 = 0; k0 < NEP; k0 += 4) {
       const int k = k0 + kk;
-      const float2 b = Ws[k * NCS + col];
+      const float2 b = Ws[k * NCS + ccol];
       const float bsum = b.x + b.y;
+      const float bst = col < 8 ? b.x : b.y;   // STK: [W_re | W_im]
       const int u0 = u_base - k * Np;
 #pragma unroll
       for (int mw = 0; mw < MW; ++mw) {
         const float2 a = He[u0 + 16 * mw];
-        const float adiff = a.x - a.y;
-        P1[mw] = mfma16(a.x, b.x, P1[mw]);
-        P2[mw] = mfma16(a.y, b.y, P2[mw]);
-        P3[mw] = mfma16(adiff, bsum, P3[mw]);
+        if constexpr (STK) {
+          P1[mw] = mfma16(a.x, bst, P1[mw]);
+          P2[mw] = mfma16(a.y, bst, P2[mw]);
+        } else {
+          const float adiff = a.x - a.y;
+          P1[mw] = mfma16(a.x, b.x, P1[mw]);
+          P2[mw] = mfma16(a.y, b.y, P2[mw]);
+          P3[mw] = mfma16(adiff, bsum, P3[mw]);
+        }
       }
     }
     if (!last_slice) continue;
-    // conj(A) W: Re = P1 + P2, Im = P3 - P1 + P2; then gamma_hat += conj(p) * Y
+    // conj(A) W: Re = P1 + P2, Im = P3 - P1 + P2; then gamma_hat += conj(p) * Y.
+    // STK: Re Y_c = P1[c] + P2[c + 8], Im Y_c = P1[c + 8] - P2[c], so lane c
+    // contributes as Y = (P1, -P2) and lane c + 8 as Y = (P2, P1).
 #pragma unroll
     for (int mw = 0; mw < MW; ++mw) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const float yr = P1[mw][v] + P2[mw][v];
-        const float yi = P3[mw][v] - P1[mw][v] + P2[mw][v];
-        const float2 p = Ps[col * PS + (ib - i_s) + 16 * mw + 4 * kk + v];
+        float yr, yi;
+        if constexpr (STK) {
+          const bool lo = col < 8;
+          yr = lo ? P1[mw][v] : P2[mw][v];
+          yi = lo ? -P2[mw][v] : P1[mw][v];
+        } else {
+          yr = P1[mw][v] + P2[mw][v];
+          yi = P3[mw][v] - P1[mw][v] + P2[mw][v];
+        }
+        const float2 p = Ps[ccol * PS + (ib - i_s) + 16 * mw + 4 * kk + v];
         gx[mw][v] += p.x * yr + p.y * yi;
         gy[mw][v] += p.x * yi - p.y * yr;
       }
@@ -612,9 +672,14 @@ void launch_adj_structured(const LaunchSet& L, const OutPtrs& outs, hipStream_t 
   const int MW = L.adj_mw;
   const size_t lds = adj_lds_bytes(MW, 16 * L.MT, L.Np);
   const dim3 grid(L.n_wg);
-#define BORN_ADJ_LAUNCH(MTv, MWv, NPv) \
-  hipLaunchKernelGGL((k_adj_s2d<MTv, MWv, NPv>), grid, dim3(256), lds, s, L.d_inst, (int)L.inst.size(), outs, \
+#define BORN_ADJ_LAUNCH2(MTv, MWv, NPv, Sv)                                                                  \
+  hipLaunchKernelGGL((k_adj_s2d<MTv, MWv, NPv, Sv>), grid, dim3(256), lds, s, L.d_inst, (int)L.inst.size(), outs, \
                      L.d_chunks, L.d_chunk_pairs)
+#define BORN_ADJ_LAUNCH(MTv, MWv, NPv)                               \
+  do {                                                               \
+    if (NPv != 0 && L.stk) BORN_ADJ_LAUNCH2(MTv, MWv, NPv, (NPv != 0)); \
+    else BORN_ADJ_LAUNCH2(MTv, MWv, NPv, false);                     \
+  } while (0)
 #define BORN_ADJ_MW(MTv, NPv)                     \
   if (MW == 2) BORN_ADJ_LAUNCH(MTv, 2, NPv);      \
   else BORN_ADJ_LAUNCH(MTv, 4, NPv);
@@ -635,19 +700,28 @@ void launch_adj_structured(const LaunchSet& L, const OutPtrs& outs, hipStream_t 
 #undef BORN_ADJ_NP
 #undef BORN_ADJ_MW
 #undef BORN_ADJ_LAUNCH
+#undef BORN_ADJ_LAUNCH2
 }
 
 // Allow dynamic LDS up to the CU's 160 KiB for the structured kernels.
 template <int MT>
 static void set_max_lds_mt(int b) {
 #define BORN_SET(K) hipFuncSetAttribute((const void*)(K), hipFuncAttributeMaxDynamicSharedMemorySize, b)
-  BORN_SET((k_fwd_s2d<MT, 4, 1, false>)); BORN_SET((k_fwd_s2d<MT, 4, 2, false>)); BORN_SET((k_fwd_s2d<MT, 4, 4, false>));
-  BORN_SET((k_fwd_s2d<MT, 2, 1, false>)); BORN_SET((k_fwd_s2d<MT, 2, 2, false>)); BORN_SET((k_fwd_s2d<MT, 2, 4, false>));
-  BORN_SET((k_fwd_s2d<MT, 0, 1, false>)); BORN_SET((k_fwd_s2d<MT, 0, 2, false>)); BORN_SET((k_fwd_s2d<MT, 0, 4, false>));
-  BORN_SET((k_fwd_s2d<MT, 4, 1, true>)); BORN_SET((k_fwd_s2d<MT, 2, 1, true>)); BORN_SET((k_fwd_s2d<MT, 0, 1, true>));
-  BORN_SET((k_adj_s2d<MT, 2, 4>)); BORN_SET((k_adj_s2d<MT, 4, 4>));
-  BORN_SET((k_adj_s2d<MT, 2, 2>)); BORN_SET((k_adj_s2d<MT, 4, 2>));
-  BORN_SET((k_adj_s2d<MT, 2, 0>)); BORN_SET((k_adj_s2d<MT, 4, 0>));
+  BORN_SET((k_fwd_s2d<MT, 4, 1, false, false>)); BORN_SET((k_fwd_s2d<MT, 4, 2, false, false>));
+  BORN_SET((k_fwd_s2d<MT, 4, 4, false, false>)); BORN_SET((k_fwd_s2d<MT, 2, 1, false, false>));
+  BORN_SET((k_fwd_s2d<MT, 2, 2, false, false>)); BORN_SET((k_fwd_s2d<MT, 2, 4, false, false>));
+  BORN_SET((k_fwd_s2d<MT, 0, 1, false, false>)); BORN_SET((k_fwd_s2d<MT, 0, 2, false, false>));
+  BORN_SET((k_fwd_s2d<MT, 0, 4, false, false>)); BORN_SET((k_fwd_s2d<MT, 4, 1, true, false>));
+  BORN_SET((k_fwd_s2d<MT, 2, 1, true, false>)); BORN_SET((k_fwd_s2d<MT, 0, 1, true, false>));
+  BORN_SET((k_fwd_s2d<MT, 4, 1, false, true>)); BORN_SET((k_fwd_s2d<MT, 4, 2, false, true>));
+  BORN_SET((k_fwd_s2d<MT, 4, 4, false, true>)); BORN_SET((k_fwd_s2d<MT, 2, 1, false, true>));
+  BORN_SET((k_fwd_s2d<MT, 2, 2, false, true>)); BORN_SET((k_fwd_s2d<MT, 2, 4, false, true>));
+  BORN_SET((k_fwd_s2d<MT, 4, 1, true, true>)); BORN_SET((k_fwd_s2d<MT, 2, 1, true, true>));
+  BORN_SET((k_adj_s2d<MT, 2, 4, false>)); BORN_SET((k_adj_s2d<MT, 4, 4, false>));
+  BORN_SET((k_adj_s2d<MT, 2, 2, false>)); BORN_SET((k_adj_s2d<MT, 4, 2, false>));
+  BORN_SET((k_adj_s2d<MT, 2, 0, false>)); BORN_SET((k_adj_s2d<MT, 4, 0, false>));
+  BORN_SET((k_adj_s2d<MT, 2, 4, true>)); BORN_SET((k_adj_s2d<MT, 4, 4, true>));
+  BORN_SET((k_adj_s2d<MT, 2, 2, true>)); BORN_SET((k_adj_s2d<MT, 4, 2, true>));
 #undef BORN_SET
 }
 

@@ -228,7 +228,10 @@ def test_config3_batch(torch_cuda):
         assert rel(fb[i], op.forward_dev(g[i])) < 1e-6
         assert rel(ab[i], op.adjoint_dev(u[i])) < 1e-6
     sides = [s.fov.N_points_axis[0] for s in specs]
-    pick = sorted({sides.index(v) for v in (128, 512) if v in sides} | {len(specs) - 1})
+    # the smallest and largest rows, the last instance and the first with 5-8
+    # waves (the stacked 8-column MFMA form)
+    mid = [k for k, s in enumerate(specs) if 5 <= s.N_measurements <= 8][:1]
+    pick = sorted({sides.index(v) for v in (128, 512) if v in sides} | {len(specs) - 1} | set(mid))
     rng = np.random.Generator(np.random.PCG64(18))
     for i in pick:
         spec, op = specs[i], ops[i]

@@ -986,3 +986,46 @@ def test_few_column_kernels(torch_cuda, dims, n_waves, n_rhs, precision, tol):
         % (dims, n_waves, n_rhs, precision, e_f, e_a, e_fd, e_ad))
     assert max(e_f, e_a, e_fd, e_ad) < tol
     op.close()
+
+
+# ------------------------------------------------- stacked 8-column chunks
+@pytest.mark.parametrize("n_waves,n_rhs", [(5, 1), (8, 1), (9, 1), (3, 2), (1, 8)])
+@pytest.mark.parametrize("dims", ["2d", "2d_dual", "3d"])
+def test_stacked_mfma_kernels(torch_cuda, dims, n_waves, n_rhs):
+    """FP32 chunks of 5-8 columns (5-8 waves; 3 waves x 2 and 1 wave x 8
+    right-hand sides packed per column) run on the plain MFMA kernels in the
+    stacked form (real parts of the 8 columns in one half of the B operand,
+    imaginary parts in the other); 9 waves run in the 16-column form.  Rows
+    of 170 points (40 elements), of 100 points (24 elements, two rows per
+    forward step) and 3-D at pitch/2, host and device entry points, against
+    the oracle."""
+    import torch
+    med = C.MediumSpec()
+    if dims != "3d":
+        N_e = 40 if dims == "2d" else 24
+        spec = tiny_spec(N_e=N_e, Nx=170 if dims == "2d" else 100, Nz=5, npp=4)
+        spec.waves = [C.qpw_wave(spec.array, a, med.c_avg) for a in np.linspace(-0.25, 0.25, n_waves)]
+        spec.mixes = [C.identity_mixes(N_e) for _ in spec.waves]
+        T = O.build_tables(spec)
+        op = OperatorBorn.from_tables(T)
+    else:
+        arr = C.ArraySpec((20, 2), (3e-4, 3e-4), (2.7e-4, 2.7e-4), (2, 2))
+        fov = C.FovSpec((130, 6, 2), (1.5e-4, 1.5e-4, 1.5e-4), 2e-3)
+        waves = [C.qsw_wave(arr, np.array([x, 0.0, -1e-2]), med.c_avg) for x in np.linspace(-6e-4, 6e-4, n_waves)]
+        spec = C.SequenceSpec("stk3", arr, med, fov, np.array([2.5e6, 4e6, 5.5e6]), waves,
+                              [C.identity_mixes(40) for _ in waves])
+        T = O.direct_tables(spec)
+        op = OperatorBorn.from_sequence(spec)
+        assert op.path == "structured"
+    g = rnd((T.N_points, n_rhs), 121)
+    u = rnd((T.N_observations(), n_rhs), 122)
+    e_f = O.rel_l2(op.forward(g), O.forward(T, g))
+    e_a = O.rel_l2(op.adjoint(u), O.adjoint(T, u))
+    gd = torch.from_numpy(g.T.astype(np.complex64)).cuda().contiguous()
+    ud = torch.from_numpy(u.T.astype(np.complex64)).cuda().contiguous()
+    e_fd = O.rel_l2(op.forward_dev(gd).cpu().numpy().T, O.forward(T, g.astype(np.complex64)))
+    e_ad = O.rel_l2(op.adjoint_dev(ud).cpu().numpy().T, O.adjoint(T, u.astype(np.complex64)))
+    log("stacked %s, %d waves, %d rhs: forward %.2e adjoint %.2e (device %.2e %.2e)"
+        % (dims, n_waves, n_rhs, e_f, e_a, e_fd, e_ad))
+    assert max(e_f, e_a, e_fd, e_ad) < TOL_FP32
+    op.close()
