@@ -284,8 +284,9 @@ def copy_probe(mib: int):
 def bench_batched(args):
     """Config 3: 24 operator instances of uneven size (FOV side 128-512 at
     pitch/4, 33-257 bins, 1-11 plane waves, config-1 array) applied as one
-    batch: one persistent contraction launch per direction, replayed from a
-    hipGraph, tile shapes chosen by the autotuner.  A step is the forward and
+    batch: a few persistent contraction launches per direction (one per group
+    of instances that take the same kernel form), replayed from a hipGraph,
+    tile shapes chosen by the autotuner.  A step is the forward and
     the adjoint of every instance."""
     import torch
     from comp_ui_toolbox_amd import OperatorBorn, OperatorBatch
@@ -369,7 +370,7 @@ def bench_batched(args):
         "data": "synthetic (config-3 instances, 30-point phantoms; tables built on the device)",
         "config": {
             "workload": "config3: 24 instances, FOV side {128,256,384,512} at pitch/4, 33-257 f, 1-11 waves, "
-                        "128 el, 1 rhs each; one persistent launch per direction, hipGraph replay, autotuned",
+                        "128 el, 1 rhs each; persistent launches per kernel form and direction, hipGraph replay, autotuned",
             "precision": prec,
             "instance_pairs_per_s": len(ops) * args.steps / elapsed,
             "table_build_s": round(t_tables, 3),

@@ -533,8 +533,9 @@ def combined_quick_gpu(operator_born, mode, input_matrix):
 
 class OperatorBatch:
     """Several operator instances applied together, one right-hand side each:
-    one persistent contraction launch per direction covers all of them
-    (born_batch_*).  The plans must stay alive while the batch exists."""
+    a few persistent contraction launches per direction cover all of them,
+    one per group of instances that take the same kernel form (born_batch_*).
+    The plans must stay alive while the batch exists."""
 
     def __init__(self, ops, graph: bool = False):
         self.ops = list(ops)

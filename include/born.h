@@ -301,8 +301,9 @@ int born_plan_get_timing(born_plan_t plan, double* ms, int64_t* launches);
 
 /* ---------------------------------------------------------------- batches
  * Several operator instances (plans of uneven FOV, frequency and wave
- * counts) applied together, one right-hand side each: one persistent
- * contraction launch per direction covers every instance.  No counterpart in
+ * counts) applied together, one right-hand side each: a few persistent
+ * contraction launches per direction, one per group of instances that take
+ * the same kernel form, cover every instance.  No counterpart in
  * the reference (each MEX call serves one operator).  The plans must be on
  * the structured path, share the device, the precision, the element padding
  * and the points per pitch, and outlive the batch; at most 64 plans.  A plan
