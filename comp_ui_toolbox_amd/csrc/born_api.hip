@@ -1663,7 +1663,10 @@ static void build_batch_launch(Batch& B) {
   for (size_t i = 0; i < B.plans.size(); ++i) {
     const Plan& P = *B.plans[i];
     const int w = fewcol_width(chunk_max_cols(P.chunks));
-    if (w > 0 && P.d_gpart && fewcol_fits(P.Nx, P.Np, P.Ne_pad, w, P.n_rows)) B.set_of[i] = w == 1 ? 1 : (w == 2 ? 2 : 3);
+    // (the split forms run few-column sets on the vector kernels only for one
+    // column, so their 2-4-column plans join the MFMA set)
+    const bool fc = w > 0 && (B.precision == BORN_PRECISION_FP32 || w == 1);
+    if (fc && P.d_gpart && fewcol_fits(P.Nx, P.Np, P.Ne_pad, w, P.n_rows)) B.set_of[i] = w == 1 ? 1 : (w == 2 ? 2 : 3);
     else if (B.precision == BORN_PRECISION_FP32) {
       const bool mw2 = adj_mw_default(P.Nx) == 2, stk = chunk_max_cols(P.chunks) <= NC / 2;
       B.set_of[i] = stk ? (mw2 ? 6 : 5) : (mw2 ? 4 : 0);
