@@ -135,6 +135,21 @@ def cpu_baseline(op, spec, n_mixes: int, threads: int):
     }
 
 
+def host_entry_rate(op, form, x0, pairs: int = 3):
+    """Pairs/s through the host-buffer entry points (born_forward /
+    born_adjoint: FP64 host arrays, conversions and PCIe transfers
+    included), timed after the device legs, outside their timed region."""
+    op.set_precision(form)
+    xh = np.asarray(x0, dtype=np.complex128).reshape(-1, 1)
+    op.adjoint(op.forward(xh))   # warm-up
+    t0 = time.perf_counter()
+    for _ in range(pairs):
+        op.adjoint(op.forward(xh))
+    dt = (time.perf_counter() - t0) / pairs
+    return {"precision": form, "pairs": pairs, "ms_per_pair": dt * 1e3, "pairs_per_s": 1.0 / dt,
+            "note": "host FP64 buffers: conversions and PCIe transfers of gamma and u included"}
+
+
 def err_vs_fp64(op, spec, forms, f_sel=(0, 128, 256)):
     """The checker: error of each numeric form against the FP64 oracle on a
     dense input over the whole FOV, at 3 of the frequencies (first, middle,
@@ -503,6 +518,11 @@ def main():
             "adjoint": rel_l2(outs[args.precision][1], outs["fp32"][1]),
         }
         result["roofline_fp32"] = roofline(F, k32, "fp32", args.config)
+    if n_rhs == 1:
+        try:
+            result["config"]["host_entry"] = host_entry_rate(op, args.precision, x0)
+        except Exception as e:  # report, do not hide
+            result["config"]["host_entry"] = {"error": repr(e)}
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         if spec.fov.N_dimensions == 2 and n_rhs == 1:
