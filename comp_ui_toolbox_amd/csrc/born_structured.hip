@@ -9,7 +9,9 @@
 // and the adjoint its conjugate transpose.  Both run on 16x16x4 FP32 MFMA with
 // the three-multiplication complex product (Re = P1 - P2 or P1 + P2, Im from
 // the product of the sums), so each complex multiply-add costs three real
-// MFMA products instead of four.
+// MFMA products instead of four.  Launches whose chunks hold at most 8
+// columns take the stacked form instead (STK): real and imaginary parts of
+// the 8 columns side by side in the 16 MFMA columns, two MFMAs per tile.
 //
 // Data movement: the next row (forward) or next chunk (adjoint) is loaded
 // from HBM into registers while the MFMAs of the current one run, then
