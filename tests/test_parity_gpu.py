@@ -990,20 +990,20 @@ def test_few_column_kernels(torch_cuda, dims, n_waves, n_rhs, precision, tol):
 
 # ------------------------------------------------- stacked 8-column chunks
 @pytest.mark.parametrize("n_waves,n_rhs", [(5, 1), (8, 1), (9, 1), (3, 2), (1, 8)])
-@pytest.mark.parametrize("dims", ["2d", "2d_dual", "3d"])
+@pytest.mark.parametrize("dims", ["2d", "2d_dual", "2d_mt1", "3d"])
 def test_stacked_mfma_kernels(torch_cuda, dims, n_waves, n_rhs):
     """FP32 chunks of 5-8 columns (5-8 waves; 3 waves x 2 and 1 wave x 8
     right-hand sides packed per column) run on the plain MFMA kernels in the
     stacked form (real parts of the 8 columns in one half of the B operand,
     imaginary parts in the other); 9 waves run in the 16-column form.  Rows
     of 170 points (40 elements), of 100 points (24 elements, two rows per
-    forward step) and 3-D at pitch/2, host and device entry points, against
-    the oracle."""
+    forward step), of 150 points with 12 elements (one element tile) and 3-D
+    at pitch/2, host and device entry points, against the oracle."""
     import torch
     med = C.MediumSpec()
     if dims != "3d":
-        N_e = 40 if dims == "2d" else 24
-        spec = tiny_spec(N_e=N_e, Nx=170 if dims == "2d" else 100, Nz=5, npp=4)
+        N_e, Nx = {"2d": (40, 170), "2d_dual": (24, 100), "2d_mt1": (12, 150)}[dims]
+        spec = tiny_spec(N_e=N_e, Nx=Nx, Nz=5, npp=4)
         spec.waves = [C.qpw_wave(spec.array, a, med.c_avg) for a in np.linspace(-0.25, 0.25, n_waves)]
         spec.mixes = [C.identity_mixes(N_e) for _ in spec.waves]
         T = O.build_tables(spec)
