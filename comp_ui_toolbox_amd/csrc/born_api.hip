@@ -1550,21 +1550,21 @@ struct Batch {
   int precision = 0;
   std::vector<Plan*> plans;
   std::vector<uint64_t> gen;     // workspace generation of each plan when the records were made
-  // instances whose chunks hold at most 1, 2 or 4 columns run in a launch of
-  // their own per width on the few-column kernels (sets 1-3), the others in
+  // instances whose chunks hold at most 1, 2, 4 or 3 columns run in a launch
+  // of their own per width on the few-column kernels (sets 1-3, 7), the others in
   // MFMA launches: set 0, and in the FP32 form set 4 for the instances whose
   // rows fill 128-point adjoint segments better than 256-point ones (rows of
   // at most 128 points; 384 points: 3 segments of 128 instead of 2 of 256,
   // one half empty); FP32 instances of at most 8 columns per chunk take the
   // stacked kernels in sets 5 (like 0) and 6 (like 4)
-  static constexpr int N_SETS = 7;
+  static constexpr int N_SETS = 8;
   LaunchSet sets[N_SETS];
   std::vector<int> set_of;         // per plan: its launch set
   std::vector<int> slot;           // per plan: its index in its launch set
   Chunk* d_chunks[N_SETS] = {};
   int* d_chunk_pairs[N_SETS] = {};
   LaunchSet& set(int k) { return sets[k]; }
-  static bool mfma_set(int k) { return k == 0 || k >= 4; }
+  static bool mfma_set(int k) { return k == 0 || (k >= 4 && k <= 6); }
   bool graph = false;
   struct Captured {
     hipGraph_t g = nullptr;
@@ -1666,7 +1666,7 @@ static void build_batch_launch(Batch& B) {
     // (the split forms run few-column sets on the vector kernels only for one
     // column, so their 2-4-column plans join the MFMA set)
     const bool fc = w > 0 && (B.precision == BORN_PRECISION_FP32 || w == 1);
-    if (fc && P.d_gpart && fewcol_fits(P.Nx, P.Np, P.Ne_pad, w, P.n_rows)) B.set_of[i] = w == 1 ? 1 : (w == 2 ? 2 : 3);
+    if (fc && P.d_gpart && fewcol_fits(P.Nx, P.Np, P.Ne_pad, w, P.n_rows)) B.set_of[i] = w == 1 ? 1 : (w == 2 ? 2 : (w == 3 ? 7 : 3));
     else if (B.precision == BORN_PRECISION_FP32) {
       const bool mw2 = adj_mw_default(P.Nx) == 2, stk = chunk_max_cols(P.chunks) <= NC / 2;
       B.set_of[i] = stk ? (mw2 ? 6 : 5) : (mw2 ? 4 : 0);

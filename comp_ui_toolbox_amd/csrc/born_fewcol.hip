@@ -5,7 +5,7 @@
 // w plane waves fills w of the 16 MFMA columns of the contraction kernels
 // (born_structured.hip, born_split.hip): with one wave, 15/16 of every MFMA
 // is padding.  For w <= 4 the Toeplitz products run here as FP32 FMAs
-// instead, W = 1, 2 or 4 columns per chunk, with the same inputs and outputs
+// instead, W = 1, 2, 3 or 4 columns per chunk, with the same inputs and outputs
 // as k_fwd_s2d / k_adj_s2d: the forward writes the same row-block partial
 // sums (summed in FP64 by k_fwd_reduce), the adjoint writes gamma_hat once
 // per right-hand side.
@@ -54,8 +54,8 @@ __host__ __device__ inline int fc_fwd_rows(const FcGeom& g) {
   while (R < 8 && R * g.tpr < 256) R <<= 1;
   return R;
 }
-// Task shapes: forward W = 1, 2: E 8 x S 16, 4: 8 x 8 (sums of E x W);
-// adjoint W = 1: E 16 x S 8, 2: 16 x 8, 4: 16 x 4 with at most fc_km
+// Task shapes: forward W = 1, 2: E 8 x S 16, 3, 4: 8 x 8 (sums of E x W);
+// adjoint W = 1: E 16 x S 8, 2: 16 x 8, 3, 4: 16 x 4 with at most fc_km
 // tasks per thread (sums of S x W per task).  (W = 1 keeps four workgroups
 // per compute unit at the cost of a 16-byte register spill: measured faster
 // than E 8, and than three workgroups without the spill.)
@@ -63,11 +63,11 @@ constexpr int FC_LDS_MAX = 78 * 1024;   // two workgroups per compute unit
 constexpr int FC_ROWS_MAX = 16;         // rows an adjoint workgroup stages (its row table in LDS)
 void fc_fwd_shape(int fc_w, int& E, int& S) {
   E = 8;
-  S = fc_w == 4 ? 8 : 16;
+  S = fc_w >= 3 ? 8 : 16;
 }
 void fc_adj_shape(int fc_w, int& E, int& S) {
   E = 16;
-  S = fc_w == 4 ? 4 : 8;
+  S = fc_w >= 3 ? 4 : 8;
 }
 // tasks per thread: W = 1 one (4 waves per SIMD), W = 2, 4 two; one with
 // right-hand sides packed per column (registers)
@@ -141,7 +141,7 @@ int fewcol_fwd_rows(int Nx, int Np, int Ne_pad, int fc_w) {
   return fc_fwd_rows(fc_geom(Nx, Np, Ne_pad, E, S));
 }
 
-int fewcol_width(int max_cols) { return max_cols <= 1 ? 1 : (max_cols <= 2 ? 2 : (max_cols <= 4 ? 4 : 0)); }
+int fewcol_width(int max_cols) { return max_cols <= 4 ? std::max(max_cols, 1) : 0; }
 
 bool fewcol_fits(int Nx, int Np, int Ne_pad, int fc_w, int n_rows) {
   if (fc_w <= 0 || !(Ne_pad == 16 || Ne_pad == 32 || Ne_pad == 64 || Ne_pad == 128)) return false;
@@ -561,6 +561,7 @@ void launch_fwd_fc(const LaunchSet& L, hipStream_t s) {
                      L.d_chunk_pairs)
   if (L.fc_w == 1) BORN_FC_F(1, 8, 16);
   else if (L.fc_w == 2) BORN_FC_F(2, 8, 16);
+  else if (L.fc_w == 3) BORN_FC_F(3, 8, 8);
   else BORN_FC_F(4, 8, 8);
 #undef BORN_FC_F
 }
@@ -579,6 +580,7 @@ void launch_adj_fc(const LaunchSet& L, const OutPtrs& outs, hipStream_t s) {
   if (L.fc_w == 1) BORN_FC_A(1, 16, 8, 1, 1);
   else if (L.fc_w == 2 && kr == 2) BORN_FC_A(2, 16, 8, 2, 1);
   else if (L.fc_w == 2) BORN_FC_A(2, 16, 8, 1, 2);
+  else if (L.fc_w == 3) BORN_FC_A(3, 16, 4, 1, 2);   // 3 columns: one rhs per chunk
   else if (kr == 4) BORN_FC_A(4, 16, 4, 4, 1);
   else if (kr == 2) BORN_FC_A(4, 16, 4, 2, 1);
   else BORN_FC_A(4, 16, 4, 1, 2);
@@ -625,10 +627,12 @@ void set_max_lds_fc(int bytes) {
 #define BORN_SET(K) (void)hipFuncSetAttribute((const void*)(K), hipFuncAttributeMaxDynamicSharedMemorySize, bytes)
   BORN_SET((k_fwd_fc<1, 8, 16>));
   BORN_SET((k_fwd_fc<2, 8, 16>));
+  BORN_SET((k_fwd_fc<3, 8, 8>));
   BORN_SET((k_fwd_fc<4, 8, 8>));
   BORN_SET((k_adj_fc<1, 16, 8, 1, 1>));
   BORN_SET((k_adj_fc<2, 16, 8, 1, 2>));
   BORN_SET((k_adj_fc<2, 16, 8, 2, 1>));
+  BORN_SET((k_adj_fc<3, 16, 4, 1, 2>));
   BORN_SET((k_adj_fc<4, 16, 4, 1, 2>));
   BORN_SET((k_adj_fc<4, 16, 4, 2, 1>));
   BORN_SET((k_adj_fc<4, 16, 4, 4, 1>));
