@@ -520,8 +520,12 @@ static void finalize_launchset(LaunchSet& L) {
   L.fc_w = L.inst.empty() || !L.fc_allowed ? 0 : fewcol_width(max_cols);
   // the plain MFMA kernels take the 8-column stacked form when every chunk fits
   L.stk = max_cols > 0 && max_cols <= NC / 2;
-  for (const Inst& I : L.inst)
+  for (const Inst& I : L.inst) {
     if (!I.gpart || !fewcol_fits(I.Nx, I.Np, 16 * L.MT, L.fc_w, I.n_rows)) L.fc_w = 0;
+    // the 3-column kernels take one rhs per chunk (right-hand sides packed
+    // per column give chunks of a multiple of 2 columns, never 3)
+    if (L.fc_w == 3 && I.col_rhs > 1) L.fc_w = 0;
+  }
   for (const Inst& I : L.inst) {
     int bt = 0, nr = 0;
     if (L.fc_w > 0) fewcol_adj_block(I.Nx, I.Np, 16 * L.MT, L.fc_w, I.col_rhs, I.n_rows, bt, nr);
