@@ -1837,7 +1837,7 @@ struct TuneResult {
 static TuneResult autotune(std::vector<Plan*>& plans, LaunchSet& L, int precision, int n_cu,
                            const std::function<void()>& rebuild, hipStream_t s) {
   const int items_c[] = {1024, 2048, 4096, 8192};
-  const int grid_c[] = {1, 2};
+  const int grid_c[] = {1, 2, 3};   // persistent workgroups per CU (3: where registers and LDS allow)
   TuneResult best{plans[0]->items_target, L.grid_mult, L.adj_mw, 1e30f, 1e30f};
   L.f16 = precision == BORN_PRECISION_SPLIT_FP16;
   OutPtrs outs;

@@ -72,6 +72,7 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(const Inst* __restrict__ ins
   constexpr int PW = 4 / EW;                 // point groups
   constexpr int MTW = MT / EW;               // element tiles per wave
   constexpr int NCF = STK ? NC / 2 : NC;     // columns loaded and filled
+  constexpr int NCB = NCF + 1;               // padded LDS stride of a B row
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int eg = w % EW, pg = w / EW;
   const int kk = lane >> 4, col = lane & 15;
@@ -177,14 +178,14 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(const Inst* __restrict__ ins
     SR[mt] = vzero();
     SI[mt] = vzero();
   }
-  const int b_base = kk * NCS + (STK ? (col & 7) : col);
+  const int b_base = kk * NCB + (STK ? (col & 7) : col);
   int it = 0;
 
   while (true) {
     // layout of the row about to be written to LDS
     const int cNp = Np, cQL = QL, cHN = HN, cXP1 = XP, cXP = ZS * XP;
     float2* Hp = lds;              // [ZS][Np][QL] polyphase extended rows
-    float2* Bs = lds + ZS * cHN;   // [ZS * XP][NCS]
+    float2* Bs = lds + ZS * cHN;   // [ZS * XP][NCB]
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < HPT; ++j) {
@@ -203,7 +204,7 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(const Inst* __restrict__ ins
           const bool ok = DUAL ? ((gvalid >> j) & 1) != 0 : (xfull || i < n_Nx);
           const float2 g = ok ? greg[j] : make_float2(0.f, 0.f);
 #pragma unroll
-          for (int c = 0; c < NCF; ++c) Bs[i * NCS + c] = cmulf(praw[j][c], g);
+          for (int c = 0; c < NCF; ++c) Bs[i * NCB + c] = cmulf(praw[j][c], g);
         }
       }
     }
@@ -252,7 +253,7 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(const Inst* __restrict__ ins
     }
     for (int s = 0; s < SW; ++s) {
       const int ks = pg * SW + s;
-      const float2 b = Bs[b_base + 4 * ks * NCS];
+      const float2 b = Bs[b_base + 4 * ks * NCB];
       const float bsum = b.x + b.y;
       const float bst = col < 8 ? b.x : b.y;   // STK: [B_re | B_im]
       int ai;
@@ -335,11 +336,16 @@ __global__ __launch_bounds__(256, 2) void k_fwd_s2d(const Inst* __restrict__ ins
 #undef BORN_FWD_INST
 }
 
-static size_t fwd_lds_bytes(int MT, int S, int Np, bool dual = false) {
+// LDS of k_fwd_s2d: the step's polyphase rows and B tile (B rows of 8
+// columns in the stacked form), or the item's reduction buffer
+// [wave][element tile of the wave][8][64] floats.
+static size_t fwd_lds_bytes(int MT, int S, int Np, bool dual = false, bool stk = false) {
   const int XP = 16 * S;
   const int QL = (XP - 1) / Np + 16 * MT;
-  const size_t body = (dual ? 2 : 1) * ((size_t)Np * QL + (size_t)XP * NCS) * sizeof(float2);
-  const size_t red = (size_t)4 * MT * 8 * 64 * sizeof(float);
+  const int ncb = stk ? NC / 2 + 1 : NCS;
+  const size_t body = (dual ? 2 : 1) * ((size_t)Np * QL + (size_t)XP * ncb) * sizeof(float2);
+  const int mtw = MT >= 2 ? MT / 2 : 1;
+  const size_t red = (size_t)4 * mtw * 8 * 64 * sizeof(float);
   return std::max(body, red);
 }
 
@@ -366,7 +372,9 @@ void launch_fwd_structured(const LaunchSet& L, int n_cu, hipStream_t s) {
   }
   const int PPT = fwd_ppt(L.S_max);
   const bool dual = fwd_dual(L.S_max);
-  const size_t lds = fwd_lds_bytes(L.MT, L.S_max, L.Np, dual);
+  // the stacked kernels serve 4 and 2 points per pitch (as the launch below)
+  const bool stk = L.stk && (L.Np == 4 || L.Np == 2);
+  const size_t lds = fwd_lds_bytes(L.MT, L.S_max, L.Np, dual, stk);
   const int grid = std::max(1, std::min(L.n_items, L.grid_mult * n_cu));
   const int Ne_pad = 16 * L.MT;
 #define BORN_FWD_LAUNCH2(MTv, NPv, PPTv, Dv, Sv)                                                              \
@@ -436,7 +444,7 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d(const Inst* __restrict__ ins
   const int L = SEG + (NEP - 1) * Np;
   float2* He = lds;                           // [L]
   float2* Ws = He + L;                        // [NEP][NCS]
-  float2* Ps = Ws + NEP * NCS;                // [NC][PS]
+  float2* Ps = Ws + NEP * NCS;                // [NCP][PS]
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int kk = lane >> 4, col = lane & 15;
@@ -650,10 +658,12 @@ __global__ __launch_bounds__(256, 2) void k_adj_s2d(const Inst* __restrict__ ins
 }
 
 
-static size_t adj_lds_bytes(int MW, int Ne_pad, int Np) {
+// LDS of k_adj_s2d: window, W tile, p_inc rows (8 in the stacked form).
+static size_t adj_lds_bytes(int MW, int Ne_pad, int Np, bool stk = false) {
   const int SEG = 64 * MW;
   const int L = SEG + (Ne_pad - 1) * Np;
-  return ((size_t)L + (size_t)Ne_pad * NCS + (size_t)NC * (SEG + 4)) * sizeof(float2);
+  const int ncp = stk ? NC / 2 : NC;
+  return ((size_t)L + (size_t)Ne_pad * NCS + (size_t)ncp * (SEG + 4)) * sizeof(float2);
 }
 
 bool adj_structured_fits(int Nx, int Ne_pad, int Np) {
@@ -672,7 +682,8 @@ int adj_mw_default(int Nx) {
 
 void launch_adj_structured(const LaunchSet& L, const OutPtrs& outs, hipStream_t s) {
   const int MW = L.adj_mw;
-  const size_t lds = adj_lds_bytes(MW, 16 * L.MT, L.Np);
+  const bool stk = L.stk && (L.Np == 4 || L.Np == 2);   // as the launch below
+  const size_t lds = adj_lds_bytes(MW, 16 * L.MT, L.Np, stk);
   const dim3 grid(L.n_wg);
 #define BORN_ADJ_LAUNCH2(MTv, MWv, NPv, Sv)                                                                  \
   hipLaunchKernelGGL((k_adj_s2d<MTv, MWv, NPv, Sv>), grid, dim3(256), lds, s, L.d_inst, (int)L.inst.size(), outs, \
