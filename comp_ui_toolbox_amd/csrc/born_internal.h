@@ -109,7 +109,7 @@ struct LaunchSet {
   bool stk = false;               // plain kernels: chunks of at most 8 columns (stacked B operand)
   int n_items = 0, n_items_s = 0;
   int n_wg = 0, n_wg_s = 0;
-  int grid_mult = 2;              // persistent forward workgroups per CU
+  int grid_mult = 0;              // persistent forward workgroups per CU (0: the kernel's default)
   bool f16 = false;               // split kernels: FP16 (true) or BF16 hi/lo operands
   int fc_w = 0;                   // > 0: chunks of at most fc_w columns run on the few-column kernels
   bool fc_allowed = true;         // false: MFMA kernels only (the allocation-free layout)

@@ -811,7 +811,7 @@ void launch_fwd_split(const LaunchSet& L, int n_cu, hipStream_t s) {
     launch_fwd_split_slices(L, n_cu, s);
     return;
   }
-  const int grid = std::max(1, std::min(L.n_items_s, L.grid_mult * n_cu));
+  const int grid = std::max(1, std::min(L.n_items_s, (L.grid_mult > 0 ? L.grid_mult : 2) * n_cu));
   bool even = true;
   for (const Inst& I : L.inst) even = even && (I.Nx % 2) == 0;
   const int n_inst = (int)L.inst.size();

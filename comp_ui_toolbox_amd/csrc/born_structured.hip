@@ -375,7 +375,9 @@ void launch_fwd_structured(const LaunchSet& L, int n_cu, hipStream_t s) {
   // the stacked kernels serve 4 and 2 points per pitch (as the launch below)
   const bool stk = L.stk && (L.Np == 4 || L.Np == 2);
   const size_t lds = fwd_lds_bytes(L.MT, L.S_max, L.Np, dual, stk);
-  const int grid = std::max(1, std::min(L.n_items, L.grid_mult * n_cu));
+  // the stacked kernels fit 3 workgroups per CU (registers and LDS), the others 2
+  const int gm = L.grid_mult > 0 ? L.grid_mult : (stk ? 3 : 2);
+  const int grid = std::max(1, std::min(L.n_items, gm * n_cu));
   const int Ne_pad = 16 * L.MT;
 #define BORN_FWD_LAUNCH2(MTv, NPv, PPTv, Dv, Sv)                                                              \
   hipLaunchKernelGGL((k_fwd_s2d<MTv, NPv, PPTv, Dv, Sv>), dim3(grid), dim3(256), lds, s, L.d_inst, (int)L.inst.size(), \

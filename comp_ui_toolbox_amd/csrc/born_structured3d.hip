@@ -245,7 +245,7 @@ static size_t fwd_s3d_lds_bytes() {
 }
 
 void launch_fwd_s3d(const LaunchSet& L, int n_cu, hipStream_t s) {
-  const int grid = std::max(1, std::min(L.n_items, L.grid_mult * n_cu));
+  const int grid = std::max(1, std::min(L.n_items, (L.grid_mult > 0 ? L.grid_mult : 2) * n_cu));
   if (L.Np == 2)
     hipLaunchKernelGGL((k_fwd_s3d<2>), dim3(grid), dim3(256), fwd_s3d_lds_bytes<2>(), s, L.d_inst,
                        (int)L.inst.size(), L.n_items, L.d_chunks, L.d_chunk_pairs);
