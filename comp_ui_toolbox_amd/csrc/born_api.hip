@@ -1355,10 +1355,11 @@ static void fp16_scales(Plan& P, bool forward, int N_rhs, hipStream_t s) {
 
 // Contraction launches of a launch set in a numeric form.  Few-column sets
 // run on the FP32 vector kernels in the FP32 form, and in the split forms
-// when every chunk has one column (where the split MFMA kernels fill one
-// column in 16 and the FP32 kernels are both faster and more accurate).
+// when every chunk has at most two columns (where the split MFMA kernels
+// fill 1-2 columns in 16 and the FP32 kernels are more accurate and no
+// slower).
 static bool fc_path(const LaunchSet& L, int precision) {
-  return L.fc_w > 0 && (precision == BORN_PRECISION_FP32 || L.fc_w == 1);
+  return L.fc_w > 0 && (precision == BORN_PRECISION_FP32 || L.fc_w <= 2);
 }
 static void fwd_contract(const LaunchSet& L, int precision, int n_cu, hipStream_t s) {
   if (L.inst.empty()) return;
@@ -1667,9 +1668,9 @@ static void build_batch_launch(Batch& B) {
   for (size_t i = 0; i < B.plans.size(); ++i) {
     const Plan& P = *B.plans[i];
     const int w = fewcol_width(chunk_max_cols(P.chunks));
-    // (the split forms run few-column sets on the vector kernels only for one
-    // column, so their 2-4-column plans join the MFMA set)
-    const bool fc = w > 0 && (B.precision == BORN_PRECISION_FP32 || w == 1);
+    // (the split forms run few-column sets on the vector kernels only for at
+    // most two columns, so their 3-4-column plans join the MFMA set)
+    const bool fc = w > 0 && (B.precision == BORN_PRECISION_FP32 || w <= 2);
     if (fc && P.d_gpart && fewcol_fits(P.Nx, P.Np, P.Ne_pad, w, P.n_rows)) B.set_of[i] = w == 1 ? 1 : (w == 2 ? 2 : (w == 3 ? 7 : 3));
     else if (B.precision == BORN_PRECISION_FP32) {
       const bool mw2 = adj_mw_default(P.Nx) == 2, stk = chunk_max_cols(P.chunks) <= NC / 2;
