@@ -949,7 +949,7 @@ def test_energy_rx_arbitrary(torch_cuda):
 def test_few_column_kernels(torch_cuda, dims, n_waves, n_rhs, precision, tol):
     """Chunks of at most 4 columns (1-4 waves per frequency; with several
     right-hand sides packed per column) run on the FP32 vector kernels: one
-    column in both forms, 2-4 in the FP32 form.  Rows whose points per phase
+    or two columns in both forms, 3-4 in the FP32 form.  Rows whose points per phase
     are not a multiple of the task length, 2-D at pitch/4 and 3-D at pitch/2
     (20 x 2 elements, 130-point rows), host and device entry points, against
     the oracle.  20 right-hand sides are too many to pack per column: each
